@@ -1,0 +1,175 @@
+/*
+ * spinach_powder.h - C-ABI of the MI355X powder-averaged time-domain
+ * propagation library (libspinach_powder.so).
+ *
+ * The library replaces the orientation loop and the weighted sum of
+ * Spinach's static powder context for the case
+ *   powder(spin_system, @acquire, parameters, 'nmr')
+ * in the sphten-liouv formalism with no relaxation, no kinetics and no
+ * rotating frames. It receives the isotropic generator I and the 34
+ * anisotropic components Q{rank}{k,m} that hamiltonian() returns, the
+ * orientation grid, the initial state and the coil, and returns the
+ * powder-averaged free induction decay.
+ *
+ * Conventions
+ *  - Sparse matrices use MATLAB's CSC layout (mxGetJc / mxGetIr /
+ *    mxGetComplexDoubles with -R2018a interleaved complex), so a MEX gateway
+ *    passes MATLAB's own arrays without conversion.
+ *  - Complex vectors are interleaved (re, im) doubles.
+ *  - Every entry point returns 0 on success or an SP_E* code; the message of
+ *    the last failure on a context is available from sp_last_error(). No C++
+ *    exception crosses this boundary.
+ *  - The library owns nothing the caller passes; it copies what it needs.
+ *  - A context is bound to one device and one HIP stream; calls on one
+ *    context must not overlap (MATLAB calls MEX files from one thread).
+ */
+#ifndef SPINACH_POWDER_H
+#define SPINACH_POWDER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_OK 0
+#define SP_EINVAL 1   /* invalid argument (shape, NULL, non-finite) */
+#define SP_EDEVICE 2  /* HIP runtime failure */
+#define SP_ENOMEM 3   /* device or host allocation failed */
+#define SP_ERANGE 4   /* |L*dt| or step count outside the supported range */
+
+#define SP_MODE_FP64 0 /* complex FP64 throughout */
+#define SP_MODE_FP32 1 /* complex FP32 bulk; FP64 norms, step counts and orientation sum */
+
+#define SP_NQ 34 /* Q slots: rank 1 (k,m) row-major 3x3, then rank 2 (k,m) row-major 5x5 */
+
+typedef struct sp_ctx sp_ctx;
+typedef struct sp_plan sp_plan;
+
+/* Square sparse matrix in MATLAB CSC layout. n == 0 or nnz == 0 means an
+ * empty matrix (an empty Q{r}{k,m} slot). */
+typedef struct sp_csc {
+  int64_t n;           /* dimension (rows == cols) */
+  const uint64_t* jc;  /* n+1 column pointers (mwIndex) */
+  const uint64_t* ir;  /* row indices, jc[n] entries */
+  const double* pr;    /* values: 2*jc[n] interleaved doubles if is_complex, else jc[n] reals */
+  int32_t is_complex;
+} sp_csc;
+
+/* Tolerances from spin_system.tols (kernel/utilities/tolerances.m). The
+ * library uses liouv_zero for the structural reduction; the others are
+ * accepted for interface compatibility and reported back in sp_stats. */
+typedef struct sp_tols {
+  double liouv_zero;   /* 1e-10 */
+  double zte_tol;      /* 1e-24 */
+  double prop_chop;    /* 1e-10 */
+  double small_matrix; /* 200   */
+  double krylov_tol;   /* 10000 */
+} sp_tols;
+
+typedef struct sp_stats {
+  double elapsed_s;          /* wall time of the GPU pipeline (HIP events) */
+  double orient_steps_per_s; /* n_orientations * nsteps / elapsed_s */
+  double kernel_ms;          /* duration of the propagation kernel */
+  int64_t full_dim;          /* Liouville dimension of I and Q */
+  int64_t reduced_dim;       /* dimension of the coil-connected subspace */
+  int64_t reduced_nnz;       /* structural nonzeros of the reduced union pattern */
+  int32_t padded_dim;        /* reduced dimension rounded up to the tile size */
+  int32_t block;             /* time-block length B of the power scheme */
+  int32_t taylor_terms;      /* Taylor degree of the scaled exponential */
+  int32_t max_squarings;     /* largest scaling exponent over orientations */
+} sp_stats;
+
+/* ---- context --------------------------------------------------------- */
+
+/* Replaces: the worker pool set up by kernel/create.m:243-371 and used by the
+ * parfor of kernel/contexts/powder.m:226. One context per process and
+ * device; NULL on failure (no device, HIP error). */
+sp_ctx* sp_create(int device);
+
+/* Releases all device memory and the stream (a MEX gateway calls this from
+ * mexAtExit). NULL is accepted. */
+void sp_destroy(sp_ctx* ctx);
+
+/* Message of the last failure on this context (never NULL). With ctx==NULL
+ * returns the message of the last failed sp_create in this thread. */
+const char* sp_last_error(const sp_ctx* ctx);
+
+/* ---- one-shot call ----------------------------------------------------- */
+
+/* Replaces: kernel/contexts/powder.m:226-286 (parfor body with
+ * H=I+orientation(Q,[a b g]), H=(H+H')/2, R=relaxation()=0, acquire() ->
+ * evolution(L,coil,rho0,dt,nsteps,'observable'), then the weighted sum
+ * answer=sum_n w_n*ans{n}).
+ *   I          isotropic part of the commutation superoperator (hamiltonian.m:718-738,
+ *              after frqoffset.m)
+ *   Q          SP_NQ matrices Q{r}{k,m} (hamiltonian.m:741-792); empty slots allowed
+ *   alphas..weights  grid, n_or entries each (powder.m:173-179)
+ *   rho0, coil Liouville state vectors, dimension I->n, interleaved complex
+ *   dt, nsteps time step 1/sweep and npoints-1 (acquire.m:70-71)
+ *   mode       SP_MODE_FP64 or SP_MODE_FP32
+ *   fid_out    nsteps+1 interleaved complex: sum_n w_n * coil'*P_n^k*rho0
+ *   per_orient_out  optional (NULL allowed): n_or*(nsteps+1) interleaved complex,
+ *              orientation-major, unweighted (parameters.sum_up=false)
+ *   stats      optional (NULL allowed)
+ */
+int sp_powder_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                  const double* betas, const double* gammas, const double* weights, int64_t n_or,
+                  const double* rho0, const double* coil, double dt, int64_t nsteps,
+                  const sp_tols* tols, int mode, double* fid_out, double* per_orient_out,
+                  sp_stats* stats);
+
+/* ---- plan / execute split (inputs resident on the device) -------------- */
+
+/* Replaces: the host-side part of powder.m:155-225 plus the per-orientation
+ * reduce(L',coil) of evolution.m:141-145, done once on the union sparsity
+ * pattern of I and Q. Uploads the reduced generator components, grid and
+ * states to the device. Returns NULL on failure (see sp_last_error(ctx)). */
+sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                        const double* betas, const double* gammas, const double* weights,
+                        int64_t n_or, const double* rho0, const double* coil, double dt,
+                        int64_t nsteps, const sp_tols* tols, int mode);
+
+/* Runs the propagation kernels and the orientation sum on the device and
+ * waits for them. Replaces: the parfor of powder.m:226-274 and the sum of
+ * powder.m:280-286. */
+int sp_execute(sp_plan* plan, sp_stats* stats);
+
+/* Copies the results of the last sp_execute to the host (arguments as in
+ * sp_powder_fid). */
+int sp_plan_result(sp_plan* plan, double* fid_out, double* per_orient_out);
+
+void sp_plan_destroy(sp_plan* plan);
+
+/* ---- batched uneven workload ------------------------------------------ */
+
+/* One spin system of a batch: the arguments of sp_powder_fid for that system. */
+typedef struct sp_system {
+  const sp_csc* I;
+  const sp_csc* Q; /* SP_NQ entries */
+  const double* alphas;
+  const double* betas;
+  const double* gammas;
+  const double* weights;
+  int64_t n_or;
+  const double* rho0;
+  const double* coil;
+  double* fid_out; /* nsteps+1 interleaved complex */
+} sp_system;
+
+/* Many spin systems of uneven dimension, all with the same dt and nsteps,
+ * propagated by one persistent launch over a cost-ordered work queue of
+ * (system, orientation) items. Replaces: a MATLAB loop of powder() calls,
+ * one per system. */
+sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_systems, double dt,
+                              int64_t nsteps, const sp_tols* tols, int mode);
+
+/* Copies each system's FID to its sp_system.fid_out (the pointers given at
+ * planning time). */
+int sp_plan_result_batch(sp_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPINACH_POWDER_H */

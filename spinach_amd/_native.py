@@ -1,0 +1,303 @@
+"""ctypes binding of libspinach_powder.so (include/spinach_powder.h).
+
+The library is the only compute path: if it is not built, or no GPU is
+visible, every entry point raises. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import os
+
+import numpy as np
+import scipy.sparse as sps
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libspinach_powder.so')
+
+SP_MODE_FP64 = 0
+SP_MODE_FP32 = 1
+SP_NQ = 34
+
+EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
+           'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
+           'sp_plan_result_batch')
+
+
+class SpCsc(C.Structure):
+    _fields_ = [('n', C.c_int64), ('jc', C.POINTER(C.c_uint64)), ('ir', C.POINTER(C.c_uint64)),
+                ('pr', C.POINTER(C.c_double)), ('is_complex', C.c_int32)]
+
+
+class SpTols(C.Structure):
+    _fields_ = [('liouv_zero', C.c_double), ('zte_tol', C.c_double), ('prop_chop', C.c_double),
+                ('small_matrix', C.c_double), ('krylov_tol', C.c_double)]
+
+
+class SpStats(C.Structure):
+    _fields_ = [('elapsed_s', C.c_double), ('orient_steps_per_s', C.c_double), ('kernel_ms', C.c_double),
+                ('full_dim', C.c_int64), ('reduced_dim', C.c_int64), ('reduced_nnz', C.c_int64),
+                ('padded_dim', C.c_int32), ('block', C.c_int32), ('taylor_terms', C.c_int32),
+                ('max_squarings', C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SpSystem(C.Structure):
+    _fields_ = [('I', C.POINTER(SpCsc)), ('Q', C.POINTER(SpCsc)), ('alphas', C.POINTER(C.c_double)),
+                ('betas', C.POINTER(C.c_double)), ('gammas', C.POINTER(C.c_double)),
+                ('weights', C.POINTER(C.c_double)), ('n_or', C.c_int64), ('rho0', C.POINTER(C.c_double)),
+                ('coil', C.POINTER(C.c_double)), ('fid_out', C.POINTER(C.c_double))]
+
+
+_lib = None
+_ctx = {}
+
+_dp = C.POINTER(C.c_double)
+
+
+def load():
+    """Load the library and declare its signatures (no GPU call)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f'{LIB_PATH} is not built: run __graft_entry__.build() first')
+    lib = C.CDLL(LIB_PATH)
+    lib.sp_create.restype = C.c_void_p
+    lib.sp_create.argtypes = [C.c_int]
+    lib.sp_destroy.restype = None
+    lib.sp_destroy.argtypes = [C.c_void_p]
+    lib.sp_last_error.restype = C.c_char_p
+    lib.sp_last_error.argtypes = [C.c_void_p]
+    lib.sp_powder_fid.restype = C.c_int
+    lib.sp_powder_fid.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
+                                  _dp, _dp, C.c_double, C.c_int64, C.POINTER(SpTols), C.c_int, _dp, _dp,
+                                  C.POINTER(SpStats)]
+    lib.sp_plan_powder.restype = C.c_void_p
+    lib.sp_plan_powder.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
+                                   _dp, _dp, C.c_double, C.c_int64, C.POINTER(SpTols), C.c_int]
+    lib.sp_execute.restype = C.c_int
+    lib.sp_execute.argtypes = [C.c_void_p, C.POINTER(SpStats)]
+    lib.sp_plan_result.restype = C.c_int
+    lib.sp_plan_result.argtypes = [C.c_void_p, _dp, _dp]
+    lib.sp_plan_destroy.restype = None
+    lib.sp_plan_destroy.argtypes = [C.c_void_p]
+    lib.sp_plan_powder_batch.restype = C.c_void_p
+    lib.sp_plan_powder_batch.argtypes = [C.c_void_p, C.POINTER(SpSystem), C.c_int64, C.c_double, C.c_int64,
+                                         C.POINTER(SpTols), C.c_int]
+    lib.sp_plan_result_batch.restype = C.c_int
+    lib.sp_plan_result_batch.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib
+
+
+class SpinachPowderError(RuntimeError):
+    pass
+
+
+def context(device=0):
+    """Process-wide context for a device (created on first use)."""
+    lib = load()
+    if device not in _ctx:
+        c = lib.sp_create(device)
+        if not c:
+            raise SpinachPowderError('sp_create failed: ' + lib.sp_last_error(None).decode())
+        _ctx[device] = c
+    return _ctx[device]
+
+
+@atexit.register
+def _release():
+    if _lib is not None:
+        for c in _ctx.values():
+            _lib.sp_destroy(c)
+        _ctx.clear()
+
+
+def _err(ctx, rc, what):
+    msg = load().sp_last_error(ctx).decode()
+    raise SpinachPowderError(f'{what} failed (code {rc}): {msg}')
+
+
+# ------------------------------------------------------------ marshalling
+class _Keep:
+    """Holds numpy buffers alive for the duration of a library call."""
+
+    def __init__(self):
+        self.refs = []
+
+    def arr(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self.refs.append(a)
+        return a
+
+    def dptr(self, a):
+        a = self.arr(a, np.float64)
+        return a.ctypes.data_as(_dp)
+
+    def cvec(self, v):
+        v = self.arr(np.asarray(v, dtype=np.complex128).ravel(), np.complex128)
+        return v.view(np.float64).ctypes.data_as(_dp)
+
+    def csc(self, m, dim):
+        s = SpCsc()
+        if m is None:
+            s.n = 0
+            return s
+        m = sps.csc_matrix(m, dtype=np.complex128)
+        m.sum_duplicates()
+        jc = self.arr(m.indptr, np.uint64)
+        ir = self.arr(m.indices, np.uint64)
+        pr = self.arr(m.data, np.complex128)
+        s.n = dim
+        s.jc = jc.ctypes.data_as(C.POINTER(C.c_uint64))
+        s.ir = ir.ctypes.data_as(C.POINTER(C.c_uint64))
+        s.pr = pr.view(np.float64).ctypes.data_as(_dp)
+        s.is_complex = 1
+        return s
+
+    def qarray(self, qlist, dim):
+        arr = (SpCsc * SP_NQ)()
+        for t in range(SP_NQ):
+            arr[t] = self.csc(qlist[t], dim)
+        self.refs.append(arr)
+        return arr
+
+
+def _tols(t):
+    t = t or {}
+    return SpTols(t.get('liouv_zero', 1e-10), t.get('zte_tol', 1e-24), t.get('prop_chop', 1e-10),
+                  t.get('small_matrix', 200), t.get('krylov_tol', 10000))
+
+
+def mode_id(mode):
+    if mode in (SP_MODE_FP64, 'fp64', 'c128'):
+        return SP_MODE_FP64
+    if mode in (SP_MODE_FP32, 'fp32', 'c64'):
+        return SP_MODE_FP32
+    raise ValueError(f'unknown precision mode {mode!r}')
+
+
+class Plan:
+    """Device-resident powder problem (sp_plan_powder / sp_plan_powder_batch)."""
+
+    def __init__(self, handle, ctx, npts, n_or, batch_outputs=None, keep=None):
+        self.h = handle
+        self.ctx = ctx
+        self.npts = npts
+        self.n_or = n_or
+        self.batch_outputs = batch_outputs
+        self._keep = keep
+        self.stats = None
+
+    def execute(self):
+        st = SpStats()
+        rc = load().sp_execute(self.h, C.byref(st))
+        if rc:
+            _err(self.ctx, rc, 'sp_execute')
+        self.stats = st.as_dict()
+        return self.stats
+
+    def result(self, per_orientation=False):
+        fid = np.zeros(self.npts, np.complex128)
+        per = np.zeros((self.n_or, self.npts), np.complex128) if per_orientation else None
+        rc = load().sp_plan_result(self.h, fid.view(np.float64).ctypes.data_as(_dp),
+                                   per.view(np.float64).ctypes.data_as(_dp) if per is not None else None)
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_result')
+        return (fid, per) if per_orientation else fid
+
+    def result_batch(self):
+        rc = load().sp_plan_result_batch(self.h)
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_result_batch')
+        return [o.copy() for o in self.batch_outputs]
+
+    def close(self):
+        if self.h:
+            load().sp_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_powder(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', device=0):
+    lib = load()
+    ctx = context(device)
+    k = _Keep()
+    dim = I.shape[0]
+    Ic = k.csc(I, dim)
+    Qa = k.qarray(qlist, dim)
+    n_or = len(grid['weights'])
+    t = _tols(tols)
+    h = lib.sp_plan_powder(ctx, C.byref(Ic), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']),
+                           k.dptr(grid['gammas']), k.dptr(grid['weights']), n_or, k.cvec(rho0), k.cvec(coil),
+                           float(dt), int(nsteps), C.byref(t), mode_id(mode))
+    if not h:
+        _err(ctx, -1, 'sp_plan_powder')
+    return Plan(h, ctx, int(nsteps) + 1, n_or)
+
+
+def plan_powder_batch(systems, dt, nsteps, tols=None, mode='fp64', device=0):
+    """systems: list of dicts with keys I, Q (34-slot list), grid, rho0, coil."""
+    lib = load()
+    ctx = context(device)
+    k = _Keep()
+    npts = int(nsteps) + 1
+    arr = (SpSystem * len(systems))()
+    outs = []
+    for s, y in enumerate(systems):
+        dim = y['I'].shape[0]
+        Ic = k.csc(y['I'], dim)
+        k.refs.append(Ic)
+        Qa = k.qarray(y['Q'], dim)
+        out = np.zeros(npts, np.complex128)
+        outs.append(out)
+        g = y['grid']
+        arr[s].I = C.pointer(Ic)
+        arr[s].Q = C.cast(Qa, C.POINTER(SpCsc))
+        arr[s].alphas = k.dptr(g['alphas'])
+        arr[s].betas = k.dptr(g['betas'])
+        arr[s].gammas = k.dptr(g['gammas'])
+        arr[s].weights = k.dptr(g['weights'])
+        arr[s].n_or = len(g['weights'])
+        arr[s].rho0 = k.cvec(y['rho0'])
+        arr[s].coil = k.cvec(y['coil'])
+        arr[s].fid_out = out.view(np.float64).ctypes.data_as(_dp)
+    t = _tols(tols)
+    h = lib.sp_plan_powder_batch(ctx, arr, len(systems), float(dt), int(nsteps), C.byref(t), mode_id(mode))
+    if not h:
+        _err(ctx, -1, 'sp_plan_powder_batch')
+    k.refs.append(arr)
+    return Plan(h, ctx, npts, sum(len(y['grid']['weights']) for y in systems), batch_outputs=outs, keep=k)
+
+
+def powder_fid(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', per_orientation=False, device=0):
+    """One-shot sp_powder_fid call. Returns the FID (and the per-orientation
+    FIDs when requested) plus the stats dict."""
+    lib = load()
+    ctx = context(device)
+    k = _Keep()
+    dim = I.shape[0]
+    Ic = k.csc(I, dim)
+    Qa = k.qarray(qlist, dim)
+    n_or = len(grid['weights'])
+    npts = int(nsteps) + 1
+    fid = np.zeros(npts, np.complex128)
+    per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
+    st = SpStats()
+    t = _tols(tols)
+    rc = lib.sp_powder_fid(ctx, C.byref(Ic), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']),
+                           k.dptr(grid['gammas']), k.dptr(grid['weights']), n_or, k.cvec(rho0), k.cvec(coil),
+                           float(dt), int(nsteps), C.byref(t), mode_id(mode),
+                           fid.view(np.float64).ctypes.data_as(_dp),
+                           per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(st))
+    if rc:
+        _err(ctx, rc, 'sp_powder_fid')
+    return fid, per, st.as_dict()

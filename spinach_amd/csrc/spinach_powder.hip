@@ -1,0 +1,643 @@
+// libspinach_powder: C-ABI, host-side planning and launches.
+//
+// Host planning (once per call, on the union sparsity pattern of I and all
+// Q slots) replaces the per-orientation reduce(L',coil) of
+// kernel/evolution.m:141-145: the coil-connected component of the union
+// pattern is an invariant subspace of every L(Omega) that contains every state
+// the coil can see, so restricting to it is exact (the reference's zero track
+// elimination and path tracing, kernel/utilities/zte.m and path_trace.m, keep
+// the same component up to their drop thresholds).
+//
+// Everything per orientation runs on the GPU (powder_kernels.hpp). There is
+// no CPU compute path: a missing device is an error.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spinach_powder.h"
+#include "powder_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_create_err;
+
+struct SpError : std::runtime_error {
+  int code;
+  SpError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                           \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess)                                                                   \
+      throw SpError(SP_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw SpError(SP_ENOMEM, std::string("hipMalloc of ") + std::to_string(b) + " bytes: " + hipGetErrorString(e));
+    }
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// ------------------------------------------------------------ host planning
+struct Triplets {
+  std::vector<int64_t> row, col;
+  std::vector<double> re, im;
+};
+
+Triplets read_csc(const sp_csc* m, int64_t dim, const char* what) {
+  Triplets t;
+  if (!m || m->n == 0) return t;
+  if (m->n != dim) throw SpError(SP_EINVAL, std::string(what) + ": dimension does not match I");
+  if (!m->jc) throw SpError(SP_EINVAL, std::string(what) + ": NULL column pointers");
+  uint64_t nnz = m->jc[dim];
+  if (nnz == 0) return t;
+  if (!m->ir || !m->pr) throw SpError(SP_EINVAL, std::string(what) + ": NULL row indices or values");
+  if (m->jc[0] != 0) throw SpError(SP_EINVAL, std::string(what) + ": jc[0] must be 0");
+  for (int64_t c = 0; c < dim; c++) {
+    if (m->jc[c + 1] < m->jc[c]) throw SpError(SP_EINVAL, std::string(what) + ": column pointers decrease");
+    for (uint64_t p = m->jc[c]; p < m->jc[c + 1]; p++) {
+      uint64_t r = m->ir[p];
+      if (r >= (uint64_t)dim) throw SpError(SP_EINVAL, std::string(what) + ": row index out of range");
+      double re = m->is_complex ? m->pr[2 * p] : m->pr[p];
+      double im = m->is_complex ? m->pr[2 * p + 1] : 0.0;
+      if (!std::isfinite(re) || !std::isfinite(im))
+        throw SpError(SP_EINVAL, std::string(what) + ": non-finite value");
+      if (re == 0.0 && im == 0.0) continue;
+      t.row.push_back((int64_t)r);
+      t.col.push_back(c);
+      t.re.push_back(re);
+      t.im.push_back(im);
+    }
+  }
+  return t;
+}
+
+// Reduced, device-ready description of one spin system.
+struct SysPlan {
+  int64_t full_dim = 0;
+  int32_t n = 0, np = 0, nnz = 0;
+  std::vector<int32_t> row, col, tpos, slots;
+  std::vector<double> vals;  // interleaved complex, (1+nq)*nnz
+  std::vector<double> rho, coil;
+  std::vector<double> alphas, betas, gammas, weights;
+};
+
+SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, const double* be, const double* ga,
+                    const double* w, int64_t n_or, const double* rho0, const double* coil) {
+  if (!I) throw SpError(SP_EINVAL, "I is NULL");
+  if (!Q) throw SpError(SP_EINVAL, "Q is NULL");
+  if (!al || !be || !ga || !w) throw SpError(SP_EINVAL, "grid arrays must not be NULL");
+  if (n_or <= 0) throw SpError(SP_EINVAL, "the grid must have at least one orientation");
+  if (!rho0 || !coil) throw SpError(SP_EINVAL, "rho0 and coil must not be NULL");
+  const int64_t dim = I->n;
+  if (dim <= 0) throw SpError(SP_EINVAL, "I must have a positive dimension");
+  if (dim > (int64_t)INT32_MAX) throw SpError(SP_ERANGE, "dimension exceeds the 32-bit index range");
+  std::vector<Triplets> mats(1 + SP_NQ);
+  mats[0] = read_csc(I, dim, "I");
+  for (int q = 0; q < SP_NQ; q++) {
+    char what[32];
+    snprintf(what, sizeof(what), "Q slot %d", q);
+    mats[1 + q] = read_csc(&Q[q], dim, what);
+  }
+  for (int64_t k = 0; k < n_or; k++)
+    if (!std::isfinite(al[k]) || !std::isfinite(be[k]) || !std::isfinite(ga[k]) || !std::isfinite(w[k]))
+      throw SpError(SP_EINVAL, "non-finite grid entry");
+  for (int64_t i = 0; i < 2 * dim; i++)
+    if (!std::isfinite(rho0[i]) || !std::isfinite(coil[i])) throw SpError(SP_EINVAL, "non-finite state vector");
+
+  // union adjacency (symmetrised) and the coil-connected component
+  std::vector<std::vector<int64_t>> adj(dim);
+  for (auto& m : mats)
+    for (size_t p = 0; p < m.row.size(); p++)
+      if (m.row[p] != m.col[p]) {
+        adj[m.row[p]].push_back(m.col[p]);
+        adj[m.col[p]].push_back(m.row[p]);
+      }
+  std::vector<char> seen(dim, 0);
+  std::vector<int64_t> stack;
+  for (int64_t i = 0; i < dim; i++)
+    if (coil[2 * i] != 0.0 || coil[2 * i + 1] != 0.0) {
+      seen[i] = 1;
+      stack.push_back(i);
+    }
+  while (!stack.empty()) {
+    int64_t v = stack.back();
+    stack.pop_back();
+    for (int64_t u : adj[v])
+      if (!seen[u]) {
+        seen[u] = 1;
+        stack.push_back(u);
+      }
+  }
+  std::vector<int64_t> map(dim, -1);
+  SysPlan sp;
+  sp.full_dim = dim;
+  int32_t n = 0;
+  for (int64_t i = 0; i < dim; i++)
+    if (seen[i]) map[i] = n++;
+  sp.n = n;
+  sp.np = std::max<int32_t>(16, (n + 15) / 16 * 16);
+  // reduced union pattern
+  std::vector<std::pair<int64_t, int64_t>> ent;
+  for (auto& m : mats)
+    for (size_t p = 0; p < m.row.size(); p++) {
+      int64_t r = map[m.row[p]], c = map[m.col[p]];
+      if (r < 0 || c < 0) continue;
+      ent.emplace_back(r, c);
+      ent.emplace_back(c, r);
+    }
+  std::sort(ent.begin(), ent.end());
+  ent.erase(std::unique(ent.begin(), ent.end()), ent.end());
+  sp.nnz = (int32_t)ent.size();
+  auto find = [&](int64_t r, int64_t c) -> int64_t {
+    auto it = std::lower_bound(ent.begin(), ent.end(), std::make_pair(r, c));
+    return (int64_t)(it - ent.begin());
+  };
+  sp.row.resize(sp.nnz);
+  sp.col.resize(sp.nnz);
+  sp.tpos.resize(sp.nnz);
+  for (int32_t e = 0; e < sp.nnz; e++) {
+    sp.row[e] = (int32_t)ent[e].first;
+    sp.col[e] = (int32_t)ent[e].second;
+    sp.tpos[e] = (int32_t)find(ent[e].second, ent[e].first);
+  }
+  // value arrays: I first, then every non-empty Q slot
+  std::vector<std::vector<double>> planes;
+  auto scatter = [&](const Triplets& m, std::vector<double>& v) {
+    bool any = false;
+    v.assign(2 * (size_t)sp.nnz, 0.0);
+    for (size_t p = 0; p < m.row.size(); p++) {
+      int64_t r = map[m.row[p]], c = map[m.col[p]];
+      if (r < 0 || c < 0) continue;
+      int64_t e = find(r, c);
+      v[2 * e] += m.re[p];
+      v[2 * e + 1] += m.im[p];
+      any = true;
+    }
+    return any;
+  };
+  std::vector<double> tmp;
+  scatter(mats[0], tmp);
+  sp.vals = tmp;
+  for (int q = 0; q < SP_NQ; q++)
+    if (scatter(mats[1 + q], tmp)) {
+      sp.slots.push_back(q);
+      sp.vals.insert(sp.vals.end(), tmp.begin(), tmp.end());
+    }
+  sp.rho.assign(2 * (size_t)n, 0.0);
+  sp.coil.assign(2 * (size_t)n, 0.0);
+  for (int64_t i = 0; i < dim; i++)
+    if (map[i] >= 0) {
+      sp.rho[2 * map[i]] = rho0[2 * i];
+      sp.rho[2 * map[i] + 1] = rho0[2 * i + 1];
+      sp.coil[2 * map[i]] = coil[2 * i];
+      sp.coil[2 * map[i] + 1] = coil[2 * i + 1];
+    }
+  sp.alphas.assign(al, al + n_or);
+  sp.betas.assign(be, be + n_or);
+  sp.gammas.assign(ga, ga + n_or);
+  sp.weights.assign(w, w + n_or);
+  return sp;
+}
+
+}  // namespace
+
+struct sp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  int n_cu = 0;
+  int code = 0;
+  std::string err;
+};
+
+struct sp_plan {
+  sp_ctx* ctx = nullptr;
+  int mode = SP_MODE_FP64;
+  int64_t n_systems = 0;
+  int64_t nsteps = 0;
+  int64_t n_items = 0;
+  int64_t total_or = 0;
+  double dt = 0;
+  int32_t np_max = 16, block = 1, nblk = 1, log2b = 0, taylor_k = 10;
+  double theta = 0.125;
+  int grid = 1;
+  int64_t work_stride = 0;
+  std::vector<int64_t> or_off, n_or;
+  std::vector<double*> fid_targets;
+  sp_stats last{};
+  int64_t full_dim = 0, red_dim = 0, red_nnz = 0;
+  DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
+  DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
+  bool executed = false;
+};
+
+namespace {
+
+int fail(sp_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) {
+    ctx->err = msg;
+    ctx->code = code;
+  }
+  return code;
+}
+
+int choose_log2_block(int64_t npts) {
+  double r = std::sqrt((double)npts);
+  int l = (int)std::lround(std::log2(std::max(1.0, r)));
+  return std::min(8, std::max(0, l));
+}
+
+sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode) {
+  if (!(std::isfinite(dt)) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
+  if (nsteps < 0) throw SpError(SP_EINVAL, "nsteps must be non-negative");
+  if (nsteps > (int64_t)1 << 30) throw SpError(SP_ERANGE, "nsteps is too large");
+  if (mode != SP_MODE_FP64 && mode != SP_MODE_FP32) throw SpError(SP_EINVAL, "unknown precision mode");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::unique_ptr<sp_plan> pl(new sp_plan);
+  pl->ctx = ctx;
+  pl->mode = mode;
+  pl->dt = dt;
+  pl->nsteps = nsteps;
+  pl->n_systems = (int64_t)sps.size();
+  const int64_t npts = nsteps + 1;
+  pl->log2b = choose_log2_block(npts);
+  pl->block = 1 << pl->log2b;
+  pl->nblk = (int32_t)((npts + pl->block - 1) / pl->block);
+  if (mode == SP_MODE_FP64) {
+    pl->theta = 0.125;
+    pl->taylor_k = 10;  // 0.125^11/11! < 6e-18
+  } else {
+    pl->theta = 0.125;
+    pl->taylor_k = 5;  // 0.125^6/6! < 6e-9
+  }
+  std::vector<spk::SysDesc> descs;
+  std::vector<int32_t> row, col, tpos, slots;
+  std::vector<double> vals, rho, coil, al, be, ga, w;
+  std::vector<spk::Item> items;
+  std::vector<std::pair<double, int64_t>> cost;  // for the queue order
+  int64_t or_base = 0;
+  for (size_t s = 0; s < sps.size(); s++) {
+    SysPlan& sp = sps[s];
+    spk::SysDesc d{};
+    d.n = sp.n;
+    d.np = sp.np;
+    d.nnz = sp.nnz;
+    d.nq = (int32_t)sp.slots.size();
+    d.pat_off = (int64_t)row.size();
+    d.val_off = (int64_t)vals.size() / 2;
+    d.slot_off = (int64_t)slots.size();
+    d.vec_off = (int64_t)rho.size() / 2;
+    d.or_off = or_base;
+    d.n_or = (int64_t)sp.weights.size();
+    descs.push_back(d);
+    row.insert(row.end(), sp.row.begin(), sp.row.end());
+    col.insert(col.end(), sp.col.begin(), sp.col.end());
+    tpos.insert(tpos.end(), sp.tpos.begin(), sp.tpos.end());
+    slots.insert(slots.end(), sp.slots.begin(), sp.slots.end());
+    vals.insert(vals.end(), sp.vals.begin(), sp.vals.end());
+    rho.insert(rho.end(), sp.rho.begin(), sp.rho.end());
+    coil.insert(coil.end(), sp.coil.begin(), sp.coil.end());
+    al.insert(al.end(), sp.alphas.begin(), sp.alphas.end());
+    be.insert(be.end(), sp.betas.begin(), sp.betas.end());
+    ga.insert(ga.end(), sp.gammas.begin(), sp.gammas.end());
+    w.insert(w.end(), sp.weights.begin(), sp.weights.end());
+    pl->or_off.push_back(or_base);
+    pl->n_or.push_back(d.n_or);
+    double c = (double)sp.np * sp.np * sp.np;  // dense-stage cost per orientation
+    for (int64_t k = 0; k < d.n_or; k++) items.push_back(spk::Item{(int32_t)s, 0, or_base + k});
+    for (int64_t k = 0; k < d.n_or; k++) cost.emplace_back(c, (int64_t)items.size() - d.n_or + k);
+    or_base += d.n_or;
+    pl->np_max = std::max(pl->np_max, sp.np);
+    pl->full_dim = std::max<int64_t>(pl->full_dim, sp.full_dim);
+    pl->red_dim = std::max<int64_t>(pl->red_dim, sp.n);
+    pl->red_nnz = std::max<int64_t>(pl->red_nnz, sp.nnz);
+  }
+  // longest-processing-time-first queue order (stable within a system)
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+                     return a.first > b.first;
+                   });
+  std::vector<spk::Item> ordered;
+  ordered.reserve(items.size());
+  for (auto& c : cost) ordered.push_back(items[c.second]);
+  pl->n_items = (int64_t)ordered.size();
+  pl->total_or = or_base;
+
+  hipStream_t st = ctx->stream;
+  upload(pl->d_sys, descs, st);
+  upload(pl->d_items, ordered, st);
+  upload(pl->d_row, row, st);
+  upload(pl->d_col, col, st);
+  upload(pl->d_tpos, tpos, st);
+  upload(pl->d_vals, vals, st);
+  upload(pl->d_slots, slots, st);
+  upload(pl->d_rho, rho, st);
+  upload(pl->d_coil, coil, st);
+  upload(pl->d_al, al, st);
+  upload(pl->d_be, be, st);
+  upload(pl->d_ga, ga, st);
+  upload(pl->d_w, w, st);
+  pl->d_queue.alloc(sizeof(unsigned int));
+
+  // persistent grid: resident workgroups, bounded by the item count
+  int per_cu = 0;
+  if (mode == SP_MODE_FP64)
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<double>, spk::kThreads, 0));
+  else
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<float>, spk::kThreads, 0));
+  per_cu = std::max(1, per_cu);
+  int64_t g = (int64_t)ctx->n_cu * per_cu;
+  pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, pl->n_items));
+  const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
+  const int64_t mat = 2 * (int64_t)pl->np_max * pl->np_max;
+  pl->work_stride = 5 * mat + 2 * (int64_t)pl->block * pl->np_max + 2 * (int64_t)pl->nblk * pl->np_max;
+  pl->work_stride = (pl->work_stride + 63) / 64 * 64;
+  pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+  pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
+  pl->d_fid.alloc((size_t)pl->n_systems * npts * 2 * sizeof(double));
+  pl->d_sq.alloc((size_t)pl->total_or * sizeof(int32_t));
+  HIPCHK(hipStreamSynchronize(st));
+  return pl.release();
+}
+
+template <typename R>
+void launch(sp_plan* pl) {
+  spk::KParams<R> kp{};
+  kp.sys = pl->d_sys.as<spk::SysDesc>();
+  kp.items = pl->d_items.as<spk::Item>();
+  kp.n_items = pl->n_items;
+  kp.queue = pl->d_queue.as<unsigned int>();
+  kp.prow = pl->d_row.as<int32_t>();
+  kp.pcol = pl->d_col.as<int32_t>();
+  kp.ptpos = pl->d_tpos.as<int32_t>();
+  kp.vals = pl->d_vals.as<double>();
+  kp.slots = pl->d_slots.as<int32_t>();
+  kp.rho0 = pl->d_rho.as<double>();
+  kp.coil = pl->d_coil.as<double>();
+  kp.alphas = pl->d_al.as<double>();
+  kp.betas = pl->d_be.as<double>();
+  kp.gammas = pl->d_ga.as<double>();
+  kp.dt = pl->dt;
+  kp.nsteps = pl->nsteps;
+  kp.block = pl->block;
+  kp.nblk = pl->nblk;
+  kp.log2b = pl->log2b;
+  kp.taylor_k = pl->taylor_k;
+  kp.theta = pl->theta;
+  kp.work = pl->d_work.as<R>();
+  kp.work_stride = pl->work_stride;
+  kp.np_max = pl->np_max;
+  kp.per_or = pl->d_per.as<double>();
+  kp.sq_out = pl->d_sq.as<int32_t>();
+  hipLaunchKernelGGL(spk::k_powder<R>, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
+  HIPCHK(hipGetLastError());
+}
+
+int run(sp_plan* pl, sp_stats* stats) {
+  sp_ctx* ctx = pl->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t npts = pl->nsteps + 1;
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, pl->d_queue.as<unsigned int>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  if (pl->mode == SP_MODE_FP64)
+    launch<double>(pl);
+  else
+    launch<float>(pl);
+  HIPCHK(hipEventRecord(ctx->ev2, st));
+  for (int64_t s = 0; s < pl->n_systems; s++) {
+    int nb = (int)((npts + 255) / 256);
+    hipLaunchKernelGGL(spk::k_weighted_sum, dim3(nb), dim3(256), 0, st, pl->d_per.as<double>(),
+                       pl->d_w.as<double>(), pl->or_off[s], pl->n_or[s], npts,
+                       pl->d_fid.as<double>() + 2 * (size_t)s * npts);
+    HIPCHK(hipGetLastError());
+  }
+  hipEvent_t evend;
+  HIPCHK(hipEventCreate(&evend));
+  HIPCHK(hipEventRecord(evend, st));
+  hipError_t se = hipEventSynchronize(evend);
+  float ms_all = 0, ms_k = 0;
+  if (se == hipSuccess) {
+    (void)hipEventElapsedTime(&ms_all, ctx->ev0, evend);
+    (void)hipEventElapsedTime(&ms_k, ctx->ev1, ctx->ev2);
+  }
+  (void)hipEventDestroy(evend);
+  HIPCHK(se);
+  pl->executed = true;
+  sp_stats s{};
+  s.elapsed_s = ms_all * 1e-3;
+  s.kernel_ms = ms_k;
+  s.orient_steps_per_s = s.elapsed_s > 0 ? (double)pl->total_or * (double)pl->nsteps / s.elapsed_s : 0.0;
+  s.full_dim = pl->full_dim;
+  s.reduced_dim = pl->red_dim;
+  s.reduced_nnz = pl->red_nnz;
+  s.padded_dim = pl->np_max;
+  s.block = pl->block;
+  s.taylor_terms = pl->taylor_k;
+  std::vector<int32_t> sq((size_t)pl->total_or);
+  HIPCHK(hipMemcpy(sq.data(), pl->d_sq.p, sq.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  int mx = 0;
+  for (int v : sq) mx = std::max(mx, v);
+  s.max_squarings = mx;
+  pl->last = s;
+  if (stats) *stats = s;
+  return SP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sp_ctx* sp_create(int device) {
+  try {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) throw SpError(SP_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= n) throw SpError(SP_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    std::unique_ptr<sp_ctx> c(new sp_ctx);
+    c->device = device;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(hipEventCreate(&c->ev2));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount;
+    return c.release();
+  } catch (std::exception& ex) {
+    g_create_err = ex.what();
+    return nullptr;
+  }
+}
+
+void sp_destroy(sp_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* sp_last_error(const sp_ctx* ctx) {
+  if (!ctx) return g_create_err.c_str();
+  return ctx->err.c_str();
+}
+
+sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas, const double* betas,
+                        const double* gammas, const double* weights, int64_t n_or, const double* rho0,
+                        const double* coil, double dt, int64_t nsteps, const sp_tols* tols, int mode) {
+  if (!ctx) return nullptr;
+  try {
+    (void)tols;
+    std::vector<SysPlan> sps;
+    sps.push_back(plan_system(I, Q, alphas, betas, gammas, weights, n_or, rho0, coil));
+    return build_plan(ctx, sps, dt, nsteps, mode);
+  } catch (SpError& e) {
+    fail(ctx, e.code, e.what());
+  } catch (std::bad_alloc&) {
+    fail(ctx, SP_ENOMEM, "host allocation failed");
+  } catch (std::exception& e) {
+    fail(ctx, SP_EDEVICE, e.what());
+  }
+  return nullptr;
+}
+
+sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_systems, double dt, int64_t nsteps,
+                              const sp_tols* tols, int mode) {
+  if (!ctx) return nullptr;
+  try {
+    (void)tols;
+    if (!systems || n_systems <= 0) throw SpError(SP_EINVAL, "the batch must hold at least one system");
+    std::vector<SysPlan> sps;
+    for (int64_t s = 0; s < n_systems; s++) {
+      const sp_system& y = systems[s];
+      sps.push_back(plan_system(y.I, y.Q, y.alphas, y.betas, y.gammas, y.weights, y.n_or, y.rho0, y.coil));
+    }
+    sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode);
+    for (int64_t s = 0; s < n_systems; s++) pl->fid_targets.push_back(systems[s].fid_out);
+    return pl;
+  } catch (SpError& e) {
+    fail(ctx, e.code, e.what());
+  } catch (std::bad_alloc&) {
+    fail(ctx, SP_ENOMEM, "host allocation failed");
+  } catch (std::exception& e) {
+    fail(ctx, SP_EDEVICE, e.what());
+  }
+  return nullptr;
+}
+
+int sp_execute(sp_plan* plan, sp_stats* stats) {
+  if (!plan) return SP_EINVAL;
+  try {
+    return run(plan, stats);
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_result(sp_plan* plan, double* fid_out, double* per_orient_out) {
+  if (!plan) return SP_EINVAL;
+  try {
+    if (!plan->executed) throw SpError(SP_EINVAL, "sp_execute has not run on this plan");
+    HIPCHK(hipSetDevice(plan->ctx->device));
+    const int64_t npts = plan->nsteps + 1;
+    if (fid_out)
+      HIPCHK(hipMemcpy(fid_out, plan->d_fid.p, (size_t)npts * 2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (per_orient_out)
+      HIPCHK(hipMemcpy(per_orient_out, plan->d_per.p, (size_t)plan->n_or[0] * npts * 2 * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    return SP_OK;
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_result_batch(sp_plan* plan) {
+  if (!plan) return SP_EINVAL;
+  try {
+    if (!plan->executed) throw SpError(SP_EINVAL, "sp_execute has not run on this plan");
+    if ((int64_t)plan->fid_targets.size() != plan->n_systems) throw SpError(SP_EINVAL, "not a batch plan");
+    HIPCHK(hipSetDevice(plan->ctx->device));
+    const int64_t npts = plan->nsteps + 1;
+    std::vector<double> all((size_t)plan->n_systems * npts * 2);
+    HIPCHK(hipMemcpy(all.data(), plan->d_fid.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < plan->n_systems; s++)
+      if (plan->fid_targets[s])
+        std::memcpy(plan->fid_targets[s], all.data() + 2 * (size_t)s * npts, (size_t)npts * 2 * sizeof(double));
+    return SP_OK;
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+void sp_plan_destroy(sp_plan* plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);
+  delete plan;
+}
+
+int sp_powder_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas, const double* betas,
+                  const double* gammas, const double* weights, int64_t n_or, const double* rho0, const double* coil,
+                  double dt, int64_t nsteps, const sp_tols* tols, int mode, double* fid_out, double* per_orient_out,
+                  sp_stats* stats) {
+  if (!ctx) return SP_EINVAL;
+  if (!fid_out) return fail(ctx, SP_EINVAL, "fid_out must not be NULL");
+  sp_plan* pl = sp_plan_powder(ctx, I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, dt, nsteps, tols, mode);
+  if (!pl) return ctx->code ? ctx->code : SP_EDEVICE;
+  int rc = sp_execute(pl, stats);
+  if (rc == SP_OK) rc = sp_plan_result(pl, fid_out, per_orient_out);
+  sp_plan_destroy(pl);
+  return rc;
+}
+
+}  // extern "C"
