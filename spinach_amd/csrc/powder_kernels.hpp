@@ -1,4 +1,4 @@
-// Device kernels of the powder propagation library (gfx950, wave64).
+// Device kernels of the powder propagation library (gfx950, wave64, MFMA).
 //
 // One work item = one (spin system, orientation) pair. A persistent grid of
 // 256-thread workgroups pulls items from a device-side queue. For each item
@@ -9,19 +9,26 @@
 //      H = I + sum_{r,k,m} D^r(k,m) Q{r}{k,m} on the union sparsity pattern
 //      (kernel/orientation.m:38-57, kernel/contexts/powder.m:235),
 //   3. takes its 1-norm in FP64 (kernel/utilities/cheap_norm.m:50) and picks
-//      the scaling exponent s so that |X/2^s| <= theta,
-//   4. forms P = expm(X) by a degree-K Taylor polynomial in Horner form and s
-//      squarings (the dense branch of kernel/propagator.m:40-42),
+//      the scaling exponent s so that |X/2^s|_1 <= theta,
+//   4. forms P = expm(X) (the dense branch of kernel/propagator.m:40-42) by a
+//      degree-K Taylor polynomial evaluated with the Paterson-Stockmeyer
+//      scheme, followed by s squarings,
 //   5. evaluates the detected trajectory f(t) = coil' * P^t * rho0,
-//      t = 0..nsteps (kernel/evolution.m:562-566), by a two-level power
-//      scheme: Q = P^B, u_j = coil'*P^j (j<B), v_k = Q^k*rho0, f(kB+j) = u_j*v_k,
+//      t = 0..nsteps (kernel/evolution.m:562-566) by a two-level power
+//      scheme: u_j = coil'*P^j (j < B) built by doubling with the powers
+//      P, P^2, .., P^(B/2) that the squarings to Q = P^B produce anyway;
+//      v_k = Q^k*rho0 by a matrix-vector chain; f(kB+j) = u_j*v_k as one
+//      matrix product U*V^T,
 //   6. writes f to the per-orientation buffer.
 // A second kernel forms the grid-weighted sum in grid order
 // (kernel/contexts/powder.m:280-286).
 //
-// Matrices are planar complex (real plane, then imaginary plane), row-major,
-// padded to a multiple of 16 with zero rows/columns (the padded block is
-// decoupled and does not change the physical block).
+// All matrix products run on the matrix cores: v_mfma_f64_16x16x4f64 in FP64
+// mode, v_mfma_f32_16x16x4f32 in FP32 mode, complex products as four real
+// MFMA accumulations. Matrices are planar complex (real plane, then
+// imaginary plane), row-major, padded to a multiple of 16 with zero rows and
+// columns (the padded block is decoupled and does not change the physical
+// block).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,10 +67,10 @@ struct KParams {
   const int32_t* prow;
   const int32_t* pcol;
   const int32_t* ptpos;
-  const double* vals;   // interleaved complex FP64
+  const double* vals;    // interleaved complex FP64
   const int32_t* slots;  // Q slot ids 0..33
-  const double* rho0;   // interleaved complex
-  const double* coil;   // interleaved complex
+  const double* rho0;    // interleaved complex
+  const double* coil;    // interleaved complex
   const double* alphas;
   const double* betas;
   const double* gammas;
@@ -72,8 +79,9 @@ struct KParams {
   int32_t block;     // B
   int32_t nblk;      // M = ceil((nsteps+1)/B)
   int32_t log2b;     // log2(B)
-  int32_t taylor_k;  // K
-  double theta;      // scaling threshold
+  int32_t taylor_k;  // degree K of the Taylor polynomial
+  int32_t ps_q;      // Paterson-Stockmeyer block length q
+  double theta;      // scaling threshold on the 1-norm
   R* work;           // per-resident-workgroup workspace
   int64_t work_stride;  // R elements per workgroup slot
   int32_t np_max;
@@ -147,9 +155,42 @@ __device__ inline void wigner_coeffs(double a, double b, double g, double* cre, 
     }
 }
 
-// ------------------------------------------------------------ workgroup GEMM
-// C = alpha*A*B (+ I if add_identity); optional transposed store of C.
-// 64x64 output tiles, k-panels of 16 staged in LDS, 4x4 micro-tile per thread.
+// -------------------------------------------------------------- MFMA traits
+template <typename R>
+struct Mma;
+
+template <>
+struct Mma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 fma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element i held by `lane` within a 16x16 tile
+  static __device__ inline int row(int lane, int i) { return 4 * i + (lane >> 4); }
+};
+
+template <>
+struct Mma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 fma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+// Planar complex matrix view: element (i,j) at re[i*ld+j], im[i*ld+j]
+// (or (j,i) when trans is set).
+template <typename R>
+struct CMat {
+  const R* re;
+  const R* im;
+  int ld;
+  bool trans;
+  __device__ inline size_t off(int i, int j) const {
+    return trans ? (size_t)j * ld + i : (size_t)i * ld + j;
+  }
+};
+
 template <typename R>
 struct GemmLds {
   R ar[16][64];
@@ -158,82 +199,93 @@ struct GemmLds {
   R bi[16][64];
 };
 
-template <typename R>
-__device__ void wg_zgemm(const R* __restrict__ A, const R* __restrict__ B, R* __restrict__ C, int np,
-                         R alpha, bool add_identity, bool transpose_out, GemmLds<R>& sm) {
+// C(MxN) = A(MxK) * B(KxN), K a multiple of 16; epi(i, j, re, im) receives
+// every output element. The workgroup walks 64x64 output tiles; each of its
+// four waves owns a 32x32 quarter as 2x2 MFMA tiles; k-panels of 16 are
+// staged through LDS.
+template <typename R, typename Epi>
+__device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, GemmLds<R>& sm) {
+  typedef typename Mma<R>::v4 v4;
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  const size_t pl = (size_t)np * np;
-  for (int i0 = 0; i0 < np; i0 += 64) {
-    for (int j0 = 0; j0 < np; j0 += 64) {
-      R cr[4][4], ci[4][4];
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  for (int i0 = 0; i0 < M; i0 += 64) {
+    for (int j0 = 0; j0 < N; j0 += 64) {
+      const bool active = (i0 + 32 * wr < M) && (j0 + 32 * wc < N);
+      v4 cr[2][2], ci[2][2];
 #pragma unroll
-      for (int m = 0; m < 4; m++)
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int n = 0; n < 4; n++) {
-          cr[m][n] = R(0);
-          ci[m][n] = R(0);
+        for (int b = 0; b < 2; b++) {
+          cr[a][b] = v4{0, 0, 0, 0};
+          ci[a][b] = v4{0, 0, 0, 0};
         }
-      for (int k0 = 0; k0 < np; k0 += 16) {
+      for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
-        for (int e = tid; e < 1024; e += kThreads) {
-          int r = e >> 4, c = e & 15;
-          int gi = i0 + r;
+        for (int rep = 0; rep < 4; rep++) {
+          int e = tid + rep * kThreads;
+          // A panel: rows i0..i0+63, cols k0..k0+15 (contiguous in k when not transposed)
+          int r, c;
+          if (!A.trans) { r = e >> 4; c = e & 15; } else { c = e >> 6; r = e & 63; }
           R va = R(0), vb = R(0);
-          if (gi < np) {
-            size_t o = (size_t)gi * np + k0 + c;
-            va = A[o];
-            vb = A[pl + o];
+          if (i0 + r < M) {
+            size_t o = A.off(i0 + r, k0 + c);
+            va = A.re[o];
+            vb = A.im[o];
           }
           sm.ar[c][r] = va;
           sm.ai[c][r] = vb;
-          int rb = e >> 6, cb = e & 63;
-          int gj = j0 + cb;
+          // B panel: rows k0..k0+15, cols j0..j0+63
+          int rb, cb;
+          if (!B.trans) { rb = e >> 6; cb = e & 63; } else { cb = e >> 4; rb = e & 15; }
           R wa = R(0), wb = R(0);
-          if (gj < np) {
-            size_t o = (size_t)(k0 + rb) * np + gj;
-            wa = B[o];
-            wb = B[pl + o];
+          if (j0 + cb < N) {
+            size_t o = B.off(k0 + rb, j0 + cb);
+            wa = B.re[o];
+            wb = B.im[o];
           }
           sm.br[rb][cb] = wa;
           sm.bi[rb][cb] = wb;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 16; kk++) {
-          R xr[4], xi[4], yr[4], yi[4];
+        if (active) {
 #pragma unroll
-          for (int m = 0; m < 4; m++) {
-            xr[m] = sm.ar[kk][ty + 16 * m];
-            xi[m] = sm.ai[kk][ty + 16 * m];
-            yr[m] = sm.br[kk][tx + 16 * m];
-            yi[m] = sm.bi[kk][tx + 16 * m];
-          }
+          for (int kk = 0; kk < 16; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            R xr[2], xi[2], nxi[2], yr[2], yi[2];
 #pragma unroll
-          for (int m = 0; m < 4; m++)
-#pragma unroll
-            for (int n = 0; n < 4; n++) {
-              cr[m][n] = fma(xr[m], yr[n], cr[m][n]);
-              cr[m][n] = fma(-xi[m], yi[n], cr[m][n]);
-              ci[m][n] = fma(xr[m], yi[n], ci[m][n]);
-              ci[m][n] = fma(xi[m], yr[n], ci[m][n]);
+            for (int t = 0; t < 2; t++) {
+              xr[t] = sm.ar[kr][32 * wr + 16 * t + (lane & 15)];
+              xi[t] = sm.ai[kr][32 * wr + 16 * t + (lane & 15)];
+              nxi[t] = -xi[t];
+              yr[t] = sm.br[kr][32 * wc + 16 * t + (lane & 15)];
+              yi[t] = sm.bi[kr][32 * wc + 16 * t + (lane & 15)];
             }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) {
+                cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
+                cr[a][b] = Mma<R>::fma(nxi[a], yi[b], cr[a][b]);
+                ci[a][b] = Mma<R>::fma(xr[a], yi[b], ci[a][b]);
+                ci[a][b] = Mma<R>::fma(xi[a], yr[b], ci[a][b]);
+              }
+          }
         }
         __syncthreads();
       }
+      if (active) {
 #pragma unroll
-      for (int m = 0; m < 4; m++)
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int n = 0; n < 4; n++) {
-          int i = i0 + ty + 16 * m, j = j0 + tx + 16 * n;
-          if (i < np && j < np) {
-            R vr = alpha * cr[m][n], vi = alpha * ci[m][n];
-            if (add_identity && i == j) vr += R(1);
-            size_t o = transpose_out ? (size_t)j * np + i : (size_t)i * np + j;
-            C[o] = vr;
-            C[pl + o] = vi;
-          }
-        }
+          for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              int i = i0 + 32 * wr + 16 * a + Mma<R>::row(lane, e);
+              int j = j0 + 32 * wc + 16 * b + (lane & 15);
+              if (i < M && j < N) epi(i, j, cr[a][b][e], ci[a][b][e]);
+            }
+      }
     }
   }
   __syncthreads();
@@ -255,8 +307,20 @@ struct SharedAll {
   GemmLds<R> g;
   double cre[34], cim[34];
   double red[4];
+  double tc[32];  // Taylor coefficients 1/k!
   int item;
   int sexp;
+};
+
+// Planar complex square matrix buffer of the workspace
+template <typename R>
+struct Buf {
+  R* p;
+  size_t pl;  // plane size np*np
+  int np;
+  __device__ inline R* re() const { return p; }
+  __device__ inline R* im() const { return p + pl; }
+  __device__ inline CMat<R> view(bool trans = false) const { return CMat<R>{p, p + pl, np, trans}; }
 };
 
 // --------------------------------------------------------- the fused kernel
@@ -266,13 +330,16 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
   const int tid = threadIdx.x;
   R* ws = p.work + (size_t)blockIdx.x * p.work_stride;
   const size_t mat = 2 * (size_t)p.np_max * p.np_max;
-  R* X = ws;
-  R* T0 = ws + mat;
-  R* T1 = ws + 2 * mat;
-  R* P = ws + 3 * mat;
-  R* QT = ws + 4 * mat;  // Q = P^B stored transposed
-  R* U = ws + 5 * mat;   // B rows x np (planar: re block, im block)
-  R* V = U + 2 * (size_t)p.block * p.np_max;  // M rows x np
+  R* bufbase[6];
+  for (int b = 0; b < 6; b++) bufbase[b] = ws + b * mat;
+  R* Ubase = ws + 6 * mat;  // (B + 16) rows x np, planar
+  R* Vbase = Ubase + 2 * (size_t)(p.block + 16) * p.np_max;  // nblk rows x np, planar
+
+  if (tid < 32) {
+    double f = 1.0;
+    for (int k = 2; k <= tid; k++) f *= k;
+    sh.tc[tid] = 1.0 / f;
+  }
 
   for (;;) {
     if (tid == 0) sh.item = (int)atomicAdd(p.queue, 1u);
@@ -285,11 +352,16 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
     const int np = sd.np;
     const size_t pl = (size_t)np * np;
     const int64_t o = item.orient;
+    Buf<R> bf[6];
+    for (int b = 0; b < 6; b++) bf[b] = Buf<R>{bufbase[b], pl, np};
 
     // 1. Wigner coefficients
     if (tid == 0) wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], sh.cre, sh.cim);
-    // 2. zero X
+    // 2. zero X and the U block
+    R* X = bf[0].p;
     for (size_t e = tid; e < 2 * pl; e += kThreads) X[e] = R(0);
+    const size_t ub = (size_t)(p.block + 16) * np;  // plane size of U
+    for (size_t e = tid; e < 2 * ub; e += kThreads) Ubase[e] = R(0);
     __syncthreads();
     // scatter the symmetrised generator: X = -i*dt*(x + x^H)/2
     {
@@ -313,7 +385,6 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
           tr += cr * wr - ci * wi;
           ti += cr * wi + ci * wr;
         }
-        // h = (x + conj(x_t))/2 ; X = -i*dt*h
         double hr = 0.5 * (xr + tr), hi = 0.5 * (xi - ti);
         size_t off = (size_t)prow[e] * np + pcol[e];
         X[off] = (R)(p.dt * hi);
@@ -342,122 +413,210 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
       __syncthreads();
     }
     const int sexp = sh.sexp;
-    const R scl = (R)ldexp(1.0, -sexp);
-    for (size_t e = tid; e < 2 * pl; e += kThreads) X[e] *= scl;
-    // 4. Taylor in Horner form: T = I + X/K ; T = I + X*T/k for k=K-1..1
     {
-      const R invk = (R)(1.0 / p.taylor_k);
-      for (size_t e = tid; e < 2 * pl; e += kThreads) {
-        size_t rem = e % pl;
-        R v = X[e] * invk;
-        if (e < pl && (rem / np) == (rem % np)) v += R(1);
-        T0[e] = v;
-      }
+      const R scl = (R)ldexp(1.0, -sexp);
+      for (size_t e = tid; e < 2 * pl; e += kThreads) X[e] *= scl;
       __syncthreads();
     }
-    R* cur = T0;
-    R* nxt = T1;
-    for (int k = p.taylor_k - 1; k >= 1; k--) {
-      wg_zgemm<R>(X, cur, nxt, np, (R)(1.0 / k), true, false, sh.g);
-      R* t = cur; cur = nxt; nxt = t;
+    // 4. Paterson-Stockmeyer: powers X^2..X^q in bf[1..q-1], then Horner in
+    //    Y = X^q over blocks B_j(X) = sum_{i<q} c_{jq+i} X^i.
+    const int q = p.ps_q;      // 2..4
+    const int K = p.taylor_k;  // degree
+    const int r = K / q;       // index of the top block
+    {
+      for (int k = 2; k <= q; k++) {
+        Buf<R> dst = bf[k - 1];
+        R* dre = dst.re();
+        R* dim_ = dst.im();
+        wg_gemm<R>(np, np, np, bf[k - 2].view(), bf[0].view(),
+                   [=](int i, int j, R vr, R vi) {
+                     size_t o2 = (size_t)i * np + j;
+                     dre[o2] = vr;
+                     dim_[o2] = vi;
+                   },
+                   sh.g);
+      }
     }
-    // squarings
-    for (int s = 0; s < sexp; s++) {
-      wg_zgemm<R>(cur, cur, nxt, np, R(1), false, false, sh.g);
-      R* t = cur; cur = nxt; nxt = t;
+    // block combination helper: c0*I + c1*X + c2*X^2 + ... (powers < q)
+    auto blockval = [&](int j, size_t o2, bool diag, R& vr, R& vi) {
+      double ar = 0.0, ai = 0.0;
+      for (int i2 = 0; i2 < q; i2++) {
+        int k = j * q + i2;
+        if (k > K) break;
+        double c = sh.tc[k];
+        if (i2 == 0) {
+          if (diag) ar += c;
+        } else {
+          ar += c * (double)bf[i2 - 1].re()[o2];
+          ai += c * (double)bf[i2 - 1].im()[o2];
+        }
+      }
+      vr = (R)ar;
+      vi = (R)ai;
+    };
+    Buf<R> acc = bf[4], tmp = bf[5];
+    // top block (may hold up to q+... terms: degrees r*q .. K, K - r*q < q)
+    for (size_t e = tid; e < pl; e += kThreads) {
+      R vr, vi;
+      blockval(r, e, (e / np) == (e % np), vr, vi);
+      acc.re()[e] = vr;
+      acc.im()[e] = vi;
     }
-    // P <- cur
-    for (size_t e = tid; e < 2 * pl; e += kThreads) P[e] = cur[e];
     __syncthreads();
-    // 5a. Q = P^B by log2(B) squarings; last one stored transposed
-    {
-      const R* src = P;
-      R* bufs[2] = {T0, T1};
-      if (p.log2b == 0) {
-        for (size_t e = tid; e < 2 * pl; e += kThreads) {
-          size_t rem = e % pl, i = rem / np, j = rem % np;
-          QT[(e < pl ? 0 : pl) + j * np + i] = P[e];
-        }
-        __syncthreads();
+    const Buf<R> Y = bf[q - 1];
+    for (int j = r - 1; j >= 0; j--) {
+      R* dre = tmp.re();
+      R* dim_ = tmp.im();
+      const Buf<R> b1 = bf[0], b2 = bf[1], b3 = bf[2];
+      const double* tc = sh.tc;
+      const int qq = q, KK = K;
+      wg_gemm<R>(np, np, np, Y.view(), acc.view(),
+                 [=](int i, int jj, R vr, R vi) {
+                   size_t o2 = (size_t)i * np + jj;
+                   double ar = (double)vr, ai = (double)vi;
+                   for (int i2 = 0; i2 < qq; i2++) {
+                     int k = j * qq + i2;
+                     if (k > KK) break;
+                     double c = tc[k];
+                     if (i2 == 0) {
+                       if (i == jj) ar += c;
+                     } else {
+                       const Buf<R>& bb = (i2 == 1) ? b1 : ((i2 == 2) ? b2 : b3);
+                       ar += c * (double)bb.re()[o2];
+                       ai += c * (double)bb.im()[o2];
+                     }
+                   }
+                   dre[o2] = (R)ar;
+                   dim_[o2] = (R)ai;
+                 },
+                 sh.g);
+      Buf<R> t2 = acc;
+      acc = tmp;
+      tmp = t2;
+    }
+    // 4b. s squarings; the free buffers are bf[0..3] and tmp
+    Buf<R> cur = acc;
+    Buf<R> spare[2] = {bf[0], bf[1]};
+    int sp_i = 0;
+    for (int s = 0; s < sexp; s++) {
+      Buf<R> dst = spare[sp_i];
+      R* dre = dst.re();
+      R* dim_ = dst.im();
+      wg_gemm<R>(np, np, np, cur.view(), cur.view(),
+                 [=](int i, int j, R vr, R vi) {
+                   size_t o2 = (size_t)i * np + j;
+                   dre[o2] = vr;
+                   dim_[o2] = vi;
+                 },
+                 sh.g);
+      spare[sp_i] = cur;
+      cur = dst;
+      sp_i ^= 1;
+    }
+    // cur = P. 5a. U doubling with P^(2^k) and squarings up to Q = P^B (stored transposed in QT)
+    const double* coilv = p.coil + 2 * sd.vec_off;
+    const double* rhov = p.rho0 + 2 * sd.vec_off;
+    R* Ure = Ubase;
+    R* Uim = Ubase + ub;
+    const size_t vb = (size_t)p.nblk * np;
+    R* Vre = Vbase;
+    R* Vim = Vbase + vb;
+    for (int c = tid; c < np; c += kThreads) {
+      R ur = R(0), ui = R(0), vr = R(0), vi = R(0);
+      if (c < sd.n) {
+        ur = (R)coilv[2 * c];
+        ui = (R)(-coilv[2 * c + 1]);
+        vr = (R)rhov[2 * c];
+        vi = (R)rhov[2 * c + 1];
       }
-      for (int s = 0; s < p.log2b; s++) {
-        bool last = (s == p.log2b - 1);
-        R* dst = last ? QT : bufs[s & 1];
-        wg_zgemm<R>(src, src, dst, np, R(1), false, last, sh.g);
-        src = dst;
+      Ure[c] = ur;
+      Uim[c] = ui;
+      Vre[c] = vr;
+      Vim[c] = vi;
+    }
+    __syncthreads();
+    Buf<R> QT = bf[2];  // free after the Taylor stage
+    if (cur.p == QT.p) QT = bf[3];
+    for (int k = 0; k < p.log2b; k++) {
+      const int rows = 1 << k;
+      // U[rows .. 2*rows) = U[0 .. rows) * P^(2^k)
+      {
+        CMat<R> Uv{Ure, Uim, np, false};
+        R* ure2 = Ure + (size_t)rows * np;
+        R* uim2 = Uim + (size_t)rows * np;
+        wg_gemm<R>(rows, np, np, Uv, cur.view(),
+                   [=](int i, int j, R vr, R vi) {
+                     ure2[(size_t)i * np + j] = vr;
+                     uim2[(size_t)i * np + j] = vi;
+                   },
+                   sh.g);
+      }
+      // next power; the last one is Q = P^B, stored transposed
+      const bool last = (k == p.log2b - 1);
+      Buf<R> dst = last ? QT : spare[sp_i];
+      if (!last && dst.p == QT.p) dst = spare[sp_i ^ 1];
+      R* dre = dst.re();
+      R* dim_ = dst.im();
+      wg_gemm<R>(np, np, np, cur.view(), cur.view(),
+                 [=](int i, int j, R vr, R vi) {
+                   size_t o2 = last ? (size_t)j * np + i : (size_t)i * np + j;
+                   dre[o2] = vr;
+                   dim_[o2] = vi;
+                 },
+                 sh.g);
+      if (!last) {
+        spare[sp_i] = cur;
+        cur = dst;
+        sp_i ^= 1;
       }
     }
-    // 5b. u_0 = conj(coil) (row), u_{j+1} = u_j * P ; v_0 = rho0, v_{k+1} = Q v_k
-    {
-      const double* coil = p.coil + 2 * sd.vec_off;
-      const double* rho = p.rho0 + 2 * sd.vec_off;
-      const size_t ub = (size_t)p.block * np;   // plane size of U
-      const size_t vb = (size_t)p.nblk * np;    // plane size of V
-      for (int c = tid; c < np; c += kThreads) {
-        R ur = R(0), ui = R(0), vr = R(0), vi = R(0);
-        if (c < sd.n) {
-          ur = (R)coil[2 * c];
-          ui = (R)(-coil[2 * c + 1]);
-          vr = (R)rho[2 * c];
-          vi = (R)rho[2 * c + 1];
-        }
-        U[c] = ur; U[ub + c] = ui;
-        V[c] = vr; V[vb + c] = vi;
+    if (p.log2b == 0) {
+      for (size_t e = tid; e < pl; e += kThreads) {
+        size_t i = e / np, j = e % np;
+        QT.re()[j * np + i] = cur.re()[e];
+        QT.im()[j * np + i] = cur.im()[e];
       }
       __syncthreads();
-      // U chain: u_{j+1}[c] = sum_r u_j[r] P[r][c]
-      for (int j = 1; j < p.block; j++) {
-        const R* ur = U + (size_t)(j - 1) * np;
-        const R* ui = U + ub + (size_t)(j - 1) * np;
-        for (int c = tid; c < np; c += kThreads) {
-          R ar = R(0), ai = R(0);
-          for (int r = 0; r < np; r++) {
-            R pr = P[(size_t)r * np + c], pi = P[pl + (size_t)r * np + c];
-            R xr = ur[r], xi = ui[r];
-            ar = fma(xr, pr, ar); ar = fma(-xi, pi, ar);
-            ai = fma(xr, pi, ai); ai = fma(xi, pr, ai);
-          }
-          U[(size_t)j * np + c] = ar;
-          U[ub + (size_t)j * np + c] = ai;
-        }
-        __syncthreads();
-      }
-      // V chain: v_{k+1}[r] = sum_c Q[r][c] v_k[c] = sum_c QT[c][r] v_k[c]
+    }
+    // 5b. V chain: v_{k+1}[r] = sum_c Q[r][c] v_k[c] = sum_c QT[c][r] v_k[c]
+    {
+      const R* qre = QT.re();
+      const R* qim = QT.im();
       for (int k = 1; k < p.nblk; k++) {
-        const R* xr_ = V + (size_t)(k - 1) * np;
-        const R* xi_ = V + vb + (size_t)(k - 1) * np;
-        for (int r = tid; r < np; r += kThreads) {
+        const R* xr_ = Vre + (size_t)(k - 1) * np;
+        const R* xi_ = Vim + (size_t)(k - 1) * np;
+        for (int rr = tid; rr < np; rr += kThreads) {
           R ar = R(0), ai = R(0);
           for (int c = 0; c < np; c++) {
-            R qr = QT[(size_t)c * np + r], qi = QT[pl + (size_t)c * np + r];
+            R qr = qre[(size_t)c * np + rr], qi = qim[(size_t)c * np + rr];
             R xr = xr_[c], xi = xi_[c];
-            ar = fma(qr, xr, ar); ar = fma(-qi, xi, ar);
-            ai = fma(qr, xi, ai); ai = fma(qi, xr, ai);
+            ar = fma(qr, xr, ar);
+            ar = fma(-qi, xi, ar);
+            ai = fma(qr, xi, ai);
+            ai = fma(qi, xr, ai);
           }
-          V[(size_t)k * np + r] = ar;
-          V[vb + (size_t)k * np + r] = ai;
+          Vre[(size_t)k * np + rr] = ar;
+          Vim[(size_t)k * np + rr] = ai;
         }
         __syncthreads();
       }
-      // 5c. f(kB+j) = u_j . v_k  (accumulated in FP64)
+    }
+    // 5c. F = U * V^T ; f(kB + j) = F[j][k]
+    {
       const int64_t npts = p.nsteps + 1;
       double* out = p.per_or + 2 * (size_t)o * npts;
-      for (int64_t t = tid; t < npts; t += kThreads) {
-        int j = (int)(t % p.block), k = (int)(t / p.block);
-        const R* ur = U + (size_t)j * np;
-        const R* ui = U + ub + (size_t)j * np;
-        const R* vr = V + (size_t)k * np;
-        const R* vi = V + vb + (size_t)k * np;
-        double sr = 0.0, si = 0.0;
-        for (int r = 0; r < np; r++) {
-          double a = (double)ur[r], b = (double)ui[r], c = (double)vr[r], d = (double)vi[r];
-          sr += a * c - b * d;
-          si += a * d + b * c;
-        }
-        out[2 * t] = sr;
-        out[2 * t + 1] = si;
-      }
-      __syncthreads();
+      const int B = p.block;
+      CMat<R> Uv{Ure, Uim, np, false};
+      CMat<R> Vt{Vre, Vim, np, true};
+      wg_gemm<R>(B, p.nblk, np, Uv, Vt,
+                 [=](int j, int k, R vr, R vi) {
+                   int64_t t = (int64_t)k * B + j;
+                   if (t < npts) {
+                     out[2 * t] = (double)vr;
+                     out[2 * t + 1] = (double)vi;
+                   }
+                 },
+                 sh.g);
     }
   }
 }

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -257,9 +258,10 @@ struct sp_plan {
   int64_t n_items = 0;
   int64_t total_or = 0;
   double dt = 0;
-  int32_t np_max = 16, block = 1, nblk = 1, log2b = 0, taylor_k = 10;
-  double theta = 0.125;
+  int32_t np_max = 16, block = 1, nblk = 1, log2b = 0, taylor_k = 18, ps_q = 4;
+  double theta = 1.0;
   int grid = 1;
+  int per_cu = 1;
   int64_t work_stride = 0;
   std::vector<int64_t> or_off, n_or;
   std::vector<double*> fid_targets;
@@ -302,12 +304,15 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   pl->log2b = choose_log2_block(npts);
   pl->block = 1 << pl->log2b;
   pl->nblk = (int32_t)((npts + pl->block - 1) / pl->block);
+  // Taylor degree K for |X|_1 <= theta = 1: the remainder is below
+  // 1.05/(K+1)!, i.e. 8.6e-18 for K=18 (FP64) and 2.6e-8 for K=10 (FP32).
+  pl->theta = 1.0;
   if (mode == SP_MODE_FP64) {
-    pl->theta = 0.125;
-    pl->taylor_k = 10;  // 0.125^11/11! < 6e-18
+    pl->taylor_k = 18;
+    pl->ps_q = 4;  // X^2..X^4 + 4 Horner steps = 7 products
   } else {
-    pl->theta = 0.125;
-    pl->taylor_k = 5;  // 0.125^6/6! < 6e-9
+    pl->taylor_k = 10;
+    pl->ps_q = 3;  // X^2, X^3 + 3 Horner steps = 5 products
   }
   std::vector<spk::SysDesc> descs;
   std::vector<int32_t> row, col, tpos, slots;
@@ -384,12 +389,17 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<double>, spk::kThreads, 0));
   else
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<float>, spk::kThreads, 0));
-  per_cu = std::max(1, per_cu);
+  // the occupancy query is conservative for this kernel; at least two
+  // workgroups per CU hide the LDS-staging latency (override:
+  // SPINACH_WG_PER_CU). Workgroups beyond residency find the queue drained.
+  per_cu = std::max(2, per_cu);
+  if (const char* e = getenv("SPINACH_WG_PER_CU")) per_cu = std::max(1, atoi(e));
+  pl->per_cu = per_cu;
   int64_t g = (int64_t)ctx->n_cu * per_cu;
   pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, pl->n_items));
   const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
   const int64_t mat = 2 * (int64_t)pl->np_max * pl->np_max;
-  pl->work_stride = 5 * mat + 2 * (int64_t)pl->block * pl->np_max + 2 * (int64_t)pl->nblk * pl->np_max;
+  pl->work_stride = 6 * mat + 2 * (int64_t)(pl->block + 16) * pl->np_max + 2 * (int64_t)pl->nblk * pl->np_max;
   pl->work_stride = (pl->work_stride + 63) / 64 * 64;
   pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
   pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
@@ -422,6 +432,7 @@ void launch(sp_plan* pl) {
   kp.nblk = pl->nblk;
   kp.log2b = pl->log2b;
   kp.taylor_k = pl->taylor_k;
+  kp.ps_q = pl->ps_q;
   kp.theta = pl->theta;
   kp.work = pl->d_work.as<R>();
   kp.work_stride = pl->work_stride;

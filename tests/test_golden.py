@@ -1,0 +1,51 @@
+"""Committed fixtures (tests/golden/make_golden.py): Spinach's own grid files,
+and oracle FIDs of seeded workloads. The oracle and the host-side builder
+must keep reproducing them; the GPU path is compared with them in
+test_gpu_parity.py."""
+import os
+
+import numpy as np
+
+import sys
+
+import pyoracle as O
+from spinach_amd import grids, spinsys as S, workloads
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+sys.path.insert(0, GOLD)
+
+
+def load_grid(name):
+    d = np.load(os.path.join(GOLD, 'grids.npz'))
+    return {k: d[f'{name}__{k}'] for k in ('alphas', 'betas', 'gammas', 'weights')}
+
+
+def test_reference_grids_are_normalised():
+    for name in ('leb_2ang_rank_5', 'rep_2ang_100pts_sph', 'rep_3ang_100pts'):
+        g = load_grid(name)
+        assert abs(g['weights'].sum() - 1.0) < 1e-12
+        assert len(g['alphas']) == len(g['weights'])
+
+
+def test_static_2spin_fixture_reproduced():
+    d = np.load(os.path.join(GOLD, 'static_2spin_fid.npz'))
+    w = workloads.static_2spin(npoints=int(d['npoints']))
+    from make_golden import fingerprint
+    assert np.allclose(fingerprint(w), d['fingerprint'], rtol=1e-13, atol=0)
+    sub = grids.subset(w['grid'], d['idx'])
+    fid = O.refcpu_powder(w['I'], w['Q'], sub, w['rho0'], w['coil'], w['dt'], w['nsteps'])
+    assert O.rel_err(fid, d['fid']) < 1e-13
+
+
+def test_ctx_powder_average_fixture():
+    # mirror of tests/kernel/test_ctx_powder_average.m: weighted sum equals
+    # the explicit sum of per-orientation FIDs
+    d = np.load(os.path.join(GOLD, 'ctx_powder_average.npz'))
+    per, fid = d['per'], d['fid']
+    w = load_grid('leb_2ang_rank_5')['weights']
+    ref = w[0] * per[0]
+    for k in range(1, len(per)):
+        ref = ref + w[k] * per[k]
+    assert np.linalg.norm(fid - ref) <= 1e-12 + 1e-12 * max(1.0, np.linalg.norm(ref))
+    # one spin: every orientation starts at coil'*rho0 = Tr(L- L+)/2 = 1/2
+    assert np.allclose(per[:, 0], 0.5, atol=1e-14)

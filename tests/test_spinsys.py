@@ -1,0 +1,234 @@
+"""Host-side setup (create/basis/assume/hamiltonian/state) pinned by
+mathematics that does not depend on its construction: Hilbert-space brute
+force, Lie-algebra identities, closed-form Wigner matrices, and the
+reference's own unit checks (tests/kernel/test_zeeman_hamiltonian.m,
+test_scalar_coupling_hamiltonian.m, test_orientation_average_suite.m,
+test_wigner_angular_suite.m)."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import pyoracle as O
+from spinach_amd import spinsys as S
+
+
+def _system(isotopes, seed=5, coords=None, scalar=None, level=None):
+    rng = np.random.default_rng(seed)
+    n = len(isotopes)
+    if coords is None:
+        coords = [np.array([0, 0, 0.]), np.array([1.2, 0.3, -0.2]), np.array([0.1, 1.4, 0.5]),
+                  np.array([1.3, 1.5, 0.9])][:n]
+    inter = {'zeeman': {'eigs': [list(rng.uniform(-100, 100, 3)) for _ in range(n)],
+                        'euler': [list(rng.uniform(0, 2 * np.pi, 3)) for _ in range(n)]},
+             'coupling': {'scalar': scalar or {}}, 'coordinates': coords}
+    ss = S.create({'magnet': 14.1, 'isotopes': isotopes}, inter)
+    bas = {'approximation': 'none'} if level is None else {'approximation': 'IK-0', 'level': level}
+    return S.assume(S.basis(ss, bas), 'nmr')
+
+
+def _basis_ops(ss):
+    t = S.irr_sph_ten_half()
+    norms = [math.sqrt(np.trace(x.conj().T @ x).real) for x in t]
+    ops = []
+    for st in ss['basis']:
+        m = np.eye(1)
+        for i in st:
+            m = np.kron(m, t[i] / norms[i])
+        ops.append(m)
+    return ops
+
+
+def _emb(op, k, n):
+    m = np.eye(1)
+    for i in range(n):
+        m = np.kron(m, op if i == k else np.eye(2))
+    return m
+
+
+def _hilbert_hamiltonian(ss, euler, convention='RART'):
+    """Secular high-field Hamiltonian built directly from the interaction
+    tensors rotated by R = euler2dcm(euler)."""
+    n = ss['nspins']
+    lp, lm, lz = S._pauli_half()
+    lx, ly = (lp + lm) / 2, (lp - lm) / 2j
+    R = S.euler2dcm(*euler)
+    rot = (lambda A: R @ A @ R.T) if convention == 'RART' else (lambda A: R.T @ A @ R)
+    H = np.zeros((2 ** n, 2 ** n), complex)
+    for a in range(n):
+        Z = rot(ss['zeeman'][a])
+        H += (Z[2, 2] - ss['basefrqs'][a]) * _emb(lz, a, n)
+    for l in range(n):
+        for s in range(n):
+            C = ss['coupling'][l][s]
+            if C is None or l == s:
+                continue
+            C = rot(C)
+            if ss['isotopes'][l] == ss['isotopes'][s]:
+                iso = np.trace(C) / 3
+                d = C[2, 2] - iso
+                xx = _emb(lx, l, n) @ _emb(lx, s, n) + _emb(ly, l, n) @ _emb(ly, s, n)
+                zz = _emb(lz, l, n) @ _emb(lz, s, n)
+                H += iso * (xx + zz) + d * (zz - 0.5 * xx)
+            else:
+                H += C[2, 2] * _emb(lz, l, n) @ _emb(lz, s, n)
+    return H
+
+
+def _liouvillian_bruteforce(ss, H):
+    B = _basis_ops(ss)
+    return np.array([[np.trace(Bk.conj().T @ (H @ Bj - Bj @ H)) for Bj in B] for Bk in B])
+
+
+def test_product_tables_identity_and_lie_algebra():
+    left, right = S.ist_product_tables_half()
+    assert np.allclose(left[0], np.eye(4)) and np.allclose(right[0], np.eye(4))
+    # commutation superoperators on one spin must represent su(2): [Lz, L+] = L+
+    ss = S.basis(S.create({'magnet': 14.1, 'isotopes': ['1H']}, {}), {'approximation': 'none'})
+    cz = S.operator(ss, 'Lz', 0).toarray()
+    cp = S.operator(ss, 'L+', 0).toarray()
+    cm = S.operator(ss, 'L-', 0).toarray()
+    assert np.allclose(cz @ cp - cp @ cz, cp, atol=1e-14)
+    assert np.allclose(cp @ cm - cm @ cp, 2 * cz, atol=1e-14)
+
+
+@pytest.mark.parametrize('n,level,dim', [(2, None, 16), (3, None, 64), (8, 3, 1789), (5, 2, 1 + 15 + 90),
+                                         (10, 3, 3676)])
+def test_basis_dimension_and_order(n, level, dim):
+    ss = S.create({'magnet': 14.1, 'isotopes': ['1H'] * n}, {})
+    bas = {'approximation': 'none'} if level is None else {'approximation': 'IK-0', 'level': level}
+    ss = S.basis(ss, bas)
+    b = ss['basis']
+    assert b.shape == (dim, n)
+    assert not b[0].any()  # unit state first
+    rows = [tuple(r) for r in b]
+    assert rows == sorted(rows)  # sortrows order
+    if level is not None:
+        assert (np.count_nonzero(b, axis=1) <= level).all()
+
+
+def test_liouvillian_matches_hilbert_bruteforce_and_rotation_convention():
+    ss = _system(['1H', '1H', '13C'], scalar={(0, 1): 7.0, (0, 2): 140.0})
+    I, Q = S.hamiltonian(ss)
+    euler = (0.3, 1.1, -0.7)
+    L = (I + S.orientation(Q, euler)).toarray()
+    L = (L + L.conj().T) / 2
+    ref = _liouvillian_bruteforce(ss, _hilbert_hamiltonian(ss, euler, 'RART'))
+    scale = np.abs(ref).max()
+    # the isotropic offset tr(Z)/3 - basefrq cancels a 3.8e9 rad/s carrier in
+    # FP64 (hamiltonian.m:99-100), so agreement is ~1e-16 * 3.8e9 in absolute terms
+    assert np.abs(L - ref).max() < 1e-5
+    assert np.abs(L - ref).max() / scale < 1e-10
+    wrong = _liouvillian_bruteforce(ss, _hilbert_hamiltonian(ss, euler, 'RTAR'))
+    assert np.abs(L - wrong).max() / scale > 1e-2
+
+
+def test_ik0_basis_is_restriction_of_complete_basis():
+    full = _system(['1H', '1H', '13C', '15N'], seed=3)
+    red = _system(['1H', '1H', '13C', '15N'], seed=3, level=2)
+    If, Qf = S.hamiltonian(full)
+    Ir, Qr = S.hamiltonian(red)
+    keys_f = list(full['basis_keys'])
+    idx = np.array([keys_f.index(k) for k in red['basis_keys']])
+    euler = (1.0, 0.4, 2.2)
+    Lf = (If + S.orientation(Qf, euler)).toarray()[np.ix_(idx, idx)]
+    Lr = (Ir + S.orientation(Qr, euler)).toarray()
+    assert np.abs(Lf - Lr).max() < 1e-6
+
+
+def test_zeeman_offset_sign_and_units():
+    # +1 ppm isotropic shift on 1H: I = 1e-6*basefrq*comm(Lz), basefrq = -gamma*B0
+    ss = S.create({'magnet': 14.1, 'isotopes': ['1H']}, {'zeeman': {'scalar': [1.0]}})
+    ss = S.assume(S.basis(ss, {'approximation': 'none'}), 'nmr')
+    I, _ = S.hamiltonian(ss)
+    lz = S.operator(ss, 'Lz', 0).toarray()
+    expect = 1e-6 * (-S.GAMMAS['1H'] * 14.1) * lz
+    assert np.abs(I.toarray() - expect).max() < 1e-9 * np.abs(expect).max() + 1e-9
+    nu = 1e-6 * S.GAMMAS['1H'] * 14.1 / (2 * math.pi)  # 1 ppm of the proton frequency, Hz
+    assert np.allclose(I.toarray(), -2 * math.pi * nu * lz, rtol=1e-9, atol=1e-6)
+
+
+def test_scalar_coupling_homo_and_hetero():
+    J = 140.0
+    for iso, strong in ((['1H', '1H'], True), (['1H', '13C'], False)):
+        ss = S.create({'magnet': 14.1, 'isotopes': iso}, {'coupling': {'scalar': {(0, 1): J}}})
+        ss = S.assume(S.basis(ss, {'approximation': 'none'}), 'nmr')
+        I, _ = S.hamiltonian(ss)
+        lz = S._pauli_half()[2]
+        lp, lm, _ = S._pauli_half()
+        lx, ly = (lp + lm) / 2, (lp - lm) / 2j
+        if strong:
+            H = 2 * math.pi * J * (np.kron(lx, lx) + np.kron(ly, ly) + np.kron(lz, lz))
+        else:
+            H = 2 * math.pi * J * np.kron(lz, lz)
+        ref = _liouvillian_bruteforce(ss, H)
+        assert np.abs(I.toarray() - ref).max() < 1e-9
+
+
+def test_orientation_at_zero_angles_is_diagonal_sum():
+    ss = _system(['1H', '13C'])
+    _, Q = S.hamiltonian(ss)
+    h = S.orientation(Q, (0.0, 0.0, 0.0)).toarray()
+    ref = None
+    for r in (1, 2):
+        for k in range(2 * r + 1):
+            q = Q[r - 1][k][k]
+            if q is not None:
+                ref = q.toarray() if ref is None else ref + q.toarray()
+    ref = (ref + ref.conj().T) / 2
+    assert np.abs(h - ref).max() < 1e-14 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize('l', [1, 2])
+def test_wigner_closed_forms(l):
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        a, b, g = rng.uniform(-3, 3, 3)
+        D = S.wigner(l, a, b, g)
+        assert np.abs(D - O.wigner_expm(l, a, b, g)).max() < 1e-13
+        assert np.abs(D - O.refcpu_wigner(l, a, b, g)).max() < 1e-13
+        assert np.abs(D @ D.conj().T - np.eye(2 * l + 1)).max() < 1e-13
+    assert np.abs(S.wigner(2, 0, 0, 0) - np.eye(5)).max() < 1e-15
+    if l == 1:
+        s = 1 / math.sqrt(2)
+        d = np.array([[0.5, -s, 0.5], [s, 0, -s], [0.5, s, 0.5]])
+        assert np.abs(S.wigner(1, 0, math.pi / 2, 0) - d).max() < 1e-15
+
+
+def test_states_and_inner_product():
+    ss = _system(['1H', '13C', '1H'])
+    rho = S.state(ss, 'Lx', '1H')
+    coil = S.state(ss, 'L+', '1H')
+    # two protons, each contributes Tr(L- Lx)/N = 1/4
+    assert abs(np.vdot(coil, rho) - 0.5) < 1e-14
+    # brute force: expand Lx(1H) / sqrt(N) in the normalised basis
+    n = 3
+    lp, lm, _ = S._pauli_half()
+    lx = (lp + lm) / 2
+    op = (_emb(lx, 0, n) + _emb(lx, 2, n)) / math.sqrt(2 ** n)
+    B = _basis_ops(ss)
+    ref = np.array([np.trace(b.conj().T @ op) for b in B])
+    assert np.abs(rho - ref).max() < 1e-14
+
+
+def test_mat2sphten_reconstructs_rank2_rotation_invariant_norm():
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(3, 3))
+    A = A + A.T
+    _, _, r2 = S.mat2sphten(A)
+    R = S.euler2dcm(*rng.uniform(0, 6, 3))
+    _, _, r2b = S.mat2sphten(R @ A @ R.T)
+    assert abs(np.linalg.norm(r2) - np.linalg.norm(r2b)) < 1e-12
+    # the m=0 component is the zz element of the traceless part times sqrt(3/2)
+    assert abs(r2[2] - math.sqrt(1.5) * (A[2, 2] - np.trace(A) / 3)) < 1e-12
+
+
+def test_dipolar_constant_and_direction():
+    ss = S.create({'magnet': 14.1, 'isotopes': ['1H', '13C']},
+                  {'coordinates': [np.zeros(3), np.array([0, 0, 1.1])]})
+    b = S.GAMMAS['1H'] * S.GAMMAS['13C'] * S.HBAR * S.MU0 / (4 * math.pi * (1.1e-10) ** 3)
+    assert abs(b - 1.43e5) / 1.43e5 < 0.02  # ~22.7 kHz * 2pi
+    D = ss['coupling'][0][1] + ss['coupling'][1][0]
+    assert np.allclose(D, b * np.diag([1, 1, -2]), rtol=1e-12)
