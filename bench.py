@@ -31,13 +31,16 @@ PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix), FLOP/s
 PEAK_FP32_VALU = 78.6e12  # non-packed FP32 vector rate (packed FP32 ops are disabled in this build)
 
 
-def algorithmic_flops_per_orientation(n, nsteps, taylor_k, sexp, block):
+def algorithmic_flops_per_orientation(n, nsteps, taylor_k, ps_q, sexp, block):
     """Real flops of the dense propagation per orientation on the reduced
     dimension n (padding excluded): complex GEMM = 8 n^3, complex matvec =
-    8 n^2, complex dot = 8 n."""
+    8 n^2, complex dot = 8 n. Matrix products: q-1 powers and K//q Horner
+    steps of the Paterson-Stockmeyer Taylor evaluation, s squarings,
+    log2(B) squarings to Q = P^B; vector-matrix work: B-1 rows of the U block
+    and M-1 steps of the V chain; detection: (nsteps+1) dots."""
     npts = nsteps + 1
     nblk = math.ceil(npts / block)
-    gemms = (taylor_k - 1) + sexp + int(round(math.log2(block)))
+    gemms = (ps_q - 1) + taylor_k // ps_q + sexp + int(round(math.log2(block)))
     return gemms * 8.0 * n ** 3 + (block - 1 + nblk - 1) * 8.0 * n ** 2 + npts * 8.0 * n
 
 
@@ -84,7 +87,8 @@ def main():
 
     # roofline of the propagation kernel
     n = st['reduced_dim']
-    flops = algorithmic_flops_per_orientation(n, nsteps, st['taylor_terms'], st['max_squarings'], st['block'])
+    mean_s = st['sum_squarings_lo'] / n_or
+    flops = algorithmic_flops_per_orientation(n, nsteps, st['taylor_terms'], st['ps_block'], mean_s, st['block'])
     k_avg_s = float(np.mean(kms)) * 1e-3
     achieved = flops * n_or / k_avg_s
     roof = {'bound': 'mfma', 'achieved': round(achieved / 1e12, 3), 'peak': round(PEAK_FP64 / 1e12, 1),

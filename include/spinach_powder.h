@@ -78,6 +78,8 @@ typedef struct sp_stats {
   int32_t block;             /* time-block length B of the power scheme */
   int32_t taylor_terms;      /* Taylor degree of the scaled exponential */
   int32_t max_squarings;     /* largest scaling exponent over orientations */
+  int32_t ps_block;          /* Paterson-Stockmeyer block length of the Taylor evaluation */
+  int32_t sum_squarings_lo;  /* sum of scaling exponents over orientations (low 32 bits) */
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */
@@ -167,6 +169,38 @@ sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_s
 /* Copies each system's FID to its sp_system.fid_out (the pointers given at
  * planning time). */
 int sp_plan_result_batch(sp_plan* plan);
+
+/* ---- magic-angle spinning, Hilbert space ------------------------------ */
+
+#define SP_MAS_FRAME_MAGNET 0 /* D_rot2lab*D_rotor*D_lab2rot*D_initial */
+#define SP_MAS_FRAME_ROTOR 1  /* D_rot2lab*D_rotor*D_initial */
+
+typedef struct sp_rotor {
+  double axis[3];   /* spinning axis in the lab frame, e.g. {1,1,1} for the magic angle */
+  double rate_hz;   /* spinning rate */
+  int32_t ticks;    /* rotor phases per period; dt = 1/(rate_hz*ticks) */
+  int32_t frame;    /* SP_MAS_FRAME_MAGNET or SP_MAS_FRAME_ROTOR */
+} sp_rotor;
+
+/* Replaces: the rotor-phase Hamiltonian stack of kernel/utilities/rotor_stack.m:63-151
+ * (H_j = I + sum D(phi_j)(k,m) Q{r}{k,m}, phi_j = 2*pi*j/ticks) propagated in
+ * Hilbert space tick by tick with P_j = expm(-1i*H_j*dt) (kernel/propagator.m:40-42),
+ * rho = P*rho*P' (kernel/step.m:115-137) and detection hdot(coil,rho)
+ * (kernel/utilities/hdot.m:23-29), summed over the powder grid with its weights
+ * (kernel/contexts/powder.m:280-286). I and Q are Hilbert-space operators
+ * (formalism 'zeeman-hilb', dimension N); rho0 and coil are dense N x N
+ * column-major interleaved complex (MATLAB layout). The FID has nsteps+1 points
+ * f(n*dt), n = 0..nsteps. */
+sp_plan* sp_plan_mas(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                     const double* betas, const double* gammas, const double* weights, int64_t n_or,
+                     const double* rho0, const double* coil, const sp_rotor* rotor, int64_t nsteps,
+                     int mode);
+
+/* One-shot MAS call: plan, execute, copy the result, destroy. */
+int sp_mas_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+               const double* betas, const double* gammas, const double* weights, int64_t n_or,
+               const double* rho0, const double* coil, const sp_rotor* rotor, int64_t nsteps,
+               int mode, double* fid_out, double* per_orient_out, sp_stats* stats);
 
 #ifdef __cplusplus
 }

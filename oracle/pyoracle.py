@@ -229,3 +229,74 @@ def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False):
 
 def rel_err(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+# ------------------------------------------------- magic-angle spinning (Hilbert)
+def rotor_wigner(r, axis, euler, phi, frame='magnet'):
+    """Rank-r rotor composition of kernel/utilities/rotor_stack.m:88-125:
+    D = D_rot2lab * D_rotor(phi) * D_lab2rot * D_initial ('magnet' frame) or
+    D_rot2lab * D_rotor(phi) * D_initial ('rotor' frame), with the rotor angles
+    from cart2sph of the axis (theta = pi/2 - elevation)."""
+    x, y, z = axis
+    rphi = math.atan2(y, x)
+    rtheta = math.pi / 2 - math.atan2(z, math.hypot(x, y))
+    d_rot2lab = wigner_expm(r, rphi, rtheta, 0.0)
+    d_lab2rot = wigner_expm(r, 0.0, -rtheta, -rphi)
+    d_init = wigner_expm(r, *euler)
+    d_rotor = wigner_expm(r, 0.0, 0.0, phi)
+    if frame == 'magnet':
+        return d_rot2lab @ d_rotor @ d_lab2rot @ d_init
+    return d_rot2lab @ d_rotor @ d_init
+
+
+def rotor_hamiltonian(I, qlist, axis, euler, phi, frame='magnet'):
+    """H(phi) = I + sum_{r,k,m} D(phi)(k,m) Q{r}{k,m} (rotor_stack.m:126-139)."""
+    h = I.toarray().astype(complex)
+    d1 = rotor_wigner(1, axis, euler, phi, frame)
+    d2 = rotor_wigner(2, axis, euler, phi, frame)
+    coeffs = list(d1.ravel()) + list(d2.ravel())
+    for c, q in zip(coeffs, qlist):
+        if q is not None:
+            h = h + c * q.toarray()
+    return h
+
+
+def mas_powder(I, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='magnet', per_orientation=False):
+    """Hilbert-space MAS powder FID by direct time stepping: at step n the
+    Hamiltonian is H(phi_{n mod ticks}), phi_j = 2*pi*j/ticks
+    (kernel/derivatives/fourdif.m:38); rho <- P rho P' with P = expm(-i H dt)
+    (kernel/step.m:115-137); f(n) = hdot(coil, rho) = sum(conj(coil).*rho)
+    (kernel/utilities/hdot.m:23-29); weighted sum over the grid. The tick
+    propagators repeat every rotor period, so each is formed once."""
+    dt = 1.0 / (rate_hz * ticks)
+    n_or = len(grid['weights'])
+    npts = int(nsteps) + 1
+    per = np.zeros((n_or, npts), complex)
+    c = np.asarray(coil, complex)
+    for o in range(n_or):
+        eul = (grid['alphas'][o], grid['betas'][o], grid['gammas'][o])
+        props = [sla.expm(-1j * dt * rotor_hamiltonian(I, qlist, axis, eul, 2 * math.pi * j / ticks, frame))
+                 for j in range(ticks)]
+        rho = np.asarray(rho0, complex)
+        for n in range(npts):
+            per[o, n] = np.sum(np.conj(c) * rho)
+            P = props[n % ticks]
+            rho = P @ rho @ P.conj().T
+    fid = (np.asarray(grid['weights'])[:, None] * per).sum(axis=0)
+    return (fid, per) if per_orientation else fid
+
+
+def static_hilbert_powder(I, qlist, grid, rho0, coil, dt, nsteps):
+    """Static Hilbert-space powder FID: H = I + orientation(Q, Omega) (with the
+    orientation.m symmetrisation), rho <- P rho P', f = hdot(coil, rho)."""
+    n_or = len(grid['weights'])
+    fid = np.zeros(int(nsteps) + 1, complex)
+    c = np.asarray(coil, complex)
+    for o in range(n_or):
+        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        P = sla.expm(-1j * dt * h)
+        rho = np.asarray(rho0, complex)
+        for n in range(int(nsteps) + 1):
+            fid[n] += grid['weights'][o] * np.sum(np.conj(c) * rho)
+            rho = P @ rho @ P.conj().T
+    return fid

@@ -21,7 +21,9 @@ SP_NQ = 34
 
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
-           'sp_plan_result_batch')
+           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid')
+SP_MAS_FRAME_MAGNET = 0
+SP_MAS_FRAME_ROTOR = 1
 
 
 class SpCsc(C.Structure):
@@ -38,10 +40,14 @@ class SpStats(C.Structure):
     _fields_ = [('elapsed_s', C.c_double), ('orient_steps_per_s', C.c_double), ('kernel_ms', C.c_double),
                 ('full_dim', C.c_int64), ('reduced_dim', C.c_int64), ('reduced_nnz', C.c_int64),
                 ('padded_dim', C.c_int32), ('block', C.c_int32), ('taylor_terms', C.c_int32),
-                ('max_squarings', C.c_int32)]
+                ('max_squarings', C.c_int32), ('ps_block', C.c_int32), ('sum_squarings_lo', C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SpRotor(C.Structure):
+    _fields_ = [('axis', C.c_double * 3), ('rate_hz', C.c_double), ('ticks', C.c_int32), ('frame', C.c_int32)]
 
 
 class SpSystem(C.Structure):
@@ -89,6 +95,12 @@ def load():
                                          C.POINTER(SpTols), C.c_int]
     lib.sp_plan_result_batch.restype = C.c_int
     lib.sp_plan_result_batch.argtypes = [C.c_void_p]
+    lib.sp_plan_mas.restype = C.c_void_p
+    lib.sp_plan_mas.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
+                                _dp, _dp, C.POINTER(SpRotor), C.c_int64, C.c_int]
+    lib.sp_mas_fid.restype = C.c_int
+    lib.sp_mas_fid.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
+                               _dp, _dp, C.POINTER(SpRotor), C.c_int64, C.c_int, _dp, _dp, C.POINTER(SpStats)]
     _lib = lib
     return lib
 
@@ -300,4 +312,62 @@ def powder_fid(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', p
                            per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(st))
     if rc:
         _err(ctx, rc, 'sp_powder_fid')
+    return fid, per, st.as_dict()
+
+
+def _rotor(axis, rate_hz, ticks, frame):
+    r = SpRotor()
+    for i in range(3):
+        r.axis[i] = float(axis[i])
+    r.rate_hz = float(rate_hz)
+    r.ticks = int(ticks)
+    r.frame = SP_MAS_FRAME_MAGNET if frame in ('magnet', SP_MAS_FRAME_MAGNET) else SP_MAS_FRAME_ROTOR
+    return r
+
+
+def _dense_colmajor(m, keep):
+    m = np.asarray(m.toarray() if sps.issparse(m) else m, dtype=np.complex128)
+    return keep.cvec(np.asfortranarray(m).ravel(order='F'))
+
+
+def plan_mas(H, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='magnet', mode='fp32', device=0):
+    """Magic-angle-spinning Hilbert-space plan (sp_plan_mas). H, Q are the
+    zeeman-hilb I and Q; rho0, coil dense N x N."""
+    lib = load()
+    ctx = context(device)
+    k = _Keep()
+    dim = H.shape[0]
+    Hc = k.csc(H, dim)
+    Qa = k.qarray(qlist, dim)
+    n_or = len(grid['weights'])
+    rot = _rotor(axis, rate_hz, ticks, frame)
+    h = lib.sp_plan_mas(ctx, C.byref(Hc), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']),
+                        k.dptr(grid['weights']), n_or, _dense_colmajor(rho0, k), _dense_colmajor(coil, k),
+                        C.byref(rot), int(nsteps), mode_id(mode))
+    if not h:
+        _err(ctx, -1, 'sp_plan_mas')
+    return Plan(h, ctx, int(nsteps) + 1, n_or)
+
+
+def mas_fid(H, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='magnet', mode='fp32',
+            per_orientation=False, device=0):
+    """One-shot sp_mas_fid call."""
+    lib = load()
+    ctx = context(device)
+    k = _Keep()
+    dim = H.shape[0]
+    Hc = k.csc(H, dim)
+    Qa = k.qarray(qlist, dim)
+    n_or = len(grid['weights'])
+    npts = int(nsteps) + 1
+    fid = np.zeros(npts, np.complex128)
+    per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
+    st = SpStats()
+    rot = _rotor(axis, rate_hz, ticks, frame)
+    rc = lib.sp_mas_fid(ctx, C.byref(Hc), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']),
+                        k.dptr(grid['weights']), n_or, _dense_colmajor(rho0, k), _dense_colmajor(coil, k),
+                        C.byref(rot), int(nsteps), mode_id(mode), fid.view(np.float64).ctypes.data_as(_dp),
+                        per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(st))
+    if rc:
+        _err(ctx, rc, 'sp_mas_fid')
     return fid, per, st.as_dict()

@@ -95,7 +95,7 @@ __global__ void k_reset(unsigned int* q) {
 
 // ----------------------------------------------------------------- Wigner
 // d^l_{m'm}(beta), rows/cols in descending m; closed forms of Brink & Satchler.
-__device__ inline void wigner_small_d2(double b, double d[5][5]) {
+__host__ __device__ inline void wigner_small_d2(double b, double d[5][5]) {
   double c = cos(b), s = sin(b);
   double ch = cos(0.5 * b), sh = sin(0.5 * b);
   const double r38 = 0.61237243569579452455, r32 = 1.2247448713915890491;
@@ -128,7 +128,7 @@ __device__ inline void wigner_small_d2(double b, double d[5][5]) {
     }
 }
 
-__device__ inline void wigner_small_d1(double b, double d[3][3]) {
+__host__ __device__ inline void wigner_small_d1(double b, double d[3][3]) {
   double c = cos(b), s = sin(b);
   const double r2 = 0.70710678118654752440;
   d[0][0] = (1 + c) * 0.5; d[0][1] = -s * r2; d[0][2] = (1 - c) * 0.5;
@@ -137,7 +137,7 @@ __device__ inline void wigner_small_d1(double b, double d[3][3]) {
 }
 
 // coefficient of Q slot t (0..33) at Euler angles (a,b,g): D^r(k,m)
-__device__ inline void wigner_coeffs(double a, double b, double g, double* cre, double* cim) {
+__host__ __device__ inline void wigner_coeffs(double a, double b, double g, double* cre, double* cim) {
   double d1[3][3], d2[5][5];
   wigner_small_d1(b, d1);
   wigner_small_d2(b, d2);
@@ -179,13 +179,14 @@ struct Mma<float> {
 };
 
 // Planar complex matrix view: element (i,j) at re[i*ld+j], im[i*ld+j]
-// (or (j,i) when trans is set).
+// (or (j,i) when trans is set); conj negates the imaginary part on load.
 template <typename R>
 struct CMat {
   const R* re;
   const R* im;
   int ld;
   bool trans;
+  bool conj = false;
   __device__ inline size_t off(int i, int j) const {
     return trans ? (size_t)j * ld + i : (size_t)i * ld + j;
   }
@@ -220,34 +221,50 @@ __device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, Gemm
           cr[a][b] = v4{0, 0, 0, 0};
           ci[a][b] = v4{0, 0, 0, 0};
         }
-      for (int k0 = 0; k0 < K; k0 += 16) {
+      // register prefetch of the next k-panel overlaps global loads with MFMA work
+      R pa_r[4], pa_i[4], pb_r[4], pb_i[4];
+      auto fetch = [&](int k0) {
 #pragma unroll
         for (int rep = 0; rep < 4; rep++) {
           int e = tid + rep * kThreads;
-          // A panel: rows i0..i0+63, cols k0..k0+15 (contiguous in k when not transposed)
           int r, c;
           if (!A.trans) { r = e >> 4; c = e & 15; } else { c = e >> 6; r = e & 63; }
           R va = R(0), vb = R(0);
           if (i0 + r < M) {
             size_t o = A.off(i0 + r, k0 + c);
             va = A.re[o];
-            vb = A.im[o];
+            vb = A.conj ? -A.im[o] : A.im[o];
           }
-          sm.ar[c][r] = va;
-          sm.ai[c][r] = vb;
-          // B panel: rows k0..k0+15, cols j0..j0+63
+          pa_r[rep] = va;
+          pa_i[rep] = vb;
           int rb, cb;
           if (!B.trans) { rb = e >> 6; cb = e & 63; } else { cb = e >> 4; rb = e & 15; }
           R wa = R(0), wb = R(0);
           if (j0 + cb < N) {
             size_t o = B.off(k0 + rb, j0 + cb);
             wa = B.re[o];
-            wb = B.im[o];
+            wb = B.conj ? -B.im[o] : B.im[o];
           }
-          sm.br[rb][cb] = wa;
-          sm.bi[rb][cb] = wb;
+          pb_r[rep] = wa;
+          pb_i[rep] = wb;
+        }
+      };
+      fetch(0);
+      for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int rep = 0; rep < 4; rep++) {
+          int e = tid + rep * kThreads;
+          int r, c;
+          if (!A.trans) { r = e >> 4; c = e & 15; } else { c = e >> 6; r = e & 63; }
+          sm.ar[c][r] = pa_r[rep];
+          sm.ai[c][r] = pa_i[rep];
+          int rb, cb;
+          if (!B.trans) { rb = e >> 6; cb = e & 63; } else { cb = e >> 4; rb = e & 15; }
+          sm.br[rb][cb] = pb_r[rep];
+          sm.bi[rb][cb] = pb_i[rep];
         }
         __syncthreads();
+        if (k0 + 16 < K) fetch(k0 + 16);
         if (active) {
 #pragma unroll
           for (int kk = 0; kk < 16; kk += 4) {
@@ -320,8 +337,101 @@ struct Buf {
   int np;
   __device__ inline R* re() const { return p; }
   __device__ inline R* im() const { return p + pl; }
-  __device__ inline CMat<R> view(bool trans = false) const { return CMat<R>{p, p + pl, np, trans}; }
+  __device__ inline CMat<R> view(bool trans = false, bool cj = false) const {
+    return CMat<R>{p, p + pl, np, trans, cj};
+  }
 };
+
+// P = expm(X) for a pre-scaled X (|X|_1 <= theta) held in bf[0], followed by
+// sexp squarings. The Taylor polynomial of degree K is evaluated by the
+// Paterson-Stockmeyer scheme: powers X^2..X^q go to bf[1..q-1], then Horner
+// in Y = X^q over blocks B_j(X) = sum_{i<q} c_{jq+i} X^i (c_k = tc[k] = 1/k!).
+// Returns the buffer that holds P; every other buffer of bf may be reused.
+template <typename R>
+__device__ Buf<R> wg_expm(Buf<R>* bf, int sexp, int q, int K, const double* tc, GemmLds<R>& sm) {
+  const int tid = threadIdx.x;
+  const int np = bf[0].np;
+  const size_t pl = bf[0].pl;
+  for (int k = 2; k <= q; k++) {
+    R* dre = bf[k - 1].re();
+    R* dim_ = bf[k - 1].im();
+    wg_gemm<R>(np, np, np, bf[k - 2].view(), bf[0].view(),
+               [=](int i, int j, R vr, R vi) {
+                 size_t o2 = (size_t)i * np + j;
+                 dre[o2] = vr;
+                 dim_[o2] = vi;
+               },
+               sm);
+  }
+  const int r = K / q;
+  Buf<R> acc = bf[4], tmp = bf[5];
+  const Buf<R> b1 = bf[0], b2 = bf[1], b3 = bf[2];
+  // top block: degrees r*q .. K
+  for (size_t e = tid; e < pl; e += kThreads) {
+    double ar = 0.0, ai = 0.0;
+    for (int i2 = 0; i2 < q; i2++) {
+      int k = r * q + i2;
+      if (k > K) break;
+      if (i2 == 0) {
+        if ((e / np) == (e % np)) ar += tc[k];
+      } else {
+        const Buf<R>& bb = (i2 == 1) ? b1 : ((i2 == 2) ? b2 : b3);
+        ar += tc[k] * (double)bb.re()[e];
+        ai += tc[k] * (double)bb.im()[e];
+      }
+    }
+    acc.re()[e] = (R)ar;
+    acc.im()[e] = (R)ai;
+  }
+  __syncthreads();
+  const Buf<R> Y = bf[q - 1];
+  for (int j = r - 1; j >= 0; j--) {
+    R* dre = tmp.re();
+    R* dim_ = tmp.im();
+    wg_gemm<R>(np, np, np, Y.view(), acc.view(),
+               [=](int i, int jj, R vr, R vi) {
+                 size_t o2 = (size_t)i * np + jj;
+                 double ar = (double)vr, ai = (double)vi;
+                 for (int i2 = 0; i2 < q; i2++) {
+                   int k = j * q + i2;
+                   if (k > K) break;
+                   if (i2 == 0) {
+                     if (i == jj) ar += tc[k];
+                   } else {
+                     const Buf<R>& bb = (i2 == 1) ? b1 : ((i2 == 2) ? b2 : b3);
+                     ar += tc[k] * (double)bb.re()[o2];
+                     ai += tc[k] * (double)bb.im()[o2];
+                   }
+                 }
+                 dre[o2] = (R)ar;
+                 dim_[o2] = (R)ai;
+               },
+               sm);
+    Buf<R> t2 = acc;
+    acc = tmp;
+    tmp = t2;
+  }
+  // squarings, ping-pong through X and X^2 (no longer needed)
+  Buf<R> cur = acc;
+  Buf<R> spare[2] = {bf[0], bf[1]};
+  int sp_i = 0;
+  for (int s = 0; s < sexp; s++) {
+    Buf<R> dst = spare[sp_i];
+    R* dre = dst.re();
+    R* dim_ = dst.im();
+    wg_gemm<R>(np, np, np, cur.view(), cur.view(),
+               [=](int i, int j, R vr, R vi) {
+                 size_t o2 = (size_t)i * np + j;
+                 dre[o2] = vr;
+                 dim_[o2] = vi;
+               },
+               sm);
+    spare[sp_i] = cur;
+    cur = dst;
+    sp_i ^= 1;
+  }
+  return cur;
+}
 
 // --------------------------------------------------------- the fused kernel
 template <typename R>
@@ -418,101 +528,15 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
       for (size_t e = tid; e < 2 * pl; e += kThreads) X[e] *= scl;
       __syncthreads();
     }
-    // 4. Paterson-Stockmeyer: powers X^2..X^q in bf[1..q-1], then Horner in
-    //    Y = X^q over blocks B_j(X) = sum_{i<q} c_{jq+i} X^i.
-    const int q = p.ps_q;      // 2..4
-    const int K = p.taylor_k;  // degree
-    const int r = K / q;       // index of the top block
+    // 4. P = expm(X): Paterson-Stockmeyer Taylor polynomial and s squarings
+    Buf<R> cur = wg_expm<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
+    Buf<R> spare[2];
     {
-      for (int k = 2; k <= q; k++) {
-        Buf<R> dst = bf[k - 1];
-        R* dre = dst.re();
-        R* dim_ = dst.im();
-        wg_gemm<R>(np, np, np, bf[k - 2].view(), bf[0].view(),
-                   [=](int i, int j, R vr, R vi) {
-                     size_t o2 = (size_t)i * np + j;
-                     dre[o2] = vr;
-                     dim_[o2] = vi;
-                   },
-                   sh.g);
-      }
+      int ns = 0;
+      for (int b2 = 0; b2 < 6 && ns < 2; b2++)
+        if (b2 != 2 && bf[b2].p != cur.p) spare[ns++] = bf[b2];
     }
-    // block combination helper: c0*I + c1*X + c2*X^2 + ... (powers < q)
-    auto blockval = [&](int j, size_t o2, bool diag, R& vr, R& vi) {
-      double ar = 0.0, ai = 0.0;
-      for (int i2 = 0; i2 < q; i2++) {
-        int k = j * q + i2;
-        if (k > K) break;
-        double c = sh.tc[k];
-        if (i2 == 0) {
-          if (diag) ar += c;
-        } else {
-          ar += c * (double)bf[i2 - 1].re()[o2];
-          ai += c * (double)bf[i2 - 1].im()[o2];
-        }
-      }
-      vr = (R)ar;
-      vi = (R)ai;
-    };
-    Buf<R> acc = bf[4], tmp = bf[5];
-    // top block (may hold up to q+... terms: degrees r*q .. K, K - r*q < q)
-    for (size_t e = tid; e < pl; e += kThreads) {
-      R vr, vi;
-      blockval(r, e, (e / np) == (e % np), vr, vi);
-      acc.re()[e] = vr;
-      acc.im()[e] = vi;
-    }
-    __syncthreads();
-    const Buf<R> Y = bf[q - 1];
-    for (int j = r - 1; j >= 0; j--) {
-      R* dre = tmp.re();
-      R* dim_ = tmp.im();
-      const Buf<R> b1 = bf[0], b2 = bf[1], b3 = bf[2];
-      const double* tc = sh.tc;
-      const int qq = q, KK = K;
-      wg_gemm<R>(np, np, np, Y.view(), acc.view(),
-                 [=](int i, int jj, R vr, R vi) {
-                   size_t o2 = (size_t)i * np + jj;
-                   double ar = (double)vr, ai = (double)vi;
-                   for (int i2 = 0; i2 < qq; i2++) {
-                     int k = j * qq + i2;
-                     if (k > KK) break;
-                     double c = tc[k];
-                     if (i2 == 0) {
-                       if (i == jj) ar += c;
-                     } else {
-                       const Buf<R>& bb = (i2 == 1) ? b1 : ((i2 == 2) ? b2 : b3);
-                       ar += c * (double)bb.re()[o2];
-                       ai += c * (double)bb.im()[o2];
-                     }
-                   }
-                   dre[o2] = (R)ar;
-                   dim_[o2] = (R)ai;
-                 },
-                 sh.g);
-      Buf<R> t2 = acc;
-      acc = tmp;
-      tmp = t2;
-    }
-    // 4b. s squarings; the free buffers are bf[0..3] and tmp
-    Buf<R> cur = acc;
-    Buf<R> spare[2] = {bf[0], bf[1]};
     int sp_i = 0;
-    for (int s = 0; s < sexp; s++) {
-      Buf<R> dst = spare[sp_i];
-      R* dre = dst.re();
-      R* dim_ = dst.im();
-      wg_gemm<R>(np, np, np, cur.view(), cur.view(),
-                 [=](int i, int j, R vr, R vi) {
-                   size_t o2 = (size_t)i * np + j;
-                   dre[o2] = vr;
-                   dim_[o2] = vi;
-                 },
-                 sh.g);
-      spare[sp_i] = cur;
-      cur = dst;
-      sp_i ^= 1;
-    }
     // cur = P. 5a. U doubling with P^(2^k) and squarings up to Q = P^B (stored transposed in QT)
     const double* coilv = p.coil + 2 * sd.vec_off;
     const double* rhov = p.rho0 + 2 * sd.vec_off;

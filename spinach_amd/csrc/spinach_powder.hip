@@ -27,6 +27,7 @@
 
 #include "../../include/spinach_powder.h"
 #include "powder_kernels.hpp"
+#include "mas_kernels.hpp"
 
 namespace {
 
@@ -71,10 +72,14 @@ struct DevBuf {
   T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Blocking copy: planning is not timed, and the host vectors may be temporaries.
 template <typename T>
 void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
   b.alloc(v.size() * sizeof(T));
-  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  if (!v.empty()) {
+    HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
 }
 
 // ------------------------------------------------------------ host planning
@@ -270,6 +275,10 @@ struct sp_plan {
   DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
   DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
   bool executed = false;
+  // magic-angle spinning plans
+  bool mas = false;
+  int32_t ticks = 0, n_rho = 0, mas_np = 0, mas_nnz = 0, mas_nq = 0;
+  DevBuf d_rot2lab, d_lab2rot;
 };
 
 namespace {
@@ -443,6 +452,192 @@ void launch(sp_plan* pl) {
   HIPCHK(hipGetLastError());
 }
 
+// Wigner matrices (rank 1 then rank 2, interleaved complex) at Euler angles.
+std::vector<double> wigner_pair(double a, double b, double g) {
+  double cre[34], cim[34];
+  spk::wigner_coeffs(a, b, g, cre, cim);
+  std::vector<double> v(68);
+  for (int t = 0; t < 34; t++) {
+    v[2 * t] = cre[t];
+    v[2 * t + 1] = cim[t];
+  }
+  return v;
+}
+
+sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al, const double* be,
+                        const double* ga, const double* w, int64_t n_or, const double* rho0, const double* coil,
+                        const sp_rotor* rot, int64_t nsteps, int mode) {
+  if (!I || !Q) throw SpError(SP_EINVAL, "I and Q must not be NULL");
+  if (!al || !be || !ga || !w || n_or <= 0) throw SpError(SP_EINVAL, "invalid grid");
+  if (!rho0 || !coil) throw SpError(SP_EINVAL, "rho0 and coil must not be NULL");
+  if (!rot) throw SpError(SP_EINVAL, "rotor must not be NULL");
+  if (!(rot->rate_hz > 0) || !std::isfinite(rot->rate_hz)) throw SpError(SP_EINVAL, "rotor rate must be positive");
+  if (rot->ticks < 1 || rot->ticks > 4096) throw SpError(SP_EINVAL, "rotor ticks must be in 1..4096");
+  if (rot->frame != SP_MAS_FRAME_MAGNET && rot->frame != SP_MAS_FRAME_ROTOR)
+    throw SpError(SP_EINVAL, "unknown MAS frame");
+  double ax = rot->axis[0], ay = rot->axis[1], az = rot->axis[2];
+  double an = std::sqrt(ax * ax + ay * ay + az * az);
+  if (!(an > 0) || !std::isfinite(an)) throw SpError(SP_EINVAL, "rotor axis must be a non-zero finite vector");
+  if (nsteps < 0 || nsteps > (int64_t)1 << 30) throw SpError(SP_EINVAL, "invalid nsteps");
+  if (mode != SP_MODE_FP64 && mode != SP_MODE_FP32) throw SpError(SP_EINVAL, "unknown precision mode");
+  const int64_t N = I->n;
+  if (N <= 0 || N > 1024) throw SpError(SP_ERANGE, "Hilbert dimension must be in 1..1024");
+  std::vector<Triplets> mats(1 + SP_NQ);
+  mats[0] = read_csc(I, N, "I");
+  for (int q = 0; q < SP_NQ; q++) mats[1 + q] = read_csc(&Q[q], N, "Q slot");
+  for (int64_t k = 0; k < n_or; k++)
+    if (!std::isfinite(al[k]) || !std::isfinite(be[k]) || !std::isfinite(ga[k]) || !std::isfinite(w[k]))
+      throw SpError(SP_EINVAL, "non-finite grid entry");
+  for (int64_t i = 0; i < 2 * N * N; i++)
+    if (!std::isfinite(rho0[i]) || !std::isfinite(coil[i])) throw SpError(SP_EINVAL, "non-finite state matrix");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::unique_ptr<sp_plan> pl(new sp_plan);
+  pl->ctx = ctx;
+  pl->mas = true;
+  pl->mode = mode;
+  pl->nsteps = nsteps;
+  pl->n_systems = 1;
+  pl->ticks = rot->ticks;
+  pl->dt = 1.0 / (rot->rate_hz * rot->ticks);
+  const int64_t npts = nsteps + 1;
+  pl->n_rho = (int32_t)((npts + rot->ticks - 1) / rot->ticks);
+  const int32_t np = (int32_t)std::max<int64_t>(16, (N + 15) / 16 * 16);
+  pl->mas_np = np;
+  pl->np_max = np;
+  pl->theta = 1.0;
+  if (mode == SP_MODE_FP64) {
+    pl->taylor_k = 18;
+    pl->ps_q = 4;
+  } else {
+    pl->taylor_k = 10;
+    pl->ps_q = 3;
+  }
+  // union pattern of I and Q (no reduction in Hilbert space)
+  std::vector<std::pair<int64_t, int64_t>> ent;
+  for (auto& m : mats)
+    for (size_t q = 0; q < m.row.size(); q++) ent.emplace_back(m.row[q], m.col[q]);
+  std::sort(ent.begin(), ent.end());
+  ent.erase(std::unique(ent.begin(), ent.end()), ent.end());
+  const int32_t nnz = (int32_t)ent.size();
+  std::vector<int32_t> row(nnz), col(nnz), slots;
+  for (int32_t e = 0; e < nnz; e++) {
+    row[e] = (int32_t)ent[e].first;
+    col[e] = (int32_t)ent[e].second;
+  }
+  auto find = [&](int64_t r, int64_t c) {
+    return (int64_t)(std::lower_bound(ent.begin(), ent.end(), std::make_pair(r, c)) - ent.begin());
+  };
+  std::vector<double> vals, tmp;
+  auto scatter = [&](const Triplets& m) {
+    tmp.assign(2 * (size_t)nnz, 0.0);
+    for (size_t q = 0; q < m.row.size(); q++) {
+      int64_t e = find(m.row[q], m.col[q]);
+      tmp[2 * e] += m.re[q];
+      tmp[2 * e + 1] += m.im[q];
+    }
+    return !m.row.empty();
+  };
+  scatter(mats[0]);
+  vals = tmp;
+  for (int q = 0; q < SP_NQ; q++)
+    if (scatter(mats[1 + q])) {
+      slots.push_back(q);
+      vals.insert(vals.end(), tmp.begin(), tmp.end());
+    }
+  pl->mas_nnz = nnz;
+  pl->mas_nq = (int32_t)slots.size();
+  // dense states: MATLAB column-major N x N -> padded row-major np x np
+  std::vector<double> r0(2 * (size_t)np * np, 0.0), c0(2 * (size_t)np * np, 0.0);
+  for (int64_t c = 0; c < N; c++)
+    for (int64_t r = 0; r < N; r++) {
+      size_t src = 2 * (size_t)(c * N + r), dst = 2 * (size_t)(r * np + c);
+      r0[dst] = rho0[src];
+      r0[dst + 1] = rho0[src + 1];
+      c0[dst] = coil[src];
+      c0[dst + 1] = coil[src + 1];
+    }
+  // rotor frame matrices (rotor_stack.m:88-105): cart2sph of the axis
+  double rphi = std::atan2(ay, ax);
+  double rtheta = M_PI / 2 - std::atan2(az, std::sqrt(ax * ax + ay * ay));
+  std::vector<double> rot2lab = wigner_pair(rphi, rtheta, 0.0);
+  std::vector<double> lab2rot(68, 0.0);
+  if (rot->frame == SP_MAS_FRAME_MAGNET) {
+    lab2rot = wigner_pair(0.0, -rtheta, -rphi);
+  } else {
+    for (int k = 0; k < 3; k++) lab2rot[2 * (k * 3 + k)] = 1.0;
+    for (int k = 0; k < 5; k++) lab2rot[2 * (9 + k * 5 + k)] = 1.0;
+  }
+  hipStream_t st = ctx->stream;
+  upload(pl->d_row, row, st);
+  upload(pl->d_col, col, st);
+  upload(pl->d_vals, vals, st);
+  upload(pl->d_slots, slots, st);
+  upload(pl->d_rho, r0, st);
+  upload(pl->d_coil, c0, st);
+  upload(pl->d_al, std::vector<double>(al, al + n_or), st);
+  upload(pl->d_be, std::vector<double>(be, be + n_or), st);
+  upload(pl->d_ga, std::vector<double>(ga, ga + n_or), st);
+  upload(pl->d_w, std::vector<double>(w, w + n_or), st);
+  upload(pl->d_rot2lab, rot2lab, st);
+  upload(pl->d_lab2rot, lab2rot, st);
+  pl->d_queue.alloc(sizeof(unsigned int));
+  pl->n_items = n_or;
+  pl->total_or = n_or;
+  pl->or_off.push_back(0);
+  pl->n_or.push_back(n_or);
+  pl->full_dim = N;
+  pl->red_dim = N;
+  pl->red_nnz = nnz;
+  int per_cu = 2;
+  if (const char* e = getenv("SPINACH_WG_PER_CU")) per_cu = std::max(1, atoi(e));
+  pl->per_cu = per_cu;
+  pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * per_cu, n_or));
+  const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
+  const int64_t mat = 2 * (int64_t)np * np;
+  pl->work_stride = 10 * mat + mat * (int64_t)pl->ticks + mat * (int64_t)pl->n_rho;
+  pl->work_stride = (pl->work_stride + 63) / 64 * 64;
+  pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+  pl->d_per.alloc((size_t)n_or * npts * 2 * sizeof(double));
+  pl->d_fid.alloc((size_t)npts * 2 * sizeof(double));
+  pl->d_sq.alloc((size_t)n_or * sizeof(int32_t));
+  HIPCHK(hipStreamSynchronize(st));
+  return pl.release();
+}
+
+template <typename R>
+void launch_mas(sp_plan* pl) {
+  spk::MasParams<R> kp{};
+  kp.prow = pl->d_row.as<int32_t>();
+  kp.pcol = pl->d_col.as<int32_t>();
+  kp.nnz = pl->mas_nnz;
+  kp.nq = pl->mas_nq;
+  kp.vals = pl->d_vals.as<double>();
+  kp.slots = pl->d_slots.as<int32_t>();
+  kp.rho0 = pl->d_rho.as<double>();
+  kp.coil = pl->d_coil.as<double>();
+  kp.alphas = pl->d_al.as<double>();
+  kp.betas = pl->d_be.as<double>();
+  kp.gammas = pl->d_ga.as<double>();
+  kp.n_or = pl->total_or;
+  kp.rot2lab = pl->d_rot2lab.as<double>();
+  kp.lab2rot = pl->d_lab2rot.as<double>();
+  kp.dt = pl->dt;
+  kp.ticks = pl->ticks;
+  kp.n_rho = pl->n_rho;
+  kp.nsteps = pl->nsteps;
+  kp.np = pl->mas_np;
+  kp.taylor_k = pl->taylor_k;
+  kp.ps_q = pl->ps_q;
+  kp.theta = pl->theta;
+  kp.work = pl->d_work.as<R>();
+  kp.work_stride = pl->work_stride;
+  kp.queue = pl->d_queue.as<unsigned int>();
+  kp.per_or = pl->d_per.as<double>();
+  kp.sq_out = pl->d_sq.as<int32_t>();
+  hipLaunchKernelGGL(spk::k_mas<R>, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
+  HIPCHK(hipGetLastError());
+}
+
 int run(sp_plan* pl, sp_stats* stats) {
   sp_ctx* ctx = pl->ctx;
   HIPCHK(hipSetDevice(ctx->device));
@@ -452,10 +647,16 @@ int run(sp_plan* pl, sp_stats* stats) {
   hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, pl->d_queue.as<unsigned int>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, st));
-  if (pl->mode == SP_MODE_FP64)
+  if (pl->mas) {
+    if (pl->mode == SP_MODE_FP64)
+      launch_mas<double>(pl);
+    else
+      launch_mas<float>(pl);
+  } else if (pl->mode == SP_MODE_FP64) {
     launch<double>(pl);
-  else
+  } else {
     launch<float>(pl);
+  }
   HIPCHK(hipEventRecord(ctx->ev2, st));
   for (int64_t s = 0; s < pl->n_systems; s++) {
     int nb = (int)((npts + 255) / 256);
@@ -489,8 +690,14 @@ int run(sp_plan* pl, sp_stats* stats) {
   std::vector<int32_t> sq((size_t)pl->total_or);
   HIPCHK(hipMemcpy(sq.data(), pl->d_sq.p, sq.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   int mx = 0;
-  for (int v : sq) mx = std::max(mx, v);
+  int64_t sum = 0;
+  for (int v : sq) {
+    mx = std::max(mx, v);
+    sum += v;
+  }
   s.max_squarings = mx;
+  s.ps_block = pl->ps_q;
+  s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   pl->last = s;
   if (stats) *stats = s;
   return SP_OK;
@@ -637,12 +844,44 @@ void sp_plan_destroy(sp_plan* plan) {
   delete plan;
 }
 
+sp_plan* sp_plan_mas(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas, const double* betas,
+                     const double* gammas, const double* weights, int64_t n_or, const double* rho0, const double* coil,
+                     const sp_rotor* rotor, int64_t nsteps, int mode) {
+  if (!ctx) return nullptr;
+  try {
+    return build_mas_plan(ctx, I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, rotor, nsteps, mode);
+  } catch (SpError& e) {
+    fail(ctx, e.code, e.what());
+  } catch (std::bad_alloc&) {
+    fail(ctx, SP_ENOMEM, "host allocation failed");
+  } catch (std::exception& e) {
+    fail(ctx, SP_EDEVICE, e.what());
+  }
+  return nullptr;
+}
+
+int sp_mas_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas, const double* betas,
+               const double* gammas, const double* weights, int64_t n_or, const double* rho0, const double* coil,
+               const sp_rotor* rotor, int64_t nsteps, int mode, double* fid_out, double* per_orient_out,
+               sp_stats* stats) {
+  if (!ctx) return SP_EINVAL;
+  if (!fid_out) return fail(ctx, SP_EINVAL, "fid_out must not be NULL");
+  ctx->code = 0;
+  sp_plan* pl = sp_plan_mas(ctx, I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, rotor, nsteps, mode);
+  if (!pl) return ctx->code ? ctx->code : SP_EDEVICE;
+  int rc = sp_execute(pl, stats);
+  if (rc == SP_OK) rc = sp_plan_result(pl, fid_out, per_orient_out);
+  sp_plan_destroy(pl);
+  return rc;
+}
+
 int sp_powder_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas, const double* betas,
                   const double* gammas, const double* weights, int64_t n_or, const double* rho0, const double* coil,
                   double dt, int64_t nsteps, const sp_tols* tols, int mode, double* fid_out, double* per_orient_out,
                   sp_stats* stats) {
   if (!ctx) return SP_EINVAL;
   if (!fid_out) return fail(ctx, SP_EINVAL, "fid_out must not be NULL");
+  ctx->code = 0;
   sp_plan* pl = sp_plan_powder(ctx, I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, dt, nsteps, tols, mode);
   if (!pl) return ctx->code ? ctx->code : SP_EDEVICE;
   int rc = sp_execute(pl, stats);

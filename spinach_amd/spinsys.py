@@ -270,8 +270,15 @@ def basis(ss, bas):
     order <= level (all spins form one chemical part). Optional
     `projections` keeps only the listed total coherence orders plus the unit
     state. Rows are sorted lexicographically (basis.m:459-465)."""
-    if bas.get('formalism', 'sphten-liouv') != 'sphten-liouv':
-        raise NotImplementedError('only the sphten-liouv formalism is supported')
+    form = bas.get('formalism', 'sphten-liouv')
+    if form == 'zeeman-hilb':
+        ss = dict(ss)
+        ss['formalism'] = 'zeeman-hilb'
+        ss['basis'] = None
+        ss['basis_keys'] = None
+        return ss
+    if form != 'sphten-liouv':
+        raise NotImplementedError(f'formalism {form} is not supported')
     n = ss['nspins']
     appr = bas.get('approximation', 'none')
     if appr == 'none':
@@ -358,9 +365,13 @@ def operator(ss, operators, spins, operator_type='comm'):
     """Superoperator of a (product) operator (kernel/operator.m:71-210).
     `operators` is a name or list of names, `spins` an isotope name, 'all',
     an int, or a list of ints (0-based) matching `operators`. Types: 'comm',
-    'left', 'right', 'acomm'. Returns scipy CSR complex128."""
-    dim = len(ss['basis_keys'])
+    'left', 'right', 'acomm'. Returns scipy CSR complex128. In the
+    zeeman-hilb formalism the Hilbert-space operator itself is returned and
+    the type is ignored (kernel/operator.m:125-160)."""
     terms = _human2opspec(ss, operators, spins)
+    if ss.get('formalism') == 'zeeman-hilb':
+        return _hilbert_operator(ss, terms)
+    dim = len(ss['basis_keys'])
     r_all, c_all, v_all = [], [], []
     for opspec, coeff in terms:
         if operator_type in ('left', 'right'):
@@ -381,6 +392,23 @@ def operator(ss, operators, spins, operator_type='comm'):
     m.sum_duplicates()
     m.eliminate_zeros()
     return m
+
+
+def _hilbert_operator(ss, terms):
+    """Sum of coeff * kron(T_{s1}, T_{s2}, ...) over the terms, with the
+    un-normalised single-spin ISTs of kernel/operators/irr_sph_ten.m and the
+    spin ordering of kron (spin 0 leftmost)."""
+    t = irr_sph_ten_half()
+    n = ss['nspins']
+    out = None
+    for spec, coeff in terms:
+        m = sps.csr_matrix(np.array([[coeff]], complex))
+        for k in range(n):
+            m = sps.kron(m, sps.csr_matrix(t[spec.get(k, 0)]), format='csr')
+        out = m if out is None else out + m
+    out = out.tocsr()
+    out.eliminate_zeros()
+    return out
 
 
 def _human2opspec(ss, operators, spins):
@@ -420,6 +448,8 @@ def _human2opspec(ss, operators, spins):
 def state(ss, states, spins):
     """Liouville-space state vector: left-side product superoperator applied
     to the unit state e_1 (kernel/state.m:94-153). Returns complex ndarray."""
+    if ss.get('formalism') == 'zeeman-hilb':
+        return operator(ss, states, spins).toarray()   # state.m:208-211
     dim = len(ss['basis_keys'])
     unit = np.zeros(dim, complex)
     unit[0] = 1.0
@@ -509,7 +539,7 @@ def hamiltonian(ss):
             descr.append((ops, (l, s), iso_terms.get(ops, 0.0),
                           _vec8(ist.get(ops, {})), irr))
     # operators (hamiltonian.m:673-690), assembly of I (:718-738) and Q (:741-792)
-    dim = len(ss['basis_keys'])
+    dim = 2 ** n if ss.get('formalism') == 'zeeman-hilb' else len(ss['basis_keys'])
     opers = []
     for ops, sp_, iso, ist, irr in descr:
         if abs(iso) < tol and (np.sum(np.abs(irr)) < tol or np.sum(np.abs(ist)) < tol):

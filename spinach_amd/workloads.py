@@ -6,6 +6,9 @@ static_8spin  5x1H, 2x13C, 1x15N, IK-0 level 3 (dim 1789), 4000 Haar
               orientations, 4096 steps at 5 us
 batch_uneven  512 systems of 2-10 spins (1H/13C/15N), IK-0 level <=3,
               100-1000 Haar orientations each, 1024 steps at 5 us
+mas_6spin     4x13C + 2x1H in Hilbert space (dim 64), magic-angle spinning at
+              12.5 kHz, 128 rotor phases per period, 64 periods, 3000 Haar
+              orientations
 
 Every system draws CSA eigenvalues uniformly in [-100,100] ppm with Euler
 angles uniform on SO(3); dipolar couplings come from coordinates.
@@ -104,3 +107,22 @@ def batch_uneven(seed=2, n_systems=512, nsteps=1024, min_or=100, max_or=1000, mi
         w['nspins'] = n
         out.append(w)
     return dict(systems=out, dt=5e-6, nsteps=nsteps, name='batch_uneven')
+
+
+def mas_6spin(seed=3, n_orient=3000, periods=64, ticks=128, rate_hz=12500.0):
+    """Hilbert-space MAS workload: rho0 = Lx(13C), coil = L+(13C) (carbon
+    detection), rotor axis [1 1 1] (magic angle), 'magnet' frame."""
+    rng = np.random.default_rng(seed)
+    iso = ['13C'] * 4 + ['1H'] * 2
+    sys = {'magnet': 14.1, 'isotopes': iso}
+    inter = {'zeeman': {'eigs': [list(rng.uniform(-100, 100, 3)) for _ in iso],
+                        'euler': [_euler(rng) for _ in iso]},
+             'coordinates': _coords(rng, len(iso), box=3.5, min_sep=1.0)}
+    ss = S.create(sys, inter)
+    ss = S.basis(ss, {'formalism': 'zeeman-hilb'})
+    I, Q = S.hamiltonian(S.assume(ss, 'nmr'))
+    rho0 = S.state(ss, 'Lx', '13C')
+    coil = S.state(ss, 'L+', '13C')
+    return dict(ss=ss, I=I, Q=S.q_list(Q), Qnested=Q, rho0=rho0, coil=coil, grid=grids.haar(n_orient, rng),
+                axis=(1.0, 1.0, 1.0), rate_hz=rate_hz, ticks=ticks, nsteps=ticks * periods,
+                dt=1.0 / (rate_hz * ticks), name='mas_6spin')

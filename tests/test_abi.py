@@ -46,7 +46,7 @@ def test_struct_layouts_match_header():
     # sp_csc: int64 n, 2 pointers, 1 pointer, int32 -> 40 bytes with padding
     assert ctypes.sizeof(_native.SpCsc) == 40
     assert ctypes.sizeof(_native.SpTols) == 5 * 8
-    assert ctypes.sizeof(_native.SpStats) == 3 * 8 + 3 * 8 + 4 * 4
+    assert ctypes.sizeof(_native.SpStats) == 3 * 8 + 3 * 8 + 6 * 4
     assert ctypes.sizeof(_native.SpSystem) == 10 * 8
 
 
