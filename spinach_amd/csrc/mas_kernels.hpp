@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
         // C_j = U' * (coil * U)
         R* tre = T.re();
         R* tim = T.im();
-        wg_gemm<R>(np, np, np, Cl.view(), U.view(),
+        wg_gemm_nn<R>(np, np, np, Cl.re(), Cl.im(), np, U.re(), U.im(), np,
                    [=](int a, int b, R vr, R vi) {
                      tre[(size_t)a * np + b] = vr;
                      tim[(size_t)a * np + b] = vi;
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
                    sh.g);
         R* cre = Cb + (size_t)j * pl;
         R* cim = Cb + cpl + (size_t)j * pl;
-        wg_gemm<R>(np, np, np, U.view(true, true), T.view(),
+        wg_gemm_t<R, true, true, false, false>(np, np, np, U.re(), U.im(), np, T.re(), T.im(), np,
                    [=](int a, int b, R vr, R vi) {
                      cre[(size_t)a * np + b] = vr;
                      cim[(size_t)a * np + b] = vi;
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
       {
         R* ure = Ut.re();
         R* uim = Ut.im();
-        wg_gemm<R>(np, np, np, P.view(), U.view(),
+        wg_gemm_nn<R>(np, np, np, P.re(), P.im(), np, U.re(), U.im(), np,
                    [=](int a, int b, R vr, R vi) {
                      ure[(size_t)a * np + b] = vr;
                      uim[(size_t)a * np + b] = vi;
@@ -252,10 +252,11 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
     }
     __syncthreads();
     for (int k = 1; k < p.n_rho; k++) {
-      CMat<R> prev{Rb + (size_t)(k - 1) * pl, Rb + rpl + (size_t)(k - 1) * pl, np, false};
+      const R* prev_re = Rb + (size_t)(k - 1) * pl;
+      const R* prev_im = Rb + rpl + (size_t)(k - 1) * pl;
       R* tre = T.re();
       R* tim = T.im();
-      wg_gemm<R>(np, np, np, U.view(), prev,
+      wg_gemm_nn<R>(np, np, np, U.re(), U.im(), np, prev_re, prev_im, np,
                  [=](int a, int b, R vr, R vi) {
                    tre[(size_t)a * np + b] = vr;
                    tim[(size_t)a * np + b] = vi;
@@ -263,7 +264,7 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
                  sh.g);
       R* rre = Rb + (size_t)k * pl;
       R* rim = Rb + rpl + (size_t)k * pl;
-      wg_gemm<R>(np, np, np, T.view(), U.view(true, true),
+      wg_gemm_t<R, false, false, true, true>(np, np, np, T.re(), T.im(), np, U.re(), U.im(), np,
                  [=](int a, int b, R vr, R vi) {
                    rre[(size_t)a * np + b] = vr;
                    rim[(size_t)a * np + b] = vi;
@@ -275,9 +276,8 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
       const int64_t npts = p.nsteps + 1;
       double* out = p.per_or + 2 * (size_t)o * npts;
       const int ticks = p.ticks;
-      CMat<R> Cv{Cb, Cb + cpl, (int)pl, false, true};
-      CMat<R> Rv{Rb, Rb + rpl, (int)pl, true, false};
-      wg_gemm<R>(ticks, p.n_rho, (int)pl, Cv, Rv,
+      wg_gemm_t<R, false, true, true, false>(ticks, p.n_rho, (int)pl, Cb, Cb + cpl, (int)pl, Rb, Rb + rpl,
+                                             (int)pl,
                  [=](int jj, int kk, R vr, R vi) {
                    int64_t t = (int64_t)kk * ticks + jj;
                    if (t < npts) {

@@ -87,7 +87,11 @@ struct KParams {
   int32_t np_max;
   double* per_or;    // per-orientation FIDs: interleaved complex FP64, (nsteps+1) per orientation
   int32_t* sq_out;   // per orientation: scaling exponent used (diagnostics)
+  unsigned long long* phase_cycles;  // optional: per workgroup, kPhases clock64() totals
 };
+
+// phases timed by the optional per-workgroup cycle counters
+constexpr int kPhases = 5;  // assemble+norm, expm, powers+U block, V chain, detection
 
 __global__ void k_reset(unsigned int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0u;
@@ -192,20 +196,29 @@ struct CMat {
   }
 };
 
+// LDS staging of one k-panel. Rows are padded by 16 bytes so that the four
+// k-rows a wave reads in one MFMA step fall into different banks.
 template <typename R>
 struct GemmLds {
-  R ar[16][64];
-  R ai[16][64];
-  R br[16][64];
-  R bi[16][64];
+  static constexpr int LD = 64 + 16 / (int)sizeof(R);
+  R ar[16][LD];
+  R ai[16][LD];
+  R br[16][LD];
+  R bi[16][LD];
+  __device__ R* flat() { return reinterpret_cast<R*>(this); }  // 4*16*LD >= 4096 elements
 };
 
-// C(MxN) = A(MxK) * B(KxN), K a multiple of 16; epi(i, j, re, im) receives
-// every output element. The workgroup walks 64x64 output tiles; each of its
-// four waves owns a 32x32 quarter as 2x2 MFMA tiles; k-panels of 16 are
-// staged through LDS.
-template <typename R, typename Epi>
-__device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, GemmLds<R>& sm) {
+// C(MxN) = op(A)(MxK) * op(B)(KxN), K a multiple of 16; epi(i, j, re, im)
+// receives every output element. TA/TB select transposed storage
+// (element (i,k) of A at A[k*lda+i]), CA/CB complex conjugation, all at
+// compile time. The workgroup walks 64x64 output tiles; each of its four
+// waves owns a 32x32 quarter as 2x2 MFMA tiles; k-panels of 16 are staged
+// through LDS, and the next panel is prefetched into registers while the
+// current one feeds the matrix cores.
+template <typename R, bool TA, bool CA, bool TB, bool CB, typename Epi>
+__device__ void wg_gemm_t(int M, int N, int K, const R* __restrict__ Are, const R* __restrict__ Aim, int lda,
+                          const R* __restrict__ Bre, const R* __restrict__ Bim, int ldb, Epi epi,
+                          GemmLds<R>& sm) {
   typedef typename Mma<R>::v4 v4;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -221,30 +234,50 @@ __device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, Gemm
           cr[a][b] = v4{0, 0, 0, 0};
           ci[a][b] = v4{0, 0, 0, 0};
         }
-      // register prefetch of the next k-panel overlaps global loads with MFMA work
+      // per-thread staging coordinates and panel-0 offsets
+      int aoff[4], boff[4];
+      bool aok[4], bok[4];
+#pragma unroll
+      for (int rep = 0; rep < 4; rep++) {
+        const int e = tid + rep * kThreads;
+        if (!TA) {
+          const int r = e >> 4, c = e & 15;
+          aok[rep] = (i0 + r < M);
+          aoff[rep] = (i0 + r) * lda + c;
+        } else {
+          const int c = e >> 6, r = e & 63;
+          aok[rep] = (i0 + r < M);
+          aoff[rep] = c * lda + i0 + r;
+        }
+        if (!TB) {
+          const int rb = e >> 6, cb = e & 63;
+          bok[rep] = (j0 + cb < N);
+          boff[rep] = rb * ldb + j0 + cb;
+        } else {
+          const int cb = e >> 4, rb = e & 15;
+          bok[rep] = (j0 + cb < N);
+          boff[rep] = (j0 + cb) * ldb + rb;
+        }
+      }
       R pa_r[4], pa_i[4], pb_r[4], pb_i[4];
       auto fetch = [&](int k0) {
+        const int da = TA ? k0 * lda : k0;
+        const int db = TB ? k0 : k0 * ldb;
 #pragma unroll
         for (int rep = 0; rep < 4; rep++) {
-          int e = tid + rep * kThreads;
-          int r, c;
-          if (!A.trans) { r = e >> 4; c = e & 15; } else { c = e >> 6; r = e & 63; }
-          R va = R(0), vb = R(0);
-          if (i0 + r < M) {
-            size_t o = A.off(i0 + r, k0 + c);
-            va = A.re[o];
-            vb = A.conj ? -A.im[o] : A.im[o];
+          R va = R(0), vb = R(0), wa = R(0), wb = R(0);
+          if (aok[rep]) {
+            va = Are[aoff[rep] + da];
+            vb = Aim[aoff[rep] + da];
+            if (CA) vb = -vb;
+          }
+          if (bok[rep]) {
+            wa = Bre[boff[rep] + db];
+            wb = Bim[boff[rep] + db];
+            if (CB) wb = -wb;
           }
           pa_r[rep] = va;
           pa_i[rep] = vb;
-          int rb, cb;
-          if (!B.trans) { rb = e >> 6; cb = e & 63; } else { cb = e >> 4; rb = e & 15; }
-          R wa = R(0), wb = R(0);
-          if (j0 + cb < N) {
-            size_t o = B.off(k0 + rb, j0 + cb);
-            wa = B.re[o];
-            wb = B.conj ? -B.im[o] : B.im[o];
-          }
           pb_r[rep] = wa;
           pb_i[rep] = wb;
         }
@@ -253,13 +286,11 @@ __device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, Gemm
       for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
         for (int rep = 0; rep < 4; rep++) {
-          int e = tid + rep * kThreads;
-          int r, c;
-          if (!A.trans) { r = e >> 4; c = e & 15; } else { c = e >> 6; r = e & 63; }
+          const int e = tid + rep * kThreads;
+          const int r = TA ? (e & 63) : (e >> 4), c = TA ? (e >> 6) : (e & 15);
           sm.ar[c][r] = pa_r[rep];
           sm.ai[c][r] = pa_i[rep];
-          int rb, cb;
-          if (!B.trans) { rb = e >> 6; cb = e & 63; } else { cb = e >> 4; rb = e & 15; }
+          const int rb = TB ? (e & 15) : (e >> 6), cb = TB ? (e >> 4) : (e & 63);
           sm.br[rb][cb] = pb_r[rep];
           sm.bi[rb][cb] = pb_i[rep];
         }
@@ -283,8 +314,13 @@ __device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, Gemm
 #pragma unroll
               for (int b = 0; b < 2; b++) {
                 cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
-                cr[a][b] = Mma<R>::fma(nxi[a], yi[b], cr[a][b]);
                 ci[a][b] = Mma<R>::fma(xr[a], yi[b], ci[a][b]);
+              }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) {
+                cr[a][b] = Mma<R>::fma(nxi[a], yi[b], cr[a][b]);
                 ci[a][b] = Mma<R>::fma(xi[a], yr[b], ci[a][b]);
               }
           }
@@ -308,6 +344,13 @@ __device__ void wg_gemm(int M, int N, int K, CMat<R> A, CMat<R> B, Epi epi, Gemm
   __syncthreads();
 }
 
+// Convenience form for square planar buffers (no transposition or conjugation).
+template <typename R, typename Epi>
+__device__ inline void wg_gemm_nn(int M, int N, int K, const R* Are, const R* Aim, int lda, const R* Bre,
+                                  const R* Bim, int ldb, Epi epi, GemmLds<R>& sm) {
+  wg_gemm_t<R, false, false, false, false>(M, N, K, Are, Aim, lda, Bre, Bim, ldb, epi, sm);
+}
+
 // ---------------------------------------------------------- block reductions
 __device__ inline double wg_max(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -322,6 +365,7 @@ __device__ inline double wg_max(double v, double* red) {
 template <typename R>
 struct SharedAll {
   GemmLds<R> g;
+  R vpart[4 * 64 * 2];  // per-wave partial sums of the matrix-vector chain
   double cre[34], cim[34];
   double red[4];
   double tc[32];  // Taylor coefficients 1/k!
@@ -355,7 +399,7 @@ __device__ Buf<R> wg_expm(Buf<R>* bf, int sexp, int q, int K, const double* tc, 
   for (int k = 2; k <= q; k++) {
     R* dre = bf[k - 1].re();
     R* dim_ = bf[k - 1].im();
-    wg_gemm<R>(np, np, np, bf[k - 2].view(), bf[0].view(),
+    wg_gemm_nn<R>(np, np, np, bf[k - 2].re(), bf[k - 2].im(), np, bf[0].re(), bf[0].im(), np,
                [=](int i, int j, R vr, R vi) {
                  size_t o2 = (size_t)i * np + j;
                  dre[o2] = vr;
@@ -388,7 +432,7 @@ __device__ Buf<R> wg_expm(Buf<R>* bf, int sexp, int q, int K, const double* tc, 
   for (int j = r - 1; j >= 0; j--) {
     R* dre = tmp.re();
     R* dim_ = tmp.im();
-    wg_gemm<R>(np, np, np, Y.view(), acc.view(),
+    wg_gemm_nn<R>(np, np, np, Y.re(), Y.im(), np, acc.re(), acc.im(), np,
                [=](int i, int jj, R vr, R vi) {
                  size_t o2 = (size_t)i * np + jj;
                  double ar = (double)vr, ai = (double)vi;
@@ -419,7 +463,7 @@ __device__ Buf<R> wg_expm(Buf<R>* bf, int sexp, int q, int K, const double* tc, 
     Buf<R> dst = spare[sp_i];
     R* dre = dst.re();
     R* dim_ = dst.im();
-    wg_gemm<R>(np, np, np, cur.view(), cur.view(),
+    wg_gemm_nn<R>(np, np, np, cur.re(), cur.im(), np, cur.re(), cur.im(), np,
                [=](int i, int j, R vr, R vi) {
                  size_t o2 = (size_t)i * np + j;
                  dre[o2] = vr;
@@ -435,7 +479,7 @@ __device__ Buf<R> wg_expm(Buf<R>* bf, int sexp, int q, int K, const double* tc, 
 
 // --------------------------------------------------------- the fused kernel
 template <typename R>
-__global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
+__global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
   __shared__ SharedAll<R> sh;
   const int tid = threadIdx.x;
   R* ws = p.work + (size_t)blockIdx.x * p.work_stride;
@@ -465,6 +509,14 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
     Buf<R> bf[6];
     for (int b = 0; b < 6; b++) bf[b] = Buf<R>{bufbase[b], pl, np};
 
+    long long t_mark = clock64();
+    auto mark = [&](int ph) {
+      if (p.phase_cycles && tid == 0) {
+        long long t = clock64();
+        p.phase_cycles[(size_t)blockIdx.x * kPhases + ph] += (unsigned long long)(t - t_mark);
+        t_mark = t;
+      }
+    };
     // 1. Wigner coefficients
     if (tid == 0) wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], sh.cre, sh.cim);
     // 2. zero X and the U block
@@ -528,6 +580,7 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
       for (size_t e = tid; e < 2 * pl; e += kThreads) X[e] *= scl;
       __syncthreads();
     }
+    mark(0);
     // 4. P = expm(X): Paterson-Stockmeyer Taylor polynomial and s squarings
     Buf<R> cur = wg_expm<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
     Buf<R> spare[2];
@@ -537,6 +590,7 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
         if (b2 != 2 && bf[b2].p != cur.p) spare[ns++] = bf[b2];
     }
     int sp_i = 0;
+    mark(1);
     // cur = P. 5a. U doubling with P^(2^k) and squarings up to Q = P^B (stored transposed in QT)
     const double* coilv = p.coil + 2 * sd.vec_off;
     const double* rhov = p.rho0 + 2 * sd.vec_off;
@@ -565,10 +619,9 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
       const int rows = 1 << k;
       // U[rows .. 2*rows) = U[0 .. rows) * P^(2^k)
       {
-        CMat<R> Uv{Ure, Uim, np, false};
         R* ure2 = Ure + (size_t)rows * np;
         R* uim2 = Uim + (size_t)rows * np;
-        wg_gemm<R>(rows, np, np, Uv, cur.view(),
+        wg_gemm_nn<R>(rows, np, np, Ure, Uim, np, cur.re(), cur.im(), np,
                    [=](int i, int j, R vr, R vi) {
                      ure2[(size_t)i * np + j] = vr;
                      uim2[(size_t)i * np + j] = vi;
@@ -581,7 +634,7 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
       if (!last && dst.p == QT.p) dst = spare[sp_i ^ 1];
       R* dre = dst.re();
       R* dim_ = dst.im();
-      wg_gemm<R>(np, np, np, cur.view(), cur.view(),
+      wg_gemm_nn<R>(np, np, np, cur.re(), cur.im(), np, cur.re(), cur.im(), np,
                  [=](int i, int j, R vr, R vi) {
                    size_t o2 = last ? (size_t)j * np + i : (size_t)i * np + j;
                    dre[o2] = vr;
@@ -602,37 +655,71 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
       }
       __syncthreads();
     }
-    // 5b. V chain: v_{k+1}[r] = sum_c Q[r][c] v_k[c] = sum_c QT[c][r] v_k[c]
+    mark(2);
+    // 5b. V chain: v_{k+1}[r] = sum_c Q[r][c] v_k[c] = sum_c QT[c][r] v_k[c].
+    //     The vector lives in LDS (the GEMM staging area is idle here); each
+    //     wave sums a quarter of the columns for 64 rows at a time with
+    //     coalesced loads of QT, and the four partial sums meet in LDS.
     {
       const R* qre = QT.re();
       const R* qim = QT.im();
-      for (int k = 1; k < p.nblk; k++) {
-        const R* xr_ = Vre + (size_t)(k - 1) * np;
-        const R* xi_ = Vim + (size_t)(k - 1) * np;
-        for (int rr = tid; rr < np; rr += kThreads) {
-          R ar = R(0), ai = R(0);
-          for (int c = 0; c < np; c++) {
-            R qr = qre[(size_t)c * np + rr], qi = qim[(size_t)c * np + rr];
-            R xr = xr_[c], xi = xi_[c];
-            ar = fma(qr, xr, ar);
-            ar = fma(-qi, xi, ar);
-            ai = fma(qr, xi, ai);
-            ai = fma(qi, xr, ai);
-          }
-          Vre[(size_t)k * np + rr] = ar;
-          Vim[(size_t)k * np + rr] = ai;
-        }
-        __syncthreads();
+      R* va = sh.g.flat();  // 4*np <= 4096 elements: two ping-pong vectors
+      R* vb = va + 2 * np;
+      const int lane = tid & 63, wave = tid >> 6;
+      const int cq = np / 4;
+      const int cb0 = wave * cq;
+      for (int c = tid; c < np; c += kThreads) {
+        va[c] = Vre[c];
+        va[np + c] = Vim[c];
       }
+      __syncthreads();
+      for (int k = 1; k < p.nblk; k++) {
+        for (int r0 = 0; r0 < np; r0 += 64) {
+          const int rr = r0 + lane;
+          R ar = R(0), ai = R(0);
+          if (rr < np) {
+            const R* q1 = qre + (size_t)cb0 * np + rr;
+            const R* q2 = qim + (size_t)cb0 * np + rr;
+#pragma unroll 8
+            for (int c = 0; c < cq; c++) {
+              const R a = q1[(size_t)c * np], b = q2[(size_t)c * np];
+              const R xr = va[cb0 + c], xi = va[np + cb0 + c];
+              ar = fma(a, xr, ar);
+              ar = fma(-b, xi, ar);
+              ai = fma(a, xi, ai);
+              ai = fma(b, xr, ai);
+            }
+          }
+          sh.vpart[2 * (wave * 64 + lane)] = ar;
+          sh.vpart[2 * (wave * 64 + lane) + 1] = ai;
+          __syncthreads();
+          if (tid < 64 && r0 + tid < np) {
+            R sr = R(0), si = R(0);
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+              sr += sh.vpart[2 * (w * 64 + tid)];
+              si += sh.vpart[2 * (w * 64 + tid) + 1];
+            }
+            vb[r0 + tid] = sr;
+            vb[np + r0 + tid] = si;
+            Vre[(size_t)k * np + r0 + tid] = sr;
+            Vim[(size_t)k * np + r0 + tid] = si;
+          }
+          __syncthreads();
+        }
+        R* t = va;
+        va = vb;
+        vb = t;
+      }
+      __syncthreads();
     }
+    mark(3);
     // 5c. F = U * V^T ; f(kB + j) = F[j][k]
     {
       const int64_t npts = p.nsteps + 1;
       double* out = p.per_or + 2 * (size_t)o * npts;
       const int B = p.block;
-      CMat<R> Uv{Ure, Uim, np, false};
-      CMat<R> Vt{Vre, Vim, np, true};
-      wg_gemm<R>(B, p.nblk, np, Uv, Vt,
+      wg_gemm_t<R, false, false, true, false>(B, p.nblk, np, Ure, Uim, np, Vre, Vim, np,
                  [=](int j, int k, R vr, R vi) {
                    int64_t t = (int64_t)k * B + j;
                    if (t < npts) {
@@ -642,6 +729,7 @@ __global__ void __launch_bounds__(kThreads) k_powder(KParams<R> p) {
                  },
                  sh.g);
     }
+    mark(4);
   }
 }
 

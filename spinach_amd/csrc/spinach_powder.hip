@@ -181,6 +181,9 @@ SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, const do
     if (seen[i]) map[i] = n++;
   sp.n = n;
   sp.np = std::max<int32_t>(16, (n + 15) / 16 * 16);
+  if (sp.np > 1024)
+    throw SpError(SP_ERANGE, "the coil-connected subspace has dimension " + std::to_string(n) +
+                                 "; the dense propagation path supports up to 1024");
   // reduced union pattern
   std::vector<std::pair<int64_t, int64_t>> ent;
   for (auto& m : mats)
@@ -279,6 +282,8 @@ struct sp_plan {
   bool mas = false;
   int32_t ticks = 0, n_rho = 0, mas_np = 0, mas_nnz = 0, mas_nq = 0;
   DevBuf d_rot2lab, d_lab2rot;
+  DevBuf d_phase;  // optional per-workgroup phase cycle counters (SPINACH_PHASE_TIMING=1)
+  std::vector<double> phase_ms;
 };
 
 namespace {
@@ -291,10 +296,29 @@ int fail(sp_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
-int choose_log2_block(int64_t npts) {
-  double r = std::sqrt((double)npts);
-  int l = (int)std::lround(std::log2(std::max(1.0, r)));
-  return std::min(8, std::max(0, l));
+// Time-block length B = 2^l of the two-level power scheme, from a cost model
+// in units of one np x np complex product: l squarings, the U-block doubling
+// steps (a step with 2^k rows costs ceil(2^k/64)/ceil(np/64) of a product),
+// and ceil(npts/B)-1 matrix-vector steps of the V chain. A V step streams the
+// np x np matrix from memory; measured on MI355X at np = 192 it costs 0.14 of
+// a product, and the ratio scales as 1/np (memory ~ np^2, product ~ np^3).
+int choose_log2_block(int64_t npts, int np) {
+  const double v_step = 0.14 * 192.0 / (double)np;
+  const double row_tiles = std::ceil(np / 64.0);
+  int best = 0;
+  double best_cost = 1e300;
+  for (int l = 0; l <= 10; l++) {
+    const int64_t B = (int64_t)1 << l;
+    if (B > npts && l > 0) break;
+    double cost = l;
+    for (int k = 0; k < l; k++) cost += std::ceil(std::ldexp(1.0, k) / 64.0) / row_tiles;
+    cost += (double)((npts + B - 1) / B - 1) * v_step;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = l;
+    }
+  }
+  return best;
 }
 
 sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode) {
@@ -310,7 +334,10 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   pl->nsteps = nsteps;
   pl->n_systems = (int64_t)sps.size();
   const int64_t npts = nsteps + 1;
-  pl->log2b = choose_log2_block(npts);
+  int np_all = 16;
+  for (auto& sp : sps) np_all = std::max(np_all, sp.np);
+  pl->log2b = choose_log2_block(npts, np_all);
+  if (const char* e = getenv("SPINACH_LOG2_BLOCK")) pl->log2b = std::max(0, std::min(10, atoi(e)));
   pl->block = 1 << pl->log2b;
   pl->nblk = (int32_t)((npts + pl->block - 1) / pl->block);
   // Taylor degree K for |X|_1 <= theta = 1: the remainder is below
@@ -414,6 +441,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)pl->n_systems * npts * 2 * sizeof(double));
   pl->d_sq.alloc((size_t)pl->total_or * sizeof(int32_t));
+  if (const char* e = getenv("SPINACH_PHASE_TIMING"))
+    if (atoi(e) > 0) pl->d_phase.alloc((size_t)pl->grid * spk::kPhases * sizeof(unsigned long long));
   HIPCHK(hipStreamSynchronize(st));
   return pl.release();
 }
@@ -448,6 +477,7 @@ void launch(sp_plan* pl) {
   kp.np_max = pl->np_max;
   kp.per_or = pl->d_per.as<double>();
   kp.sq_out = pl->d_sq.as<int32_t>();
+  kp.phase_cycles = pl->d_phase.p ? pl->d_phase.as<unsigned long long>() : nullptr;
   hipLaunchKernelGGL(spk::k_powder<R>, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
   HIPCHK(hipGetLastError());
 }
@@ -646,6 +676,8 @@ int run(sp_plan* pl, sp_stats* stats) {
   HIPCHK(hipEventRecord(ctx->ev0, st));
   hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, pl->d_queue.as<unsigned int>());
   HIPCHK(hipGetLastError());
+  if (pl->d_phase.p)
+    HIPCHK(hipMemsetAsync(pl->d_phase.p, 0, pl->d_phase.bytes, st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   if (pl->mas) {
     if (pl->mode == SP_MODE_FP64)
@@ -697,6 +729,17 @@ int run(sp_plan* pl, sp_stats* stats) {
   }
   s.max_squarings = mx;
   s.ps_block = pl->ps_q;
+  if (pl->d_phase.p) {
+    std::vector<unsigned long long> ph((size_t)pl->grid * spk::kPhases);
+    HIPCHK(hipMemcpy(ph.data(), pl->d_phase.p, ph.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    fprintf(stderr, "[spinach_powder] phase cycles summed over %d workgroups:", pl->grid);
+    for (int k = 0; k < spk::kPhases; k++) {
+      unsigned long long t = 0;
+      for (int g = 0; g < pl->grid; g++) t += ph[(size_t)g * spk::kPhases + k];
+      fprintf(stderr, " %llu", t);
+    }
+    fprintf(stderr, "\n");
+  }
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   pl->last = s;
   if (stats) *stats = s;
