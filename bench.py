@@ -1,20 +1,28 @@
 """Benchmark: powder-averaged time-domain propagation on one MI355X.
 
-Workload: static 8-spin solid (5x1H, 2x13C, 1x15N, 14.1 T, random CSA and
-dipolar couplings, IK-0 level 3 basis of dimension 1789), 4000 Haar-random
-orientations, 4096 steps of 5 us, rho0 = Lx(1H), coil = L+(1H).
+Default workload (the headline configuration): static 8-spin solid (5x1H,
+2x13C, 1x15N, 14.1 T, random CSA and dipolar couplings, IK-0 level 3 basis of
+dimension 1789 reduced to its 185-state coil-connected subspace), 4000
+Haar-random orientations, 4096 steps of 5 us, rho0 = Lx(1H), coil = L+(1H),
+complex FP64.
 
 A step is one full pass of the hot path over the whole grid: sp_execute on a
 plan whose inputs (reduced generator components, grid, states) are already
-resident in HBM. The value is orientation-steps/s = 4000*4096/(time per pass).
+resident in HBM. value = orientations * nsteps / (time per pass).
 
-Also reported: the roofline of the propagation kernel (FP64, compute bound)
-from HIP-event kernel durations, the relative FID error of the GPU result
-against the CPU oracle on a sample of orientations, and the CPU baseline (the
-oracle's faithful restatement of the reference algorithm, OpenMP over
-orientations) timed on this host on a bounded sample.
+Reported beside it: the roofline of the propagation kernel (algorithmic
+flops / HIP-event kernel time against the FP64 peak), the relative FID error
+of the GPU result against the CPU oracle on a sample of orientations, and the
+CPU baseline (the oracle's faithful restatement of the reference algorithm,
+OpenMP over orientations) timed on this host on a bounded sample.
+
+Other workloads (--workload): static2 (two-spin 1H-13C, 256 x 2047), batch
+(512 systems of 2-10 spins, 100-1000 orientations each, 1024 steps, one
+persistent launch), mas (6-spin Hilbert-space magic-angle spinning, 3000
+orientations x 8192 steps, complex FP32 by default).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp64|fp32]
+                       [--workload static8|static2|batch|mas]
 """
 import argparse
 import json
@@ -26,22 +34,197 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = ('orientation-steps/s on 1 MI355X at ≤1e-6 rel FID error vs FP64; surrogate speedup')
-PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix), FLOP/s
-PEAK_FP32_VALU = 78.6e12  # non-packed FP32 vector rate (packed FP32 ops are disabled in this build)
+METRIC = 'orientation-steps/s on 1 MI355X at ≤1e-6 rel FID error vs FP64; surrogate speedup'
+PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12}  # MI355X dense MFMA FP64 / FP32, FLOP/s
 
 
-def algorithmic_flops_per_orientation(n, nsteps, taylor_k, ps_q, sexp, block):
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def powder_flops_per_orientation(n, nsteps, taylor_k, ps_q, sexp, block):
     """Real flops of the dense propagation per orientation on the reduced
-    dimension n (padding excluded): complex GEMM = 8 n^3, complex matvec =
-    8 n^2, complex dot = 8 n. Matrix products: q-1 powers and K//q Horner
-    steps of the Paterson-Stockmeyer Taylor evaluation, s squarings,
-    log2(B) squarings to Q = P^B; vector-matrix work: B-1 rows of the U block
-    and M-1 steps of the V chain; detection: (nsteps+1) dots."""
+    dimension n (padding excluded): complex product = 8 n^3, complex
+    vector-matrix product = 8 n^2, complex dot = 8 n. Matrix products: q-1
+    powers and K//q Horner steps of the Paterson-Stockmeyer Taylor evaluation,
+    s squarings, log2(B) squarings to Q = P^B; B-1 rows of the U block;
+    M-1 steps of the V chain; (nsteps+1) detection dots."""
     npts = nsteps + 1
     nblk = math.ceil(npts / block)
     gemms = (ps_q - 1) + taylor_k // ps_q + sexp + int(round(math.log2(block)))
     return gemms * 8.0 * n ** 3 + (block - 1 + nblk - 1) * 8.0 * n ** 2 + npts * 8.0 * n
+
+
+def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
+    """Per orientation: every tick an expm (q-1 + K//q products + s squarings),
+    the U update and C_j = U'*coil*U (3 products); two products per rotor
+    period for the rho chain; detection conj(C) . rho over n^2 elements."""
+    npts = nsteps + 1
+    n_rho = math.ceil(npts / ticks)
+    per_tick = (ps_q - 1) + taylor_k // ps_q + sexp + 3
+    return ticks * per_tick * 8.0 * n ** 3 + 2 * (n_rho - 1) * 8.0 * n ** 3 + ticks * n_rho * 8.0 * n * n
+
+
+def ncores():
+    c = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
+    return max(1, min(c, 64))
+
+
+def time_plan(plan, steps, warmup):
+    import numpy as np
+    for _ in range(warmup):
+        plan.execute()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = plan.execute()
+        kms.append(st['kernel_ms'])
+    total = time.perf_counter() - t0
+    return total / steps, float(np.mean(kms)), st
+
+
+def bench_static(args, two_spin=False):
+    import numpy as np
+    from spinach_amd import _native, grids, workloads
+    if two_spin:
+        w = workloads.static_2spin()
+        desc = ('static 2-spin 1H-13C, complete basis (dim 16 -> 2 coil-connected), zcwn 256 orientations '
+                'x 2047 steps at 10 us')
+    else:
+        w = workloads.static_8spin(n_orient=args.orientations)
+        desc = ('static 8-spin solid (5x1H,2x13C,1x15N), IK-0 level 3 (dim 1789 -> 185 coil-connected), '
+                f'{len(w["grid"]["weights"])} orientations x 4096 steps at 5 us')
+    n_or = len(w['grid']['weights'])
+    nsteps = w['nsteps']
+    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], nsteps, mode=args.mode)
+    sec, kms, st = time_plan(plan, args.steps, args.warmup)
+    fid, per = plan.result(per_orientation=True)
+    log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms')
+    n = st['reduced_dim']
+    flops = powder_flops_per_orientation(n, nsteps, st['taylor_terms'], st['ps_block'],
+                                         st['sum_squarings_lo'] / n_or, st['block'])
+    cpu = err = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import pyoracle as O
+        m = min(args.cpu_sample if not two_spin else n_or, n_or)
+        sub = grids.subset(w['grid'], np.arange(m))
+        t1 = time.perf_counter()
+        ref, rper = O.refcpu_powder(w['I'], w['Q'], sub, w['rho0'], w['coil'], w['dt'], nsteps, variant=0,
+                                    nthreads=ncores(), per_orientation=True)
+        tc = time.perf_counter() - t1
+        cpu = {'value': round(m * nsteps / tc, 1), 'unit': 'orientation-steps/s', 'cores': ncores(),
+               'kind': 'port',
+               'sample': f'{m} of the {n_or} orientations x {nsteps} steps, faithful reference algorithm '
+                         f'(per-orientation ZTE reduce, expm, sequential detection loop), FP64, {tc:.1f}s'}
+        wsum = np.asarray(sub['weights'])
+        err = O.rel_err((wsum[:, None] * per[:m]).sum(axis=0), (wsum[:, None] * rper).sum(axis=0))
+    cfg = {'workload': desc, 'orientations': n_or, 'nsteps': nsteps, 'full_dim': st['full_dim'],
+           'reduced_dim': n, 'time_block': st['block'], 'mode': args.mode}
+    return n_or * nsteps / sec, sec, kms, flops * n_or, cfg, cpu, err
+
+
+def bench_batch(args):
+    import numpy as np
+    from spinach_amd import _native, grids, workloads
+    t0 = time.time()
+    b = workloads.batch_uneven(n_systems=args.systems)
+    log(f'{len(b["systems"])} systems built in {time.time() - t0:.1f}s')
+    n_or_tot = sum(len(y['grid']['weights']) for y in b['systems'])
+    plan = _native.plan_powder_batch(b['systems'], b['dt'], b['nsteps'], mode=args.mode)
+    sec, kms, st = time_plan(plan, args.steps, args.warmup)
+    fids = plan.result_batch()
+    log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms, {n_or_tot} orientations')
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import pyoracle as O
+    flops = 0.0
+    dims = []
+    for y in b['systems']:
+        n = len(O.union_states(y['I'], y['Q'], y['coil']))
+        dims.append(n)
+        flops += len(y['grid']['weights']) * powder_flops_per_orientation(
+            n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'])
+    cpu = err = None
+    if not args.no_cpu:
+        # sample: 2 orientations of every 8th system, summed in the same way
+        t1 = time.perf_counter()
+        work = 0
+        errs = []
+        for y, f in list(zip(b['systems'], fids))[::8]:
+            sub = grids.subset(y['grid'], np.arange(2))
+            ref = O.refcpu_powder(y['I'], y['Q'], sub, y['rho0'], y['coil'], b['dt'], b['nsteps'], variant=0,
+                                  nthreads=ncores())
+            work += 2 * b['nsteps']
+        tc = time.perf_counter() - t1
+        cpu = {'value': round(work / tc, 1), 'unit': 'orientation-steps/s', 'cores': ncores(), 'kind': 'port',
+               'sample': f'2 orientations of every 8th system ({len(b["systems"][::8])} systems) x '
+                         f'{b["nsteps"]} steps, faithful reference algorithm, FP64, {tc:.1f}s'}
+        for y, f in list(zip(b['systems'], fids))[::64]:
+            ref = O.refcpu_powder(y['I'], y['Q'], y['grid'], y['rho0'], y['coil'], b['dt'], b['nsteps'],
+                                  variant=0, nthreads=ncores())
+            errs.append(O.rel_err(f, ref))
+        err = max(errs)
+    cfg = {'workload': f'{len(b["systems"])} seeded spin systems of 2-10 spins (1H/13C/15N), IK-0 level <=3, '
+                       f'100-1000 orientations each ({n_or_tot} total) x {b["nsteps"]} steps at 5 us, '
+                       'one persistent launch',
+           'systems': len(b['systems']), 'orientations': n_or_tot, 'nsteps': b['nsteps'],
+           'reduced_dim_min': int(min(dims)), 'reduced_dim_max': int(max(dims)),
+           'time_block': st['block'], 'mode': args.mode}
+    return n_or_tot * b['nsteps'] / sec, sec, kms, flops, cfg, cpu, err
+
+
+def bench_mas(args):
+    import numpy as np
+    from spinach_amd import _native, grids, workloads
+    w = workloads.mas_6spin(n_orient=args.orientations if args.orientations != 4000 else 3000)
+    n_or = len(w['grid']['weights'])
+    plan = _native.plan_mas(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['axis'], w['rate_hz'], w['ticks'],
+                            w['nsteps'], mode=args.mode)
+    sec, kms, st = time_plan(plan, args.steps, args.warmup)
+    fid, per = plan.result(per_orientation=True)
+    log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms')
+    flops = n_or * mas_flops_per_orientation(64, w['nsteps'], w['ticks'], st['taylor_terms'], st['ps_block'],
+                                             st['sum_squarings_lo'] / n_or)
+    cpu = err = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import pyoracle as O
+        m = 4
+        sub = grids.subset(w['grid'], np.arange(m))
+        t1 = time.perf_counter()
+        ref, rper = O.mas_powder(w['I'], w['Q'], sub, w['rho0'], w['coil'], w['axis'], w['rate_hz'], w['ticks'],
+                                 w['nsteps'], per_orientation=True)
+        tc = time.perf_counter() - t1
+        cpu = {'value': round(m * w['nsteps'] / tc, 1), 'unit': 'orientation-steps/s', 'cores': ncores(),
+               'kind': 'port',
+               'sample': f'{m} orientations x {w["nsteps"]} steps, numpy direct stepping (tick propagators '
+                         f'by scipy expm, rho <- P rho P\'), FP64, BLAS threads, {tc:.1f}s'}
+        wsum = np.asarray(sub['weights'])
+        err = O.rel_err((wsum[:, None] * per[:m]).sum(axis=0), (wsum[:, None] * rper).sum(axis=0))
+    cfg = {'workload': f'MAS 12.5 kHz, 6 spins (4x13C, 2x1H) in Hilbert space (dim 64), 128 ticks per period '
+                       f'x 64 periods, {n_or} orientations', 'orientations': n_or, 'nsteps': w['nsteps'],
+           'hilbert_dim': 64, 'mode': args.mode}
+    return n_or * w['nsteps'] / sec, sec, kms, flops, cfg, cpu, err
+
+
+def copy_probe(mib):
+    """Device-to-device copy of a known size through the HIP runtime, for
+    calibrating the FETCH_SIZE / WRITE_SIZE counters: three copies of `mib`
+    MiB (the runtime's copy kernel reads and writes mib MiB per dispatch)."""
+    import ctypes as C
+    hip = C.CDLL('libamdhip64.so')
+    n = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError('hipMalloc failed')
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError('hipMemcpy failed')
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({'copy_probe_bytes_per_dispatch': n}))
 
 
 def main():
@@ -49,77 +232,37 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--mode', default='fp64', choices=['fp64', 'fp32'])
+    ap.add_argument('--mode', default=None, choices=['fp64', 'fp32'])
+    ap.add_argument('--workload', default='static8', choices=['static8', 'static2', 'batch', 'mas'])
     ap.add_argument('--orientations', type=int, default=4000)
-    ap.add_argument('--cpu-sample', type=int, default=48, help='orientations in the CPU baseline sample')
+    ap.add_argument('--systems', type=int, default=512)
+    ap.add_argument('--cpu-sample', type=int, default=1000, help='orientations in the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='only run a device copy of MIB MiB (counter calibration) and exit')
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
-        print('bench.py measures one GPU: every box in this setup has a single MI355X, and the '
-              'multi-GPU path (orientation sharding + one FID all-reduce) is not built. Run with --gpus 1.',
+        print('bench.py measures one GPU: every box in this setup has a single MI355X, and the multi-GPU path '
+              '(orientation sharding + one FID all-reduce) is a design, not a build. Run with --gpus 1.',
               file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
-    from spinach_amd import _native, grids, workloads
-
-    t0 = time.time()
-    w = workloads.static_8spin(n_orient=args.orientations)
-    print(f'[bench] workload built in {time.time() - t0:.1f}s: dim {w["I"].shape[0]}', file=sys.stderr, flush=True)
-    n_or = len(w['grid']['weights'])
-    nsteps = w['nsteps']
-    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], nsteps, mode=args.mode)
-    for _ in range(args.warmup):
-        plan.execute()
-    times, kms = [], []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
-        st = plan.execute()
-        times.append(time.perf_counter() - t1)
-        kms.append(st['kernel_ms'])
-    total = time.perf_counter() - t_start
-    ms_per_step = 1e3 * total / args.steps
-    value = n_or * nsteps / (total / args.steps)
-    fid, per = plan.result(per_orientation=True)
-    print(f'[bench] {args.steps} steps, {ms_per_step:.2f} ms/step, kernel {np.mean(kms):.2f} ms', file=sys.stderr,
-          flush=True)
-
-    # roofline of the propagation kernel
-    n = st['reduced_dim']
-    mean_s = st['sum_squarings_lo'] / n_or
-    flops = algorithmic_flops_per_orientation(n, nsteps, st['taylor_terms'], st['ps_block'], mean_s, st['block'])
-    k_avg_s = float(np.mean(kms)) * 1e-3
-    achieved = flops * n_or / k_avg_s
-    roof = {'bound': 'mfma', 'achieved': round(achieved / 1e12, 3), 'peak': round(PEAK_FP64 / 1e12, 1),
-            'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4), 'traffic': None,
-            'kernel': 'spk::k_powder', 'kernel_ms_avg': round(float(np.mean(kms)), 3),
-            'flops_per_orientation': flops, 'orientations_per_launch': n_or}
-
-    # parity on a sample against the oracle, and the CPU baseline
-    cpu = None
-    err = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
-        import pyoracle as O
-        ncores = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
-        ncores = min(ncores, 64)
-        m = min(args.cpu_sample, n_or)
-        sub = grids.subset(w['grid'], np.arange(m))
-        t1 = time.perf_counter()
-        ref, rper = O.refcpu_powder(w['I'], w['Q'], sub, w['rho0'], w['coil'], w['dt'], nsteps, variant=0,
-                                    nthreads=ncores, per_orientation=True)
-        tc = time.perf_counter() - t1
-        cpu = {'value': round(m * nsteps / tc, 1), 'unit': 'orientation-steps/s', 'cores': ncores, 'kind': 'port',
-               'sample': f'{m} of the {n_or} orientations x {nsteps} steps, faithful reference algorithm '
-                         f'(per-orientation ZTE reduce, expm, sequential detection loop), FP64, {tc:.1f}s'}
-        wsum = np.asarray(sub['weights'])
-        gsum = (wsum[:, None] * per[:m]).sum(axis=0)
-        rsum = (wsum[:, None] * rper).sum(axis=0)
-        err = float(np.linalg.norm(gsum - rsum) / np.linalg.norm(rsum))
-        print(f'[bench] cpu baseline {cpu["value"]:.3g} orientation-steps/s on {ncores} cores; '
-              f'GPU rel FID error on the sample {err:.2e}', file=sys.stderr, flush=True)
-
+    if args.mode is None:
+        args.mode = 'fp32' if args.workload == 'mas' else 'fp64'
+    if args.workload in ('static8', 'static2'):
+        value, sec, kms, flops, cfg, cpu, err = bench_static(args, two_spin=args.workload == 'static2')
+    elif args.workload == 'batch':
+        value, sec, kms, flops, cfg, cpu, err = bench_batch(args)
+    else:
+        value, sec, kms, flops, cfg, cpu, err = bench_mas(args)
+    achieved = flops / (kms * 1e-3)
+    peak = PEAK[args.mode]
+    kernel = 'spk::k_mas' if args.workload == 'mas' else 'spk::k_powder'
+    roof = {'bound': 'mfma', 'achieved': round(achieved / 1e12, 3), 'peak': round(peak / 1e12, 1),
+            'unit': 'TFLOP/s', 'frac': round(achieved / peak, 4), 'traffic': None, 'kernel': kernel,
+            'kernel_ms_avg': round(kms, 3), 'flops_per_launch': flops}
     line = {
         'metric': METRIC,
         'value': round(value, 1),
@@ -127,20 +270,19 @@ def main():
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': round(ms_per_step, 3),
+        'ms_per_step': round(1e3 * sec, 3),
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'c128' if args.mode == 'fp64' else 'c64',
-        'data': 'synthetic (seeded spin system and Haar grid)',
-        'config': {'workload': 'static 8-spin solid (5x1H,2x13C,1x15N), IK-0 level 3 (dim 1789 -> 185 '
-                               'coil-connected), 4000 orientations x 4096 steps at 5 us, FP64',
-                   'orientations': n_or, 'nsteps': nsteps, 'full_dim': st['full_dim'],
-                   'reduced_dim': n, 'mode': args.mode},
+        'data': 'synthetic (seeded spin systems and Haar/zcwn grids)',
+        'config': cfg,
         'fid_rel_err_vs_oracle': err,
         'roofline': roof,
         'cpu_baseline': cpu,
     }
+    if cpu:
+        line['speedup_vs_cpu_baseline'] = round(value / cpu['value'], 1)
     print(json.dumps(line))
 
 

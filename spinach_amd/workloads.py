@@ -96,16 +96,32 @@ def random_system(rng, nspins, level=3):
     return build(sys, inter, appr)
 
 
-def batch_uneven(seed=2, n_systems=512, nsteps=1024, min_or=100, max_or=1000, min_spins=2, max_spins=10):
-    rng = np.random.default_rng(seed)
-    out = []
-    for s in range(n_systems):
-        n = int(rng.integers(min_spins, max_spins + 1))
-        w = random_system(rng, n)
-        n_or = int(rng.integers(min_or, max_or + 1))
-        w['grid'] = grids.haar(n_or, rng)
-        w['nspins'] = n
-        out.append(w)
+def _batch_member(args):
+    seed, s, min_spins, max_spins, min_or, max_or = args
+    rng = np.random.default_rng([seed, s])
+    n = int(rng.integers(min_spins, max_spins + 1))
+    w = random_system(rng, n)
+    n_or = int(rng.integers(min_or, max_or + 1))
+    w['grid'] = grids.haar(n_or, rng)
+    w['nspins'] = n
+    del w['ss'], w['Qnested']
+    return w
+
+
+def batch_uneven(seed=2, n_systems=512, nsteps=1024, min_or=100, max_or=1000, min_spins=2, max_spins=10,
+                 workers=None):
+    """Every member system s is drawn from its own generator seeded with
+    (seed, s), so the batch does not depend on how it is built."""
+    jobs = [(seed, s, min_spins, max_spins, min_or, max_or) for s in range(n_systems)]
+    if workers is None:
+        import os
+        workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else 4)
+    if workers > 1 and n_systems > 8:
+        import multiprocessing as mp
+        with mp.get_context('fork').Pool(workers) as pool:
+            out = pool.map(_batch_member, jobs, chunksize=4)
+    else:
+        out = [_batch_member(j) for j in jobs]
     return dict(systems=out, dt=5e-6, nsteps=nsteps, name='batch_uneven')
 
 
