@@ -36,6 +36,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'orientation-steps/s on 1 MI355X at ≤1e-6 rel FID error vs FP64; surrogate speedup'
 PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12}  # MI355X dense MFMA FP64 / FP32, FLOP/s
+# HBM bytes per launch of the propagation kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs of this script (counters scaled by a device copy of known
+# size: FETCH x2.000, WRITE x1.000); see profiles/k_powder_static8spin_fp64.md
+TRAFFIC = {('static8', 'fp64'): 165512902 * 1024 * 2.0 + 53150294 * 1024 * 1.0}
 
 
 def log(msg):
@@ -261,7 +265,8 @@ def main():
     peak = PEAK[args.mode]
     kernel = 'spk::k_mas' if args.workload == 'mas' else 'spk::k_powder'
     roof = {'bound': 'mfma', 'achieved': round(achieved / 1e12, 3), 'peak': round(peak / 1e12, 1),
-            'unit': 'TFLOP/s', 'frac': round(achieved / peak, 4), 'traffic': None, 'kernel': kernel,
+            'unit': 'TFLOP/s', 'frac': round(achieved / peak, 4),
+            'traffic': TRAFFIC.get((args.workload, args.mode)), 'kernel': kernel,
             'kernel_ms_avg': round(kms, 3), 'flops_per_launch': flops}
     line = {
         'metric': METRIC,
