@@ -371,3 +371,116 @@ def mas_fid(H, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='mag
     if rc:
         _err(ctx, rc, 'sp_mas_fid')
     return fid, per, st.as_dict()
+
+
+# ------------------------------------------------------------- surrogate
+def _load_surrogate():
+    lib = load()
+    if getattr(lib, '_sg_ready', False):
+        return lib
+    lib.sp_surrogate_create.restype = C.c_void_p
+    lib.sp_surrogate_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64]
+    lib.sp_surrogate_destroy.restype = None
+    lib.sp_surrogate_destroy.argtypes = [C.c_void_p]
+    lib.sp_surrogate_last_error.restype = C.c_char_p
+    lib.sp_surrogate_last_error.argtypes = [C.c_void_p]
+    lib.sp_surrogate_num_params.restype = C.c_int64
+    lib.sp_surrogate_num_params.argtypes = [C.c_void_p]
+    _fp = C.POINTER(C.c_float)
+    _ip = C.POINTER(C.c_int32)
+    lib.sp_surrogate_get_params.restype = C.c_int
+    lib.sp_surrogate_get_params.argtypes = [C.c_void_p, _fp]
+    lib.sp_surrogate_set_params.restype = C.c_int
+    lib.sp_surrogate_set_params.argtypes = [C.c_void_p, _fp]
+    lib.sp_surrogate_grad.restype = C.c_int
+    lib.sp_surrogate_grad.argtypes = [C.c_void_p, _dp, _dp, _dp, _ip, C.c_int64, C.c_double, _fp, _fp, _fp]
+    lib.sp_surrogate_train.restype = C.c_int
+    lib.sp_surrogate_train.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int64, _dp, C.c_int64, C.c_double, C.c_int64,
+                                       C.c_int64, C.c_double, C.c_uint64, _fp]
+    lib.sp_surrogate_powder.restype = C.c_int
+    lib.sp_surrogate_powder.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, _dp, _dp,
+                                        _dp]
+    lib._sg_ready = True
+    return lib
+
+
+class SurrogateNet:
+    """Device-resident Fourier-feature MLP (sp_surrogate_*)."""
+
+    def __init__(self, t_max, width=256, depth=4, ka=3, kt=22, seed=0, device=0):
+        lib = _load_surrogate()
+        context(device)
+        self.h = lib.sp_surrogate_create(device, width, depth, ka, kt, float(t_max), int(seed))
+        if not self.h:
+            raise SpinachPowderError('sp_surrogate_create failed')
+        self.width, self.depth, self.ka, self.kt, self.t_max = width, depth, ka, kt, float(t_max)
+        self.nparam = lib.sp_surrogate_num_params(self.h)
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = _load_surrogate().sp_surrogate_last_error(self.h).decode()
+            raise SpinachPowderError(f'{what} failed (code {rc}): {msg}')
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        self._chk(_load_surrogate().sp_surrogate_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))),
+                  'get_params')
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._chk(_load_surrogate().sp_surrogate_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))),
+                  'set_params')
+
+    def grad(self, a, b, g, tidx, dt, target):
+        """Loss and gradient for explicit points (one angle triple per point)."""
+        n = len(a)
+        k = _Keep()
+        t = np.ascontiguousarray(tidx, np.int32)
+        tg = np.ascontiguousarray(np.asarray(target, np.float32).reshape(n, 2))
+        loss = C.c_float()
+        gr = np.zeros(self.nparam, np.float32)
+        rc = _load_surrogate().sp_surrogate_grad(
+            self.h, k.dptr(a), k.dptr(b), k.dptr(g), t.ctypes.data_as(C.POINTER(C.c_int32)), n, float(dt),
+            tg.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss), gr.ctypes.data_as(C.POINTER(C.c_float)))
+        self._chk(rc, 'grad')
+        return loss.value, gr
+
+    def train(self, grid, fids, dt, steps, batch=4096, lr=1e-3, seed=1):
+        """Adam on random (orientation, time) points of the training FIDs."""
+        k = _Keep()
+        f = np.ascontiguousarray(fids, np.complex128)
+        n_or, npts = f.shape
+        trace = np.zeros(max(steps, 1), np.float32)
+        rc = _load_surrogate().sp_surrogate_train(
+            self.h, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']), n_or,
+            f.view(np.float64).ctypes.data_as(_dp), npts, float(dt), int(steps), int(batch), float(lr), int(seed),
+            trace.ctypes.data_as(C.POINTER(C.c_float)))
+        self._chk(rc, 'train')
+        return trace[:steps]
+
+    def powder(self, grid, npts, dt, per_orientation=False):
+        """Weighted powder sum of the network's FIDs over a grid (FP64 sum);
+        returns (fid, per-orientation or None, device ms)."""
+        k = _Keep()
+        n_or = len(grid['weights'])
+        fid = np.zeros(npts, np.complex128)
+        per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
+        ms = C.c_double()
+        rc = _load_surrogate().sp_surrogate_powder(
+            self.h, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']), k.dptr(grid['weights']),
+            n_or, int(npts), float(dt), fid.view(np.float64).ctypes.data_as(_dp),
+            per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(ms))
+        self._chk(rc, 'powder')
+        return fid, per, ms.value
+
+    def close(self):
+        if self.h:
+            _load_surrogate().sp_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

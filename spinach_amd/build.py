@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, 'libspinach_powder.so')
-SOURCES = [os.path.join(HERE, 'csrc', 'spinach_powder.hip')]
+SOURCES = [os.path.join(HERE, 'csrc', 'spinach_powder.hip'), os.path.join(HERE, 'csrc', 'surrogate.hip')]
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 
 FLAGS = [
