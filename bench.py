@@ -39,7 +39,7 @@ PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12}  # MI355X dense MFMA FP64 / FP32, FLO
 # HBM bytes per launch of the propagation kernel, from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE runs of this script (counters scaled by a device copy of known
 # size: FETCH x2.000, WRITE x1.000); see profiles/k_powder_static8spin_fp64.md
-TRAFFIC = {('static8', 'fp64'): 165512902 * 1024 * 2.0 + 53150294 * 1024 * 1.0}
+TRAFFIC = {('static8', 'fp64', 'spk::k_powder'): 165512902 * 1024 * 2.0 + 53150294 * 1024 * 1.0}
 
 
 def log(msg):
@@ -69,6 +69,18 @@ def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
     return ticks * per_tick * 8.0 * n ** 3 + 2 * (n_rho - 1) * 8.0 * n ** 3 + ticks * n_rho * 8.0 * n * n
 
 
+def spectral_stage_flops(n, npts):
+    """Algorithmic flops per orientation of the spectral path on the reduced
+    dimension n: Householder tridiagonalisation 4/3 n^3 (symmetric
+    matrix-vector product 2 m^2 and rank-2 update 2 m^2 per column of a
+    trailing block of size m) plus 4 n^2 per transformed vector (two complex
+    vectors = four real ones); the tridiagonal QR iteration is O(n^2) scalar
+    work, counted as 0 (it is bound by its dependent rotation chain, not by
+    throughput); synthesis 8 n flops per time point (complex multiply-add)."""
+    return {'spk::k_tridiag': 4.0 / 3.0 * n ** 3 + 16.0 * n ** 2, 'spk::k_tql': 0.0,
+            'spk::k_synth': 8.0 * n * npts}
+
+
 def ncores():
     c = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
     return max(1, min(c, 64))
@@ -78,13 +90,27 @@ def time_plan(plan, steps, warmup):
     import numpy as np
     for _ in range(warmup):
         plan.execute()
-    kms = []
+    kms, stages = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         st = plan.execute()
         kms.append(st['kernel_ms'])
+        stages.append(st['stage_ms'])
     total = time.perf_counter() - t0
+    st['stage_ms'] = [float(x) for x in np.mean(np.asarray(stages), axis=0)]
     return total / steps, float(np.mean(kms)), st
+
+
+def stage_table(st, prop_flops, spec_flops, prop_kernel='spk::k_powder'):
+    """{kernel: (ms, flops)} for the kernels that ran in a pass."""
+    ms = st['stage_ms']
+    out = {}
+    if prop_flops:
+        out[prop_kernel] = (ms[0], prop_flops)
+    if spec_flops:
+        for k, name in enumerate(('spk::k_tridiag', 'spk::k_tql', 'spk::k_synth')):
+            out[name] = (ms[1 + k], spec_flops[name])
+    return out
 
 
 def bench_static(args, two_spin=False):
@@ -100,13 +126,17 @@ def bench_static(args, two_spin=False):
                 f'{len(w["grid"]["weights"])} orientations x 4096 steps at 5 us')
     n_or = len(w['grid']['weights'])
     nsteps = w['nsteps']
-    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], nsteps, mode=args.mode)
+    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], nsteps,
+                               mode=args.mode + args.path_flag)
     sec, kms, st = time_plan(plan, args.steps, args.warmup)
     fid, per = plan.result(per_orientation=True)
-    log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms')
+    log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernels {kms:.2f} ms, stages {st["stage_ms"]}')
     n = st['reduced_dim']
-    flops = powder_flops_per_orientation(n, nsteps, st['taylor_terms'], st['ps_block'],
-                                         st['sum_squarings_lo'] / n_or, st['block'])
+    if st['spectral_systems']:
+        stages = stage_table(st, 0.0, {k: v * n_or for k, v in spectral_stage_flops(n, nsteps + 1).items()})
+    else:
+        stages = stage_table(st, n_or * powder_flops_per_orientation(
+            n, nsteps, st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or, st['block']), None)
     cpu = err = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
@@ -124,8 +154,11 @@ def bench_static(args, two_spin=False):
         wsum = np.asarray(sub['weights'])
         err = O.rel_err((wsum[:, None] * per[:m]).sum(axis=0), (wsum[:, None] * rper).sum(axis=0))
     cfg = {'workload': desc, 'orientations': n_or, 'nsteps': nsteps, 'full_dim': st['full_dim'],
-           'reduced_dim': n, 'time_block': st['block'], 'mode': args.mode}
-    return n_or * nsteps / sec, sec, kms, flops * n_or, cfg, cpu, err
+           'reduced_dim': n, 'mode': args.mode,
+           'path': 'spectral' if st['spectral_systems'] else 'propagator'}
+    if not st['spectral_systems']:
+        cfg['time_block'] = st['block']
+    return n_or * nsteps / sec, sec, kms, stages, cfg, cpu, err
 
 
 def bench_batch(args):
@@ -135,19 +168,27 @@ def bench_batch(args):
     b = workloads.batch_uneven(n_systems=args.systems)
     log(f'{len(b["systems"])} systems built in {time.time() - t0:.1f}s')
     n_or_tot = sum(len(y['grid']['weights']) for y in b['systems'])
-    plan = _native.plan_powder_batch(b['systems'], b['dt'], b['nsteps'], mode=args.mode)
+    plan = _native.plan_powder_batch(b['systems'], b['dt'], b['nsteps'], mode=args.mode + args.path_flag)
     sec, kms, st = time_plan(plan, args.steps, args.warmup)
     fids = plan.result_batch()
     log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms, {n_or_tot} orientations')
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import pyoracle as O
-    flops = 0.0
+    prop_flops = 0.0
+    spec_flops = {k: 0.0 for k in ('spk::k_tridiag', 'spk::k_tql', 'spk::k_synth')}
     dims = []
+    spectral_ok = st['spectral_systems'] > 0
     for y in b['systems']:
         n = len(O.union_states(y['I'], y['Q'], y['coil']))
         dims.append(n)
-        flops += len(y['grid']['weights']) * powder_flops_per_orientation(
-            n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'])
+        n_or = len(y['grid']['weights'])
+        if spectral_ok and n <= 192:
+            for k, v in spectral_stage_flops(n, b['nsteps'] + 1).items():
+                spec_flops[k] += n_or * v
+        else:
+            prop_flops += n_or * powder_flops_per_orientation(
+                n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'])
+    stages = stage_table(st, prop_flops, spec_flops if spectral_ok else None)
     cpu = err = None
     if not args.no_cpu:
         # sample: 2 orientations of every 8th system, summed in the same way
@@ -173,8 +214,10 @@ def bench_batch(args):
                        'one persistent launch',
            'systems': len(b['systems']), 'orientations': n_or_tot, 'nsteps': b['nsteps'],
            'reduced_dim_min': int(min(dims)), 'reduced_dim_max': int(max(dims)),
-           'time_block': st['block'], 'mode': args.mode}
-    return n_or_tot * b['nsteps'] / sec, sec, kms, flops, cfg, cpu, err
+           'mode': args.mode, 'spectral_systems': st['spectral_systems']}
+    if prop_flops:
+        cfg['time_block'] = st['block']
+    return n_or_tot * b['nsteps'] / sec, sec, kms, stages, cfg, cpu, err
 
 
 def bench_mas(args):
@@ -208,7 +251,7 @@ def bench_mas(args):
     cfg = {'workload': f'MAS 12.5 kHz, 6 spins (4x13C, 2x1H) in Hilbert space (dim 64), 128 ticks per period '
                        f'x 64 periods, {n_or} orientations', 'orientations': n_or, 'nsteps': w['nsteps'],
            'hilbert_dim': 64, 'mode': args.mode}
-    return n_or * w['nsteps'] / sec, sec, kms, flops, cfg, cpu, err
+    return n_or * w['nsteps'] / sec, sec, kms, stage_table(st, flops, None, 'spk::k_mas'), cfg, cpu, err
 
 
 def bench_surrogate(args):
@@ -239,7 +282,8 @@ def bench_surrogate(args):
            'surrogate_ms': info['surrogate_fine_ms'], 'plain_ms': info['plain_fine_ms'],
            'surrogate_speedup': info['speedup_surrogate_vs_plain'], 'train_seconds': info['train_seconds'],
            'mode': 'bf16 network / fp64 plain'}
-    return n_or * nsteps / sec, sec, kms, flops, cfg, None, (info['heldout_rel_err'] if info['decision'] ==
+    stages = {'sgk::k_gemm_bf16' if info['decision'] == 'surrogate' else 'plain FP64 path': (kms, flops)}
+    return n_or * nsteps / sec, sec, kms, stages, cfg, None, (info['heldout_rel_err'] if info['decision'] ==
                                                               'surrogate' else 0.0)
 
 
@@ -276,6 +320,9 @@ def main():
     ap.add_argument('--systems', type=int, default=512)
     ap.add_argument('--cpu-sample', type=int, default=1000, help='orientations in the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--path', default='auto', choices=['auto', 'propagator', 'complex'],
+                    help='auto: spectral path for real generators (FP64, dim <= 192); propagator: the power '
+                         'scheme with real products; complex: the general complex power scheme')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a device copy of MIB MiB (counter calibration) and exit')
     args = ap.parse_args()
@@ -289,25 +336,28 @@ def main():
         sys.exit(2)
     if args.mode is None:
         args.mode = 'fp32' if args.workload == 'mas' else 'fp64'
+    args.path_flag = '' if args.path == 'auto' else '+' + args.path
     if args.workload == 'surrogate':
         args.steps, args.warmup = 1, 0
     if args.workload in ('static8', 'static2'):
-        value, sec, kms, flops, cfg, cpu, err = bench_static(args, two_spin=args.workload == 'static2')
+        value, sec, kms, stages, cfg, cpu, err = bench_static(args, two_spin=args.workload == 'static2')
     elif args.workload == 'batch':
-        value, sec, kms, flops, cfg, cpu, err = bench_batch(args)
+        value, sec, kms, stages, cfg, cpu, err = bench_batch(args)
     elif args.workload == 'surrogate':
-        value, sec, kms, flops, cfg, cpu, err = bench_surrogate(args)
+        value, sec, kms, stages, cfg, cpu, err = bench_surrogate(args)
     else:
-        value, sec, kms, flops, cfg, cpu, err = bench_mas(args)
-    achieved = flops / (kms * 1e-3)
-    peak = PEAK[args.mode]
-    kernel = {'mas': 'spk::k_mas', 'surrogate': 'sgk::k_gemm_bf16'}.get(args.workload, 'spk::k_powder')
-    if args.workload == 'surrogate':
-        peak = 2.5e15
+        value, sec, kms, stages, cfg, cpu, err = bench_mas(args)
+    peak = 2.5e15 if args.workload == 'surrogate' else PEAK[args.mode]
+    # roofline of the dominant kernel (largest share of the pass), HIP-event timed
+    kernel = max(stages, key=lambda k: stages[k][0])
+    k_ms, k_flops = stages[kernel]
+    achieved = k_flops / (k_ms * 1e-3) if k_ms > 0 else 0.0
     roof = {'bound': 'mfma', 'achieved': round(achieved / 1e12, 3), 'peak': round(peak / 1e12, 1),
             'unit': 'TFLOP/s', 'frac': round(achieved / peak, 4),
-            'traffic': TRAFFIC.get((args.workload, args.mode)), 'kernel': kernel,
-            'kernel_ms_avg': round(kms, 3), 'flops_per_launch': flops}
+            'traffic': TRAFFIC.get((args.workload, args.mode, kernel)), 'kernel': kernel,
+            'kernel_ms_avg': round(k_ms, 3), 'flops_per_launch': k_flops}
+    stage_json = {k: {'ms': round(v[0], 3), 'tflops': round(v[1] / (v[0] * 1e-3) / 1e12, 3) if v[0] > 0 else None}
+                  for k, v in stages.items()}
     line = {
         'metric': METRIC,
         'value': round(value, 1),
@@ -324,6 +374,7 @@ def main():
         'config': cfg,
         'fid_rel_err_vs_oracle': err,
         'roofline': roof,
+        'kernels': stage_json,
         'cpu_baseline': cpu,
     }
     if cpu:

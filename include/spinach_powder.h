@@ -40,6 +40,20 @@ extern "C" {
 
 #define SP_MODE_FP64 0 /* complex FP64 throughout */
 #define SP_MODE_FP32 1 /* complex FP32 bulk; FP64 norms, step counts and orientation sum */
+/* Flag OR-ed into the mode: keep the general complex-generator arithmetic
+ * even where the generator is real. Without it, a system whose generator is
+ * real at every orientation (I real and Q{r}{k,m} = (-1)^(k-m) conj(Q{r}{-k,-m}),
+ * which secular 'nmr' Hamiltonians in the sphten-liouv basis satisfy) is
+ * propagated with real matrix products, P = cos(L dt) - i sin(L dt); the
+ * result agrees with the complex path to rounding. */
+#define SP_MODE_COMPLEX 16
+/* Flag OR-ed into the mode: use the propagator power scheme for every system.
+ * Without it, FP64 systems with a real generator and a coil-connected
+ * dimension of at most 192 take the spectral path: one Householder
+ * tridiagonalisation and one tridiagonal QR iteration per orientation give
+ * f(t) = sum_k g_k exp(-i lambda_k t) directly (agreement with the
+ * propagator path: ~1e-12 relative over thousands of steps). */
+#define SP_MODE_PROPAGATOR 32
 
 #define SP_NQ 34 /* Q slots: rank 1 (k,m) row-major 3x3, then rank 2 (k,m) row-major 5x5 */
 
@@ -80,6 +94,11 @@ typedef struct sp_stats {
   int32_t max_squarings;     /* largest scaling exponent over orientations */
   int32_t ps_block;          /* Paterson-Stockmeyer block length of the Taylor evaluation */
   int32_t sum_squarings_lo;  /* sum of scaling exponents over orientations (low 32 bits) */
+  int32_t real_systems;      /* systems whose generator is real at every orientation */
+  int32_t spectral_systems;  /* systems evaluated on the spectral path */
+  int32_t eig_max_sweeps;    /* most QR sweeps any orientation of the spectral path needed */
+  int32_t reserved;
+  double stage_ms[4];        /* propagator kernel, tridiagonalisation, tridiagonal QR, synthesis */
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */
@@ -94,8 +113,18 @@ sp_ctx* sp_create(int device);
 void sp_destroy(sp_ctx* ctx);
 
 /* Message of the last failure on this context (never NULL). With ctx==NULL
- * returns the message of the last failed sp_create in this thread. */
+ * returns the message of the last failed context-free call (sp_create,
+ * sp_generator_is_real) in this thread. */
 const char* sp_last_error(const sp_ctx* ctx);
+
+/* Host-only query (no device needed): sets *is_real to 1 when
+ * H(Omega) = I + sum D^r(k,m)(Omega) Q{r}{k,m} is real at every orientation,
+ * i.e. I is real and Q{r}{k,m} = (-1)^(k-m) conj(Q{r}{-k,-m}) for every slot,
+ * to 8 ulp of the largest entry; those systems take the real-product path
+ * (see SP_MODE_COMPLEX). Replaces: nothing in the reference, which always
+ * propagates in complex arithmetic (kernel/propagator.m:40-42); it reports
+ * the path choice the planner makes. */
+int sp_generator_is_real(const sp_csc* I, const sp_csc* Q, int32_t* is_real);
 
 /* ---- one-shot call ----------------------------------------------------- */
 

@@ -17,11 +17,13 @@ LIB_PATH = os.path.join(HERE, 'libspinach_powder.so')
 
 SP_MODE_FP64 = 0
 SP_MODE_FP32 = 1
+SP_MODE_COMPLEX = 16  # flag: keep complex arithmetic for a real generator
+SP_MODE_PROPAGATOR = 32  # flag: propagator power scheme for every system (no spectral path)
 SP_NQ = 34
 
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
-           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid')
+           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real')
 SP_MAS_FRAME_MAGNET = 0
 SP_MAS_FRAME_ROTOR = 1
 
@@ -40,10 +42,14 @@ class SpStats(C.Structure):
     _fields_ = [('elapsed_s', C.c_double), ('orient_steps_per_s', C.c_double), ('kernel_ms', C.c_double),
                 ('full_dim', C.c_int64), ('reduced_dim', C.c_int64), ('reduced_nnz', C.c_int64),
                 ('padded_dim', C.c_int32), ('block', C.c_int32), ('taylor_terms', C.c_int32),
-                ('max_squarings', C.c_int32), ('ps_block', C.c_int32), ('sum_squarings_lo', C.c_int32)]
+                ('max_squarings', C.c_int32), ('ps_block', C.c_int32), ('sum_squarings_lo', C.c_int32),
+                ('real_systems', C.c_int32), ('spectral_systems', C.c_int32), ('eig_max_sweeps', C.c_int32),
+                ('reserved', C.c_int32), ('stage_ms', C.c_double * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {k: getattr(self, k) for k, _ in self._fields_}
+        out['stage_ms'] = list(self.stage_ms)
+        return out
 
 
 class SpRotor(C.Structure):
@@ -101,6 +107,8 @@ def load():
     lib.sp_mas_fid.restype = C.c_int
     lib.sp_mas_fid.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
                                _dp, _dp, C.POINTER(SpRotor), C.c_int64, C.c_int, _dp, _dp, C.POINTER(SpStats)]
+    lib.sp_generator_is_real.restype = C.c_int
+    lib.sp_generator_is_real.argtypes = [C.POINTER(SpCsc), C.POINTER(SpCsc), C.POINTER(C.c_int32)]
     _lib = lib
     return lib
 
@@ -145,35 +153,46 @@ class _Keep:
         self.refs.append(a)
         return a
 
-    def dptr(self, a):
-        a = self.arr(a, np.float64)
+    def dptr(self, a, n=None):
+        a = self.arr(np.asarray(a, dtype=np.float64).ravel(), np.float64)
+        if n is not None and a.size != n:
+            raise ValueError(f'expected {n} values, got {a.size}')
         return a.ctypes.data_as(_dp)
 
-    def cvec(self, v):
+    def cvec(self, v, n):
         v = self.arr(np.asarray(v, dtype=np.complex128).ravel(), np.complex128)
+        if v.size != n:
+            raise ValueError(f'state vector has {v.size} entries, the operators have dimension {n}')
         return v.view(np.float64).ctypes.data_as(_dp)
 
-    def csc(self, m, dim):
+    def grid(self, g):
+        """(alphas, betas, gammas, weights) pointers and the orientation count."""
+        n = len(g['weights'])
+        return tuple(self.dptr(g[key], n) for key in ('alphas', 'betas', 'gammas', 'weights')) + (n,)
+
+    def csc(self, m):
         s = SpCsc()
         if m is None:
             s.n = 0
             return s
         m = sps.csc_matrix(m, dtype=np.complex128)
+        if m.shape[0] != m.shape[1]:
+            raise ValueError(f'sparse operand must be square, got shape {m.shape}')
         m.sum_duplicates()
         jc = self.arr(m.indptr, np.uint64)
         ir = self.arr(m.indices, np.uint64)
         pr = self.arr(m.data, np.complex128)
-        s.n = dim
+        s.n = m.shape[1]  # the library checks it against I's dimension
         s.jc = jc.ctypes.data_as(C.POINTER(C.c_uint64))
         s.ir = ir.ctypes.data_as(C.POINTER(C.c_uint64))
         s.pr = pr.view(np.float64).ctypes.data_as(_dp)
         s.is_complex = 1
         return s
 
-    def qarray(self, qlist, dim):
+    def qarray(self, qlist):
         arr = (SpCsc * SP_NQ)()
         for t in range(SP_NQ):
-            arr[t] = self.csc(qlist[t], dim)
+            arr[t] = self.csc(qlist[t])
         self.refs.append(arr)
         return arr
 
@@ -185,6 +204,21 @@ def _tols(t):
 
 
 def mode_id(mode):
+    """'fp64' / 'fp32' (or SP_MODE_*), with optional '+complex' (SP_MODE_COMPLEX:
+    complex arithmetic even for a real generator) and '+propagator'
+    (SP_MODE_PROPAGATOR: no spectral path) flags."""
+    flags = {'complex': SP_MODE_COMPLEX, 'propagator': SP_MODE_PROPAGATOR}
+    if isinstance(mode, str) and '+' in mode:
+        base, *extra = mode.split('+')
+        out = mode_id(base)
+        for f in extra:
+            if f not in flags:
+                raise ValueError(f'unknown mode flag {f!r}')
+            out |= flags[f]
+        return out
+    mask = SP_MODE_COMPLEX | SP_MODE_PROPAGATOR
+    if isinstance(mode, (int, np.integer)) and not isinstance(mode, bool) and int(mode) & mask:
+        return mode_id(int(mode) & ~mask) | (int(mode) & mask)
     if mode in (SP_MODE_FP64, 'fp64', 'c128'):
         return SP_MODE_FP64
     if mode in (SP_MODE_FP32, 'fp32', 'c64'):
@@ -239,17 +273,29 @@ class Plan:
             pass
 
 
+def generator_is_real(I, qlist):
+    """sp_generator_is_real: host-only test whether H(Omega) = I + sum D Q is
+    real at every orientation (the planner then uses real products)."""
+    lib = load()
+    k = _Keep()
+    dim = I.shape[0]
+    out = C.c_int32(-1)
+    rc = lib.sp_generator_is_real(C.byref(k.csc(I)), k.qarray(qlist), C.byref(out))
+    if rc:
+        raise SpinachPowderError(f'sp_generator_is_real failed (code {rc}): ' + lib.sp_last_error(None).decode())
+    return bool(out.value)
+
+
 def plan_powder(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', device=0):
     lib = load()
     ctx = context(device)
     k = _Keep()
     dim = I.shape[0]
-    Ic = k.csc(I, dim)
-    Qa = k.qarray(qlist, dim)
+    Ic = k.csc(I)
+    Qa = k.qarray(qlist)
     n_or = len(grid['weights'])
     t = _tols(tols)
-    h = lib.sp_plan_powder(ctx, C.byref(Ic), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']),
-                           k.dptr(grid['gammas']), k.dptr(grid['weights']), n_or, k.cvec(rho0), k.cvec(coil),
+    h = lib.sp_plan_powder(ctx, C.byref(Ic), Qa, *k.grid(grid), k.cvec(rho0, dim), k.cvec(coil, dim),
                            float(dt), int(nsteps), C.byref(t), mode_id(mode))
     if not h:
         _err(ctx, -1, 'sp_plan_powder')
@@ -266,21 +312,17 @@ def plan_powder_batch(systems, dt, nsteps, tols=None, mode='fp64', device=0):
     outs = []
     for s, y in enumerate(systems):
         dim = y['I'].shape[0]
-        Ic = k.csc(y['I'], dim)
+        Ic = k.csc(y['I'])
         k.refs.append(Ic)
-        Qa = k.qarray(y['Q'], dim)
+        Qa = k.qarray(y['Q'])
         out = np.zeros(npts, np.complex128)
         outs.append(out)
         g = y['grid']
         arr[s].I = C.pointer(Ic)
         arr[s].Q = C.cast(Qa, C.POINTER(SpCsc))
-        arr[s].alphas = k.dptr(g['alphas'])
-        arr[s].betas = k.dptr(g['betas'])
-        arr[s].gammas = k.dptr(g['gammas'])
-        arr[s].weights = k.dptr(g['weights'])
-        arr[s].n_or = len(g['weights'])
-        arr[s].rho0 = k.cvec(y['rho0'])
-        arr[s].coil = k.cvec(y['coil'])
+        arr[s].alphas, arr[s].betas, arr[s].gammas, arr[s].weights, arr[s].n_or = k.grid(g)
+        arr[s].rho0 = k.cvec(y['rho0'], dim)
+        arr[s].coil = k.cvec(y['coil'], dim)
         arr[s].fid_out = out.view(np.float64).ctypes.data_as(_dp)
     t = _tols(tols)
     h = lib.sp_plan_powder_batch(ctx, arr, len(systems), float(dt), int(nsteps), C.byref(t), mode_id(mode))
@@ -297,16 +339,15 @@ def powder_fid(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', p
     ctx = context(device)
     k = _Keep()
     dim = I.shape[0]
-    Ic = k.csc(I, dim)
-    Qa = k.qarray(qlist, dim)
+    Ic = k.csc(I)
+    Qa = k.qarray(qlist)
     n_or = len(grid['weights'])
     npts = int(nsteps) + 1
     fid = np.zeros(npts, np.complex128)
     per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
     st = SpStats()
     t = _tols(tols)
-    rc = lib.sp_powder_fid(ctx, C.byref(Ic), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']),
-                           k.dptr(grid['gammas']), k.dptr(grid['weights']), n_or, k.cvec(rho0), k.cvec(coil),
+    rc = lib.sp_powder_fid(ctx, C.byref(Ic), Qa, *k.grid(grid), k.cvec(rho0, dim), k.cvec(coil, dim),
                            float(dt), int(nsteps), C.byref(t), mode_id(mode),
                            fid.view(np.float64).ctypes.data_as(_dp),
                            per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(st))
@@ -325,9 +366,11 @@ def _rotor(axis, rate_hz, ticks, frame):
     return r
 
 
-def _dense_colmajor(m, keep):
+def _dense_colmajor(m, keep, dim):
     m = np.asarray(m.toarray() if sps.issparse(m) else m, dtype=np.complex128)
-    return keep.cvec(np.asfortranarray(m).ravel(order='F'))
+    if m.shape != (dim, dim):
+        raise ValueError(f'state matrix has shape {m.shape}, the operators have dimension {dim}')
+    return keep.cvec(np.asfortranarray(m).ravel(order='F'), dim * dim)
 
 
 def plan_mas(H, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='magnet', mode='fp32', device=0):
@@ -337,12 +380,11 @@ def plan_mas(H, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='ma
     ctx = context(device)
     k = _Keep()
     dim = H.shape[0]
-    Hc = k.csc(H, dim)
-    Qa = k.qarray(qlist, dim)
+    Hc = k.csc(H)
+    Qa = k.qarray(qlist)
     n_or = len(grid['weights'])
     rot = _rotor(axis, rate_hz, ticks, frame)
-    h = lib.sp_plan_mas(ctx, C.byref(Hc), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']),
-                        k.dptr(grid['weights']), n_or, _dense_colmajor(rho0, k), _dense_colmajor(coil, k),
+    h = lib.sp_plan_mas(ctx, C.byref(Hc), Qa, *k.grid(grid), _dense_colmajor(rho0, k, dim), _dense_colmajor(coil, k, dim),
                         C.byref(rot), int(nsteps), mode_id(mode))
     if not h:
         _err(ctx, -1, 'sp_plan_mas')
@@ -356,16 +398,15 @@ def mas_fid(H, qlist, grid, rho0, coil, axis, rate_hz, ticks, nsteps, frame='mag
     ctx = context(device)
     k = _Keep()
     dim = H.shape[0]
-    Hc = k.csc(H, dim)
-    Qa = k.qarray(qlist, dim)
+    Hc = k.csc(H)
+    Qa = k.qarray(qlist)
     n_or = len(grid['weights'])
     npts = int(nsteps) + 1
     fid = np.zeros(npts, np.complex128)
     per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
     st = SpStats()
     rot = _rotor(axis, rate_hz, ticks, frame)
-    rc = lib.sp_mas_fid(ctx, C.byref(Hc), Qa, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']),
-                        k.dptr(grid['weights']), n_or, _dense_colmajor(rho0, k), _dense_colmajor(coil, k),
+    rc = lib.sp_mas_fid(ctx, C.byref(Hc), Qa, *k.grid(grid), _dense_colmajor(rho0, k, dim), _dense_colmajor(coil, k, dim),
                         C.byref(rot), int(nsteps), mode_id(mode), fid.view(np.float64).ctypes.data_as(_dp),
                         per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(st))
     if rc:
@@ -428,7 +469,9 @@ class SurrogateNet:
         return out
 
     def set_params(self, p):
-        p = np.ascontiguousarray(p, np.float32)
+        p = np.ascontiguousarray(p, np.float32).ravel()
+        if p.size != self.nparam:
+            raise ValueError(f'expected {self.nparam} parameters, got {p.size}')
         self._chk(_load_surrogate().sp_surrogate_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))),
                   'set_params')
 
@@ -436,12 +479,14 @@ class SurrogateNet:
         """Loss and gradient for explicit points (one angle triple per point)."""
         n = len(a)
         k = _Keep()
-        t = np.ascontiguousarray(tidx, np.int32)
+        t = np.ascontiguousarray(tidx, np.int32).ravel()
+        if t.size != n:
+            raise ValueError(f'expected {n} time indices, got {t.size}')
         tg = np.ascontiguousarray(np.asarray(target, np.float32).reshape(n, 2))
         loss = C.c_float()
         gr = np.zeros(self.nparam, np.float32)
         rc = _load_surrogate().sp_surrogate_grad(
-            self.h, k.dptr(a), k.dptr(b), k.dptr(g), t.ctypes.data_as(C.POINTER(C.c_int32)), n, float(dt),
+            self.h, k.dptr(a, n), k.dptr(b, n), k.dptr(g, n), t.ctypes.data_as(C.POINTER(C.c_int32)), n, float(dt),
             tg.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss), gr.ctypes.data_as(C.POINTER(C.c_float)))
         self._chk(rc, 'grad')
         return loss.value, gr
@@ -453,7 +498,7 @@ class SurrogateNet:
         n_or, npts = f.shape
         trace = np.zeros(max(steps, 1), np.float32)
         rc = _load_surrogate().sp_surrogate_train(
-            self.h, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']), n_or,
+            self.h, k.dptr(grid['alphas'], n_or), k.dptr(grid['betas'], n_or), k.dptr(grid['gammas'], n_or), n_or,
             f.view(np.float64).ctypes.data_as(_dp), npts, float(dt), int(steps), int(batch), float(lr), int(seed),
             trace.ctypes.data_as(C.POINTER(C.c_float)))
         self._chk(rc, 'train')
@@ -468,8 +513,7 @@ class SurrogateNet:
         per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
         ms = C.c_double()
         rc = _load_surrogate().sp_surrogate_powder(
-            self.h, k.dptr(grid['alphas']), k.dptr(grid['betas']), k.dptr(grid['gammas']), k.dptr(grid['weights']),
-            n_or, int(npts), float(dt), fid.view(np.float64).ctypes.data_as(_dp),
+            self.h, *k.grid(grid), int(npts), float(dt), fid.view(np.float64).ctypes.data_as(_dp),
             per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(ms))
         self._chk(rc, 'powder')
         return fid, per, ms.value

@@ -23,6 +23,13 @@
 // A second kernel forms the grid-weighted sum in grid order
 // (kernel/contexts/powder.m:280-286).
 //
+// Real-generator path: when H is real at every orientation (the host checks
+// the Q symmetry that guarantees it; secular 'nmr' Hamiltonians in the
+// spherical-tensor Liouville basis have it), A = dt*H is real symmetric and
+// P = exp(-i*A) = cos(A) - i*sin(A). The Taylor stage then uses real powers
+// of A with complex coefficients, and every squaring of P uses the fact that
+// its planes commute: two real products instead of four.
+//
 // All matrix products run on the matrix cores: v_mfma_f64_16x16x4f64 in FP64
 // mode, v_mfma_f32_16x16x4f32 in FP32 mode, complex products as four real
 // MFMA accumulations. Matrices are planar complex (real plane, then
@@ -50,6 +57,8 @@ struct SysDesc {
   int64_t vec_off;   // offset (complex elements) into rho0/coil arrays (n entries each)
   int64_t or_off;    // first orientation of this system in the global orientation arrays
   int64_t n_or;      // number of orientations
+  int32_t real;      // 1: the generator is real at every orientation (real-product path)
+  int32_t pad2;
 };
 
 struct Item {
@@ -93,8 +102,10 @@ struct KParams {
 // phases timed by the optional per-workgroup cycle counters
 constexpr int kPhases = 5;  // assemble+norm, expm, powers+U block, V chain, detection
 
+// zeroes the kQueues work-queue heads of a plan
+constexpr int kQueues = 4;
 __global__ void k_reset(unsigned int* q) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < kQueues) q[threadIdx.x] = 0u;
 }
 
 // ----------------------------------------------------------------- Wigner
@@ -208,18 +219,30 @@ struct GemmLds {
   __device__ R* flat() { return reinterpret_cast<R*>(this); }  // 4*16*LD >= 4096 elements
 };
 
+// Operand modes of wg_gemm_t
+constexpr int kCplx = 0;  // complex A, complex B: cr = Ar Br - Ai Bi, ci = Ar Bi + Ai Br
+constexpr int kRR = 1;    // real A, real B: cr = Ar Br (imaginary planes are not read)
+constexpr int kRC = 2;    // real A, complex B: cr = Ar Br, ci = Ar Bi
+constexpr int kSq = 3;    // squaring of P = Pr + i Pi whose planes commute (P a function of one
+                          // real symmetric matrix): cr = (Ar - Ai)(Br + Bi) = Re(P^2),
+                          // ci = Ar Bi = Im(P^2)/2; the epilogue doubles ci
+
 // C(MxN) = op(A)(MxK) * op(B)(KxN), K a multiple of 16; epi(i, j, re, im)
 // receives every output element. TA/TB select transposed storage
-// (element (i,k) of A at A[k*lda+i]), CA/CB complex conjugation, all at
-// compile time. The workgroup walks 64x64 output tiles; each of its four
-// waves owns a 32x32 quarter as 2x2 MFMA tiles; k-panels of 16 are staged
-// through LDS, and the next panel is prefetched into registers while the
-// current one feeds the matrix cores.
-template <typename R, bool TA, bool CA, bool TB, bool CB, typename Epi>
+// (element (i,k) of A at A[k*lda+i]), CA/CB complex conjugation, MODE the
+// operand mode above, all at compile time. The workgroup walks 64x64 output
+// tiles; each of its four waves owns a 32x32 quarter as 2x2 MFMA tiles;
+// k-panels of 16 are staged through LDS, and the next panel is prefetched
+// into registers while the current one feeds the matrix cores.
+template <typename R, bool TA, bool CA, bool TB, bool CB, int MODE = kCplx, typename Epi>
 __device__ void wg_gemm_t(int M, int N, int K, const R* __restrict__ Are, const R* __restrict__ Aim, int lda,
                           const R* __restrict__ Bre, const R* __restrict__ Bim, int ldb, Epi epi,
                           GemmLds<R>& sm) {
   typedef typename Mma<R>::v4 v4;
+  constexpr bool a_cplx = (MODE == kCplx || MODE == kSq);
+  constexpr bool b_cplx = (MODE != kRR);
+  constexpr bool c_cplx = (MODE != kRR);
+  static_assert(MODE != kSq || (!CA && !CB), "the squaring mode takes no conjugation");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -268,18 +291,30 @@ __device__ void wg_gemm_t(int M, int N, int K, const R* __restrict__ Are, const 
           R va = R(0), vb = R(0), wa = R(0), wb = R(0);
           if (aok[rep]) {
             va = Are[aoff[rep] + da];
-            vb = Aim[aoff[rep] + da];
-            if (CA) vb = -vb;
+            if constexpr (a_cplx) {
+              vb = Aim[aoff[rep] + da];
+              if (CA) vb = -vb;
+            }
           }
           if (bok[rep]) {
             wa = Bre[boff[rep] + db];
-            wb = Bim[boff[rep] + db];
-            if (CB) wb = -wb;
+            if constexpr (b_cplx) {
+              wb = Bim[boff[rep] + db];
+              if (CB) wb = -wb;
+            }
           }
-          pa_r[rep] = va;
-          pa_i[rep] = vb;
-          pb_r[rep] = wa;
-          pb_i[rep] = wb;
+          if constexpr (MODE == kSq) {
+            // stage (Ar - Ai, Ar) and (Br + Bi, Bi)
+            pa_r[rep] = va - vb;
+            pa_i[rep] = va;
+            pb_r[rep] = wa + wb;
+            pb_i[rep] = wb;
+          } else {
+            pa_r[rep] = va;
+            pa_i[rep] = vb;
+            pb_r[rep] = wa;
+            pb_i[rep] = wb;
+          }
         }
       };
       fetch(0);
@@ -289,10 +324,10 @@ __device__ void wg_gemm_t(int M, int N, int K, const R* __restrict__ Are, const 
           const int e = tid + rep * kThreads;
           const int r = TA ? (e & 63) : (e >> 4), c = TA ? (e >> 6) : (e & 15);
           sm.ar[c][r] = pa_r[rep];
-          sm.ai[c][r] = pa_i[rep];
+          if constexpr (a_cplx) sm.ai[c][r] = pa_i[rep];
           const int rb = TB ? (e & 15) : (e >> 6), cb = TB ? (e >> 4) : (e & 63);
           sm.br[rb][cb] = pb_r[rep];
-          sm.bi[rb][cb] = pb_i[rep];
+          if constexpr (b_cplx) sm.bi[rb][cb] = pb_i[rep];
         }
         __syncthreads();
         if (k0 + 16 < K) fetch(k0 + 16);
@@ -304,25 +339,48 @@ __device__ void wg_gemm_t(int M, int N, int K, const R* __restrict__ Are, const 
 #pragma unroll
             for (int t = 0; t < 2; t++) {
               xr[t] = sm.ar[kr][32 * wr + 16 * t + (lane & 15)];
-              xi[t] = sm.ai[kr][32 * wr + 16 * t + (lane & 15)];
+              xi[t] = a_cplx ? sm.ai[kr][32 * wr + 16 * t + (lane & 15)] : R(0);
               nxi[t] = -xi[t];
               yr[t] = sm.br[kr][32 * wc + 16 * t + (lane & 15)];
-              yi[t] = sm.bi[kr][32 * wc + 16 * t + (lane & 15)];
+              yi[t] = b_cplx ? sm.bi[kr][32 * wc + 16 * t + (lane & 15)] : R(0);
             }
+            if constexpr (MODE == kCplx) {
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+              for (int a = 0; a < 2; a++)
 #pragma unroll
-              for (int b = 0; b < 2; b++) {
-                cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
-                ci[a][b] = Mma<R>::fma(xr[a], yi[b], ci[a][b]);
-              }
+                for (int b = 0; b < 2; b++) {
+                  cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
+                  ci[a][b] = Mma<R>::fma(xr[a], yi[b], ci[a][b]);
+                }
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+              for (int a = 0; a < 2; a++)
 #pragma unroll
-              for (int b = 0; b < 2; b++) {
-                cr[a][b] = Mma<R>::fma(nxi[a], yi[b], cr[a][b]);
-                ci[a][b] = Mma<R>::fma(xi[a], yr[b], ci[a][b]);
-              }
+                for (int b = 0; b < 2; b++) {
+                  cr[a][b] = Mma<R>::fma(nxi[a], yi[b], cr[a][b]);
+                  ci[a][b] = Mma<R>::fma(xi[a], yr[b], ci[a][b]);
+                }
+            } else if constexpr (MODE == kRR) {
+#pragma unroll
+              for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
+            } else if constexpr (MODE == kRC) {
+#pragma unroll
+              for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                  cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
+                  ci[a][b] = Mma<R>::fma(xr[a], yi[b], ci[a][b]);
+                }
+            } else {
+#pragma unroll
+              for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                  cr[a][b] = Mma<R>::fma(xr[a], yr[b], cr[a][b]);
+                  ci[a][b] = Mma<R>::fma(xi[a], yi[b], ci[a][b]);
+                }
+            }
           }
         }
         __syncthreads();
@@ -336,7 +394,7 @@ __device__ void wg_gemm_t(int M, int N, int K, const R* __restrict__ Are, const 
             for (int e = 0; e < 4; e++) {
               int i = i0 + 32 * wr + 16 * a + Mma<R>::row(lane, e);
               int j = j0 + 32 * wc + 16 * b + (lane & 15);
-              if (i < M && j < N) epi(i, j, cr[a][b][e], ci[a][b][e]);
+              if (i < M && j < N) epi(i, j, cr[a][b][e], c_cplx ? ci[a][b][e] : R(0));
             }
       }
     }
@@ -477,6 +535,96 @@ __device__ Buf<R> wg_expm(Buf<R>* bf, int sexp, int q, int K, const double* tc, 
   return cur;
 }
 
+// Squaring dst = src^2 of a propagator whose real and imaginary planes
+// commute (two real products instead of four); `trans` stores the result
+// transposed.
+template <typename R>
+__device__ inline void wg_square_commuting(const Buf<R>& src, const Buf<R>& dst, bool trans, GemmLds<R>& sm) {
+  const int np = src.np;
+  R* dre = dst.re();
+  R* dim_ = dst.im();
+  wg_gemm_t<R, false, false, false, false, kSq>(np, np, np, src.re(), src.im(), np, src.re(), src.im(), np,
+                                                [=](int i, int j, R vr, R vi) {
+                                                  size_t o2 = trans ? (size_t)j * np + i : (size_t)i * np + j;
+                                                  dre[o2] = vr;
+                                                  dim_[o2] = vi + vi;
+                                                },
+                                                sm);
+}
+
+// P = exp(-i*A) for a real symmetric, pre-scaled A (|A|_1 <= theta) held in
+// the real plane of bf[0], followed by sexp squarings. The Taylor polynomial
+// sum_k c_k A^k with c_k = (-i)^k/k! has real powers and complex
+// coefficients: powers A^2..A^q are real products into the real planes of
+// bf[1..q-1]; each Horner step Y*acc (Y = A^q real, acc complex) is two real
+// products; each squaring of P = cos(A) - i sin(A) is two real products,
+// Re(P^2) = (C+S)(C-S) and Im(P^2) = -2CS. Returns the buffer that holds P.
+template <typename R>
+__device__ Buf<R> wg_expm_real(Buf<R>* bf, int sexp, int q, int K, const double* tc, GemmLds<R>& sm) {
+  const int tid = threadIdx.x;
+  const int np = bf[0].np;
+  const size_t pl = bf[0].pl;
+  for (int k = 2; k <= q; k++) {
+    R* dre = bf[k - 1].re();
+    wg_gemm_t<R, false, false, false, false, kRR>(np, np, np, bf[k - 2].re(), nullptr, np, bf[0].re(), nullptr, np,
+                                                  [=](int i, int j, R vr, R) { dre[(size_t)i * np + j] = vr; }, sm);
+  }
+  // c_k = (-i)^k/k!: real part +tc[k] (k%4==0), -tc[k] (k%4==2); imaginary -tc[k] (k%4==1), +tc[k] (k%4==3)
+  const R* p1 = bf[0].re();
+  const R* p2 = bf[1].re();
+  const R* p3 = bf[2].re();
+  auto add_block = [=](int jb, size_t e, bool diag, double& ar, double& ai) {
+    for (int i2 = 0; i2 < q; i2++) {
+      const int k = jb * q + i2;
+      if (k > K) break;
+      const double c = ((k & 2) ? -tc[k] : tc[k]);
+      const R* pw = (i2 == 1) ? p1 : ((i2 == 2) ? p2 : p3);
+      const double v = (i2 == 0) ? (diag ? 1.0 : 0.0) : (double)pw[e];
+      if (k & 1)
+        ai -= c * v;
+      else
+        ar += c * v;
+    }
+  };
+  const int r = K / q;
+  Buf<R> acc = bf[4], tmp = bf[5];
+  for (size_t e = tid; e < pl; e += kThreads) {
+    double ar = 0.0, ai = 0.0;
+    add_block(r, e, (e / np) == (e % np), ar, ai);
+    acc.re()[e] = (R)ar;
+    acc.im()[e] = (R)ai;
+  }
+  __syncthreads();
+  const R* Y = bf[q - 1].re();
+  for (int j = r - 1; j >= 0; j--) {
+    R* dre = tmp.re();
+    R* dim_ = tmp.im();
+    wg_gemm_t<R, false, false, false, false, kRC>(np, np, np, Y, nullptr, np, acc.re(), acc.im(), np,
+                                                  [=](int i, int jj, R vr, R vi) {
+                                                    size_t o2 = (size_t)i * np + jj;
+                                                    double ar = (double)vr, ai = (double)vi;
+                                                    add_block(j, o2, i == jj, ar, ai);
+                                                    dre[o2] = (R)ar;
+                                                    dim_[o2] = (R)ai;
+                                                  },
+                                                  sm);
+    Buf<R> t2 = acc;
+    acc = tmp;
+    tmp = t2;
+  }
+  Buf<R> cur = acc;
+  Buf<R> spare[2] = {bf[0], bf[1]};
+  int sp_i = 0;
+  for (int s = 0; s < sexp; s++) {
+    Buf<R> dst = spare[sp_i];
+    wg_square_commuting<R>(cur, dst, false, sm);
+    spare[sp_i] = cur;
+    cur = dst;
+    sp_i ^= 1;
+  }
+  return cur;
+}
+
 // --------------------------------------------------------- the fused kernel
 template <typename R>
 __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
@@ -549,8 +697,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
         }
         double hr = 0.5 * (xr + tr), hi = 0.5 * (xi - ti);
         size_t off = (size_t)prow[e] * np + pcol[e];
-        X[off] = (R)(p.dt * hi);
-        X[pl + off] = (R)(-p.dt * hr);
+        if (sd.real) {
+          X[off] = (R)(p.dt * hr);  // A = dt*H; the imaginary parts cancel pairwise
+        } else {
+          X[off] = (R)(p.dt * hi);
+          X[pl + off] = (R)(-p.dt * hr);
+        }
       }
     }
     __syncthreads();
@@ -582,7 +734,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
     }
     mark(0);
     // 4. P = expm(X): Paterson-Stockmeyer Taylor polynomial and s squarings
-    Buf<R> cur = wg_expm<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
+    Buf<R> cur = sd.real ? wg_expm_real<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g)
+                         : wg_expm<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
     Buf<R> spare[2];
     {
       int ns = 0;
@@ -632,15 +785,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
       const bool last = (k == p.log2b - 1);
       Buf<R> dst = last ? QT : spare[sp_i];
       if (!last && dst.p == QT.p) dst = spare[sp_i ^ 1];
-      R* dre = dst.re();
-      R* dim_ = dst.im();
-      wg_gemm_nn<R>(np, np, np, cur.re(), cur.im(), np, cur.re(), cur.im(), np,
-                 [=](int i, int j, R vr, R vi) {
-                   size_t o2 = last ? (size_t)j * np + i : (size_t)i * np + j;
-                   dre[o2] = vr;
-                   dim_[o2] = vi;
-                 },
-                 sh.g);
+      if (sd.real) {
+        wg_square_commuting<R>(cur, dst, last, sh.g);
+      } else {
+        R* dre = dst.re();
+        R* dim_ = dst.im();
+        wg_gemm_nn<R>(np, np, np, cur.re(), cur.im(), np, cur.re(), cur.im(), np,
+                   [=](int i, int j, R vr, R vi) {
+                     size_t o2 = last ? (size_t)j * np + i : (size_t)i * np + j;
+                     dre[o2] = vr;
+                     dim_[o2] = vi;
+                   },
+                   sh.g);
+      }
       if (!last) {
         spare[sp_i] = cur;
         cur = dst;

@@ -28,6 +28,7 @@
 #include "../../include/spinach_powder.h"
 #include "powder_kernels.hpp"
 #include "mas_kernels.hpp"
+#include "spectral_kernels.hpp"
 
 namespace {
 
@@ -124,7 +125,52 @@ struct SysPlan {
   std::vector<double> vals;  // interleaved complex, (1+nq)*nnz
   std::vector<double> rho, coil;
   std::vector<double> alphas, betas, gammas, weights;
+  bool real = false;  // generator real at every orientation
 };
+
+// Partner slot and sign of the conjugation symmetry of the Wigner matrices:
+// conj(D^r(k,m)) = (-1)^(k-m) D^r(-k,-m) (slot order: descending k and m).
+inline void conj_partner(int t, int& partner, double& sign) {
+  if (t < 9) {
+    const int i = t / 3, j = t % 3;
+    partner = (2 - i) * 3 + (2 - j);
+    sign = ((i - j) & 1) ? -1.0 : 1.0;
+  } else {
+    const int i = (t - 9) / 5, j = (t - 9) % 5;
+    partner = 9 + (4 - i) * 5 + (4 - j);
+    sign = ((i - j) & 1) ? -1.0 : 1.0;
+  }
+}
+
+// H(Omega) = I + sum D^r(k,m)(Omega) Q{r}{k,m} is real at every orientation
+// exactly when I is real and Q{r}{k,m} = (-1)^(k-m) conj(Q{r}{-k,-m}) for
+// every slot (the D^r(k,m) are linearly independent functions of Omega).
+// Checked on value planes laid out as plan_system lays them out (I first,
+// then the non-empty slots), to 8 ulp of the largest magnitude: a mismatch
+// at that level is below the rounding of the complex path itself.
+bool generator_is_real(const std::vector<int32_t>& slots, const std::vector<double>& vals, size_t nnz) {
+  double scale = 0.0;
+  for (double v : vals) scale = std::max(scale, std::fabs(v));
+  const double tol = 8.0 * 2.220446049250313e-16 * scale;
+  for (size_t e = 0; e < nnz; e++)
+    if (std::fabs(vals[2 * e + 1]) > tol) return false;
+  std::vector<const double*> plane(SP_NQ, nullptr);
+  for (size_t q = 0; q < slots.size(); q++) plane[slots[q]] = vals.data() + 2 * (q + 1) * nnz;
+  for (int t = 0; t < SP_NQ; t++) {
+    int u;
+    double sg;
+    conj_partner(t, u, sg);
+    const double* a = plane[t];
+    const double* b = plane[u];
+    if (!a && !b) continue;
+    for (size_t e = 0; e < nnz; e++) {
+      const double ar = a ? a[2 * e] : 0.0, ai = a ? a[2 * e + 1] : 0.0;
+      const double br = b ? b[2 * e] : 0.0, bi = b ? b[2 * e + 1] : 0.0;
+      if (std::fabs(ar - sg * br) > tol || std::fabs(ai + sg * bi) > tol) return false;
+    }
+  }
+  return true;
+}
 
 SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, const double* be, const double* ga,
                     const double* w, int64_t n_or, const double* rho0, const double* coil) {
@@ -240,11 +286,49 @@ SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, const do
       sp.coil[2 * map[i]] = coil[2 * i];
       sp.coil[2 * map[i] + 1] = coil[2 * i + 1];
     }
+  sp.real = generator_is_real(sp.slots, sp.vals, (size_t)sp.nnz);
   sp.alphas.assign(al, al + n_or);
   sp.betas.assign(be, be + n_or);
   sp.gammas.assign(ga, ga + n_or);
   sp.weights.assign(w, w + n_or);
   return sp;
+}
+
+// The realness test on the full (unreduced) matrices, for sp_generator_is_real.
+bool generator_is_real_full(const sp_csc* I, const sp_csc* Q) {
+  if (!I) throw SpError(SP_EINVAL, "I is NULL");
+  if (!Q) throw SpError(SP_EINVAL, "Q is NULL");
+  const int64_t dim = I->n;
+  if (dim <= 0) throw SpError(SP_EINVAL, "I must have a positive dimension");
+  std::vector<Triplets> mats(1 + SP_NQ);
+  mats[0] = read_csc(I, dim, "I");
+  for (int q = 0; q < SP_NQ; q++) {
+    char what[32];
+    snprintf(what, sizeof(what), "Q slot %d", q);
+    mats[1 + q] = read_csc(&Q[q], dim, what);
+  }
+  std::vector<std::pair<int64_t, int64_t>> ent;
+  for (auto& m : mats)
+    for (size_t p = 0; p < m.row.size(); p++) ent.emplace_back(m.row[p], m.col[p]);
+  std::sort(ent.begin(), ent.end());
+  ent.erase(std::unique(ent.begin(), ent.end()), ent.end());
+  const size_t nnz = ent.size();
+  std::vector<int32_t> slots;
+  std::vector<double> vals;
+  for (int q = 0; q <= SP_NQ; q++) {
+    const Triplets& m = mats[q];
+    if (q > 0 && m.row.empty()) continue;
+    std::vector<double> v(2 * nnz, 0.0);
+    for (size_t p = 0; p < m.row.size(); p++) {
+      const size_t e = (size_t)(std::lower_bound(ent.begin(), ent.end(), std::make_pair(m.row[p], m.col[p])) -
+                                ent.begin());
+      v[2 * e] += m.re[p];
+      v[2 * e + 1] += m.im[p];
+    }
+    if (q > 0) slots.push_back(q - 1);
+    vals.insert(vals.end(), v.begin(), v.end());
+  }
+  return generator_is_real(slots, vals, nnz);
 }
 
 }  // namespace
@@ -253,6 +337,7 @@ struct sp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};  // stage boundaries inside the propagation
   int n_cu = 0;
   int code = 0;
   std::string err;
@@ -266,7 +351,7 @@ struct sp_plan {
   int64_t n_items = 0;
   int64_t total_or = 0;
   double dt = 0;
-  int32_t np_max = 16, block = 1, nblk = 1, log2b = 0, taylor_k = 18, ps_q = 4;
+  int32_t np_max = 16, np_prop = 16, block = 1, nblk = 1, log2b = 0, taylor_k = 18, ps_q = 4;
   double theta = 1.0;
   int grid = 1;
   int per_cu = 1;
@@ -275,6 +360,13 @@ struct sp_plan {
   std::vector<double*> fid_targets;
   sp_stats last{};
   int64_t full_dim = 0, red_dim = 0, red_nnz = 0;
+  int32_t real_systems = 0;
+  // spectral path (real generators, FP64, padded dimension <= spk::kTriNP)
+  int64_t n_prop = 0, n_spec = 0;
+  int32_t spectral_systems = 0;
+  int grid_tri = 0, grid_syn = 0;
+  int64_t tri_stride = 0, syn_stride = 0;
+  DevBuf d_spec, d_status, d_work_tri, d_work_syn;
   DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
   DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
   bool executed = false;
@@ -322,6 +414,9 @@ int choose_log2_block(int64_t npts, int np) {
 }
 
 sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode) {
+  const bool force_complex = (mode & SP_MODE_COMPLEX) != 0;
+  const bool force_prop = force_complex || (mode & SP_MODE_PROPAGATOR) != 0;
+  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR);
   if (!(std::isfinite(dt)) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
   if (nsteps < 0) throw SpError(SP_EINVAL, "nsteps must be non-negative");
   if (nsteps > (int64_t)1 << 30) throw SpError(SP_ERANGE, "nsteps is too large");
@@ -334,9 +429,18 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   pl->nsteps = nsteps;
   pl->n_systems = (int64_t)sps.size();
   const int64_t npts = nsteps + 1;
-  int np_all = 16;
-  for (auto& sp : sps) np_all = std::max(np_all, sp.np);
-  pl->log2b = choose_log2_block(npts, np_all);
+  // path of each system: spectral (real generator, FP64, fits the register
+  // layout of the tridiagonalisation) or the propagator power scheme
+  std::vector<char> spectral(sps.size(), 0);
+  int np_prop = 16, np_spec = 16;
+  for (size_t s = 0; s < sps.size(); s++) {
+    spectral[s] = (!force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriNP) ? 1 : 0;
+    if (spectral[s])
+      np_spec = std::max(np_spec, sps[s].np);
+    else
+      np_prop = std::max(np_prop, sps[s].np);
+  }
+  pl->log2b = choose_log2_block(npts, np_prop);
   if (const char* e = getenv("SPINACH_LOG2_BLOCK")) pl->log2b = std::max(0, std::min(10, atoi(e)));
   pl->block = 1 << pl->log2b;
   pl->nblk = (int32_t)((npts + pl->block - 1) / pl->block);
@@ -353,8 +457,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   std::vector<spk::SysDesc> descs;
   std::vector<int32_t> row, col, tpos, slots;
   std::vector<double> vals, rho, coil, al, be, ga, w;
-  std::vector<spk::Item> items;
-  std::vector<std::pair<double, int64_t>> cost;  // for the queue order
+  // queue order: longest-processing-time first within each path (stable within a system)
+  std::vector<std::pair<double, spk::Item>> prop_items, spec_items;
   int64_t or_base = 0;
   for (size_t s = 0; s < sps.size(); s++) {
     SysPlan& sp = sps[s];
@@ -369,6 +473,9 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     d.vec_off = (int64_t)rho.size() / 2;
     d.or_off = or_base;
     d.n_or = (int64_t)sp.weights.size();
+    d.real = (sp.real && !force_complex) ? 1 : 0;
+    pl->real_systems += d.real;
+    pl->spectral_systems += spectral[s];
     descs.push_back(d);
     row.insert(row.end(), sp.row.begin(), sp.row.end());
     col.insert(col.end(), sp.col.begin(), sp.col.end());
@@ -383,24 +490,27 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     w.insert(w.end(), sp.weights.begin(), sp.weights.end());
     pl->or_off.push_back(or_base);
     pl->n_or.push_back(d.n_or);
-    double c = (double)sp.np * sp.np * sp.np;  // dense-stage cost per orientation
-    for (int64_t k = 0; k < d.n_or; k++) items.push_back(spk::Item{(int32_t)s, 0, or_base + k});
-    for (int64_t k = 0; k < d.n_or; k++) cost.emplace_back(c, (int64_t)items.size() - d.n_or + k);
+    const double c = (double)sp.np * sp.np * sp.np;  // dense-stage cost per orientation
+    auto& dst = spectral[s] ? spec_items : prop_items;
+    for (int64_t k = 0; k < d.n_or; k++) dst.emplace_back(c, spk::Item{(int32_t)s, 0, or_base + k});
     or_base += d.n_or;
     pl->np_max = std::max(pl->np_max, sp.np);
     pl->full_dim = std::max<int64_t>(pl->full_dim, sp.full_dim);
     pl->red_dim = std::max<int64_t>(pl->red_dim, sp.n);
     pl->red_nnz = std::max<int64_t>(pl->red_nnz, sp.nnz);
   }
-  // longest-processing-time-first queue order (stable within a system)
-  std::stable_sort(cost.begin(), cost.end(),
-                   [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
-                     return a.first > b.first;
-                   });
+  auto by_cost = [](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
+    return a.first > b.first;
+  };
+  std::stable_sort(prop_items.begin(), prop_items.end(), by_cost);
+  std::stable_sort(spec_items.begin(), spec_items.end(), by_cost);
   std::vector<spk::Item> ordered;
-  ordered.reserve(items.size());
-  for (auto& c : cost) ordered.push_back(items[c.second]);
-  pl->n_items = (int64_t)ordered.size();
+  ordered.reserve(prop_items.size() + spec_items.size());
+  for (auto& c : prop_items) ordered.push_back(c.second);
+  for (auto& c : spec_items) ordered.push_back(c.second);
+  pl->n_prop = (int64_t)prop_items.size();
+  pl->n_spec = (int64_t)spec_items.size();
+  pl->n_items = pl->n_prop + pl->n_spec;
   pl->total_or = or_base;
 
   hipStream_t st = ctx->stream;
@@ -417,32 +527,48 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   upload(pl->d_be, be, st);
   upload(pl->d_ga, ga, st);
   upload(pl->d_w, w, st);
-  pl->d_queue.alloc(sizeof(unsigned int));
-
-  // persistent grid: resident workgroups, bounded by the item count
-  int per_cu = 0;
-  if (mode == SP_MODE_FP64)
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<double>, spk::kThreads, 0));
-  else
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<float>, spk::kThreads, 0));
-  // the occupancy query is conservative for this kernel; at least two
-  // workgroups per CU hide the LDS-staging latency (override:
-  // SPINACH_WG_PER_CU). Workgroups beyond residency find the queue drained.
-  per_cu = std::max(2, per_cu);
-  if (const char* e = getenv("SPINACH_WG_PER_CU")) per_cu = std::max(1, atoi(e));
-  pl->per_cu = per_cu;
-  int64_t g = (int64_t)ctx->n_cu * per_cu;
-  pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, pl->n_items));
-  const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
-  const int64_t mat = 2 * (int64_t)pl->np_max * pl->np_max;
-  pl->work_stride = 6 * mat + 2 * (int64_t)(pl->block + 16) * pl->np_max + 2 * (int64_t)pl->nblk * pl->np_max;
-  pl->work_stride = (pl->work_stride + 63) / 64 * 64;
-  pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+  pl->d_queue.alloc(spk::kQueues * sizeof(unsigned int));
   pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)pl->n_systems * npts * 2 * sizeof(double));
-  pl->d_sq.alloc((size_t)pl->total_or * sizeof(int32_t));
-  if (const char* e = getenv("SPINACH_PHASE_TIMING"))
-    if (atoi(e) > 0) pl->d_phase.alloc((size_t)pl->grid * spk::kPhases * sizeof(unsigned long long));
+  pl->d_sq.alloc((size_t)std::max<int64_t>(1, pl->total_or) * sizeof(int32_t));
+  HIPCHK(hipMemsetAsync(pl->d_sq.p, 0, pl->d_sq.bytes, st));
+
+  if (pl->n_prop > 0) {
+    // persistent grid: resident workgroups, bounded by the item count
+    int per_cu = 0;
+    if (mode == SP_MODE_FP64)
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<double>, spk::kThreads, 0));
+    else
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<float>, spk::kThreads, 0));
+    // the occupancy query is conservative for this kernel; at least two
+    // workgroups per CU hide the LDS-staging latency (override:
+    // SPINACH_WG_PER_CU). Workgroups beyond residency find the queue drained.
+    per_cu = std::max(2, per_cu);
+    if (const char* e = getenv("SPINACH_WG_PER_CU")) per_cu = std::max(1, atoi(e));
+    pl->per_cu = per_cu;
+    int64_t g = (int64_t)ctx->n_cu * per_cu;
+    pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, pl->n_prop));
+    const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
+    const int64_t mat = 2 * (int64_t)np_prop * np_prop;
+    pl->np_prop = np_prop;
+    pl->work_stride = 6 * mat + 2 * (int64_t)(pl->block + 16) * np_prop + 2 * (int64_t)pl->nblk * np_prop;
+    pl->work_stride = (pl->work_stride + 63) / 64 * 64;
+    pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+    if (const char* e = getenv("SPINACH_PHASE_TIMING"))
+      if (atoi(e) > 0) pl->d_phase.alloc((size_t)pl->grid * spk::kPhases * sizeof(unsigned long long));
+  }
+  if (pl->n_spec > 0) {
+    // one 512-thread workgroup per CU holds a 192 x 192 matrix in registers
+    pl->grid_tri = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->n_cu, pl->n_spec));
+    pl->tri_stride = ((int64_t)np_spec * np_spec + 63) / 64 * 64;
+    pl->d_work_tri.alloc((size_t)pl->grid_tri * pl->tri_stride * sizeof(double));
+    HIPCHK(hipMemsetAsync(pl->d_work_tri.p, 0, pl->d_work_tri.bytes, st));  // kept all-zero between items
+    pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)ctx->n_cu, pl->n_spec));
+    pl->syn_stride = (4 * (int64_t)np_spec * spk::kSynthB + 63) / 64 * 64;
+    pl->d_work_syn.alloc((size_t)pl->grid_syn * pl->syn_stride * sizeof(double));
+    pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
+    pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
+  }
   HIPCHK(hipStreamSynchronize(st));
   return pl.release();
 }
@@ -452,7 +578,7 @@ void launch(sp_plan* pl) {
   spk::KParams<R> kp{};
   kp.sys = pl->d_sys.as<spk::SysDesc>();
   kp.items = pl->d_items.as<spk::Item>();
-  kp.n_items = pl->n_items;
+  kp.n_items = pl->n_prop;
   kp.queue = pl->d_queue.as<unsigned int>();
   kp.prow = pl->d_row.as<int32_t>();
   kp.pcol = pl->d_col.as<int32_t>();
@@ -474,12 +600,55 @@ void launch(sp_plan* pl) {
   kp.theta = pl->theta;
   kp.work = pl->d_work.as<R>();
   kp.work_stride = pl->work_stride;
-  kp.np_max = pl->np_max;
+  kp.np_max = pl->np_prop;
   kp.per_or = pl->d_per.as<double>();
   kp.sq_out = pl->d_sq.as<int32_t>();
   kp.phase_cycles = pl->d_phase.p ? pl->d_phase.as<unsigned long long>() : nullptr;
   hipLaunchKernelGGL(spk::k_powder<R>, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
   HIPCHK(hipGetLastError());
+}
+
+spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
+  spk::SpecParams sp{};
+  sp.sys = pl->d_sys.as<spk::SysDesc>();
+  sp.items = pl->d_items.as<spk::Item>() + pl->n_prop;
+  sp.n_items = pl->n_spec;
+  sp.queue = pl->d_queue.as<unsigned int>() + 1;
+  sp.prow = pl->d_row.as<int32_t>();
+  sp.pcol = pl->d_col.as<int32_t>();
+  sp.ptpos = pl->d_tpos.as<int32_t>();
+  sp.vals = pl->d_vals.as<double>();
+  sp.slots = pl->d_slots.as<int32_t>();
+  sp.rho0 = pl->d_rho.as<double>();
+  sp.coil = pl->d_coil.as<double>();
+  sp.alphas = pl->d_al.as<double>();
+  sp.betas = pl->d_be.as<double>();
+  sp.gammas = pl->d_ga.as<double>();
+  sp.dt = pl->dt;
+  sp.nsteps = pl->nsteps;
+  sp.work = work;
+  sp.work_stride = stride;
+  sp.spec = pl->d_spec.as<double>();
+  sp.status = pl->d_status.as<int32_t>();
+  sp.per_or = pl->d_per.as<double>();
+  return sp;
+}
+
+// The three spectral kernels; ev[0..2] are recorded after each of them.
+void launch_spectral(sp_plan* pl, hipEvent_t* ev) {
+  hipStream_t st = pl->ctx->stream;
+  spk::SpecParams tri = spec_params(pl, pl->d_work_tri.as<double>(), pl->tri_stride);
+  hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[0], st));
+  const int nb = (int)((pl->n_spec + spk::kTqlLanes - 1) / spk::kTqlLanes);
+  hipLaunchKernelGGL(spk::k_tql, dim3(nb), dim3(64 * spk::kTqlWaves), 0, st, tri);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[1], st));
+  spk::SpecParams syn = spec_params(pl, pl->d_work_syn.as<double>(), pl->syn_stride);
+  hipLaunchKernelGGL(spk::k_synth, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[2], st));
 }
 
 // Wigner matrices (rank 1 then rank 2, interleaved complex) at Euler angles.
@@ -509,6 +678,7 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   double an = std::sqrt(ax * ax + ay * ay + az * az);
   if (!(an > 0) || !std::isfinite(an)) throw SpError(SP_EINVAL, "rotor axis must be a non-zero finite vector");
   if (nsteps < 0 || nsteps > (int64_t)1 << 30) throw SpError(SP_EINVAL, "invalid nsteps");
+  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR);  // the Hilbert-space rotor path is always complex propagation
   if (mode != SP_MODE_FP64 && mode != SP_MODE_FP32) throw SpError(SP_EINVAL, "unknown precision mode");
   const int64_t N = I->n;
   if (N <= 0 || N > 1024) throw SpError(SP_ERANGE, "Hilbert dimension must be in 1..1024");
@@ -684,10 +854,20 @@ int run(sp_plan* pl, sp_stats* stats) {
       launch_mas<double>(pl);
     else
       launch_mas<float>(pl);
-  } else if (pl->mode == SP_MODE_FP64) {
-    launch<double>(pl);
+    for (auto& e : ctx->evs) HIPCHK(hipEventRecord(e, st));
   } else {
-    launch<float>(pl);
+    if (pl->n_prop > 0) {
+      if (pl->mode == SP_MODE_FP64)
+        launch<double>(pl);
+      else
+        launch<float>(pl);
+    }
+    HIPCHK(hipEventRecord(ctx->evs[0], st));
+    if (pl->n_spec > 0) {
+      launch_spectral(pl, ctx->evs + 1);
+    } else {
+      for (int k = 1; k < 4; k++) HIPCHK(hipEventRecord(ctx->evs[k], st));
+    }
   }
   HIPCHK(hipEventRecord(ctx->ev2, st));
   for (int64_t s = 0; s < pl->n_systems; s++) {
@@ -701,13 +881,24 @@ int run(sp_plan* pl, sp_stats* stats) {
   HIPCHK(hipEventCreate(&evend));
   HIPCHK(hipEventRecord(evend, st));
   hipError_t se = hipEventSynchronize(evend);
-  float ms_all = 0, ms_k = 0;
+  float ms_all = 0, ms_k = 0, ms_stage[4] = {0, 0, 0, 0};
   if (se == hipSuccess) {
     (void)hipEventElapsedTime(&ms_all, ctx->ev0, evend);
     (void)hipEventElapsedTime(&ms_k, ctx->ev1, ctx->ev2);
+    (void)hipEventElapsedTime(&ms_stage[0], ctx->ev1, ctx->evs[0]);
+    for (int k = 1; k < 4; k++) (void)hipEventElapsedTime(&ms_stage[k], ctx->evs[k - 1], ctx->evs[k]);
   }
   (void)hipEventDestroy(evend);
   HIPCHK(se);
+  int32_t eig_failures = 0, max_sweeps = 0;
+  if (pl->n_spec > 0) {
+    std::vector<int32_t> stv((size_t)pl->n_spec);
+    HIPCHK(hipMemcpy(stv.data(), pl->d_status.p, stv.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int32_t v : stv) {
+      if (v < 0) eig_failures++;
+      max_sweeps = std::max(max_sweeps, v);
+    }
+  }
   pl->executed = true;
   sp_stats s{};
   s.elapsed_s = ms_all * 1e-3;
@@ -729,6 +920,10 @@ int run(sp_plan* pl, sp_stats* stats) {
   }
   s.max_squarings = mx;
   s.ps_block = pl->ps_q;
+  s.real_systems = pl->real_systems;
+  s.spectral_systems = pl->spectral_systems;
+  s.eig_max_sweeps = max_sweeps;
+  for (int k = 0; k < 4; k++) s.stage_ms[k] = ms_stage[k];
   if (pl->d_phase.p) {
     std::vector<unsigned long long> ph((size_t)pl->grid * spk::kPhases);
     HIPCHK(hipMemcpy(ph.data(), pl->d_phase.p, ph.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -743,6 +938,9 @@ int run(sp_plan* pl, sp_stats* stats) {
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   pl->last = s;
   if (stats) *stats = s;
+  if (eig_failures > 0)
+    throw SpError(SP_ERANGE, "the tridiagonal eigenvalue iteration did not converge for " +
+                                 std::to_string(eig_failures) + " orientation(s)");
   return SP_OK;
 }
 
@@ -763,6 +961,7 @@ sp_ctx* sp_create(int device) {
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
     HIPCHK(hipEventCreate(&c->ev2));
+    for (auto& e : c->evs) HIPCHK(hipEventCreate(&e));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
@@ -779,8 +978,27 @@ void sp_destroy(sp_ctx* ctx) {
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+  for (auto& e : ctx->evs)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+int sp_generator_is_real(const sp_csc* I, const sp_csc* Q, int32_t* is_real) {
+  try {
+    if (!is_real) throw SpError(SP_EINVAL, "is_real must not be NULL");
+    *is_real = generator_is_real_full(I, Q) ? 1 : 0;
+    return SP_OK;
+  } catch (SpError& e) {
+    g_create_err = e.what();
+    return e.code;
+  } catch (std::bad_alloc&) {
+    g_create_err = "host allocation failed";
+    return SP_ENOMEM;
+  } catch (std::exception& e) {
+    g_create_err = e.what();
+    return SP_EINVAL;
+  }
 }
 
 const char* sp_last_error(const sp_ctx* ctx) {

@@ -4,6 +4,7 @@ import ctypes
 import glob
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -42,12 +43,29 @@ def test_binding_declares_signatures():
     assert lib.sp_powder_fid.argtypes is not None
 
 
-def test_struct_layouts_match_header():
-    # sp_csc: int64 n, 2 pointers, 1 pointer, int32 -> 40 bytes with padding
-    assert ctypes.sizeof(_native.SpCsc) == 40
-    assert ctypes.sizeof(_native.SpTols) == 5 * 8
-    assert ctypes.sizeof(_native.SpStats) == 3 * 8 + 3 * 8 + 6 * 4
-    assert ctypes.sizeof(_native.SpSystem) == 10 * 8
+STRUCTS = {'sp_csc': _native.SpCsc, 'sp_tols': _native.SpTols, 'sp_stats': _native.SpStats,
+           'sp_system': _native.SpSystem, 'sp_rotor': _native.SpRotor}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors have the size and field offsets a C compiler gives
+    the header's structs (compiled here with gcc against include/)."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "spinach_powder.h"', 'int main(void) {']
+    for cname, py in STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / 'layout.c'
+    src.write_text('\n'.join(lines) + '\n')
+    exe = tmp_path / 'layout'
+    subprocess.run(['gcc', '-std=c99', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split('\n')
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in STRUCTS.items():
+        assert got[(cname, 'size')] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def test_last_error_without_context_is_a_string():
