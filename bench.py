@@ -74,10 +74,11 @@ def spectral_stage_flops(n, npts):
     dimension n: Householder tridiagonalisation 4/3 n^3 (symmetric
     matrix-vector product 2 m^2 and rank-2 update 2 m^2 per column of a
     trailing block of size m) plus 4 n^2 per transformed vector (two complex
-    vectors = four real ones); the tridiagonal QR iteration is O(n^2) scalar
-    work, counted as 0 (it is bound by its dependent rotation chain, not by
-    throughput); synthesis 8 n flops per time point (complex multiply-add)."""
-    return {'spk::k_tridiag': 4.0 / 3.0 * n ** 3 + 16.0 * n ** 2, 'spk::k_tql': 0.0,
+    vectors = four real ones); the divide-and-conquer tridiagonal eigensolver
+    is O(n^2) scalar work per merge level, counted as 0 (bound by its
+    secular-equation iterations, not by throughput); synthesis 8 n flops per
+    time point (complex multiply-add)."""
+    return {'spk::k_tridiag': 4.0 / 3.0 * n ** 3 + 16.0 * n ** 2, 'spk::k_dceig': 0.0,
             'spk::k_synth': 8.0 * n * npts}
 
 
@@ -108,7 +109,7 @@ def stage_table(st, prop_flops, spec_flops, prop_kernel='spk::k_powder'):
     if prop_flops:
         out[prop_kernel] = (ms[0], prop_flops)
     if spec_flops:
-        for k, name in enumerate(('spk::k_tridiag', 'spk::k_tql', 'spk::k_synth')):
+        for k, name in enumerate(('spk::k_tridiag', 'spk::k_dceig', 'spk::k_synth')):
             out[name] = (ms[1 + k], spec_flops[name])
     return out
 
@@ -175,7 +176,7 @@ def bench_batch(args):
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import pyoracle as O
     prop_flops = 0.0
-    spec_flops = {k: 0.0 for k in ('spk::k_tridiag', 'spk::k_tql', 'spk::k_synth')}
+    spec_flops = {k: 0.0 for k in ('spk::k_tridiag', 'spk::k_dceig', 'spk::k_synth')}
     dims = []
     spectral_ok = st['spectral_systems'] > 0
     for y in b['systems']:

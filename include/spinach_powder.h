@@ -50,8 +50,8 @@ extern "C" {
 /* Flag OR-ed into the mode: use the propagator power scheme for every system.
  * Without it, FP64 systems with a real generator and a coil-connected
  * dimension of at most 192 take the spectral path: one Householder
- * tridiagonalisation and one tridiagonal QR iteration per orientation give
- * f(t) = sum_k g_k exp(-i lambda_k t) directly (agreement with the
+ * tridiagonalisation and one divide-and-conquer tridiagonal eigensolution per
+ * orientation give f(t) = sum_k g_k exp(-i lambda_k t) directly (agreement with the
  * propagator path: ~1e-12 relative over thousands of steps). */
 #define SP_MODE_PROPAGATOR 32
 
@@ -96,9 +96,9 @@ typedef struct sp_stats {
   int32_t sum_squarings_lo;  /* sum of scaling exponents over orientations (low 32 bits) */
   int32_t real_systems;      /* systems whose generator is real at every orientation */
   int32_t spectral_systems;  /* systems evaluated on the spectral path */
-  int32_t eig_max_sweeps;    /* most QR sweeps any orientation of the spectral path needed */
+  int32_t eig_max_sweeps;    /* most secular-equation iterations any root of the spectral path needed */
   int32_t reserved;
-  double stage_ms[4];        /* propagator kernel, tridiagonalisation, tridiagonal QR, synthesis */
+  double stage_ms[4];        /* propagator kernel, tridiagonalisation, tridiagonal eigensolver, synthesis */
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */

@@ -15,9 +15,9 @@
 //              thread 3 rows x 24 columns, cyclic), so the symmetric
 //              matrix-vector products and rank-2 updates of every step run
 //              from registers with the Householder vectors broadcast from LDS.
-//   k_tql      implicit symmetric QR with Wilkinson shifts on T (one lane per
-//              orientation), the Givens rotations applied to the two
-//              transformed vectors only: the eigenvectors are never formed.
+//   k_dceig    eigenvalues of T and the projections g_k by divide and
+//              conquer (QR on leaves of <= 12 rows, secular-equation merges
+//              with one thread per root); the eigenvectors are never formed.
 //   k_synth    f(64*i + j) = sum_k [g_k e^{-i lambda_k 64 i}] [e^{-i lambda_k j}]
 //              as one complex matrix product per 64 x 64 block of time
 //              points, on the matrix cores.
@@ -221,16 +221,24 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         se[j] = alpha;
       }
       if (tau != 0.0) {  // uniform
-        auto vat = [&](int i) -> double { return (i == j + 1) ? v1 : ((i > j + 1) ? x[i] : 0.0); };
-        // p = tau*A*v on the trailing rows and the per-wave parts of v'p
+        // v_i (i > j+1: row j, i = j+1: v1, else 0); the loads are unconditional so
+        // that a chunk's loads issue together
+        auto vat = [&](int i) -> double {
+          const double xi = x[i];
+          return (i > j + 1) ? xi : ((i == j + 1) ? v1 : 0.0);
+        };
+        // p = tau*A*v on the trailing rows, in chunks of 4 column groups (32 columns)
         double part[kTriRowsPer] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int b = 0; b < kTriColsPer; b++) {
-          if (8 * b + 7 > j) {
-            const double vb = vat(tc + 8 * b);
+        for (int bc = 0; bc < kTriColsPer / 4; bc++) {
+          if (32 * bc + 31 > j) {
+            double vb[4];
 #pragma unroll
-            for (int a = 0; a < kTriRowsPer; a++)
-              if (64 * a + 8 * wave + 7 > j) part[a] = fma(R[a][b], vb, part[a]);
+            for (int q = 0; q < 4; q++) vb[q] = vat(tc + 8 * (4 * bc + q));
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+              for (int a = 0; a < kTriRowsPer; a++) part[a] = fma(R[a][4 * bc + q], vb[q], part[a]);
           }
         }
         double qsum = 0.0;
@@ -247,10 +255,19 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         if (lane == 0) sq[wave] = qw;
         // Q^T applied to conj(coil) and rho0, one vector per wave: u <- u - tau*v*(v'u)
         if (wave < 4) {
+          double vv[kTriNP / 64], uu[kTriNP / 64];
           double s = 0.0;
-          for (int i = j + 1 + lane; i < n; i += 64) s = fma(vat(i), su[wave][i], s);
+#pragma unroll
+          for (int r = 0; r < kTriNP / 64; r++) {
+            const int i = lane + 64 * r;
+            vv[r] = vat(i);
+            uu[r] = su[wave][i];
+            s = fma(vv[r], uu[r], s);
+          }
           s = wave_sum_dpp(s) * tau;
-          for (int i = j + 1 + lane; i < n; i += 64) su[wave][i] -= s * vat(i);
+#pragma unroll
+          for (int r = 0; r < kTriNP / 64; r++)
+            if (vv[r] != 0.0) su[wave][lane + 64 * r] = uu[r] - s * vv[r];
         }
         __syncthreads();
         // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
@@ -266,12 +283,20 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           wi[a] = sp[i] - K * vi[a];
         }
 #pragma unroll
-        for (int b = 0; b < kTriColsPer; b++) {
-          if (8 * b + 7 > j) {
-            const double vb = vat(tc + 8 * b), wb = sp[tc + 8 * b] - K * vb;
+        for (int bc = 0; bc < kTriColsPer / 4; bc++) {
+          if (32 * bc + 31 > j) {
+            double vb[4], wb[4];
 #pragma unroll
-            for (int a = 0; a < kTriRowsPer; a++)
-              if (64 * a + 8 * wave + 7 > j) R[a][b] = fma(-vi[a], wb, fma(-wi[a], vb, R[a][b]));
+            for (int q = 0; q < 4; q++) {
+              const int c = tc + 8 * (4 * bc + q);
+              vb[q] = vat(c);
+              wb[q] = sp[c] - K * vb[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+              for (int a = 0; a < kTriRowsPer; a++)
+                R[a][4 * bc + q] = fma(-vi[a], wb[q], fma(-wi[a], vb[q], R[a][4 * bc + q]));
           }
         }
       }
@@ -303,136 +328,464 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
   }
 }
 
-// ------------------------------------------------------------ k_tql
-// Implicit symmetric tridiagonal QR with Wilkinson shifts (Golub & Van Loan,
-// Matrix Computations, algorithms 8.3.2/8.3.3), deflating at |e_i| <= eps*|T|
-// (a normwise backward-stable test). Each Givens rotation is applied to the
-// four real vectors (conj(coil) and rho0, real and imaginary parts) instead
-// of an eigenvector matrix. One lane per orientation: the rotations form a
-// dependent chain, so its operands live in LDS (kTqlLanes orientations per
-// workgroup, rows padded so that lanes spread over the banks). A chain is
-// bound by instruction issue (~100 FP64 instructions per rotation), so the
-// orientations of a workgroup share one wave (measured on MI355X for the
-// 185-state case: 16 per wave 14.6 ms, 4 waves x 4 25.5 ms, 16 waves x 1
-// 27.2 ms for 4000 orientations). On exit the item's d holds the eigenvalues
-// and its first two vectors hold g_k = (coil' v_k)(v_k^T rho0).
-constexpr int kTqlPerWave = 16;
-constexpr int kTqlWaves = 1;
-constexpr int kTqlLanes = kTqlPerWave * kTqlWaves;  // orientations per workgroup
-constexpr int kTqlStride = 6 * kTriNP + 2;          // doubles per orientation
+// ------------------------------------------------------------ k_dceig
+// Eigenvalues of the tridiagonal T and the projections of the transformed
+// coil and rho0 on its eigenvectors, by divide and conquer (Cuppen's method
+// with the eigenvector formula of Gu & Eisenstat): T is torn into leaves of
+// at most kDcLeaf rows by rank-one tears, T = diag(T1, T2) + |b| w w^T
+// (w = e_m + sign(b) e_{m+1}, b the coupling at the tear); each leaf is
+// solved by implicit QR with Wilkinson shifts (one thread per leaf); the
+// leaves are merged level by level, every merge solving
+//   D + rho z z^T,  rho = 2|b|,  z = [last row of Q1, sign(b) first row of Q2]/sqrt(2)
+// through the secular equation 1 + rho sum z_i^2/(d_i - lambda) = 0 (one
+// thread per root). Eigenvectors are never stored: each subproblem carries
+// six vectors over its eigen-index, Q^T applied to conj(coil) and rho0 (real
+// and imaginary parts) and its first and last eigenvector rows (needed by the
+// parent merge). One 256-thread workgroup per orientation, everything in LDS.
+constexpr int kDcThreads = 256;
+constexpr int kDcLeaf = 12;
+constexpr int kDcMaxLevels = 5;  // up to 32 leaves
+constexpr int kDcMaxIter = 100;  // secular-equation iterations per root (typically 3-8)
 
-__global__ void __launch_bounds__(64 * kTqlWaves) k_tql(SpecParams p) {
-  __shared__ double lds[kTqlLanes * kTqlStride];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane >= kTqlPerWave) return;
-  const int slot = wave * kTqlPerWave + lane;
-  const int64_t it = (int64_t)blockIdx.x * kTqlLanes + slot;
-  if (it >= p.n_items) return;
-  const int n = p.sys[p.items[it].sys].n;
-  double* gsrc = p.spec + (size_t)it * kSpecStride;
-  double* d = lds + slot * kTqlStride;
-  double* e = d + kTriNP;
-  double* u0 = d + 2 * kTriNP;
-  double* u1 = d + 3 * kTriNP;
-  double* u2 = d + 4 * kTriNP;
-  double* u3 = d + 5 * kTriNP;
-  for (int q = 0; q < 6; q++)
-    for (int i = 0; i < n; i++) d[q * kTriNP + i] = gsrc[q * kTriNP + i];
+struct DcShared {
+  double d[kTriNP], e[kTriNP];
+  double X[6][kTriNP];  // tracked vectors over the eigen-index of each segment
+  double Y[6][kTriNP];  // merge input, sorted
+  double D[kTriNP], z[kTriNP];
+  double Dc[kTriNP], Zc[kTriNP], zh[kTriNP], tau[kTriNP];
+  double stage_lam[kTriNP];     // per-thread results of M5/M6 between their two halves
+  double stage_x[6][kTriNP];
+  int stage_pos[kTriNP];
+  int org[kTriNP], keep[kTriNP], defl[kTriNP];
+  int bnd[kDcMaxLevels + 1][(1 << kDcMaxLevels) + 1];  // segment boundaries per level
+  double beta[(1 << kDcMaxLevels)];                     // rho = 2|coupling| of the merges of a level
+  int K[(1 << kDcMaxLevels)];
+  int its[kTriNP];
+  int nlev;
+  int fail;
+};
+
+// The kernel is a sequence of phases separated by workgroup barriers; within
+// a phase every thread works on its own data. Each phase is a
+// __host__ __device__ function of the thread id, so that a CPU build can run
+// the same code phase by phase (a loop over thread ids per phase).
+
+// implicit QR on the segment [a, b) of (d, e) with the six tracked vectors
+// (Golub & Van Loan, algorithm 8.3.3); deflation at |e_i| <= eps*|T_seg|
+__host__ __device__ inline int dc_leaf_qr(DcShared& S, int a, int b) {
+  const int n = b - a;
   double anorm = 0.0;
-  for (int i = 0; i < n; i++) {
-    double r = fabs(d[i]);
-    if (i + 1 < n) r += fabs(e[i]);
-    if (i > 0) r += fabs(e[i - 1]);
+  for (int i = a; i < b; i++) {
+    double r = fabs(S.d[i]);
+    if (i + 1 < b) r += fabs(S.e[i]);
+    if (i > a) r += fabs(S.e[i - 1]);
     anorm = fmax(anorm, r);
   }
   const double tol = 2.220446049250313e-16 * anorm;
-  int hi = n - 1;
-  int sweeps = 0;
-  bool failed = false;
-  while (hi > 0) {
-    if (fabs(e[hi - 1]) <= tol) {
-      e[hi - 1] = 0.0;
+  int hi = b - 1, sweeps = 0;
+  while (hi > a) {
+    if (fabs(S.e[hi - 1]) <= tol) {
+      S.e[hi - 1] = 0.0;
       hi--;
       continue;
     }
     int lo = hi - 1;
-    while (lo > 0 && fabs(e[lo - 1]) > tol) lo--;
-    if (++sweeps > 40 * n) {
-      failed = true;
-      break;
-    }
-    // Wilkinson shift from the trailing 2 x 2 block
-    const double dd = 0.5 * (d[hi - 1] - d[hi]), bb = e[hi - 1];
+    while (lo > a && fabs(S.e[lo - 1]) > tol) lo--;
+    if (++sweeps > 40 * n) return -1;
+    const double dd = 0.5 * (S.d[hi - 1] - S.d[hi]), bb = S.e[hi - 1];
     const double h = sqrt(dd * dd + bb * bb);
-    const double mu = d[hi] - bb * bb / (dd + (dd >= 0.0 ? h : -h));
-    double x = d[lo] - mu, z = e[lo];
-    double dk = d[lo], ek = e[lo];
-    double dk1 = d[lo + 1], ek1 = (lo + 1 < hi) ? e[lo + 1] : 0.0;
-    double a0 = u0[lo], a1 = u1[lo], a2 = u2[lo], a3 = u3[lo];
-    double b0 = u0[lo + 1], b1 = u1[lo + 1], b2 = u2[lo + 1], b3 = u3[lo + 1];
+    const double mu = S.d[hi] - bb * bb / (dd + (dd >= 0.0 ? h : -h));
+    double x = S.d[lo] - mu, zz = S.e[lo];
     for (int k = lo; k < hi; k++) {
-      // operands of the next rotation, loaded ahead of the dependent chain
-      const bool more = (k + 1 < hi);
-      double nd = 0.0, ne = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
-      if (more) {
-        nd = d[k + 2];
-        ne = (k + 2 < hi) ? e[k + 2] : 0.0;
-        n0 = u0[k + 2];
-        n1 = u1[k + 2];
-        n2 = u2[k + 2];
-        n3 = u3[k + 2];
+      const double r = sqrt(x * x + zz * zz);
+      double c = 1.0, s = 0.0;
+      if (r > 0.0) {
+        c = x / r;
+        s = zz / r;
       }
-      const double r2 = x * x + z * z;
-      double c = 1.0, s = 0.0, r = 0.0;
-      if (r2 > 0.0) {
-        const double ri = rsqrt(r2);
-        c = x * ri;
-        s = z * ri;
-        r = r2 * ri;
-      }
-      if (k > lo) e[k - 1] = r;
+      if (k > lo) S.e[k - 1] = r;
+      const double dk = S.d[k], dk1 = S.d[k + 1], ek = S.e[k];
       const double cc = c * c, ss = s * s, cs = c * s;
-      const double dkn = cc * dk + 2.0 * cs * ek + ss * dk1;
-      const double dk1n = ss * dk - 2.0 * cs * ek + cc * dk1;
-      const double ekn = cs * (dk1 - dk) + (cc - ss) * ek;
-      d[k] = dkn;
-      u0[k] = c * a0 + s * b0;
-      u1[k] = c * a1 + s * b1;
-      u2[k] = c * a2 + s * b2;
-      u3[k] = c * a3 + s * b3;
-      a0 = c * b0 - s * a0;
-      a1 = c * b1 - s * a1;
-      a2 = c * b2 - s * a2;
-      a3 = c * b3 - s * a3;
-      if (more) {
-        x = ekn;
-        z = s * ek1;
-        ek = c * ek1;
-        dk = dk1n;
-        dk1 = nd;
-        ek1 = ne;
-        b0 = n0;
-        b1 = n1;
-        b2 = n2;
-        b3 = n3;
-      } else {
-        d[k + 1] = dk1n;
-        e[k] = ekn;
-        u0[k + 1] = a0;
-        u1[k + 1] = a1;
-        u2[k + 1] = a2;
-        u3[k + 1] = a3;
+      S.d[k] = cc * dk + 2.0 * cs * ek + ss * dk1;
+      S.d[k + 1] = ss * dk - 2.0 * cs * ek + cc * dk1;
+      S.e[k] = cs * (dk1 - dk) + (cc - ss) * ek;
+      if (k < hi - 1) {
+        x = S.e[k];
+        zz = s * S.e[k + 1];
+        S.e[k + 1] = c * S.e[k + 1];
+      }
+      for (int q = 0; q < 6; q++) {
+        const double u0 = S.X[q][k], u1 = S.X[q][k + 1];
+        S.X[q][k] = c * u0 + s * u1;
+        S.X[q][k + 1] = c * u1 - s * u0;
       }
     }
   }
-  // eigenvalues and g_k = a_k * b_k (a = transformed conj(coil), b = transformed rho0)
-  for (int k = 0; k < n; k++) {
-    const double ar = u0[k], ai = u1[k], br = u2[k], bi = u3[k];
-    gsrc[k] = d[k];
+  return sweeps;
+}
+
+// insertion sort of the segment [a, b) by eigenvalue, carrying the vectors
+__host__ __device__ inline void dc_sort_segment(DcShared& S, int a, int b) {
+  for (int i = a + 1; i < b; i++) {
+    const double v = S.d[i];
+    double x[6];
+    for (int q = 0; q < 6; q++) x[q] = S.X[q][i];
+    int j = i - 1;
+    while (j >= a && S.d[j] > v) {
+      S.d[j + 1] = S.d[j];
+      for (int q = 0; q < 6; q++) S.X[q][j + 1] = S.X[q][j];
+      j--;
+    }
+    S.d[j + 1] = v;
+    for (int q = 0; q < 6; q++) S.X[q][j + 1] = x[q];
+  }
+}
+
+// Root k of 1 + rho sum_{i<K} Zc_i^2/(Dc_i - lambda) = 0 (Dc ascending) as
+// lambda = Dc[org] + tau. Interior roots: the two-pole rational model of the
+// "middle way" (R.-C. Li, 1993); the outermost root: Newton on
+// F(x) = x (1/rho + psi(x)) - z_K^2, x = lambda - Dc[K-1], which has no pole at
+// x = 0. Both safeguarded by the bracket. Returns the iteration count
+// (kDcMaxIter + 1 if it did not converge).
+__host__ __device__ inline int dc_secular_root(const double* Dc, const double* Zc, int K, int k, double rho, int& org,
+                                               double& tau) {
+  const double eps = 2.220446049250313e-16;
+  double tlo, thi;
+  if (k < K - 1) {
+    const double mid = 0.5 * (Dc[k + 1] - Dc[k]);
+    double f = 1.0 / rho;
+    for (int i = 0; i < K; i++) f += Zc[i] * Zc[i] / ((Dc[i] - Dc[k]) - mid);
+    if (f >= 0.0) {
+      org = k;
+      tlo = 0.0;
+      thi = mid;
+    } else {
+      org = k + 1;
+      tlo = -mid;
+      thi = 0.0;
+    }
+  } else {
+    org = k;
+    double zs = 0.0;
+    for (int i = 0; i < K; i++) zs += Zc[i] * Zc[i];
+    tlo = 0.0;
+    thi = rho * zs;
+  }
+  const double dorg = Dc[org];
+  tau = 0.5 * (tlo + thi);
+  int it = 1;
+  for (; it <= kDcMaxIter; it++) {
+    double f = 1.0 / rho, wabs = 0.0, psi = 0.0, psip = 0.0, phip = 0.0;
+    for (int i = 0; i < K; i++) {
+      const double delta = (Dc[i] - dorg) - tau;
+      const double rd = 1.0 / delta;
+      const double w = Zc[i] * Zc[i] * rd;
+      f += w;
+      wabs += fabs(w);
+      if (k < K - 1) {
+        if (i <= k)
+          psip += w * rd;
+        else
+          phip += w * rd;
+      } else if (i < K - 1) {
+        psi += w;
+        psip += w * rd;
+      }
+    }
+    if (f > 0.0)
+      thi = tau;
+    else
+      tlo = tau;
+    if (fabs(f) <= 8.0 * eps * (1.0 / rho + wabs) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) break;
+    double eta;
+    if (k < K - 1) {
+      const double d1 = (Dc[k] - dorg) - tau, d2 = (Dc[k + 1] - dorg) - tau;
+      const double c = f - psip * d1 - phip * d2;
+      const double B = c * (d1 + d2) + psip * d1 * d1 + phip * d2 * d2;
+      const double C = d1 * d2 * f;
+      const double disc = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
+      if (B >= 0.0)
+        eta = (B + disc != 0.0) ? 2.0 * C / (B + disc) : 0.0;
+      else
+        eta = (c != 0.0) ? (B - disc) / (2.0 * c) : 0.0;
+    } else {
+      const double F = tau * (1.0 / rho + psi) - Zc[K - 1] * Zc[K - 1];
+      const double Fp = 1.0 / rho + psi + tau * psip;
+      eta = (Fp != 0.0) ? -F / Fp : 0.0;
+    }
+    double nt = tau + eta;
+    if (!(nt > tlo && nt < thi)) nt = 0.5 * (tlo + thi);
+    tau = nt;
+  }
+  return it;
+}
+
+// merge index of element i at level l (bounded by the level's segment count)
+__host__ __device__ inline int dc_segment_of(const DcShared& S, int l, int i) {
+  const int last = (1 << l) - 1;
+  int s2 = 0;
+  while (s2 < last && S.bnd[l][s2 + 1] <= i) s2++;
+  return s2;
+}
+
+// phase 0: load T and the four vectors (every thread)
+__host__ __device__ inline void dc_phase_load(DcShared& S, const double* gsrc, int tid, int nthreads) {
+  for (int i = tid; i < kTriNP; i += nthreads) {
+    S.d[i] = gsrc[i];
+    S.e[i] = gsrc[kTriNP + i];
+    for (int q = 0; q < 4; q++) S.X[q][i] = gsrc[(2 + q) * kTriNP + i];
+    S.X[4][i] = 0.0;
+    S.X[5][i] = 0.0;
+    S.its[i] = 0;
+  }
+}
+
+// phase 1 (one thread): the segment tree and the rank-one tears
+__host__ __device__ inline void dc_phase_plan(DcShared& S, int n) {
+  int L = 0;
+  while (L < kDcMaxLevels && ((n + (1 << L) - 1) >> L) > kDcLeaf) L++;
+  S.nlev = L;
+  S.fail = 0;
+  S.bnd[0][0] = 0;
+  S.bnd[0][1] = n;
+  for (int l = 0; l < L; l++) {
+    for (int s2 = 0; s2 < (1 << l); s2++) {
+      const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1];
+      S.bnd[l + 1][2 * s2] = a;
+      S.bnd[l + 1][2 * s2 + 1] = a + (b - a) / 2;
+    }
+    S.bnd[l + 1][1 << (l + 1)] = n;
+  }
+  for (int l = 0; l < L; l++)
+    for (int s2 = 0; s2 < (1 << l); s2++) {
+      const int m = S.bnd[l + 1][2 * s2 + 1];
+      if (m <= S.bnd[l][s2] || m >= S.bnd[l][s2 + 1]) continue;  // empty child
+      const double b = fabs(S.e[m - 1]);
+      S.d[m - 1] -= b;
+      S.d[m] -= b;
+    }
+}
+
+// phase 2: one thread per leaf
+__host__ __device__ inline void dc_phase_leaf(DcShared& S, int tid) {
+  const int L = S.nlev;
+  if (tid >= (1 << L)) return;
+  const int a = S.bnd[L][tid], b = S.bnd[L][tid + 1];
+  if (b <= a) return;
+  S.X[4][a] = 1.0;
+  S.X[5][b - 1] = 1.0;
+  if (dc_leaf_qr(S, a, b) < 0) S.fail = 1;
+  dc_sort_segment(S, a, b);
+}
+
+// M1: merge-path of the two sorted children into Y, D, z (one thread per element)
+__host__ __device__ inline void dc_phase_m1(DcShared& S, int l, int n, int i) {
+  if (i >= n) return;
+  const int s2 = dc_segment_of(S, l, i);
+  const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1], m = S.bnd[l + 1][2 * s2 + 1];
+  const double bcoup = (m > a && m < b) ? S.e[m - 1] : 0.0;
+  const double sg = (bcoup >= 0.0) ? 1.0 : -1.0;
+  const bool left = (i < m);
+  const double v = S.d[i];
+  // rank in the other child: a left element counts right < v, a right one left <= v
+  int lo2 = left ? m : a, hi2 = left ? b : m;
+  while (lo2 < hi2) {
+    const int md = (lo2 + hi2) >> 1;
+    const bool before = left ? (S.d[md] < v) : (S.d[md] <= v);
+    if (before)
+      lo2 = md + 1;
+    else
+      hi2 = md;
+  }
+  const int pos = a + (i - (left ? a : m)) + (lo2 - (left ? m : a));
+  S.D[pos] = v;
+  S.z[pos] = (left ? S.X[5][i] : sg * S.X[4][i]) * 0.70710678118654752440;
+  for (int q = 0; q < 4; q++) S.Y[q][pos] = S.X[q][i];
+  S.Y[4][pos] = left ? S.X[4][i] : 0.0;
+  S.Y[5][pos] = left ? 0.0 : S.X[5][i];
+}
+
+// M2: deflation scan (one thread per merge): small z_i, and pairs of close
+// poles rotated so that one z component vanishes
+__host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
+  if (s2 >= (1 << l)) return;
+  const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1], m = S.bnd[l + 1][2 * s2 + 1];
+  const double bcoup = (m > a && m < b) ? S.e[m - 1] : 0.0;
+  const double rho = 2.0 * fabs(bcoup);
+  S.beta[s2] = rho;
+  double dmax = 0.0;
+  for (int i = a; i < b; i++) dmax = fmax(dmax, fabs(S.D[i]));
+  const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, rho);
+  int K = 0, nd = 0, prev = -1;
+  for (int i = a; i < b; i++) {
+    if (rho == 0.0 || rho * fabs(S.z[i]) <= tol) {
+      S.defl[a + nd++] = i;
+      continue;
+    }
+    if (prev >= 0) {
+      const double t = S.D[i] - S.D[prev];
+      const double tz = sqrt(S.z[prev] * S.z[prev] + S.z[i] * S.z[i]);
+      const double c = S.z[i] / tz, s = -S.z[prev] / tz;
+      if (fabs(t * c * s) <= tol) {
+        S.z[i] = tz;
+        S.z[prev] = 0.0;
+        const double dp = S.D[prev], di = S.D[i];
+        S.D[prev] = dp * c * c + di * s * s;
+        S.D[i] = dp * s * s + di * c * c;
+        for (int q = 0; q < 6; q++) {
+          const double u0 = S.Y[q][prev], u1 = S.Y[q][i];
+          S.Y[q][prev] = c * u0 + s * u1;
+          S.Y[q][i] = c * u1 - s * u0;
+        }
+        S.defl[a + nd++] = prev;
+        K--;
+      }
+    }
+    S.keep[a + K++] = i;
+    prev = i;
+  }
+  S.K[s2] = K;
+  for (int k = 0; k < K; k++) {
+    S.Dc[a + k] = S.D[S.keep[a + k]];
+    S.Zc[a + k] = S.z[S.keep[a + k]];
+  }
+}
+
+// M3: secular roots (one thread per (merge, root))
+__host__ __device__ inline void dc_phase_m3(DcShared& S, int l, int n, int i) {
+  if (i >= n) return;
+  const int s2 = dc_segment_of(S, l, i);
+  const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
+  if (k >= K) return;
+  int og;
+  double tu;
+  const int its = dc_secular_root(S.Dc + a, S.Zc + a, K, k, S.beta[s2], og, tu);
+  if (its > kDcMaxIter) S.fail = 1;
+  if (its > S.its[i]) S.its[i] = its;
+  S.org[a + k] = og;
+  S.tau[a + k] = tu;
+}
+
+// M4: z recomputed from the computed roots (Gu & Eisenstat), one thread per (merge, pole)
+__host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
+  if (t >= n) return;
+  const int s2 = dc_segment_of(S, l, t);
+  const int a = S.bnd[l][s2], i = t - a, K = S.K[s2];
+  if (i >= K) return;
+  const double di = S.Dc[a + i];
+  double w = 1.0;
+  for (int k = 0; k < K; k++) {
+    const double lmd = (S.Dc[a + S.org[a + k]] - di) + S.tau[a + k];  // lambda_k - d_i
+    w *= (k != i) ? lmd / (S.Dc[a + k] - di) : lmd;
+  }
+  const double zz = sqrt(fmax(w, 0.0) / S.beta[s2]);
+  S.zh[a + i] = (S.Zc[a + i] >= 0.0) ? zz : -zz;
+}
+
+// M5: eigenpairs of each merge into d, X: roots in slots 0..K-1, deflated pairs after
+__host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
+  if (t >= n) return;
+  const int s2 = dc_segment_of(S, l, t);
+  const int a = S.bnd[l][s2], slot = t - a, K = S.K[s2];
+  double out[6];
+  double lam;
+  if (slot < K) {
+    const int og = S.org[a + slot];
+    const double tu = S.tau[a + slot], dorg = S.Dc[a + og];
+    double nrm = 0.0;
+    for (int q = 0; q < 6; q++) out[q] = 0.0;
+    for (int i = 0; i < K; i++) {
+      const double u = S.zh[a + i] / ((S.Dc[a + i] - dorg) - tu);
+      nrm = fma(u, u, nrm);
+      const int src = S.keep[a + i];
+      for (int q = 0; q < 6; q++) out[q] = fma(u, S.Y[q][src], out[q]);
+    }
+    const double inv = 1.0 / sqrt(nrm);
+    for (int q = 0; q < 6; q++) out[q] *= inv;
+    lam = dorg + tu;
+  } else {
+    const int src = S.defl[a + slot - K];
+    for (int q = 0; q < 6; q++) out[q] = S.Y[q][src];
+    lam = S.D[src];
+  }
+  S.d[t] = lam;
+  for (int q = 0; q < 6; q++) S.X[q][t] = out[q];
+}
+
+// M6a/M6b: rank sort of each merged segment (the rank into stage_*, then the scatter)
+__host__ __device__ inline void dc_phase_m6a(DcShared& S, int l, int n, int t) {
+  if (t >= kTriNP) return;
+  S.stage_pos[t] = -1;
+  if (t >= n) return;
+  const int s2 = dc_segment_of(S, l, t);
+  const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1];
+  const double lam = S.d[t];
+  int r = 0;
+  for (int j = a; j < b; j++) {
+    const double v = S.d[j];
+    r += (v < lam || (v == lam && j < t)) ? 1 : 0;
+  }
+  S.stage_pos[t] = a + r;
+  S.stage_lam[t] = lam;
+  for (int q = 0; q < 6; q++) S.stage_x[q][t] = S.X[q][t];
+}
+
+__host__ __device__ inline void dc_phase_m6b(DcShared& S, int t) {
+  if (t >= kTriNP) return;
+  const int pos = S.stage_pos[t];
+  if (pos < 0) return;
+  S.d[pos] = S.stage_lam[t];
+  for (int q = 0; q < 6; q++) S.X[q][pos] = S.stage_x[q][t];
+}
+
+// output: eigenvalues and g_k = a_k * b_k (a = transformed conj(coil), b = transformed rho0)
+__host__ __device__ inline void dc_phase_store(const DcShared& S, double* gsrc, int n, int tid, int nthreads) {
+  for (int k = tid; k < n; k += nthreads) {
+    const double ar = S.X[0][k], ai = S.X[1][k], br = S.X[2][k], bi = S.X[3][k];
+    gsrc[k] = S.d[k];
     gsrc[2 * kTriNP + k] = ar * br - ai * bi;
     gsrc[3 * kTriNP + k] = ar * bi + ai * br;
   }
-  p.status[it] = failed ? -1 : sweeps;
+}
+
+__global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
+  __shared__ DcShared S;
+  const int tid = threadIdx.x;
+  const int64_t it = blockIdx.x;
+  if (it >= p.n_items) return;
+  const int n = p.sys[p.items[it].sys].n;
+  double* gsrc = p.spec + (size_t)it * kSpecStride;
+  dc_phase_load(S, gsrc, tid, kDcThreads);
+  __syncthreads();
+  if (tid == 0) dc_phase_plan(S, n);
+  __syncthreads();
+  dc_phase_leaf(S, tid);
+  __syncthreads();
+  for (int l = S.nlev - 1; l >= 0; l--) {
+    dc_phase_m1(S, l, n, tid);
+    __syncthreads();
+    dc_phase_m2(S, l, tid);
+    __syncthreads();
+    dc_phase_m3(S, l, n, tid);
+    __syncthreads();
+    dc_phase_m4(S, l, n, tid);
+    __syncthreads();
+    dc_phase_m5(S, l, n, tid);
+    __syncthreads();
+    if (l > 0) {
+      dc_phase_m6a(S, l, n, tid);
+      __syncthreads();
+      dc_phase_m6b(S, tid);
+      __syncthreads();
+    }
+  }
+  dc_phase_store(S, gsrc, n, tid, kDcThreads);
+  __syncthreads();
+  if (tid == 0) {
+    int mx = 0;
+    for (int i = 0; i < kTriNP; i++) mx = (S.its[i] > mx) ? S.its[i] : mx;
+    p.status[it] = S.fail ? -1 : mx;
+  }
 }
 
 // ------------------------------------------------------------ k_synth

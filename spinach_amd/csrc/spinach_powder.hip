@@ -641,8 +641,7 @@ void launch_spectral(sp_plan* pl, hipEvent_t* ev) {
   hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[0], st));
-  const int nb = (int)((pl->n_spec + spk::kTqlLanes - 1) / spk::kTqlLanes);
-  hipLaunchKernelGGL(spk::k_tql, dim3(nb), dim3(64 * spk::kTqlWaves), 0, st, tri);
+  hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[1], st));
   spk::SpecParams syn = spec_params(pl, pl->d_work_syn.as<double>(), pl->syn_stride);
@@ -939,7 +938,7 @@ int run(sp_plan* pl, sp_stats* stats) {
   pl->last = s;
   if (stats) *stats = s;
   if (eig_failures > 0)
-    throw SpError(SP_ERANGE, "the tridiagonal eigenvalue iteration did not converge for " +
+    throw SpError(SP_ERANGE, "the tridiagonal eigensolver did not converge for " +
                                  std::to_string(eig_failures) + " orientation(s)");
   return SP_OK;
 }

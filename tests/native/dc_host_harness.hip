@@ -1,0 +1,34 @@
+// Host build of the divide-and-conquer eigensolver of the spectral path: the
+// same phase functions as the k_dceig kernel, run phase by phase with a loop
+// over thread ids (a phase's threads touch disjoint data, so this order is
+// one valid schedule). Built by tests/test_dc_host.py; not part of the library.
+#include "../../spinach_amd/csrc/spectral_kernels.hpp"
+
+using namespace spk;
+
+extern "C" int dc_host_solve(const double* in, int n, double* out, int* status, int* levels) {
+  static DcShared S;
+  if (n < 0 || n > kTriNP) return 1;
+  const int T = kDcThreads;
+  for (int i = 0; i < kSpecStride; i++) out[i] = in[i];
+  for (int t = 0; t < T; t++) dc_phase_load(S, out, t, T);
+  dc_phase_plan(S, n);
+  *levels = S.nlev;
+  for (int t = 0; t < T; t++) dc_phase_leaf(S, t);
+  for (int l = S.nlev - 1; l >= 0; l--) {
+    for (int t = 0; t < T; t++) dc_phase_m1(S, l, n, t);
+    for (int t = 0; t < T; t++) dc_phase_m2(S, l, t);
+    for (int t = 0; t < T; t++) dc_phase_m3(S, l, n, t);
+    for (int t = 0; t < T; t++) dc_phase_m4(S, l, n, t);
+    for (int t = 0; t < T; t++) dc_phase_m5(S, l, n, t);
+    if (l > 0) {
+      for (int t = 0; t < T; t++) dc_phase_m6a(S, l, n, t);
+      for (int t = 0; t < T; t++) dc_phase_m6b(S, t);
+    }
+  }
+  for (int t = 0; t < T; t++) dc_phase_store(S, out, n, t, T);
+  int mx = 0;
+  for (int i = 0; i < kTriNP; i++) mx = S.its[i] > mx ? S.its[i] : mx;
+  *status = S.fail ? -1 : mx;
+  return 0;
+}
