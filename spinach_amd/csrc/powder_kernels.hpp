@@ -890,21 +890,49 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
   }
 }
 
-// Grid-weighted orientation sum in grid order, FP64 (powder.m:280-286).
-__global__ void k_weighted_sum(const double* __restrict__ per_or, const double* __restrict__ weights,
-                               int64_t or_off, int64_t n_or, int64_t npts, double* __restrict__ fid) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Grid-weighted orientation sum (kernel/contexts/powder.m:280-286) in two
+// fixed-order FP64 passes that fill the GPU: chunks of up to kSumChunk
+// consecutive orientations of one system (one grid row per chunk, one thread
+// per time point), then the chunks of each system in order.
+constexpr int kSumChunk = 64;
+
+struct SumChunk {
+  int64_t first;  // global index of the chunk's first orientation
+  int32_t count;
+  int32_t pad;
+};
+
+__global__ void k_chunk_sum(const double* __restrict__ per_or, const double* __restrict__ weights,
+                            const SumChunk* __restrict__ chunks, int64_t npts, double* __restrict__ partial) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npts) return;
-  const double* base = per_or + 2 * (size_t)or_off * npts;
-  double sr = weights[or_off] * base[2 * t];
-  double si = weights[or_off] * base[2 * t + 1];
-  for (int64_t k = 1; k < n_or; k++) {
-    double w = weights[or_off + k];
-    sr += w * base[2 * ((size_t)k * npts + t)];
-    si += w * base[2 * ((size_t)k * npts + t) + 1];
+  const SumChunk ch = chunks[blockIdx.y];
+  double sr = 0.0, si = 0.0;
+  for (int k = 0; k < ch.count; k++) {
+    const int64_t o = ch.first + k;
+    const double w = weights[o];
+    const double* f = per_or + 2 * ((size_t)o * npts + t);
+    sr = fma(w, f[0], sr);
+    si = fma(w, f[1], si);
   }
-  fid[2 * t] = sr;
-  fid[2 * t + 1] = si;
+  double* out = partial + 2 * ((size_t)blockIdx.y * npts + t);
+  out[0] = sr;
+  out[1] = si;
+}
+
+__global__ void k_system_sum(const double* __restrict__ partial, const int64_t* __restrict__ chunk_off, int64_t npts,
+                             double* __restrict__ fid) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  const int64_t s = blockIdx.y;
+  double sr = 0.0, si = 0.0;
+  for (int64_t c = chunk_off[s]; c < chunk_off[s + 1]; c++) {
+    const double* f = partial + 2 * ((size_t)c * npts + t);
+    sr += f[0];
+    si += f[1];
+  }
+  fid[2 * ((size_t)s * npts + t)] = sr;
+  fid[2 * ((size_t)s * npts + t) + 1] = si;
 }
 
 }  // namespace spk

@@ -16,7 +16,7 @@
 //              matrix-vector products and rank-2 updates of every step run
 //              from registers with the Householder vectors broadcast from LDS.
 //   k_dceig    eigenvalues of T and the projections g_k by divide and
-//              conquer (QR on leaves of <= 12 rows, secular-equation merges
+//              conquer (QR on leaves of <= 6 rows, secular-equation merges
 //              with one thread per root); the eigenvectors are never formed.
 //   k_synth    f(64*i + j) = sum_k [g_k e^{-i lambda_k 64 i}] [e^{-i lambda_k j}]
 //              as one complex matrix product per 64 x 64 block of time
@@ -28,9 +28,10 @@
 namespace spk {
 
 constexpr int kTriNP = 192;      // largest padded dimension of the register layout
-constexpr int kTriThreads = 512;
-constexpr int kTriRowsPer = 3;   // kTriNP / 64 row groups
-constexpr int kTriColsPer = 24;  // kTriNP / 8 column groups
+constexpr int kTriTC = 8;                   // column groups of the register layout (8 measured faster than 16)
+constexpr int kTriThreads = 64 * kTriTC;    // 64 row groups x kTriTC column groups
+constexpr int kTriRowsPer = kTriNP / 64;    // rows per thread (3)
+constexpr int kTriColsPer = kTriNP / kTriTC;  // columns per thread
 constexpr int kSpecStride = 6 * kTriNP;  // per item: d | e | four vectors
 constexpr int kSynthB = 64;      // time block of the synthesis product
 
@@ -57,7 +58,14 @@ struct SpecParams {
   double* spec;          // kSpecStride doubles per spectral item
   int32_t* status;       // per item: QR sweeps used, or -1 if the iteration did not converge
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
+  unsigned long long* tri_cycles;  // optional: per workgroup, kTriPhases clock64() totals of k_tridiag
 };
+
+// phases of k_tridiag timed by the optional cycle counters: prologue
+// (assembly, register load), Householder vector + matrix-vector product +
+// vector transforms (to the mid-step barrier), rank-2 update + next row (to
+// the end-of-step barrier), epilogue
+constexpr int kTriPhases = 4;
 
 // Scatter (sign = +1) or clear (sign = 0) the real symmetric generator
 // A = dt*(x + x^T)/2 of one orientation into a dense row-major np x np buffer.
@@ -105,6 +113,14 @@ __device__ __forceinline__ double sum8(double v) {
   return v;
 }
 
+// sum over each aligned group of kTriTC lanes (one row group of the layout)
+__device__ __forceinline__ double sum_tc(double v) {
+  v = sum8(v);
+  if constexpr (kTriTC == 16) v += dpp_f64<0x140>(v);  // row_mirror: the other 8-group
+  static_assert(kTriTC == 8 || kTriTC == 16, "column groups: 8 or 16");
+  return v;
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
@@ -120,7 +136,7 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 
 // Row jj of the register-resident matrix into xb and the sum of squares of
 // its entries right of the first subdiagonal position into *sig. The owners
-// are the eight threads of row group jj % 64 (one aligned 8-lane group);
+// are the kTriTC threads of row group jj % 64 (one aligned lane group);
 // every thread calls this (the DPP sum needs the whole wave active). Indices
 // into R are compile-time constants, so R stays in registers.
 __device__ __forceinline__ void extract_row(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr, int tc,
@@ -132,15 +148,18 @@ __device__ __forceinline__ void extract_row(const double (&R)[kTriRowsPer][kTriC
 #pragma unroll
       for (int b = 0; b < kTriColsPer; b++) {
         const double v = R[a][b];
-        xb[tc + 8 * b] = v;
-        if (tc + 8 * b >= jj + 2) part = fma(v, v, part);
+        xb[tc + kTriTC * b] = v;
+        if (tc + kTriTC * b >= jj + 2) part = fma(v, v, part);
       }
     }
   }
-  part = sum8(part);
+  part = sum_tc(part);
   if (tr == (jj & 63) && tc == 0) *sig = part;
 }
 
+// kTiming: the variant with the cycle counters (SPINACH_PHASE_TIMING); the
+// production instantiation carries none of their state.
+template <bool kTiming>
 __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
   __shared__ double sx[2][kTriNP];  // rows j (Householder source), double-buffered
   __shared__ double ssig[2];
@@ -152,7 +171,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
   __shared__ int sitem;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int tr = tid >> 3, tc = tid & 7;  // row group 0..63, column group 0..7
+  const int tr = tid / kTriTC, tc = tid % kTriTC;  // row group 0..63, column group
   double* W = p.work + (size_t)blockIdx.x * p.work_stride;
 
   for (;;) {
@@ -165,6 +184,16 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     const SysDesc sdesc = p.sys[item.sys];
     const int n = sdesc.n, np = sdesc.np;
     const int64_t o = item.orient;
+    long long t_mark = kTiming ? clock64() : 0;
+    auto mark = [&](int ph) {
+      if constexpr (kTiming) {
+        if (tid == 0) {
+          const long long t = clock64();
+          p.tri_cycles[(size_t)blockIdx.x * kTriPhases + ph] += (unsigned long long)(t - t_mark);
+          t_mark = t;
+        }
+      }
+    };
     if (tid == 0) wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], cre, cim);
     __syncthreads();
     scatter_real_generator(p, sdesc, cre, cim, W, false);
@@ -193,7 +222,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     for (int a = 0; a < kTriRowsPer; a++)
 #pragma unroll
       for (int b = 0; b < kTriColsPer; b++) {
-        const int i = tr + 64 * a, j = tc + 8 * b;
+        const int i = tr + 64 * a, j = tc + kTriTC * b;
         R[a][b] = (i < np && j < np) ? W[(size_t)i * np + j] : 0.0;
       }
     __syncthreads();
@@ -201,6 +230,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     scatter_real_generator(p, sdesc, cre, cim, W, true);
     extract_row(R, 0, tr, tc, sx[0], &ssig[0]);
     __syncthreads();
+    mark(0);
 
     for (int j = 0; j < n - 2; j++) {
       const int cb = j & 1;
@@ -227,14 +257,14 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           const double xi = x[i];
           return (i > j + 1) ? xi : ((i == j + 1) ? v1 : 0.0);
         };
-        // p = tau*A*v on the trailing rows, in chunks of 4 column groups (32 columns)
+        // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread
         double part[kTriRowsPer] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int bc = 0; bc < kTriColsPer / 4; bc++) {
-          if (32 * bc + 31 > j) {
+          if (4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
             double vb[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) vb[q] = vat(tc + 8 * (4 * bc + q));
+            for (int q = 0; q < 4; q++) vb[q] = vat(tc + kTriTC * (4 * bc + q));
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
@@ -245,13 +275,13 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
 #pragma unroll
         for (int a = 0; a < kTriRowsPer; a++) {
           const int i = tr + 64 * a;
-          double s = sum8(part[a]) * tau;
+          double s = sum_tc(part[a]) * tau;
           if (i <= j) s = 0.0;
           if (tc == 0) sp[i] = s;
           qsum = fma(vat(i), s, qsum);
         }
-        // the eight lanes of a group hold the same qsum
-        const double qw = wave_sum_dpp(qsum) * 0.125;
+        // the kTriTC lanes of a group hold the same qsum
+        const double qw = wave_sum_dpp(qsum) * (1.0 / kTriTC);
         if (lane == 0) sq[wave] = qw;
         // Q^T applied to conj(coil) and rho0, one vector per wave: u <- u - tau*v*(v'u)
         if (wave < 4) {
@@ -270,6 +300,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
             if (vv[r] != 0.0) su[wave][lane + 64 * r] = uu[r] - s * vv[r];
         }
         __syncthreads();
+        mark(1);
         // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
         double qt = 0.0;
 #pragma unroll
@@ -284,11 +315,11 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         }
 #pragma unroll
         for (int bc = 0; bc < kTriColsPer / 4; bc++) {
-          if (32 * bc + 31 > j) {
+          if (4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
             double vb[4], wb[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-              const int c = tc + 8 * (4 * bc + q);
+              const int c = tc + kTriTC * (4 * bc + q);
               vb[q] = vat(c);
               wb[q] = sp[c] - K * vb[q];
             }
@@ -303,6 +334,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       // the owners of row j+1 publish it (after their own update) for the next step
       extract_row(R, j + 1, tr, tc, sx[cb ^ 1], &ssig[cb ^ 1]);
       __syncthreads();
+      mark(2);
     }
     // trailing 2 x 2 (or 1 x 1) block: row max(n-2, 0) is in its buffer; publish row n-1
     if (n >= 2) {
@@ -325,6 +357,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       for (int q = 0; q < 4; q++) out[(2 + q) * kTriNP + i] = su[q][i];
     }
     __syncthreads();
+    mark(3);
   }
 }
 
@@ -343,7 +376,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
 // and imaginary parts) and its first and last eigenvector rows (needed by the
 // parent merge). One 256-thread workgroup per orientation, everything in LDS.
 constexpr int kDcThreads = 256;
-constexpr int kDcLeaf = 12;
+constexpr int kDcLeaf = 6;
 constexpr int kDcMaxLevels = 5;  // up to 32 leaves
 constexpr int kDcMaxIter = 100;  // secular-equation iterations per root (typically 3-8)
 
@@ -353,9 +386,6 @@ struct DcShared {
   double Y[6][kTriNP];  // merge input, sorted
   double D[kTriNP], z[kTriNP];
   double Dc[kTriNP], Zc[kTriNP], zh[kTriNP], tau[kTriNP];
-  double stage_lam[kTriNP];     // per-thread results of M5/M6 between their two halves
-  double stage_x[6][kTriNP];
-  int stage_pos[kTriNP];
   int org[kTriNP], keep[kTriNP], defl[kTriNP];
   int bnd[kDcMaxLevels + 1][(1 << kDcMaxLevels) + 1];  // segment boundaries per level
   double beta[(1 << kDcMaxLevels)];                     // rho = 2|coupling| of the merges of a level
@@ -364,6 +394,19 @@ struct DcShared {
   int nlev;
   int fail;
 };
+
+// 1/x: on the device v_rcp_f64 refined by two Newton steps (full double
+// precision from the hardware estimate), on the host a division
+__host__ __device__ inline double dc_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+#else
+  return 1.0 / x;
+#endif
+}
 
 // The kernel is a sequence of phases separated by workgroup barriers; within
 // a phase every thread works on its own data. Each phase is a
@@ -478,7 +521,7 @@ __host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z
     double f = 1.0 / rho, wabs = 0.0, psi = 0.0, psip = 0.0, phip = 0.0;
     for (int i = 0; i < K; i++) {
       const double delta = (Dc[i] - dorg) - tau;
-      const double rd = 1.0 / delta;
+      const double rd = dc_rcp(delta);
       const double w = Zc[i] * Zc[i] * rd;
       f += w;
       wabs += fabs(w);
@@ -676,7 +719,7 @@ __host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
   double w = 1.0;
   for (int k = 0; k < K; k++) {
     const double lmd = (S.Dc[a + S.org[a + k]] - di) + S.tau[a + k];  // lambda_k - d_i
-    w *= (k != i) ? lmd / (S.Dc[a + k] - di) : lmd;
+    w *= (k != i) ? lmd * dc_rcp(S.Dc[a + k] - di) : lmd;
   }
   const double zz = sqrt(fmax(w, 0.0) / S.beta[s2]);
   S.zh[a + i] = (S.Zc[a + i] >= 0.0) ? zz : -zz;
@@ -695,7 +738,7 @@ __host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
     double nrm = 0.0;
     for (int q = 0; q < 6; q++) out[q] = 0.0;
     for (int i = 0; i < K; i++) {
-      const double u = S.zh[a + i] / ((S.Dc[a + i] - dorg) - tu);
+      const double u = S.zh[a + i] * dc_rcp((S.Dc[a + i] - dorg) - tu);
       nrm = fma(u, u, nrm);
       const int src = S.keep[a + i];
       for (int q = 0; q < 6; q++) out[q] = fma(u, S.Y[q][src], out[q]);
@@ -713,9 +756,10 @@ __host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
 }
 
 // M6a/M6b: rank sort of each merged segment (the rank into stage_*, then the scatter)
+// (the ranks and copies are staged in org, D and Y, which the merge no longer needs)
 __host__ __device__ inline void dc_phase_m6a(DcShared& S, int l, int n, int t) {
   if (t >= kTriNP) return;
-  S.stage_pos[t] = -1;
+  S.org[t] = -1;
   if (t >= n) return;
   const int s2 = dc_segment_of(S, l, t);
   const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1];
@@ -725,17 +769,17 @@ __host__ __device__ inline void dc_phase_m6a(DcShared& S, int l, int n, int t) {
     const double v = S.d[j];
     r += (v < lam || (v == lam && j < t)) ? 1 : 0;
   }
-  S.stage_pos[t] = a + r;
-  S.stage_lam[t] = lam;
-  for (int q = 0; q < 6; q++) S.stage_x[q][t] = S.X[q][t];
+  S.org[t] = a + r;
+  S.D[t] = lam;
+  for (int q = 0; q < 6; q++) S.Y[q][t] = S.X[q][t];
 }
 
 __host__ __device__ inline void dc_phase_m6b(DcShared& S, int t) {
   if (t >= kTriNP) return;
-  const int pos = S.stage_pos[t];
+  const int pos = S.org[t];
   if (pos < 0) return;
-  S.d[pos] = S.stage_lam[t];
-  for (int q = 0; q < 6; q++) S.X[q][pos] = S.stage_x[q][t];
+  S.d[pos] = S.D[t];
+  for (int q = 0; q < 6; q++) S.X[q][pos] = S.Y[q][t];
 }
 
 // output: eigenvalues and g_k = a_k * b_k (a = transformed conj(coil), b = transformed rho0)

@@ -367,6 +367,10 @@ struct sp_plan {
   int grid_tri = 0, grid_syn = 0;
   int64_t tri_stride = 0, syn_stride = 0;
   DevBuf d_spec, d_status, d_work_tri, d_work_syn;
+  DevBuf d_tri_phase;  // optional k_tridiag cycle counters (SPINACH_PHASE_TIMING=1)
+  // orientation sum: chunks of consecutive orientations, chunk ranges per system
+  int64_t n_chunks = 0;
+  DevBuf d_chunks, d_chunk_off, d_partial;
   DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
   DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
   bool executed = false;
@@ -386,6 +390,23 @@ int fail(sp_ctx* ctx, int code, const std::string& msg) {
     ctx->code = code;
   }
   return code;
+}
+
+// Chunk table and partial-sum buffer of the two-pass orientation sum.
+void setup_sum(sp_plan* pl, hipStream_t st) {
+  if (pl->n_systems > 65535) throw SpError(SP_ERANGE, "at most 65535 systems per plan");
+  std::vector<spk::SumChunk> chunks;
+  std::vector<int64_t> off(1, 0);
+  for (int64_t s = 0; s < pl->n_systems; s++) {
+    for (int64_t k = 0; k < pl->n_or[s]; k += spk::kSumChunk)
+      chunks.push_back(spk::SumChunk{pl->or_off[s] + k, (int32_t)std::min<int64_t>(spk::kSumChunk, pl->n_or[s] - k), 0});
+    off.push_back((int64_t)chunks.size());
+  }
+  if (chunks.size() > 65535) throw SpError(SP_ERANGE, "too many orientations for one plan");
+  pl->n_chunks = (int64_t)chunks.size();
+  upload(pl->d_chunks, chunks, st);
+  upload(pl->d_chunk_off, off, st);
+  pl->d_partial.alloc((size_t)std::max<int64_t>(1, pl->n_chunks) * (pl->nsteps + 1) * 2 * sizeof(double));
 }
 
 // Time-block length B = 2^l of the two-level power scheme, from a cost model
@@ -567,8 +588,11 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->syn_stride = (4 * (int64_t)np_spec * spk::kSynthB + 63) / 64 * 64;
     pl->d_work_syn.alloc((size_t)pl->grid_syn * pl->syn_stride * sizeof(double));
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
+    if (const char* e = getenv("SPINACH_PHASE_TIMING"))
+      if (atoi(e) > 0) pl->d_tri_phase.alloc((size_t)pl->grid_tri * spk::kTriPhases * sizeof(unsigned long long));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
+  setup_sum(pl.get(), st);
   HIPCHK(hipStreamSynchronize(st));
   return pl.release();
 }
@@ -631,6 +655,7 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
   sp.spec = pl->d_spec.as<double>();
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
+  sp.tri_cycles = pl->d_tri_phase.p ? pl->d_tri_phase.as<unsigned long long>() : nullptr;
   return sp;
 }
 
@@ -638,7 +663,10 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
 void launch_spectral(sp_plan* pl, hipEvent_t* ev) {
   hipStream_t st = pl->ctx->stream;
   spk::SpecParams tri = spec_params(pl, pl->d_work_tri.as<double>(), pl->tri_stride);
-  hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
+  if (tri.tri_cycles)
+    hipLaunchKernelGGL(spk::k_tridiag<true>, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
+  else
+    hipLaunchKernelGGL(spk::k_tridiag<false>, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
@@ -799,6 +827,7 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   pl->d_per.alloc((size_t)n_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)npts * 2 * sizeof(double));
   pl->d_sq.alloc((size_t)n_or * sizeof(int32_t));
+  setup_sum(pl.get(), st);
   HIPCHK(hipStreamSynchronize(st));
   return pl.release();
 }
@@ -847,6 +876,8 @@ int run(sp_plan* pl, sp_stats* stats) {
   HIPCHK(hipGetLastError());
   if (pl->d_phase.p)
     HIPCHK(hipMemsetAsync(pl->d_phase.p, 0, pl->d_phase.bytes, st));
+  if (pl->d_tri_phase.p)
+    HIPCHK(hipMemsetAsync(pl->d_tri_phase.p, 0, pl->d_tri_phase.bytes, st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   if (pl->mas) {
     if (pl->mode == SP_MODE_FP64)
@@ -869,11 +900,13 @@ int run(sp_plan* pl, sp_stats* stats) {
     }
   }
   HIPCHK(hipEventRecord(ctx->ev2, st));
-  for (int64_t s = 0; s < pl->n_systems; s++) {
-    int nb = (int)((npts + 255) / 256);
-    hipLaunchKernelGGL(spk::k_weighted_sum, dim3(nb), dim3(256), 0, st, pl->d_per.as<double>(),
-                       pl->d_w.as<double>(), pl->or_off[s], pl->n_or[s], npts,
-                       pl->d_fid.as<double>() + 2 * (size_t)s * npts);
+  {
+    const unsigned nb = (unsigned)((npts + 255) / 256);
+    hipLaunchKernelGGL(spk::k_chunk_sum, dim3(nb, (unsigned)pl->n_chunks), dim3(256), 0, st, pl->d_per.as<double>(),
+                       pl->d_w.as<double>(), pl->d_chunks.as<spk::SumChunk>(), npts, pl->d_partial.as<double>());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(spk::k_system_sum, dim3(nb, (unsigned)pl->n_systems), dim3(256), 0, st,
+                       pl->d_partial.as<double>(), pl->d_chunk_off.as<int64_t>(), npts, pl->d_fid.as<double>());
     HIPCHK(hipGetLastError());
   }
   hipEvent_t evend;
@@ -930,6 +963,17 @@ int run(sp_plan* pl, sp_stats* stats) {
     for (int k = 0; k < spk::kPhases; k++) {
       unsigned long long t = 0;
       for (int g = 0; g < pl->grid; g++) t += ph[(size_t)g * spk::kPhases + k];
+      fprintf(stderr, " %llu", t);
+    }
+    fprintf(stderr, "\n");
+  }
+  if (pl->d_tri_phase.p) {
+    std::vector<unsigned long long> ph((size_t)pl->grid_tri * spk::kTriPhases);
+    HIPCHK(hipMemcpy(ph.data(), pl->d_tri_phase.p, ph.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    fprintf(stderr, "[spinach_powder] k_tridiag cycles summed over %d workgroups:", pl->grid_tri);
+    for (int k = 0; k < spk::kTriPhases; k++) {
+      unsigned long long t = 0;
+      for (int g = 0; g < pl->grid_tri; g++) t += ph[(size_t)g * spk::kTriPhases + k];
       fprintf(stderr, " %llu", t);
     }
     fprintf(stderr, "\n");
