@@ -858,28 +858,54 @@ __global__ void __launch_bounds__(kThreads, 2) k_synth(SpecParams p) {
     double* Wim = ws + (size_t)np * kSynthB;
     double* Ure = ws + 2 * (size_t)np * kSynthB;
     double* Uim = Ure + (size_t)kSynthB * np;
-    for (int e = tid; e < np * kSynthB; e += kThreads) {
-      const int k = e / kSynthB, j = e % kSynthB;
-      double c = 0.0, s = 0.0;
-      if (k < n) sincos(lam[k] * (double)j, &s, &c);
-      Wre[e] = c;
-      Wim[e] = -s;
+    // phases by angle addition: one thread per (k, group of 8 consecutive
+    // exponents) takes sincos at the group's first exponent and at the step,
+    // then multiplies (at most 7 rounded products per entry)
+    for (int e = tid; e < np * (kSynthB / 8); e += kThreads) {
+      const int k = e / (kSynthB / 8), a8 = e % (kSynthB / 8);
+      double br = 0.0, bi = 0.0, sr = 1.0, si = 0.0;
+      if (k < n) {
+        double s, c;
+        sincos(lam[k] * (double)(8 * a8), &s, &c);
+        br = c;
+        bi = -s;
+        sincos(lam[k], &s, &c);
+        sr = c;
+        si = -s;
+      }
+      for (int b = 0; b < 8; b++) {
+        const size_t o2 = (size_t)k * kSynthB + 8 * a8 + b;
+        Wre[o2] = br;
+        Wim[o2] = bi;
+        const double t = br * sr - bi * si;
+        bi = br * si + bi * sr;
+        br = t;
+      }
     }
     for (int64_t i0 = 0; i0 < nfull; i0 += kSynthB) {
       const int rows = (int)((nfull - i0 < kSynthB) ? (nfull - i0) : kSynthB);
       __syncthreads();
       // U[i][k] = g_k exp(-i lambda_k 64 (i0 + i))
-      for (int e = tid; e < rows * np; e += kThreads) {
-        const int i = e / np, k = e % np;
-        double ur = 0.0, ui = 0.0;
+      for (int e = tid; e < np * ((rows + 7) / 8); e += kThreads) {
+        const int k = e % np, a8 = e / np;
+        double br = 0.0, bi = 0.0, sr = 1.0, si = 0.0;
         if (k < n) {
           double s, c;
-          sincos(lam[k] * (double)(kSynthB * (i0 + i)), &s, &c);
-          ur = gre[k] * c + gim[k] * s;
-          ui = gim[k] * c - gre[k] * s;
+          sincos(lam[k] * (double)(kSynthB * (i0 + 8 * a8)), &s, &c);
+          br = gre[k] * c + gim[k] * s;
+          bi = gim[k] * c - gre[k] * s;
+          sincos(lam[k] * (double)kSynthB, &s, &c);
+          sr = c;
+          si = -s;
         }
-        Ure[e] = ur;
-        Uim[e] = ui;
+        for (int b = 0; b < 8 && 8 * a8 + b < rows; b++) {
+          const size_t o2 = (size_t)(8 * a8 + b) * np + k;
+          Ure[o2] = br;
+          Uim[o2] = bi;
+          const double t = br * sr - bi * si;
+          bi = br * si + bi * sr;
+          br = t;
+        }
       }
       __syncthreads();
       double* ob = out + 2 * (size_t)i0 * kSynthB;
