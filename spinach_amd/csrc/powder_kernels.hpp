@@ -102,7 +102,9 @@ struct KParams {
 // phases timed by the optional per-workgroup cycle counters
 constexpr int kPhases = 5;  // assemble+norm, expm, powers+U block, V chain, detection
 
-// zeroes the kQueues work-queue heads of a plan
+// zeroes the kQueues work-queue heads of a plan: [0] propagator items with a
+// complex generator, [1] tridiagonalisation, [2] synthesis, [3] propagator
+// items with a real generator
 constexpr int kQueues = 4;
 __global__ void k_reset(unsigned int* q) {
   if (blockIdx.x == 0 && threadIdx.x < kQueues) q[threadIdx.x] = 0u;
@@ -626,7 +628,10 @@ __device__ Buf<R> wg_expm_real(Buf<R>* bf, int sexp, int q, int K, const double*
 }
 
 // --------------------------------------------------------- the fused kernel
-template <typename R>
+// REAL: the real-generator instantiation (real-product expm and commuting
+// squarings); each instantiation carries one variant, so the host launches the
+// real and the complex items of a plan separately.
+template <typename R, bool REAL>
 __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
   __shared__ SharedAll<R> sh;
   const int tid = threadIdx.x;
@@ -697,7 +702,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
         }
         double hr = 0.5 * (xr + tr), hi = 0.5 * (xi - ti);
         size_t off = (size_t)prow[e] * np + pcol[e];
-        if (sd.real) {
+        if constexpr (REAL) {
           X[off] = (R)(p.dt * hr);  // A = dt*H; the imaginary parts cancel pairwise
         } else {
           X[off] = (R)(p.dt * hi);
@@ -734,8 +739,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
     }
     mark(0);
     // 4. P = expm(X): Paterson-Stockmeyer Taylor polynomial and s squarings
-    Buf<R> cur = sd.real ? wg_expm_real<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g)
-                         : wg_expm<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
+    Buf<R> cur;
+    if constexpr (REAL)
+      cur = wg_expm_real<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
+    else
+      cur = wg_expm<R>(bf, sexp, p.ps_q, p.taylor_k, sh.tc, sh.g);
     Buf<R> spare[2];
     {
       int ns = 0;
@@ -785,7 +793,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
       const bool last = (k == p.log2b - 1);
       Buf<R> dst = last ? QT : spare[sp_i];
       if (!last && dst.p == QT.p) dst = spare[sp_i ^ 1];
-      if (sd.real) {
+      if constexpr (REAL) {
         wg_square_commuting<R>(cur, dst, last, sh.g);
       } else {
         R* dre = dst.re();

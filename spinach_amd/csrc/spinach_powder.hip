@@ -363,6 +363,7 @@ struct sp_plan {
   int32_t real_systems = 0;
   // spectral path (real generators, FP64, padded dimension <= spk::kTriNP)
   int64_t n_prop = 0, n_spec = 0;
+  int64_t n_prop_complex = 0;  // the first n_prop_complex propagator items have a complex generator
   int32_t spectral_systems = 0;
   int grid_tri = 0, grid_syn = 0;
   int64_t tri_stride = 0, syn_stride = 0;
@@ -523,8 +524,17 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   auto by_cost = [](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
     return a.first > b.first;
   };
-  std::stable_sort(prop_items.begin(), prop_items.end(), by_cost);
+  // propagator items: complex generators first, then real ones (two launches)
+  std::stable_sort(prop_items.begin(), prop_items.end(),
+                   [&](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
+                     const int ra = descs[a.second.sys].real, rb = descs[b.second.sys].real;
+                     if (ra != rb) return ra < rb;
+                     return a.first > b.first;
+                   });
   std::stable_sort(spec_items.begin(), spec_items.end(), by_cost);
+  pl->n_prop_complex = 0;
+  for (auto& c : prop_items)
+    if (!descs[c.second.sys].real) pl->n_prop_complex++;
   std::vector<spk::Item> ordered;
   ordered.reserve(prop_items.size() + spec_items.size());
   for (auto& c : prop_items) ordered.push_back(c.second);
@@ -558,9 +568,9 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     // persistent grid: resident workgroups, bounded by the item count
     int per_cu = 0;
     if (mode == SP_MODE_FP64)
-      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<double>, spk::kThreads, 0));
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<double, false>, spk::kThreads, 0));
     else
-      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<float>, spk::kThreads, 0));
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spk::k_powder<float, false>, spk::kThreads, 0));
     // the occupancy query is conservative for this kernel; at least two
     // workgroups per CU hide the LDS-staging latency (override:
     // SPINACH_WG_PER_CU). Workgroups beyond residency find the queue drained.
@@ -628,8 +638,22 @@ void launch(sp_plan* pl) {
   kp.per_or = pl->d_per.as<double>();
   kp.sq_out = pl->d_sq.as<int32_t>();
   kp.phase_cycles = pl->d_phase.p ? pl->d_phase.as<unsigned long long>() : nullptr;
-  hipLaunchKernelGGL(spk::k_powder<R>, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
-  HIPCHK(hipGetLastError());
+  // complex-generator items first (queue 0), then real-generator items (queue 3)
+  const int64_t n_cplx = pl->n_prop_complex, n_real = pl->n_prop - pl->n_prop_complex;
+  if (n_cplx > 0) {
+    kp.n_items = n_cplx;
+    const int g = (int)std::min<int64_t>(pl->grid, n_cplx);
+    hipLaunchKernelGGL((spk::k_powder<R, false>), dim3(g), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
+    HIPCHK(hipGetLastError());
+  }
+  if (n_real > 0) {
+    kp.items = pl->d_items.as<spk::Item>() + n_cplx;
+    kp.n_items = n_real;
+    kp.queue = pl->d_queue.as<unsigned int>() + 3;
+    const int g = (int)std::min<int64_t>(pl->grid, n_real);
+    hipLaunchKernelGGL((spk::k_powder<R, true>), dim3(g), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
@@ -807,7 +831,7 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   upload(pl->d_w, std::vector<double>(w, w + n_or), st);
   upload(pl->d_rot2lab, rot2lab, st);
   upload(pl->d_lab2rot, lab2rot, st);
-  pl->d_queue.alloc(sizeof(unsigned int));
+  pl->d_queue.alloc(spk::kQueues * sizeof(unsigned int));
   pl->n_items = n_or;
   pl->total_or = n_or;
   pl->or_off.push_back(0);
