@@ -36,27 +36,38 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'orientation-steps/s on 1 MI355X at ≤1e-6 rel FID error vs FP64; surrogate speedup'
 PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12}  # MI355X dense MFMA FP64 / FP32, FLOP/s
-# HBM bytes per launch of the propagation kernel, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE runs of this script (counters scaled by a device copy of known
-# size: FETCH x2.000, WRITE x1.000); see profiles/k_powder_static8spin_fp64.md
-TRAFFIC = {('static8', 'fp64', 'spk::k_powder'): 165512902 * 1024 * 2.0 + 53150294 * 1024 * 1.0}
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script (counters scaled by a device copy of known size:
+# FETCH x2.000, WRITE x1.000); see profiles/k_spectral_static8spin_fp64.md and
+# profiles/k_powder_static8spin_fp64.md
+TRAFFIC = {
+    ('static8', 'fp64', 'spk::k_tridiag'): 600112 * 1024 * 2.0 + 286884 * 1024 * 1.0,
+    ('static8', 'fp64', 'spk::k_powder<double,false>'): 165530392 * 1024 * 2.0 + 53150413 * 1024 * 1.0,
+    ('static8', 'fp64', 'spk::k_powder<double,true>'): 139595089 * 1024 * 2.0 + 49487957 * 1024 * 1.0,
+}
 
 
 def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
-def powder_flops_per_orientation(n, nsteps, taylor_k, ps_q, sexp, block):
+def powder_flops_per_orientation(n, nsteps, taylor_k, ps_q, sexp, block, real=False):
     """Real flops of the dense propagation per orientation on the reduced
     dimension n (padding excluded): complex product = 8 n^3, complex
     vector-matrix product = 8 n^2, complex dot = 8 n. Matrix products: q-1
     powers and K//q Horner steps of the Paterson-Stockmeyer Taylor evaluation,
     s squarings, log2(B) squarings to Q = P^B; B-1 rows of the U block;
-    M-1 steps of the V chain; (nsteps+1) detection dots."""
+    M-1 steps of the V chain; (nsteps+1) detection dots. real: the
+    real-generator kernel, whose powers are real products (2 n^3), Horner
+    steps real x complex (4 n^3) and squarings two real products (4 n^3)."""
     npts = nsteps + 1
     nblk = math.ceil(npts / block)
-    gemms = (ps_q - 1) + taylor_k // ps_q + sexp + int(round(math.log2(block)))
-    return gemms * 8.0 * n ** 3 + (block - 1 + nblk - 1) * 8.0 * n ** 2 + npts * 8.0 * n
+    sq = sexp + int(round(math.log2(block)))
+    if real:
+        mat = (ps_q - 1) * 2.0 * n ** 3 + (taylor_k // ps_q) * 4.0 * n ** 3 + sq * 4.0 * n ** 3
+    else:
+        mat = ((ps_q - 1) + taylor_k // ps_q + sq) * 8.0 * n ** 3
+    return mat + (block - 1 + nblk - 1) * 8.0 * n ** 2 + npts * 8.0 * n
 
 
 def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
@@ -102,11 +113,13 @@ def time_plan(plan, steps, warmup):
     return total / steps, float(np.mean(kms)), st
 
 
-def stage_table(st, prop_flops, spec_flops, prop_kernel='spk::k_powder'):
+def stage_table(st, prop_flops, spec_flops, prop_kernel=None):
     """{kernel: (ms, flops)} for the kernels that ran in a pass."""
     ms = st['stage_ms']
     out = {}
     if prop_flops:
+        if prop_kernel is None:
+            prop_kernel = 'spk::k_powder<double,%s>' % ('true' if st['real_systems'] else 'false')
         out[prop_kernel] = (ms[0], prop_flops)
     if spec_flops:
         for k, name in enumerate(('spk::k_tridiag', 'spk::k_dceig', 'spk::k_synth')):
@@ -137,7 +150,8 @@ def bench_static(args, two_spin=False):
         stages = stage_table(st, 0.0, {k: v * n_or for k, v in spectral_stage_flops(n, nsteps + 1).items()})
     else:
         stages = stage_table(st, n_or * powder_flops_per_orientation(
-            n, nsteps, st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or, st['block']), None)
+            n, nsteps, st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or, st['block'],
+            real=st['real_systems'] > 0), None)
     cpu = err = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
@@ -188,7 +202,8 @@ def bench_batch(args):
                 spec_flops[k] += n_or * v
         else:
             prop_flops += n_or * powder_flops_per_orientation(
-                n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'])
+                n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'],
+                real=st['real_systems'] > 0)
     stages = stage_table(st, prop_flops, spec_flops if spectral_ok else None)
     cpu = err = None
     if not args.no_cpu:
