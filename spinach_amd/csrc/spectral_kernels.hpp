@@ -58,14 +58,7 @@ struct SpecParams {
   double* spec;          // kSpecStride doubles per spectral item
   int32_t* status;       // per item: QR sweeps used, or -1 if the iteration did not converge
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
-  unsigned long long* tri_cycles;  // optional: per workgroup, kTriPhases clock64() totals of k_tridiag
 };
-
-// phases of k_tridiag timed by the optional cycle counters: prologue
-// (assembly, register load), Householder vector + matrix-vector product +
-// vector transforms (to the mid-step barrier), rank-2 update + next row (to
-// the end-of-step barrier), epilogue
-constexpr int kTriPhases = 4;
 
 // Scatter (sign = +1) or clear (sign = 0) the real symmetric generator
 // A = dt*(x + x^T)/2 of one orientation into a dense row-major np x np buffer.
@@ -157,9 +150,6 @@ __device__ __forceinline__ void extract_row(const double (&R)[kTriRowsPer][kTriC
   if (tr == (jj & 63) && tc == 0) *sig = part;
 }
 
-// kTiming: the variant with the cycle counters (SPINACH_PHASE_TIMING); the
-// production instantiation carries none of their state.
-template <bool kTiming>
 __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
   __shared__ double sx[2][kTriNP];  // rows j (Householder source), double-buffered
   __shared__ double ssig[2];
@@ -184,16 +174,6 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     const SysDesc sdesc = p.sys[item.sys];
     const int n = sdesc.n, np = sdesc.np;
     const int64_t o = item.orient;
-    long long t_mark = kTiming ? clock64() : 0;
-    auto mark = [&](int ph) {
-      if constexpr (kTiming) {
-        if (tid == 0) {
-          const long long t = clock64();
-          p.tri_cycles[(size_t)blockIdx.x * kTriPhases + ph] += (unsigned long long)(t - t_mark);
-          t_mark = t;
-        }
-      }
-    };
     if (tid == 0) wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], cre, cim);
     __syncthreads();
     scatter_real_generator(p, sdesc, cre, cim, W, false);
@@ -230,7 +210,6 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     scatter_real_generator(p, sdesc, cre, cim, W, true);
     extract_row(R, 0, tr, tc, sx[0], &ssig[0]);
     __syncthreads();
-    mark(0);
 
     for (int j = 0; j < n - 2; j++) {
       const int cb = j & 1;
@@ -300,7 +279,6 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
             if (vv[r] != 0.0) su[wave][lane + 64 * r] = uu[r] - s * vv[r];
         }
         __syncthreads();
-        mark(1);
         // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
         double qt = 0.0;
 #pragma unroll
@@ -334,7 +312,6 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       // the owners of row j+1 publish it (after their own update) for the next step
       extract_row(R, j + 1, tr, tc, sx[cb ^ 1], &ssig[cb ^ 1]);
       __syncthreads();
-      mark(2);
     }
     // trailing 2 x 2 (or 1 x 1) block: row max(n-2, 0) is in its buffer; publish row n-1
     if (n >= 2) {
@@ -357,7 +334,6 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       for (int q = 0; q < 4; q++) out[(2 + q) * kTriNP + i] = su[q][i];
     }
     __syncthreads();
-    mark(3);
   }
 }
 

@@ -368,7 +368,6 @@ struct sp_plan {
   int grid_tri = 0, grid_syn = 0;
   int64_t tri_stride = 0, syn_stride = 0;
   DevBuf d_spec, d_status, d_work_tri, d_work_syn;
-  DevBuf d_tri_phase;  // optional k_tridiag cycle counters (SPINACH_PHASE_TIMING=1)
   // orientation sum: chunks of consecutive orientations, chunk ranges per system
   int64_t n_chunks = 0;
   DevBuf d_chunks, d_chunk_off, d_partial;
@@ -598,8 +597,6 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->syn_stride = (4 * (int64_t)np_spec * spk::kSynthB + 63) / 64 * 64;
     pl->d_work_syn.alloc((size_t)pl->grid_syn * pl->syn_stride * sizeof(double));
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
-    if (const char* e = getenv("SPINACH_PHASE_TIMING"))
-      if (atoi(e) > 0) pl->d_tri_phase.alloc((size_t)pl->grid_tri * spk::kTriPhases * sizeof(unsigned long long));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
   setup_sum(pl.get(), st);
@@ -679,7 +676,6 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
   sp.spec = pl->d_spec.as<double>();
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
-  sp.tri_cycles = pl->d_tri_phase.p ? pl->d_tri_phase.as<unsigned long long>() : nullptr;
   return sp;
 }
 
@@ -687,10 +683,7 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
 void launch_spectral(sp_plan* pl, hipEvent_t* ev) {
   hipStream_t st = pl->ctx->stream;
   spk::SpecParams tri = spec_params(pl, pl->d_work_tri.as<double>(), pl->tri_stride);
-  if (tri.tri_cycles)
-    hipLaunchKernelGGL(spk::k_tridiag<true>, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
-  else
-    hipLaunchKernelGGL(spk::k_tridiag<false>, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
+  hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
@@ -900,8 +893,6 @@ int run(sp_plan* pl, sp_stats* stats) {
   HIPCHK(hipGetLastError());
   if (pl->d_phase.p)
     HIPCHK(hipMemsetAsync(pl->d_phase.p, 0, pl->d_phase.bytes, st));
-  if (pl->d_tri_phase.p)
-    HIPCHK(hipMemsetAsync(pl->d_tri_phase.p, 0, pl->d_tri_phase.bytes, st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   if (pl->mas) {
     if (pl->mode == SP_MODE_FP64)
@@ -987,17 +978,6 @@ int run(sp_plan* pl, sp_stats* stats) {
     for (int k = 0; k < spk::kPhases; k++) {
       unsigned long long t = 0;
       for (int g = 0; g < pl->grid; g++) t += ph[(size_t)g * spk::kPhases + k];
-      fprintf(stderr, " %llu", t);
-    }
-    fprintf(stderr, "\n");
-  }
-  if (pl->d_tri_phase.p) {
-    std::vector<unsigned long long> ph((size_t)pl->grid_tri * spk::kTriPhases);
-    HIPCHK(hipMemcpy(ph.data(), pl->d_tri_phase.p, ph.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    fprintf(stderr, "[spinach_powder] k_tridiag cycles summed over %d workgroups:", pl->grid_tri);
-    for (int k = 0; k < spk::kTriPhases; k++) {
-      unsigned long long t = 0;
-      for (int g = 0; g < pl->grid_tri; g++) t += ph[(size_t)g * spk::kTriPhases + k];
       fprintf(stderr, " %llu", t);
     }
     fprintf(stderr, "\n");
