@@ -361,10 +361,12 @@ struct DcShared {
   double X[6][kTriNP];  // tracked vectors over the eigen-index of each segment
   double Y[6][kTriNP];  // merge input, sorted
   double D[kTriNP], z[kTriNP];
-  double Dc[kTriNP], Zc[kTriNP], zh[kTriNP], tau[kTriNP];
+  double Dc[kTriNP], Zc[kTriNP], Z2c[kTriNP], zh[kTriNP], tau[kTriNP];
+  double lorg[kTriNP];  // Dc[org] of each root: lambda = lorg + tau
   int org[kTriNP], keep[kTriNP], defl[kTriNP];
   int bnd[kDcMaxLevels + 1][(1 << kDcMaxLevels) + 1];  // segment boundaries per level
   double beta[(1 << kDcMaxLevels)];                     // rho = 2|coupling| of the merges of a level
+  double zsum[(1 << kDcMaxLevels)];                     // sum of the kept z^2 of each merge
   int K[(1 << kDcMaxLevels)];
   int its[kTriNP];
   int nlev;
@@ -460,81 +462,140 @@ __host__ __device__ inline void dc_sort_segment(DcShared& S, int a, int b) {
   }
 }
 
-// Root k of 1 + rho sum_{i<K} Zc_i^2/(Dc_i - lambda) = 0 (Dc ascending) as
-// lambda = Dc[org] + tau. Interior roots: the two-pole rational model of the
-// "middle way" (R.-C. Li, 1993); the outermost root: Newton on
-// F(x) = x (1/rho + psi(x)) - z_K^2, x = lambda - Dc[K-1], which has no pole at
-// x = 0. Both safeguarded by the bracket. Returns the iteration count
+// Root k of 1/rho + sum_{i<K} Z2_i/(Dc_i - lambda) = 0 (Dc ascending, Z2 = z^2 > 0,
+// zs = sum Z2) as lambda = Dc[org] + tau, org the nearer pole. Start: the two-pole model
+// through poles kk, kk+1 with the other terms frozen at the middle of the interval (the
+// interval (Dc[K-1], Dc[K-1] + rho zs) for the outermost root), as LAPACK dlaed4 starts.
+// Steps: the rational models of R.-C. Li ("Solving secular equations stably and
+// efficiently", 1993) through the same two poles -- fixed weight (the origin pole's own
+// weight, the other terms' slope given to the far pole) and middle way (the slopes of the
+// two sides given to the two poles) -- switched when |f| does not fall tenfold with an
+// unchanged sign; for the outermost root the model's root beyond the last pole. Both models
+// match f and f' at the current point, so they converge at least quadratically: a step
+// below 1e-9 |tau| is taken and ends the iteration. The bracket guards every step (a
+// bisection replaces a step that leaves it). Returns the number of evaluations of f
 // (kDcMaxIter + 1 if it did not converge).
-__host__ __device__ inline int dc_secular_root(const double* Dc, const double* Zc, int K, int k, double rho, int& org,
-                                               double& tau) {
+__host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z2, int K, int k, double rho, double zs,
+                                               int& org, double& tau) {
   const double eps = 2.220446049250313e-16;
+  const double rinv = 1.0 / rho;
+  if (K == 1) {
+    org = 0;
+    tau = rho * Z2[0];
+    return 1;
+  }
+  const bool outer = (k == K - 1);
+  const int kk = outer ? K - 2 : k;  // the model's poles: kk and kk + 1
+  const double gap = Dc[kk + 1] - Dc[kk];
+  const int base = outer ? K - 1 : k;
+  const double x0 = outer ? 0.5 * rho * zs : 0.5 * gap;
+  const double dbase = Dc[base];
+  double f = rinv;
+#pragma unroll 4
+  for (int i = 0; i < K; i++) f += Z2[i] * dc_rcp((Dc[i] - dbase) - x0);
+  const double s2 = Z2[kk] + Z2[kk + 1];
   double tlo, thi;
-  if (k < K - 1) {
-    const double mid = 0.5 * (Dc[k + 1] - Dc[k]);
-    double f = 1.0 / rho;
-    for (int i = 0; i < K; i++) f += Zc[i] * Zc[i] / ((Dc[i] - Dc[k]) - mid);
-    if (f >= 0.0) {
+  if (!outer) {
+    // the other terms at the midpoint: f less Z2_kk/(-x0) and Z2_kk+1/x0
+    const double c = f + (Z2[kk] - Z2[kk + 1]) * dc_rcp(x0);
+    if (f >= 0.0) {  // root in (Dc[k], midpoint]: c t^2 - bq t + aq = 0, t = lambda - Dc[k]
       org = k;
       tlo = 0.0;
-      thi = mid;
-    } else {
+      thi = x0;
+      const double bq = c * gap + s2, aq = Z2[kk] * gap;
+      const double r = sqrt(fabs(bq * bq - 4.0 * aq * c));
+      tau = (bq > 0.0) ? 2.0 * aq / (bq + r) : (bq - r) / (2.0 * c);
+    } else {  // root in (midpoint, Dc[k+1]): c t^2 - aq t - bq = 0, t = lambda - Dc[k+1]
       org = k + 1;
-      tlo = -mid;
+      tlo = -x0;
       thi = 0.0;
+      const double aq = s2 - c * gap, bq = Z2[kk + 1] * gap;
+      const double r = sqrt(fabs(aq * aq + 4.0 * c * bq));
+      tau = (aq > 0.0) ? -2.0 * bq / (aq + r) : (aq - r) / (2.0 * c);
     }
-  } else {
-    org = k;
-    double zs = 0.0;
-    for (int i = 0; i < K; i++) zs += Zc[i] * Zc[i];
-    tlo = 0.0;
-    thi = rho * zs;
+  } else {  // c t^2 - aq t - bq = 0 for t = lambda - Dc[K-1] > 0
+    org = K - 1;
+    const double c = f + Z2[kk] * dc_rcp(gap + x0) + Z2[kk + 1] * dc_rcp(x0);
+    if (f > 0.0) {
+      tlo = 0.0;
+      thi = x0;
+    } else {
+      tlo = x0;
+      thi = 2.0 * x0;
+    }
+    const double aq = s2 - c * gap, bq = Z2[kk + 1] * gap;
+    const double r = sqrt(fabs(aq * aq + 4.0 * c * bq));
+    tau = (aq >= 0.0) ? (aq + r) / (2.0 * c) : 2.0 * bq / (r - aq);
   }
+  if (!(tau > tlo && tau < thi)) tau = 0.5 * (tlo + thi);
   const double dorg = Dc[org];
-  tau = 0.5 * (tlo + thi);
+  const double e1 = Dc[kk] - dorg, e2 = Dc[kk + 1] - dorg;  // the model's poles from the origin
+  bool fixed_weight = true;
+  double fprev = 0.0;
   int it = 1;
   for (; it <= kDcMaxIter; it++) {
-    double f = 1.0 / rho, wabs = 0.0, psi = 0.0, psip = 0.0, phip = 0.0;
-    for (int i = 0; i < K; i++) {
-      const double delta = (Dc[i] - dorg) - tau;
-      const double rd = dc_rcp(delta);
-      const double w = Zc[i] * Zc[i] * rd;
-      f += w;
-      wabs += fabs(w);
-      if (k < K - 1) {
-        if (i <= k)
-          psip += w * rd;
-        else
-          phip += w * rd;
-      } else if (i < K - 1) {
-        psi += w;
-        psip += w * rd;
-      }
+    // terms split by sign: the poles below lambda give w < 0 (psi), those above w > 0 (phi)
+    double wpos = 0.0, wneg = 0.0, dpos = 0.0, dneg = 0.0;
+#pragma unroll 4
+    for (int i = 0; i < K - 1; i++) {
+      const double rd = dc_rcp((Dc[i] - dorg) - tau);
+      const double w = Z2[i] * rd;
+      const double wp = fmax(w, 0.0), wn = fmin(w, 0.0);
+      wpos += wp;
+      wneg += wn;
+      dpos = fma(wp, rd, dpos);
+      dneg = fma(wn, rd, dneg);
     }
-    if (f > 0.0)
+    // the last pole: the far pole of the outermost root's model, else a term as the rest
+    const double rl = dc_rcp((Dc[K - 1] - dorg) - tau);
+    const double wl = Z2[K - 1] * rl;
+    double psip, phip;
+    if (outer) {
+      wneg += wl;
+      psip = dneg;
+      phip = wl * rl;
+    } else {
+      const double wp = fmax(wl, 0.0), wn = fmin(wl, 0.0);
+      wpos += wp;
+      wneg += wn;
+      psip = fma(wn, rl, dneg);
+      phip = fma(wp, rl, dpos);
+    }
+    const double fv = rinv + wpos + wneg;
+    if (fv > 0.0)
       thi = tau;
     else
       tlo = tau;
-    if (fabs(f) <= 8.0 * eps * (1.0 / rho + wabs) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) break;
-    double eta;
-    if (k < K - 1) {
-      const double d1 = (Dc[k] - dorg) - tau, d2 = (Dc[k + 1] - dorg) - tau;
-      const double c = f - psip * d1 - phip * d2;
-      const double B = c * (d1 + d2) + psip * d1 * d1 + phip * d2 * d2;
-      const double C = d1 * d2 * f;
-      const double disc = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
-      if (B >= 0.0)
-        eta = (B + disc != 0.0) ? 2.0 * C / (B + disc) : 0.0;
-      else
-        eta = (c != 0.0) ? (B - disc) / (2.0 * c) : 0.0;
+    if (fabs(fv) <= 8.0 * eps * (rinv + wpos - wneg) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) break;
+    const double d1 = e1 - tau, d2 = e2 - tau;
+    const double dw = psip + phip;
+    if (fv * fprev > 0.0 && fabs(fv) > 0.1 * fabs(fprev)) fixed_weight = !fixed_weight;
+    fprev = fv;
+    double c;
+    if (!outer && fixed_weight) {
+      const bool lo = (org == k);
+      const double dn = lo ? d1 : d2, dfar = lo ? d2 : d1;
+      const double r1 = dc_rcp(dn);
+      const double zo = lo ? Z2[kk] : Z2[kk + 1];
+      c = fv - dfar * dw - (lo ? -gap : gap) * (zo * r1 * r1);
     } else {
-      const double F = tau * (1.0 / rho + psi) - Zc[K - 1] * Zc[K - 1];
-      const double Fp = 1.0 / rho + psi + tau * psip;
-      eta = (Fp != 0.0) ? -F / Fp : 0.0;
+      c = fv - psip * d1 - phip * d2;
     }
-    double nt = tau + eta;
-    if (!(nt > tlo && nt < thi)) nt = 0.5 * (tlo + thi);
+    const double B = (d1 + d2) * fv - d1 * d2 * dw, C = d1 * d2 * fv;
+    const double disc = sqrt(fabs(B * B - 4.0 * c * C));
+    double eta;
+    if (!outer)
+      eta = (B >= 0.0) ? 2.0 * C / (B + disc) : (B - disc) / (2.0 * c);
+    else
+      eta = (B >= 0.0) ? (B + disc) / (2.0 * fabs(c)) : 2.0 * C / (B - disc);
+    const double nt = tau + eta;
+    if (!(nt > tlo && nt < thi)) {
+      tau = 0.5 * (tlo + thi);
+      continue;
+    }
+    const bool small = fabs(eta) <= 1e-9 * fabs(tau);
     tau = nt;
+    if (small) break;
   }
   return it;
 }
@@ -625,27 +686,34 @@ __host__ __device__ inline void dc_phase_m1(DcShared& S, int l, int n, int i) {
 }
 
 // M2: deflation scan (one thread per merge): small z_i, and pairs of close
-// poles rotated so that one z component vanishes
+// poles rotated so that one z component vanishes; the kept poles are compacted
+// into Dc, Zc, Z2c as they are found (a later rotation only removes the last kept
+// one, whose slot the rotated pole takes)
 __host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
   if (s2 >= (1 << l)) return;
   const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1], m = S.bnd[l + 1][2 * s2 + 1];
   const double bcoup = (m > a && m < b) ? S.e[m - 1] : 0.0;
   const double rho = 2.0 * fabs(bcoup);
   S.beta[s2] = rho;
-  double dmax = 0.0;
-  for (int i = a; i < b; i++) dmax = fmax(dmax, fabs(S.D[i]));
+  // D is ascending on [a, b): the largest magnitude is at one of the ends
+  const double dmax = (b > a) ? fmax(fabs(S.D[a]), fabs(S.D[b - 1])) : 0.0;
   const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, rho);
   int K = 0, nd = 0, prev = -1;
+  double zs = 0.0;  // sum of the kept z^2 (a rotation keeps the sum of the pair's squares)
   for (int i = a; i < b; i++) {
-    if (rho == 0.0 || rho * fabs(S.z[i]) <= tol) {
+    const double zi = S.z[i];
+    if (rho == 0.0 || rho * fabs(zi) <= tol) {
       S.defl[a + nd++] = i;
       continue;
     }
+    zs = fma(zi, zi, zs);
     if (prev >= 0) {
-      const double t = S.D[i] - S.D[prev];
-      const double tz = sqrt(S.z[prev] * S.z[prev] + S.z[i] * S.z[i]);
-      const double c = S.z[i] / tz, s = -S.z[prev] / tz;
-      if (fabs(t * c * s) <= tol) {
+      const double zp = S.z[prev], t = S.D[i] - S.D[prev];
+      const double zz = fma(zp, zp, zi * zi);
+      // |t c s| <= tol for the rotation c = zi/|z|, s = -zp/|z|, tested without the root
+      if (fabs(t * zi * zp) <= tol * zz) {
+        const double tz = sqrt(zz);
+        const double c = zi / tz, s = -zp / tz;
         S.z[i] = tz;
         S.z[prev] = 0.0;
         const double dp = S.D[prev], di = S.D[i];
@@ -660,14 +728,16 @@ __host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
         K--;
       }
     }
-    S.keep[a + K++] = i;
+    const double zk = S.z[i];
+    S.keep[a + K] = i;
+    S.Dc[a + K] = S.D[i];
+    S.Zc[a + K] = zk;
+    S.Z2c[a + K] = zk * zk;
+    K++;
     prev = i;
   }
   S.K[s2] = K;
-  for (int k = 0; k < K; k++) {
-    S.Dc[a + k] = S.D[S.keep[a + k]];
-    S.Zc[a + k] = S.z[S.keep[a + k]];
-  }
+  S.zsum[s2] = zs;
 }
 
 // M3: secular roots (one thread per (merge, root))
@@ -678,25 +748,35 @@ __host__ __device__ inline void dc_phase_m3(DcShared& S, int l, int n, int i) {
   if (k >= K) return;
   int og;
   double tu;
-  const int its = dc_secular_root(S.Dc + a, S.Zc + a, K, k, S.beta[s2], og, tu);
+  const int its = dc_secular_root(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], S.zsum[s2], og, tu);
   if (its > kDcMaxIter) S.fail = 1;
   if (its > S.its[i]) S.its[i] = its;
   S.org[a + k] = og;
   S.tau[a + k] = tu;
+  S.lorg[a + k] = S.Dc[a + og];
 }
 
-// M4: z recomputed from the computed roots (Gu & Eisenstat), one thread per (merge, pole)
+// M4: z recomputed from the computed roots (Gu & Eisenstat), one thread per (merge, pole):
+// z_i^2 = (lambda_i - d_i)/rho prod_{k != i} (lambda_k - d_i)/(d_k - d_i), the products
+// taken four factors at a time with one reciprocal per group
 __host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
   if (t >= n) return;
   const int s2 = dc_segment_of(S, l, t);
   const int a = S.bnd[l][s2], i = t - a, K = S.K[s2];
   if (i >= K) return;
   const double di = S.Dc[a + i];
-  double w = 1.0;
+  double w = 1.0, num = 1.0, den = 1.0;
+#pragma unroll 4
   for (int k = 0; k < K; k++) {
-    const double lmd = (S.Dc[a + S.org[a + k]] - di) + S.tau[a + k];  // lambda_k - d_i
-    w *= (k != i) ? lmd * dc_rcp(S.Dc[a + k] - di) : lmd;
+    num *= (S.lorg[a + k] - di) + S.tau[a + k];  // lambda_k - d_i
+    den *= (k != i) ? S.Dc[a + k] - di : 1.0;    // d_k - d_i
+    if ((k & 3) == 3) {
+      w *= num * dc_rcp(den);
+      num = 1.0;
+      den = 1.0;
+    }
   }
+  w *= num * dc_rcp(den);
   const double zz = sqrt(fmax(w, 0.0) / S.beta[s2]);
   S.zh[a + i] = (S.Zc[a + i] >= 0.0) ? zz : -zz;
 }
@@ -709,8 +789,7 @@ __host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
   double out[6];
   double lam;
   if (slot < K) {
-    const int og = S.org[a + slot];
-    const double tu = S.tau[a + slot], dorg = S.Dc[a + og];
+    const double tu = S.tau[a + slot], dorg = S.lorg[a + slot];
     double nrm = 0.0;
     for (int q = 0; q < 6; q++) out[q] = 0.0;
     for (int i = 0; i < K; i++) {
