@@ -127,32 +127,62 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
-// Row jj of the register-resident matrix into xb and the sum of squares of
-// its entries right of the first subdiagonal position into *sig. The owners
-// are the kTriTC threads of row group jj % 64 (one aligned lane group);
-// every thread calls this (the DPP sum needs the whole wave active). Indices
-// into R are compile-time constants, so R stays in registers.
-__device__ __forceinline__ void extract_row(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr, int tc,
-                                            double* xb, double* sig) {
-  double part = 0.0;
+// The owners of row jj (the kTriTC lanes of row group jj % 64, all in one
+// wave) copy it to srow; that wave then forms the Householder reflection that
+// clears row jj right of its first subdiagonal entry:
+//   sigma = sum_{c >= jj+2} x_c^2,  alpha = -sign(x_{jj+1}) sqrt(x_{jj+1}^2 + sigma),
+//   v = (0, ..., 0, x_{jj+1} - alpha, x_{jj+2}, ...),  tau = 2/(v'v)
+// (tau = 0, alpha = x_{jj+1} when sigma = 0: no reflection), and writes v to
+// vb, tau to *ptau and T's d_jj = x_jj, e_jj = alpha. Every thread calls it;
+// the other waves return at once. Indices into R are compile-time constants,
+// so R stays in registers; the copy is read back by the same wave, whose LDS
+// operations complete in order.
+__device__ __forceinline__ void householder_publish(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr,
+                                                    int tc, int lane, int wave, double* srow, double* vb,
+                                                    double* ptau, double* sd, double* se) {
+  if (wave != ((jj & 63) * kTriTC) >> 6) return;
+  const int aj = jj >> 6;
 #pragma unroll
   for (int a = 0; a < kTriRowsPer; a++) {
-    if (tr + 64 * a == jj) {
+    if (a == aj && tr == (jj & 63)) {
 #pragma unroll
-      for (int b = 0; b < kTriColsPer; b++) {
-        const double v = R[a][b];
-        xb[tc + kTriTC * b] = v;
-        if (tc + kTriTC * b >= jj + 2) part = fma(v, v, part);
-      }
+      for (int b = 0; b < kTriColsPer; b++) srow[tc + kTriTC * b] = R[a][b];
     }
   }
-  part = sum_tc(part);
-  if (tr == (jj & 63) && tc == 0) *sig = part;
+  __builtin_amdgcn_wave_barrier();
+  double x[kTriNP / 64];
+  double part = 0.0;
+#pragma unroll
+  for (int r = 0; r < kTriNP / 64; r++) {
+    const int c = lane + 64 * r;
+    x[r] = srow[c];
+    if (c >= jj + 2) part = fma(x[r], x[r], part);
+  }
+  const double sigma = wave_sum_dpp(part);
+  const double xd = srow[jj], x1 = srow[jj + 1];  // srow[kTriNP] stays zero
+  double tau = 0.0, alpha = x1, v1 = 0.0;
+  if (sigma > 0.0) {
+    const double nrm = sqrt(x1 * x1 + sigma);
+    alpha = (x1 > 0.0) ? -nrm : nrm;
+    v1 = x1 - alpha;
+    tau = 2.0 / (v1 * v1 + sigma);
+  }
+#pragma unroll
+  for (int r = 0; r < kTriNP / 64; r++) {
+    const int c = lane + 64 * r;
+    vb[c] = (c > jj + 1) ? x[r] : ((c == jj + 1) ? v1 : 0.0);
+  }
+  if (lane == 0) {
+    *ptau = tau;
+    sd[jj] = xd;
+    se[jj] = alpha;
+  }
 }
 
 __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
-  __shared__ double sx[2][kTriNP];  // rows j (Householder source), double-buffered
-  __shared__ double ssig[2];
+  __shared__ double sv[2][kTriNP];  // Householder vectors of rows j, double-buffered
+  __shared__ double stau[2];
+  __shared__ double srow[kTriNP + 1];  // the row being published; srow[kTriNP] = 0
   __shared__ double sp[kTriNP];     // p = tau*A*v
   __shared__ double sq[kTriThreads / 64];  // per-wave parts of v'p
   __shared__ double su[4][kTriNP];
@@ -195,6 +225,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         sd[i] = 0.0;
         se[i] = 0.0;
       }
+      if (tid == 0) srow[kTriNP] = 0.0;
     }
     __syncthreads();
     double R[kTriRowsPer][kTriColsPer];
@@ -208,34 +239,14 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     __syncthreads();
     // restore the all-zero workspace for the next item
     scatter_real_generator(p, sdesc, cre, cim, W, true);
-    extract_row(R, 0, tr, tc, sx[0], &ssig[0]);
+    householder_publish(R, 0, tr, tc, lane, wave, srow, sv[0], &stau[0], sd, se);
     __syncthreads();
 
     for (int j = 0; j < n - 2; j++) {
       const int cb = j & 1;
-      const double* x = sx[cb];
-      // Householder vector v (v_{j+1} = x_1 - alpha, v_i = x_i below), tau = 2/v'v,
-      // formed redundantly by every thread from row j
-      const double sigma = ssig[cb];
-      const double x1 = x[j + 1];
-      double tau = 0.0, alpha = x1, v1 = 0.0;
-      if (sigma > 0.0) {
-        const double nrm = sqrt(x1 * x1 + sigma);
-        alpha = (x1 > 0.0) ? -nrm : nrm;
-        v1 = x1 - alpha;
-        tau = 2.0 / (v1 * v1 + sigma);
-      }
-      if (tid == 0) {
-        sd[j] = x[j];
-        se[j] = alpha;
-      }
+      const double* v = sv[cb];
+      const double tau = stau[cb];  // reflection of row j (published by the owners of row j)
       if (tau != 0.0) {  // uniform
-        // v_i (i > j+1: row j, i = j+1: v1, else 0); the loads are unconditional so
-        // that a chunk's loads issue together
-        auto vat = [&](int i) -> double {
-          const double xi = x[i];
-          return (i > j + 1) ? xi : ((i == j + 1) ? v1 : 0.0);
-        };
         // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread
         double part[kTriRowsPer] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -243,7 +254,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           if (4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
             double vb[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) vb[q] = vat(tc + kTriTC * (4 * bc + q));
+            for (int q = 0; q < 4; q++) vb[q] = v[tc + kTriTC * (4 * bc + q)];
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
@@ -257,7 +268,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           double s = sum_tc(part[a]) * tau;
           if (i <= j) s = 0.0;
           if (tc == 0) sp[i] = s;
-          qsum = fma(vat(i), s, qsum);
+          qsum = fma(v[i], s, qsum);
         }
         // the kTriTC lanes of a group hold the same qsum
         const double qw = wave_sum_dpp(qsum) * (1.0 / kTriTC);
@@ -269,7 +280,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
 #pragma unroll
           for (int r = 0; r < kTriNP / 64; r++) {
             const int i = lane + 64 * r;
-            vv[r] = vat(i);
+            vv[r] = v[i];
             uu[r] = su[wave][i];
             s = fma(vv[r], uu[r], s);
           }
@@ -288,7 +299,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
 #pragma unroll
         for (int a = 0; a < kTriRowsPer; a++) {
           const int i = tr + 64 * a;
-          vi[a] = vat(i);
+          vi[a] = v[i];
           wi[a] = sp[i] - K * vi[a];
         }
 #pragma unroll
@@ -298,7 +309,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
               const int c = tc + kTriTC * (4 * bc + q);
-              vb[q] = vat(c);
+              vb[q] = v[c];
               wb[q] = sp[c] - K * vb[q];
             }
 #pragma unroll
@@ -309,22 +320,12 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           }
         }
       }
-      // the owners of row j+1 publish it (after their own update) for the next step
-      extract_row(R, j + 1, tr, tc, sx[cb ^ 1], &ssig[cb ^ 1]);
+      // the owners of row j+1 publish its reflection (after their own update) for the next step
+      householder_publish(R, j + 1, tr, tc, lane, wave, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
       __syncthreads();
     }
-    // trailing 2 x 2 (or 1 x 1) block: row max(n-2, 0) is in its buffer; publish row n-1
-    if (n >= 2) {
-      extract_row(R, n - 1, tr, tc, sx[(n - 1) & 1], &ssig[(n - 1) & 1]);
-      __syncthreads();
-      if (tid == 0) {
-        sd[n - 2] = sx[(n - 2) & 1][n - 2];
-        se[n - 2] = sx[(n - 2) & 1][n - 1];
-        sd[n - 1] = sx[(n - 1) & 1][n - 1];
-      }
-    } else if (n == 1) {
-      if (tid == 0) sd[0] = sx[0][0];
-    }
+    // the last row (d_{n-1}; rows up to n-2 are published by now)
+    if (n >= 2) householder_publish(R, n - 1, tr, tc, lane, wave, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
     __syncthreads();
     double* out = p.spec + (size_t)it * kSpecStride;
     for (int i = tid; i < kTriNP; i += kTriThreads) {
