@@ -127,39 +127,36 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
-// The owners of row jj (the kTriTC lanes of row group jj % 64, all in one
-// wave) copy it to srow; that wave then forms the Householder reflection that
-// clears row jj right of its first subdiagonal entry:
-//   sigma = sum_{c >= jj+2} x_c^2,  alpha = -sign(x_{jj+1}) sqrt(x_{jj+1}^2 + sigma),
-//   v = (0, ..., 0, x_{jj+1} - alpha, x_{jj+2}, ...),  tau = 2/(v'v)
-// (tau = 0, alpha = x_{jj+1} when sigma = 0: no reflection), and writes v to
-// vb, tau to *ptau and T's d_jj = x_jj, e_jj = alpha. Every thread calls it;
-// the other waves return at once. Indices into R are compile-time constants,
-// so R stays in registers; the copy is read back by the same wave, whose LDS
-// operations complete in order.
-__device__ __forceinline__ void householder_publish(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr,
-                                                    int tc, int lane, int wave, double* srow, double* vb,
-                                                    double* ptau, double* sd, double* se) {
-  if (wave != ((jj & 63) * kTriTC) >> 6) return;
-  const int aj = jj >> 6;
+// The wave of the owners of row jj: the kTriTC lanes of row group jj % 64.
+__device__ __forceinline__ int row_owner_wave(int jj) { return ((jj & 63) * kTriTC) >> 6; }
+
+// The owners of row jj copy it from their registers to srow (indices into R
+// are compile-time constants, so R stays in registers). Called by the owner wave.
+__device__ __forceinline__ void copy_row(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr, int tc,
+                                         double* srow) {
 #pragma unroll
   for (int a = 0; a < kTriRowsPer; a++) {
-    if (a == aj && tr == (jj & 63)) {
+    if (a == (jj >> 6) && tr == (jj & 63)) {
 #pragma unroll
       for (int b = 0; b < kTriColsPer; b++) srow[tc + kTriTC * b] = R[a][b];
     }
   }
-  __builtin_amdgcn_wave_barrier();
-  double x[kTriNP / 64];
+}
+
+// One wave forms the Householder reflection that clears row jj right of its
+// first subdiagonal entry, from the row's entries x[r] at columns lane + 64 r
+// and its entries xd = x_jj, x1 = x_jj+1 (uniform):
+//   sigma = sum_{c >= jj+2} x_c^2,  alpha = -sign(x1) sqrt(x1^2 + sigma),
+//   v = (0, ..., 0, x1 - alpha, x_{jj+2}, ...),  tau = 2/(v'v)
+// (tau = 0, alpha = x1 when sigma = 0: no reflection), and writes v to vb,
+// tau to *ptau and T's d_jj = xd, e_jj = alpha. Every lane of the wave active.
+__device__ __forceinline__ void householder_form(const double (&x)[kTriNP / 64], double xd, double x1, int jj,
+                                                 int lane, double* vb, double* ptau, double* sd, double* se) {
   double part = 0.0;
 #pragma unroll
-  for (int r = 0; r < kTriNP / 64; r++) {
-    const int c = lane + 64 * r;
-    x[r] = srow[c];
-    if (c >= jj + 2) part = fma(x[r], x[r], part);
-  }
+  for (int r = 0; r < kTriNP / 64; r++)
+    if (lane + 64 * r >= jj + 2) part = fma(x[r], x[r], part);
   const double sigma = wave_sum_dpp(part);
-  const double xd = srow[jj], x1 = srow[jj + 1];  // srow[kTriNP] stays zero
   double tau = 0.0, alpha = x1, v1 = 0.0;
   if (sigma > 0.0) {
     const double nrm = sqrt(x1 * x1 + sigma);
@@ -177,6 +174,22 @@ __device__ __forceinline__ void householder_publish(const double (&R)[kTriRowsPe
     sd[jj] = xd;
     se[jj] = alpha;
   }
+}
+
+// The reflection of row jj as it stands in the registers: the owners copy the
+// row to srow and their wave forms the reflection from the copy (read back by
+// the same wave, whose LDS operations complete in order). Every thread calls
+// it; the other waves return at once.
+__device__ __forceinline__ void householder_publish(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr,
+                                                    int tc, int lane, int wave, double* srow, double* vb,
+                                                    double* ptau, double* sd, double* se) {
+  if (wave != row_owner_wave(jj)) return;
+  copy_row(R, jj, tr, tc, srow);
+  __builtin_amdgcn_wave_barrier();
+  double x[kTriNP / 64];
+#pragma unroll
+  for (int r = 0; r < kTriNP / 64; r++) x[r] = srow[lane + 64 * r];
+  householder_form(x, srow[jj], srow[jj + 1], jj, lane, vb, ptau, sd, se);  // srow[kTriNP] stays zero
 }
 
 __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
@@ -246,7 +259,11 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       const int cb = j & 1;
       const double* v = sv[cb];
       const double tau = stau[cb];  // reflection of row j (published by the owners of row j)
+      const int jn = j + 1;         // the next row to reflect (jn + 1 <= n - 1 < kTriNP)
+      const int own = row_owner_wave(jn);
       if (tau != 0.0) {  // uniform
+        // the owners of row jn copy it (as the previous step left it) for the look-ahead below
+        if (wave == own) copy_row(R, jn, tr, tc, srow);
         // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread
         double part[kTriRowsPer] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -273,28 +290,47 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         // the kTriTC lanes of a group hold the same qsum
         const double qw = wave_sum_dpp(qsum) * (1.0 / kTriTC);
         if (lane == 0) sq[wave] = qw;
-        // Q^T applied to conj(coil) and rho0, one vector per wave: u <- u - tau*v*(v'u)
-        if (wave < 4) {
-          double vv[kTriNP / 64], uu[kTriNP / 64];
-          double s = 0.0;
-#pragma unroll
-          for (int r = 0; r < kTriNP / 64; r++) {
-            const int i = lane + 64 * r;
-            vv[r] = v[i];
-            uu[r] = su[wave][i];
-            s = fma(vv[r], uu[r], s);
-          }
-          s = wave_sum_dpp(s) * tau;
-#pragma unroll
-          for (int r = 0; r < kTriNP / 64; r++)
-            if (vv[r] != 0.0) su[wave][lane + 64 * r] = uu[r] - s * vv[r];
-        }
         __syncthreads();
         // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
         double qt = 0.0;
 #pragma unroll
         for (int w8 = 0; w8 < kTriThreads / 64; w8++) qt += sq[w8];
         const double K = 0.5 * tau * qt;
+        if (wave == own) {
+          // look-ahead: row jn after this update, from its copy, and its reflection for the
+          // next step (the same operations as the register update below)
+          const double vn = v[jn], wn = sp[jn] - K * vn;
+          double x[kTriNP / 64];
+#pragma unroll
+          for (int r = 0; r < kTriNP / 64; r++) {
+            const int c = lane + 64 * r;
+            const double vc = v[c], wc = sp[c] - K * vc;
+            x[r] = fma(-vn, wc, fma(-wn, vc, srow[c]));
+          }
+          const double xd = fma(-vn, wn, fma(-wn, vn, srow[jn]));
+          const double v1 = v[jn + 1], w1 = sp[jn + 1] - K * v1;
+          const double x1 = fma(-vn, w1, fma(-wn, v1, srow[jn + 1]));
+          householder_form(x, xd, x1, jn, lane, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+        } else {
+          // Q^T applied to conj(coil) and rho0 (four vectors, one per wave other than
+          // the owners'): u <- u - tau*v*(v'u)
+          const int q = (wave - own - 1) & 7;
+          if (q < 4) {
+            double vv[kTriNP / 64], uu[kTriNP / 64];
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < kTriNP / 64; r++) {
+              const int i = lane + 64 * r;
+              vv[r] = v[i];
+              uu[r] = su[q][i];
+              s = fma(vv[r], uu[r], s);
+            }
+            s = wave_sum_dpp(s) * tau;
+#pragma unroll
+            for (int r = 0; r < kTriNP / 64; r++)
+              if (vv[r] != 0.0) su[q][lane + 64 * r] = uu[r] - s * vv[r];
+          }
+        }
         double vi[kTriRowsPer], wi[kTriRowsPer];
 #pragma unroll
         for (int a = 0; a < kTriRowsPer; a++) {
@@ -319,9 +355,10 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
                 R[a][4 * bc + q] = fma(-vi[a], wb[q], fma(-wi[a], vb[q], R[a][4 * bc + q]));
           }
         }
+      } else {
+        // no reflection at this step: row jn stands as it is in the registers
+        householder_publish(R, jn, tr, tc, lane, wave, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
       }
-      // the owners of row j+1 publish its reflection (after their own update) for the next step
-      householder_publish(R, j + 1, tr, tc, lane, wave, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
       __syncthreads();
     }
     // the last row (d_{n-1}; rows up to n-2 are published by now)
