@@ -52,8 +52,7 @@ struct SpecParams {
   const double* gammas;
   double dt;
   int64_t nsteps;
-  double* work;          // per-workgroup workspace (k_tridiag: dense np x np, all zero between uses;
-                         // k_synth: U and W blocks)
+  double* work;          // per-workgroup workspace of k_tridiag: dense np x np, all zero between uses
   int64_t work_stride;   // doubles per workgroup
   double* spec;          // kSpecStride doubles per spectral item
   int32_t* status;       // per item: QR sweeps used, or -1 if the iteration did not converge
@@ -926,17 +925,41 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
 }
 
 // ------------------------------------------------------------ k_synth
-__global__ void __launch_bounds__(kThreads, 2) k_synth(SpecParams p) {
-  __shared__ GemmLds<double> g;
-  __shared__ int sitem;
+// f(64 (i0 + i) + j) = sum_k U[i][k] W[k][j] for one row tile i < 64 of
+// 64-point rows, U[i][k] = g_k e^{-i lambda_k 64 (i0 + i)}, W[k][j] =
+// e^{-i lambda_k j}: a complex 64 x 64 x np product on the matrix cores, its
+// operands generated panel by panel straight into the LDS staging buffers.
+// Each eigenvalue's phase steps e^{-i lambda m} for m = 1, 8, 64, 512 come from
+// one sincos each; an entry is its group's start (a binary power of the 8- or
+// 512-step, at most four rounded products) advanced by the 1- or 64-step (at
+// most seven more), so every phase factor carries at most ~12 roundings.
+struct SynthLds {
+  double s1r[kTriNP], s1i[kTriNP], s8r[kTriNP], s8i[kTriNP];
+  double s64r[kTriNP], s64i[kTriNP], s512r[kTriNP], s512i[kTriNP];
+  double gr[kTriNP], gi[kTriNP];  // g_k e^{-i lambda_k 64 i0} for the current row tile
+  GemmLds<double> g;
+  double red[kThreads / 64][2];  // per-wave parts of a tail point
+  int sitem;
+};
+
+__device__ __forceinline__ void cmul(double& ar, double& ai, double br, double bi) {
+  const double t = ar * br - ai * bi;
+  ai = ar * bi + ai * br;
+  ar = t;
+}
+
+__global__ void __launch_bounds__(kThreads, 3) k_synth(SpecParams p) {
+  typedef Mma<double>::v4 v4;
+  __shared__ SynthLds S;
   const int tid = threadIdx.x;
-  double* ws = p.work + (size_t)blockIdx.x * p.work_stride;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;  // the wave's 32 x 32 quarter of the 64 x 64 tile
   const int64_t npts = p.nsteps + 1;
-  const int64_t nfull = npts / kSynthB;
+  const int64_t nfull = npts / kSynthB;  // rows of 64 points
   for (;;) {
-    if (tid == 0) sitem = (int)atomicAdd(&p.queue[1], 1u);
+    if (tid == 0) S.sitem = (int)atomicAdd(&p.queue[1], 1u);
     __syncthreads();
-    const int64_t it = sitem;
+    const int64_t it = S.sitem;
     __syncthreads();
     if (it >= p.n_items) break;
     const Item item = p.items[it];
@@ -946,81 +969,167 @@ __global__ void __launch_bounds__(kThreads, 2) k_synth(SpecParams p) {
     const double* gre = lam + 2 * kTriNP;
     const double* gim = lam + 3 * kTriNP;
     double* out = p.per_or + 2 * (size_t)item.orient * npts;
-    // W[k][j] = exp(-i lambda_k j), j < 64 (rows k >= n are zero)
-    double* Wre = ws;
-    double* Wim = ws + (size_t)np * kSynthB;
-    double* Ure = ws + 2 * (size_t)np * kSynthB;
-    double* Uim = Ure + (size_t)kSynthB * np;
-    // phases by angle addition: one thread per (k, group of 8 consecutive
-    // exponents) takes sincos at the group's first exponent and at the step,
-    // then multiplies (at most 7 rounded products per entry)
-    for (int e = tid; e < np * (kSynthB / 8); e += kThreads) {
-      const int k = e / (kSynthB / 8), a8 = e % (kSynthB / 8);
-      double br = 0.0, bi = 0.0, sr = 1.0, si = 0.0;
+    for (int k = tid; k < np; k += kThreads) {
+      double c1 = 1.0, n1 = 0.0, c8 = 1.0, n8 = 0.0, c64 = 1.0, n64 = 0.0, c512 = 1.0, n512 = 0.0;
       if (k < n) {
-        double s, c;
-        sincos(lam[k] * (double)(8 * a8), &s, &c);
-        br = c;
-        bi = -s;
-        sincos(lam[k], &s, &c);
-        sr = c;
-        si = -s;
+        double sn;
+        sincos(lam[k], &sn, &c1);
+        n1 = -sn;
+        sincos(lam[k] * 8.0, &sn, &c8);
+        n8 = -sn;
+        sincos(lam[k] * 64.0, &sn, &c64);
+        n64 = -sn;
+        sincos(lam[k] * 512.0, &sn, &c512);
+        n512 = -sn;
       }
-      for (int b = 0; b < 8; b++) {
-        const size_t o2 = (size_t)k * kSynthB + 8 * a8 + b;
-        Wre[o2] = br;
-        Wim[o2] = bi;
-        const double t = br * sr - bi * si;
-        bi = br * si + bi * sr;
-        br = t;
-      }
+      S.s1r[k] = c1;
+      S.s1i[k] = n1;
+      S.s8r[k] = c8;
+      S.s8i[k] = n8;
+      S.s64r[k] = c64;
+      S.s64i[k] = n64;
+      S.s512r[k] = c512;
+      S.s512i[k] = n512;
     }
     for (int64_t i0 = 0; i0 < nfull; i0 += kSynthB) {
       const int rows = (int)((nfull - i0 < kSynthB) ? (nfull - i0) : kSynthB);
-      __syncthreads();
-      // U[i][k] = g_k exp(-i lambda_k 64 (i0 + i))
-      for (int e = tid; e < np * ((rows + 7) / 8); e += kThreads) {
-        const int k = e % np, a8 = e / np;
-        double br = 0.0, bi = 0.0, sr = 1.0, si = 0.0;
+      for (int k = tid; k < np; k += kThreads) {
+        double br = 0.0, bi = 0.0;
         if (k < n) {
-          double s, c;
-          sincos(lam[k] * (double)(kSynthB * (i0 + 8 * a8)), &s, &c);
-          br = gre[k] * c + gim[k] * s;
-          bi = gim[k] * c - gre[k] * s;
-          sincos(lam[k] * (double)kSynthB, &s, &c);
-          sr = c;
-          si = -s;
+          double sn = 0.0, cs = 1.0;
+          if (i0 > 0) sincos(lam[k] * (double)(kSynthB * i0), &sn, &cs);
+          br = gre[k] * cs + gim[k] * sn;
+          bi = gim[k] * cs - gre[k] * sn;
         }
-        for (int b = 0; b < 8 && 8 * a8 + b < rows; b++) {
-          const size_t o2 = (size_t)(8 * a8 + b) * np + k;
-          Ure[o2] = br;
-          Uim[o2] = bi;
-          const double t = br * sr - bi * si;
-          bi = br * si + bi * sr;
-          br = t;
+        S.gr[k] = br;
+        S.gi[k] = bi;
+      }
+      v4 cr[2][2], ci[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          cr[a][b] = v4{0, 0, 0, 0};
+          ci[a][b] = v4{0, 0, 0, 0};
         }
+      const bool active = (32 * wr < rows);
+      for (int k0 = 0; k0 < np; k0 += 16) {
+        __syncthreads();  // the step tables are written / the previous panel is consumed
+        {
+          // waves 0-1: W[k0 + kk][8a .. 8a+7]; waves 2-3: U[8a .. 8a+7][k0 + kk]
+          const bool wpanel = (tid < 128);
+          const int t7 = tid & 127, kk = t7 >> 3, a = t7 & 7, k = k0 + kk;
+          double qr, qi, sr, si, br, bi;
+          if (wpanel) {
+            qr = S.s8r[k];
+            qi = S.s8i[k];
+            sr = S.s1r[k];
+            si = S.s1i[k];
+            br = 1.0;
+            bi = 0.0;
+          } else {
+            qr = S.s512r[k];
+            qi = S.s512i[k];
+            sr = S.s64r[k];
+            si = S.s64i[k];
+            br = S.gr[k];
+            bi = S.gi[k];
+          }
+          // the group start: base * q^a
+#pragma unroll
+          for (int m = 0; m < 3; m++) {
+            if ((a >> m) & 1) cmul(br, bi, qr, qi);
+            if (m < 2) cmul(qr, qi, qr, qi);
+          }
+#pragma unroll
+          for (int b = 0; b < 8; b++) {
+            const int e = 8 * a + b;
+            if (wpanel) {
+              S.g.br[kk][e] = br;
+              S.g.bi[kk][e] = bi;
+            } else {
+              const bool ok = (e < rows);
+              S.g.ar[kk][e] = ok ? br : 0.0;
+              S.g.ai[kk][e] = ok ? bi : 0.0;
+            }
+            cmul(br, bi, sr, si);
+          }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+          for (int kk = 0; kk < 16; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            double xr[2], xi[2], nxi[2], yr[2], yi[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+              xr[t] = S.g.ar[kr][32 * wr + 16 * t + (lane & 15)];
+              xi[t] = S.g.ai[kr][32 * wr + 16 * t + (lane & 15)];
+              nxi[t] = -xi[t];
+              yr[t] = S.g.br[kr][32 * wc + 16 * t + (lane & 15)];
+              yi[t] = S.g.bi[kr][32 * wc + 16 * t + (lane & 15)];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) {
+                cr[a][b] = Mma<double>::fma(xr[a], yr[b], cr[a][b]);
+                ci[a][b] = Mma<double>::fma(xr[a], yi[b], ci[a][b]);
+              }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) {
+                cr[a][b] = Mma<double>::fma(nxi[a], yi[b], cr[a][b]);
+                ci[a][b] = Mma<double>::fma(xi[a], yr[b], ci[a][b]);
+              }
+          }
+        }
+      }
+      if (active) {
+        double* ob = out + 2 * (size_t)i0 * kSynthB;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              const int i = 32 * wr + 16 * a + Mma<double>::row(lane, e);
+              const int j = 32 * wc + 16 * b + (lane & 15);
+              if (i < rows) {
+                const size_t t = (size_t)i * kSynthB + j;
+                ob[2 * t] = cr[a][b][e];
+                ob[2 * t + 1] = ci[a][b][e];
+              }
+            }
+      }
+    }
+    // the points after the last full row: one term per thread, summed over the workgroup
+    for (int64_t t = nfull * kSynthB; t < npts; t++) {
+      double fr = 0.0, fi = 0.0;
+      for (int k = tid; k < n; k += kThreads) {
+        double sn, cs;
+        sincos(lam[k] * (double)t, &sn, &cs);
+        fr += gre[k] * cs + gim[k] * sn;
+        fi += gim[k] * cs - gre[k] * sn;
+      }
+      fr = wave_sum_dpp(fr);
+      fi = wave_sum_dpp(fi);
+      if (lane == 0) {
+        S.red[wave][0] = fr;
+        S.red[wave][1] = fi;
       }
       __syncthreads();
-      double* ob = out + 2 * (size_t)i0 * kSynthB;
-      wg_gemm_t<double, false, false, false, false>(rows, kSynthB, np, Ure, Uim, np, Wre, Wim, kSynthB,
-                                                    [=](int i, int j, double vr, double vi) {
-                                                      const size_t t = (size_t)i * kSynthB + j;
-                                                      ob[2 * t] = vr;
-                                                      ob[2 * t + 1] = vi;
-                                                    },
-                                                    g);
-    }
-    // the points after the last full block
-    for (int64_t t = nfull * kSynthB + tid; t < npts; t += kThreads) {
-      double fr = 0.0, fi = 0.0;
-      for (int k = 0; k < n; k++) {
-        double s, c;
-        sincos(lam[k] * (double)t, &s, &c);
-        fr += gre[k] * c + gim[k] * s;
-        fi += gim[k] * c - gre[k] * s;
+      if (tid == 0) {
+        double sr = 0.0, si = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) {
+          sr += S.red[w][0];
+          si += S.red[w][1];
+        }
+        out[2 * t] = sr;
+        out[2 * t + 1] = si;
       }
-      out[2 * t] = fr;
-      out[2 * t + 1] = fi;
+      __syncthreads();
     }
     __syncthreads();
   }

@@ -366,8 +366,8 @@ struct sp_plan {
   int64_t n_prop_complex = 0;  // the first n_prop_complex propagator items have a complex generator
   int32_t spectral_systems = 0;
   int grid_tri = 0, grid_syn = 0;
-  int64_t tri_stride = 0, syn_stride = 0;
-  DevBuf d_spec, d_status, d_work_tri, d_work_syn;
+  int64_t tri_stride = 0;
+  DevBuf d_spec, d_status, d_work_tri;
   // orientation sum: chunks of consecutive orientations, chunk ranges per system
   int64_t n_chunks = 0;
   DevBuf d_chunks, d_chunk_off, d_partial;
@@ -593,9 +593,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->tri_stride = ((int64_t)np_spec * np_spec + 63) / 64 * 64;
     pl->d_work_tri.alloc((size_t)pl->grid_tri * pl->tri_stride * sizeof(double));
     HIPCHK(hipMemsetAsync(pl->d_work_tri.p, 0, pl->d_work_tri.bytes, st));  // kept all-zero between items
-    pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)ctx->n_cu, pl->n_spec));
-    pl->syn_stride = (4 * (int64_t)np_spec * spk::kSynthB + 63) / 64 * 64;
-    pl->d_work_syn.alloc((size_t)pl->grid_syn * pl->syn_stride * sizeof(double));
+    pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, pl->n_spec));
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
@@ -689,7 +687,7 @@ void launch_spectral(sp_plan* pl, hipEvent_t* ev) {
   hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[1], st));
-  spk::SpecParams syn = spec_params(pl, pl->d_work_syn.as<double>(), pl->syn_stride);
+  spk::SpecParams syn = spec_params(pl, nullptr, 0);  // the synthesis needs no workspace
   hipLaunchKernelGGL(spk::k_synth, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[2], st));
