@@ -54,6 +54,15 @@ extern "C" {
  * orientation give f(t) = sum_k g_k exp(-i lambda_k t) directly (agreement with the
  * propagator path: ~1e-12 relative over thousands of steps). */
 #define SP_MODE_PROPAGATOR 32
+/* Flag OR-ed into the mode of a plan (sp_plan_powder, sp_plan_powder_batch,
+ * sp_plan_mas): the first sp_execute records the plan's whole pass (queue reset,
+ * propagation kernels, orientation sum) as a HIP graph on the context's stream,
+ * and every sp_execute launches that graph (one launch per pass instead of one
+ * per kernel). Results are identical to the plan without the flag. The stats of
+ * a graph execute carry the whole pass in kernel_ms and zeros in stage_ms.
+ * Replaces: nothing in the reference (an execution option for callers that
+ * replay a plan). */
+#define SP_MODE_GRAPH 64
 
 #define SP_NQ 34 /* Q slots: rank 1 (k,m) row-major 3x3, then rank 2 (k,m) row-major 5x5 */
 

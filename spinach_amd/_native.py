@@ -19,6 +19,7 @@ SP_MODE_FP64 = 0
 SP_MODE_FP32 = 1
 SP_MODE_COMPLEX = 16  # flag: keep complex arithmetic for a real generator
 SP_MODE_PROPAGATOR = 32  # flag: propagator power scheme for every system (no spectral path)
+SP_MODE_GRAPH = 64  # flag: record the pass once as a HIP graph and replay it
 SP_NQ = 34
 
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
@@ -205,9 +206,10 @@ def _tols(t):
 
 def mode_id(mode):
     """'fp64' / 'fp32' (or SP_MODE_*), with optional '+complex' (SP_MODE_COMPLEX:
-    complex arithmetic even for a real generator) and '+propagator'
-    (SP_MODE_PROPAGATOR: no spectral path) flags."""
-    flags = {'complex': SP_MODE_COMPLEX, 'propagator': SP_MODE_PROPAGATOR}
+    complex arithmetic even for a real generator), '+propagator'
+    (SP_MODE_PROPAGATOR: no spectral path) and '+graph' (SP_MODE_GRAPH: the pass
+    recorded once as a HIP graph and replayed) flags."""
+    flags = {'complex': SP_MODE_COMPLEX, 'propagator': SP_MODE_PROPAGATOR, 'graph': SP_MODE_GRAPH}
     if isinstance(mode, str) and '+' in mode:
         base, *extra = mode.split('+')
         out = mode_id(base)
@@ -216,7 +218,7 @@ def mode_id(mode):
                 raise ValueError(f'unknown mode flag {f!r}')
             out |= flags[f]
         return out
-    mask = SP_MODE_COMPLEX | SP_MODE_PROPAGATOR
+    mask = SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH
     if isinstance(mode, (int, np.integer)) and not isinstance(mode, bool) and int(mode) & mask:
         return mode_id(int(mode) & ~mask) | (int(mode) & mask)
     if mode in (SP_MODE_FP64, 'fp64', 'c128'):

@@ -374,6 +374,8 @@ struct sp_plan {
   DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
   DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
   bool executed = false;
+  bool graph = false;                // SP_MODE_GRAPH: the pass is recorded once and replayed
+  hipGraphExec_t gexec = nullptr;
   // magic-angle spinning plans
   bool mas = false;
   int32_t ticks = 0, n_rho = 0, mas_np = 0, mas_nnz = 0, mas_nq = 0;
@@ -437,7 +439,8 @@ int choose_log2_block(int64_t npts, int np) {
 sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode) {
   const bool force_complex = (mode & SP_MODE_COMPLEX) != 0;
   const bool force_prop = force_complex || (mode & SP_MODE_PROPAGATOR) != 0;
-  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR);
+  const bool graph = (mode & SP_MODE_GRAPH) != 0;
+  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH);
   if (!(std::isfinite(dt)) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
   if (nsteps < 0) throw SpError(SP_EINVAL, "nsteps must be non-negative");
   if (nsteps > (int64_t)1 << 30) throw SpError(SP_ERANGE, "nsteps is too large");
@@ -446,6 +449,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   std::unique_ptr<sp_plan> pl(new sp_plan);
   pl->ctx = ctx;
   pl->mode = mode;
+  pl->graph = graph;
   pl->dt = dt;
   pl->nsteps = nsteps;
   pl->n_systems = (int64_t)sps.size();
@@ -678,19 +682,19 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
 }
 
 // The three spectral kernels; ev[0..2] are recorded after each of them.
-void launch_spectral(sp_plan* pl, hipEvent_t* ev) {
+void launch_spectral(sp_plan* pl, hipEvent_t* ev) {  // ev == nullptr: no stage events (graph recording)
   hipStream_t st = pl->ctx->stream;
   spk::SpecParams tri = spec_params(pl, pl->d_work_tri.as<double>(), pl->tri_stride);
   hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[0], st));
+  if (ev) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[1], st));
+  if (ev) HIPCHK(hipEventRecord(ev[1], st));
   spk::SpecParams syn = spec_params(pl, nullptr, 0);  // the synthesis needs no workspace
   hipLaunchKernelGGL(spk::k_synth, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[2], st));
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
 }
 
 // Wigner matrices (rank 1 then rank 2, interleaved complex) at Euler angles.
@@ -720,7 +724,8 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   double an = std::sqrt(ax * ax + ay * ay + az * az);
   if (!(an > 0) || !std::isfinite(an)) throw SpError(SP_EINVAL, "rotor axis must be a non-zero finite vector");
   if (nsteps < 0 || nsteps > (int64_t)1 << 30) throw SpError(SP_EINVAL, "invalid nsteps");
-  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR);  // the Hilbert-space rotor path is always complex propagation
+  const bool graph = (mode & SP_MODE_GRAPH) != 0;
+  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH);  // the rotor path is always complex propagation
   if (mode != SP_MODE_FP64 && mode != SP_MODE_FP32) throw SpError(SP_EINVAL, "unknown precision mode");
   const int64_t N = I->n;
   if (N <= 0 || N > 1024) throw SpError(SP_ERANGE, "Hilbert dimension must be in 1..1024");
@@ -737,6 +742,7 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   pl->ctx = ctx;
   pl->mas = true;
   pl->mode = mode;
+  pl->graph = graph;
   pl->nsteps = nsteps;
   pl->n_systems = 1;
   pl->ticks = rot->ticks;
@@ -881,23 +887,24 @@ void launch_mas(sp_plan* pl) {
   HIPCHK(hipGetLastError());
 }
 
-int run(sp_plan* pl, sp_stats* stats) {
+// One pass on the context's stream: queue reset, propagation, orientation sum.
+// With stage events (eager execution) the boundaries ev1, evs[0..3], ev2 are
+// recorded; without (graph recording) nothing but the kernels is enqueued.
+void enqueue_pass(sp_plan* pl, bool events) {
   sp_ctx* ctx = pl->ctx;
-  HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int64_t npts = pl->nsteps + 1;
-  HIPCHK(hipEventRecord(ctx->ev0, st));
   hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, pl->d_queue.as<unsigned int>());
   HIPCHK(hipGetLastError());
-  if (pl->d_phase.p)
-    HIPCHK(hipMemsetAsync(pl->d_phase.p, 0, pl->d_phase.bytes, st));
-  HIPCHK(hipEventRecord(ctx->ev1, st));
+  if (events && pl->d_phase.p) HIPCHK(hipMemsetAsync(pl->d_phase.p, 0, pl->d_phase.bytes, st));
+  if (events) HIPCHK(hipEventRecord(ctx->ev1, st));
   if (pl->mas) {
     if (pl->mode == SP_MODE_FP64)
       launch_mas<double>(pl);
     else
       launch_mas<float>(pl);
-    for (auto& e : ctx->evs) HIPCHK(hipEventRecord(e, st));
+    if (events)
+      for (auto& e : ctx->evs) HIPCHK(hipEventRecord(e, st));
   } else {
     if (pl->n_prop > 0) {
       if (pl->mode == SP_MODE_FP64)
@@ -905,23 +912,56 @@ int run(sp_plan* pl, sp_stats* stats) {
       else
         launch<float>(pl);
     }
-    HIPCHK(hipEventRecord(ctx->evs[0], st));
+    if (events) HIPCHK(hipEventRecord(ctx->evs[0], st));
     if (pl->n_spec > 0) {
-      launch_spectral(pl, ctx->evs + 1);
-    } else {
+      launch_spectral(pl, events ? ctx->evs + 1 : nullptr);
+    } else if (events) {
       for (int k = 1; k < 4; k++) HIPCHK(hipEventRecord(ctx->evs[k], st));
     }
   }
-  HIPCHK(hipEventRecord(ctx->ev2, st));
-  {
-    const unsigned nb = (unsigned)((npts + 255) / 256);
-    hipLaunchKernelGGL(spk::k_chunk_sum, dim3(nb, (unsigned)pl->n_chunks), dim3(256), 0, st, pl->d_per.as<double>(),
-                       pl->d_w.as<double>(), pl->d_chunks.as<spk::SumChunk>(), npts, pl->d_partial.as<double>());
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(spk::k_system_sum, dim3(nb, (unsigned)pl->n_systems), dim3(256), 0, st,
-                       pl->d_partial.as<double>(), pl->d_chunk_off.as<int64_t>(), npts, pl->d_fid.as<double>());
-    HIPCHK(hipGetLastError());
+  if (events) HIPCHK(hipEventRecord(ctx->ev2, st));
+  const unsigned nb = (unsigned)((npts + 255) / 256);
+  hipLaunchKernelGGL(spk::k_chunk_sum, dim3(nb, (unsigned)pl->n_chunks), dim3(256), 0, st, pl->d_per.as<double>(),
+                     pl->d_w.as<double>(), pl->d_chunks.as<spk::SumChunk>(), npts, pl->d_partial.as<double>());
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(spk::k_system_sum, dim3(nb, (unsigned)pl->n_systems), dim3(256), 0, st,
+                     pl->d_partial.as<double>(), pl->d_chunk_off.as<int64_t>(), npts, pl->d_fid.as<double>());
+  HIPCHK(hipGetLastError());
+}
+
+// Records the pass as a graph (once per plan); the stream is left out of
+// capture mode whatever happens while recording.
+void record_pass(sp_plan* pl) {
+  hipStream_t st = pl->ctx->stream;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_pass(pl, false);
+  } catch (...) {
+    if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    throw;
   }
+  HIPCHK(hipStreamEndCapture(st, &g));
+  hipError_t ie = hipGraphInstantiate(&pl->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    pl->gexec = nullptr;
+    HIPCHK(ie);
+  }
+}
+
+int run(sp_plan* pl, sp_stats* stats) {
+  sp_ctx* ctx = pl->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  // phase timing (a debugging aid) clears its counters with a memset: eager only
+  const bool graph = pl->graph && !pl->d_phase.p;
+  if (graph && !pl->gexec) record_pass(pl);
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  if (graph)
+    HIPCHK(hipGraphLaunch(pl->gexec, st));
+  else
+    enqueue_pass(pl, true);
   hipEvent_t evend;
   HIPCHK(hipEventCreate(&evend));
   HIPCHK(hipEventRecord(evend, st));
@@ -929,9 +969,13 @@ int run(sp_plan* pl, sp_stats* stats) {
   float ms_all = 0, ms_k = 0, ms_stage[4] = {0, 0, 0, 0};
   if (se == hipSuccess) {
     (void)hipEventElapsedTime(&ms_all, ctx->ev0, evend);
-    (void)hipEventElapsedTime(&ms_k, ctx->ev1, ctx->ev2);
-    (void)hipEventElapsedTime(&ms_stage[0], ctx->ev1, ctx->evs[0]);
-    for (int k = 1; k < 4; k++) (void)hipEventElapsedTime(&ms_stage[k], ctx->evs[k - 1], ctx->evs[k]);
+    if (graph) {
+      ms_k = ms_all;
+    } else {
+      (void)hipEventElapsedTime(&ms_k, ctx->ev1, ctx->ev2);
+      (void)hipEventElapsedTime(&ms_stage[0], ctx->ev1, ctx->evs[0]);
+      for (int k = 1; k < 4; k++) (void)hipEventElapsedTime(&ms_stage[k], ctx->evs[k - 1], ctx->evs[k]);
+    }
   }
   (void)hipEventDestroy(evend);
   HIPCHK(se);
@@ -1147,6 +1191,7 @@ int sp_plan_result_batch(sp_plan* plan) {
 void sp_plan_destroy(sp_plan* plan) {
   if (!plan) return;
   (void)hipSetDevice(plan->ctx->device);
+  if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
   delete plan;
 }
 

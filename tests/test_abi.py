@@ -71,3 +71,10 @@ def test_struct_layouts_match_header(tmp_path):
 def test_last_error_without_context_is_a_string():
     lib = _native.load()
     assert isinstance(lib.sp_last_error(None), bytes)
+
+
+def test_graph_flag_in_mode_strings():
+    assert _native.mode_id('fp64+graph') == _native.SP_MODE_FP64 | _native.SP_MODE_GRAPH
+    assert _native.mode_id('fp32+propagator+graph') == (_native.SP_MODE_FP32 | _native.SP_MODE_PROPAGATOR
+                                                          | _native.SP_MODE_GRAPH)
+    assert _native.mode_id(_native.SP_MODE_GRAPH | _native.SP_MODE_FP32) == _native.SP_MODE_FP32 | _native.SP_MODE_GRAPH
