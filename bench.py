@@ -113,13 +113,14 @@ def time_plan(plan, steps, warmup):
     return total / steps, float(np.mean(kms)), st
 
 
-def stage_table(st, prop_flops, spec_flops, prop_kernel=None):
+def stage_table(st, prop_flops, spec_flops, prop_kernel=None, mode='fp64'):
     """{kernel: (ms, flops)} for the kernels that ran in a pass."""
     ms = st['stage_ms']
     out = {}
     if prop_flops:
         if prop_kernel is None:
-            prop_kernel = 'spk::k_powder<double,%s>' % ('true' if st['real_systems'] else 'false')
+            prop_kernel = 'spk::k_powder<%s,%s>' % ('float' if mode == 'fp32' else 'double',
+                                                    'true' if st['real_systems'] else 'false')
         out[prop_kernel] = (ms[0], prop_flops)
     if spec_flops:
         for k, name in enumerate(('spk::k_tridiag', 'spk::k_dceig', 'spk::k_synth')):
@@ -151,7 +152,7 @@ def bench_static(args, two_spin=False):
     else:
         stages = stage_table(st, n_or * powder_flops_per_orientation(
             n, nsteps, st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or, st['block'],
-            real=st['real_systems'] > 0), None)
+            real=st['real_systems'] > 0), None, mode=args.mode)
     cpu = err = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
@@ -204,7 +205,7 @@ def bench_batch(args):
             prop_flops += n_or * powder_flops_per_orientation(
                 n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'],
                 real=st['real_systems'] > 0)
-    stages = stage_table(st, prop_flops, spec_flops if spectral_ok else None)
+    stages = stage_table(st, prop_flops, spec_flops if spectral_ok else None, mode=args.mode)
     cpu = err = None
     if not args.no_cpu:
         # sample: 2 orientations of every 8th system, summed in the same way
