@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     const SysDesc sdesc = p.sys[item.sys];
     const int n = sdesc.n, np = sdesc.np;
     const int64_t o = item.orient;
-    if (tid == 0) wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], cre, cim);
+    if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
     __syncthreads();
     scatter_real_generator(p, sdesc, cre, cim, W, false);
     {

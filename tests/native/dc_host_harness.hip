@@ -32,3 +32,14 @@ extern "C" int dc_host_solve(const double* in, int n, double* out, int* status, 
   *status = S.fail ? -1 : mx;
   return 0;
 }
+
+// the per-slot Wigner coefficients (one thread per slot on the device) against
+// the whole-set function, bit for bit: returns the number of differing entries
+extern "C" int wigner_slot_mismatches(double a, double b, double g) {
+  double cre[34], cim[34], sre[34], sim[34];
+  spk::wigner_coeffs(a, b, g, cre, cim);
+  for (int t = 0; t < 34; t++) spk::wigner_coeff_slot(a, b, g, t, sre, sim);
+  int bad = 0;
+  for (int t = 0; t < 34; t++) bad += (cre[t] != sre[t]) + (cim[t] != sim[t]);
+  return bad;
+}

@@ -96,3 +96,13 @@ def test_8spin_orientations(harness):
         le, fe, stat, lev = solve(harness, 5e-6 * H.real, w['coil'][st], w['rho0'][st])
         assert lev >= 1 and stat > 0
         assert le < 1e-13 and fe < 1e-10, (le, fe)
+
+
+def test_wigner_slot_matches_whole_set(harness):
+    """k_tridiag computes the 34 Wigner coefficients one slot per thread; the
+    per-slot function must reproduce the whole-set one exactly."""
+    harness.wigner_slot_mismatches.argtypes = [ctypes.c_double] * 3
+    harness.wigner_slot_mismatches.restype = ctypes.c_int
+    rng = np.random.default_rng(11)
+    for a, b, g in rng.uniform([0, 0, 0], [2 * np.pi, np.pi, 2 * np.pi], (50, 3)):
+        assert harness.wigner_slot_mismatches(a, b, g) == 0
