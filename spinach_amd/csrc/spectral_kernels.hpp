@@ -55,7 +55,7 @@ struct SpecParams {
   double* work;          // per-workgroup workspace of k_tridiag: dense np x np, all zero between uses
   int64_t work_stride;   // doubles per workgroup
   double* spec;          // kSpecStride doubles per spectral item
-  int32_t* status;       // per item: QR sweeps used, or -1 if the iteration did not converge
+  int32_t* status;       // per item: most secular evaluations of a root, or -1 if a solve failed
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
 };
 
