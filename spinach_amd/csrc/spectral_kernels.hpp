@@ -571,41 +571,42 @@ __host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z
   double fprev = 0.0;
   int it = 1;
   for (; it <= kDcMaxIter; it++) {
-    // terms split by sign: the poles below lambda give w < 0 (psi), those above w > 0 (phi)
-    double wpos = 0.0, wneg = 0.0, dpos = 0.0, dneg = 0.0;
+    // f, sum |w|, f' = sum w/delta and sum w/|delta|: the poles below lambda (psi side,
+    // delta < 0) enter the last with a minus sign, so the two slopes are the half-difference
+    // and half-sum of the derivative sums (two FMAs instead of a split by sign)
+    double fs = 0.0, wabs = 0.0, dw = 0.0, sq = 0.0;
 #pragma unroll 4
     for (int i = 0; i < K - 1; i++) {
       const double rd = dc_rcp((Dc[i] - dorg) - tau);
       const double w = Z2[i] * rd;
-      const double wp = fmax(w, 0.0), wn = fmin(w, 0.0);
-      wpos += wp;
-      wneg += wn;
-      dpos = fma(wp, rd, dpos);
-      dneg = fma(wn, rd, dneg);
+      fs += w;
+      wabs += fabs(w);
+      dw = fma(w, rd, dw);
+      sq = fma(w, fabs(rd), sq);
     }
     // the last pole: the far pole of the outermost root's model, else a term as the rest
     const double rl = dc_rcp((Dc[K - 1] - dorg) - tau);
     const double wl = Z2[K - 1] * rl;
+    fs += wl;
+    wabs += fabs(wl);
     double psip, phip;
     if (outer) {
-      wneg += wl;
-      psip = dneg;
+      psip = dw;  // every other pole lies below lambda
       phip = wl * rl;
+      dw += phip;
     } else {
-      const double wp = fmax(wl, 0.0), wn = fmin(wl, 0.0);
-      wpos += wp;
-      wneg += wn;
-      psip = fma(wn, rl, dneg);
-      phip = fma(wp, rl, dpos);
+      dw = fma(wl, rl, dw);
+      sq = fma(wl, fabs(rl), sq);
+      psip = 0.5 * (dw - sq);
+      phip = 0.5 * (dw + sq);
     }
-    const double fv = rinv + wpos + wneg;
+    const double fv = rinv + fs;
     if (fv > 0.0)
       thi = tau;
     else
       tlo = tau;
-    if (fabs(fv) <= 8.0 * eps * (rinv + wpos - wneg) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) break;
+    if (fabs(fv) <= 8.0 * eps * (rinv + wabs) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) break;
     const double d1 = e1 - tau, d2 = e2 - tau;
-    const double dw = psip + phip;
     if (fv * fprev > 0.0 && fabs(fv) > 0.1 * fabs(fprev)) fixed_weight = !fixed_weight;
     fprev = fv;
     double c;
