@@ -392,6 +392,7 @@ constexpr int kDcThreads = 256;
 constexpr int kDcLeaf = 6;
 constexpr int kDcMaxLevels = 5;  // up to 32 leaves
 constexpr int kDcMaxIter = 100;  // secular-equation iterations per root (typically 3-8)
+constexpr int kDcPass1 = 4;      // evaluations per root before the open roots are packed (see M3a)
 
 struct DcShared {
   double d[kTriNP], e[kTriNP];
@@ -406,6 +407,7 @@ struct DcShared {
   double zsum[(1 << kDcMaxLevels)];                     // sum of the kept z^2 of each merge
   int K[(1 << kDcMaxLevels)];
   int its[kTriNP];
+  unsigned long long pend[kDcThreads / 64];  // roots the first secular pass left open, by thread
   int nlev;
   int fail;
 };
@@ -510,16 +512,31 @@ __host__ __device__ inline void dc_sort_segment(DcShared& S, int a, int b) {
 // unchanged sign; for the outermost root the model's root beyond the last pole. Both models
 // match f and f' at the current point, so they converge at least quadratically: a step
 // below 1e-9 |tau| is taken and ends the iteration. The bracket guards every step (a
-// bisection replaces a step that leaves it). Returns the number of evaluations of f
-// (kDcMaxIter + 1 if it did not converge).
-__host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z2, int K, int k, double rho, double zs,
-                                               int& org, double& tau) {
-  const double eps = 2.220446049250313e-16;
+// bisection replaces a step that leaves it). dc_secular_start and dc_secular_run below
+// split this into the start and a resumable iteration.
+
+// The state of one root's iteration, so that it can stop after a given number of
+// evaluations of f and resume (in another lane) later.
+struct SecState {
+  double tau, tlo, thi, fprev;
+  int org, it, fixed_weight;
+};
+
+// The start of root k (org, tau, bracket); returns true if that is already the
+// root (K == 1: lambda = d + rho z^2).
+__host__ __device__ __attribute__((always_inline)) inline bool dc_secular_start(const double* Dc, const double* Z2,
+                                                                                 int K, int k, double rho, double zs,
+                                                                                 SecState& st) {
   const double rinv = 1.0 / rho;
+  st.it = 1;
+  st.fixed_weight = 1;
+  st.fprev = 0.0;
   if (K == 1) {
-    org = 0;
-    tau = rho * Z2[0];
-    return 1;
+    st.org = 0;
+    st.tau = rho * Z2[0];
+    st.tlo = 0.0;
+    st.thi = st.tau;
+    return true;
   }
   const bool outer = (k == K - 1);
   const int kk = outer ? K - 2 : k;  // the model's poles: kk and kk + 1
@@ -531,46 +548,57 @@ __host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z
 #pragma unroll 4
   for (int i = 0; i < K; i++) f += Z2[i] * dc_rcp((Dc[i] - dbase) - x0);
   const double s2 = Z2[kk] + Z2[kk + 1];
-  double tlo, thi;
+  // the bracket and the origin as values (no branch assigns them, so they stay in registers)
+  const bool left = outer ? true : (f >= 0.0);  // interior: root in (Dc[k], midpoint]
+  const int org = outer ? K - 1 : (left ? k : k + 1);
+  const double tlo = outer ? ((f > 0.0) ? 0.0 : x0) : (left ? 0.0 : -x0);
+  const double thi = outer ? ((f > 0.0) ? x0 : 2.0 * x0) : (left ? x0 : 0.0);
+  double tau;
   if (!outer) {
     // the other terms at the midpoint: f less Z2_kk/(-x0) and Z2_kk+1/x0
     const double c = f + (Z2[kk] - Z2[kk + 1]) * dc_rcp(x0);
-    if (f >= 0.0) {  // root in (Dc[k], midpoint]: c t^2 - bq t + aq = 0, t = lambda - Dc[k]
-      org = k;
-      tlo = 0.0;
-      thi = x0;
+    if (left) {  // c t^2 - bq t + aq = 0, t = lambda - Dc[k]
       const double bq = c * gap + s2, aq = Z2[kk] * gap;
       const double r = sqrt(fabs(bq * bq - 4.0 * aq * c));
       tau = (bq > 0.0) ? 2.0 * aq / (bq + r) : (bq - r) / (2.0 * c);
     } else {  // root in (midpoint, Dc[k+1]): c t^2 - aq t - bq = 0, t = lambda - Dc[k+1]
-      org = k + 1;
-      tlo = -x0;
-      thi = 0.0;
       const double aq = s2 - c * gap, bq = Z2[kk + 1] * gap;
       const double r = sqrt(fabs(aq * aq + 4.0 * c * bq));
       tau = (aq > 0.0) ? -2.0 * bq / (aq + r) : (aq - r) / (2.0 * c);
     }
   } else {  // c t^2 - aq t - bq = 0 for t = lambda - Dc[K-1] > 0
-    org = K - 1;
     const double c = f + Z2[kk] * dc_rcp(gap + x0) + Z2[kk + 1] * dc_rcp(x0);
-    if (f > 0.0) {
-      tlo = 0.0;
-      thi = x0;
-    } else {
-      tlo = x0;
-      thi = 2.0 * x0;
-    }
     const double aq = s2 - c * gap, bq = Z2[kk + 1] * gap;
     const double r = sqrt(fabs(aq * aq + 4.0 * c * bq));
     tau = (aq >= 0.0) ? (aq + r) / (2.0 * c) : 2.0 * bq / (r - aq);
   }
   if (!(tau > tlo && tau < thi)) tau = 0.5 * (tlo + thi);
+  st.org = org;
+  st.tau = tau;
+  st.tlo = tlo;
+  st.thi = thi;
+  return false;
+}
+
+// Evaluations of f and model steps from the state st until root k has converged
+// (returns true; st.tau final, st.it the evaluations used) or st.it exceeds
+// limit (returns false; st holds everything needed to resume).
+__host__ __device__ __attribute__((always_inline)) inline bool dc_secular_run(const double* Dc, const double* Z2, int K,
+                                                                               int k, double rho, SecState& st,
+                                                                               int limit) {
+  const double eps = 2.220446049250313e-16;
+  const double rinv = 1.0 / rho;
+  const bool outer = (k == K - 1);
+  const int kk = outer ? K - 2 : k;  // the model's poles: kk and kk + 1
+  const double gap = Dc[kk + 1] - Dc[kk];
+  const int org = st.org;
+  double tau = st.tau, tlo = st.tlo, thi = st.thi, fprev = st.fprev;
+  bool fixed_weight = st.fixed_weight != 0;
   const double dorg = Dc[org];
   const double e1 = Dc[kk] - dorg, e2 = Dc[kk + 1] - dorg;  // the model's poles from the origin
-  bool fixed_weight = true;
-  double fprev = 0.0;
-  int it = 1;
-  for (; it <= kDcMaxIter; it++) {
+  int it = st.it;
+  bool done = false;
+  for (; it <= limit; it++) {
     // f, sum |w|, f' = sum w/delta and sum w/|delta|: the poles below lambda (psi side,
     // delta < 0) enter the last with a minus sign, so the two slopes are the half-difference
     // and half-sum of the derivative sums (two FMAs instead of a split by sign)
@@ -605,7 +633,10 @@ __host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z
       thi = tau;
     else
       tlo = tau;
-    if (fabs(fv) <= 8.0 * eps * (rinv + wabs) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) break;
+    if (fabs(fv) <= 8.0 * eps * (rinv + wabs) || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) {
+      done = true;
+      break;
+    }
     const double d1 = e1 - tau, d2 = e2 - tau;
     if (fv * fprev > 0.0 && fabs(fv) > 0.1 * fabs(fprev)) fixed_weight = !fixed_weight;
     fprev = fv;
@@ -633,10 +664,21 @@ __host__ __device__ inline int dc_secular_root(const double* Dc, const double* Z
     }
     const bool small = fabs(eta) <= 1e-9 * fabs(tau);
     tau = nt;
-    if (small) break;
+    if (small) {
+      done = true;
+      break;
+    }
   }
-  return it;
+  st.tau = tau;
+  st.tlo = tlo;
+  st.thi = thi;
+  st.fprev = fprev;
+  st.fixed_weight = fixed_weight ? 1 : 0;
+  st.it = it;
+  return done;
 }
+
+
 
 // merge index of element i at level l (bounded by the level's segment count)
 __host__ __device__ inline int dc_segment_of(const DcShared& S, int l, int i) {
@@ -729,6 +771,8 @@ __host__ __device__ inline void dc_phase_m1(DcShared& S, int l, int n, int i) {
 // one, whose slot the rotated pole takes)
 __host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
   if (s2 >= (1 << l)) return;
+  if (s2 == 0)
+    for (int w = 0; w < kDcThreads / 64; w++) S.pend[w] = 0;
   const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1], m = S.bnd[l + 1][2 * s2 + 1];
   const double bcoup = (m > a && m < b) ? S.e[m - 1] : 0.0;
   const double rho = 2.0 * fabs(bcoup);
@@ -778,20 +822,83 @@ __host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
   S.zsum[s2] = zs;
 }
 
-// M3: secular roots (one thread per (merge, root))
-__host__ __device__ inline void dc_phase_m3(DcShared& S, int l, int n, int i) {
+// the position of the r-th (from 0) set bit of m
+__host__ __device__ inline int dc_select_bit(unsigned long long m, int r) {
+  int pos = 0;
+  for (int width = 32; width >= 1; width >>= 1) {
+    const unsigned long long low = m & ((1ull << width) - 1);
+    const int c = __builtin_popcountll(low);
+    if (r >= c) {
+      r -= c;
+      m >>= width;
+      pos += width;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+__host__ __device__ inline void dc_root_store(DcShared& S, int i, int a, int k, const SecState& st) {
+  if (st.it > S.its[i]) S.its[i] = st.it;
+  S.org[a + k] = st.org;
+  S.tau[a + k] = st.tau;
+  S.lorg[a + k] = S.Dc[a + st.org];
+}
+
+// M3a: secular roots, first pass (one thread per (merge, root)): at most kDcPass1
+// evaluations of f per root. A root still open leaves its state in X (unused
+// between M1 and M5) at its thread index and its bit in pend, so that M3b can
+// finish the open roots packed into as few waves as they fill: a wave's
+// iterations cost its slowest lane, and most roots converge in 2-4.
+__host__ __device__ inline void dc_phase_m3a(DcShared& S, int l, int n, int i) {
   if (i >= n) return;
   const int s2 = dc_segment_of(S, l, i);
   const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
   if (k >= K) return;
-  int og;
-  double tu;
-  const int its = dc_secular_root(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], S.zsum[s2], og, tu);
-  if (its > kDcMaxIter) S.fail = 1;
-  if (its > S.its[i]) S.its[i] = its;
-  S.org[a + k] = og;
-  S.tau[a + k] = tu;
-  S.lorg[a + k] = S.Dc[a + og];
+  SecState st;
+  bool done = dc_secular_start(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], S.zsum[s2], st);
+  if (!done) done = dc_secular_run(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], st, kDcPass1);
+  if (done) {
+    dc_root_store(S, i, a, k, st);
+    return;
+  }
+  S.X[0][i] = st.tau;
+  S.X[1][i] = st.tlo;
+  S.X[2][i] = st.thi;
+  S.X[3][i] = st.fprev;
+  S.X[4][i] = (double)st.org;
+  S.X[5][i] = (double)(2 * st.it + st.fixed_weight);
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicOr(&S.pend[i >> 6], 1ull << (i & 63));
+#else
+  S.pend[i >> 6] |= 1ull << (i & 63);
+#endif
+}
+
+// M3b: thread u finishes the u-th open root (in thread order)
+__host__ __device__ inline void dc_phase_m3b(DcShared& S, int l, int u) {
+  int w = 0, base = 0;
+  for (; w < kDcThreads / 64; w++) {
+    const int c = __builtin_popcountll(S.pend[w]);
+    if (u < base + c) break;
+    base += c;
+  }
+  if (w == kDcThreads / 64) return;
+  const int i = 64 * w + dc_select_bit(S.pend[w], u - base);
+  const int s2 = dc_segment_of(S, l, i);
+  const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
+  SecState st;
+  st.tau = S.X[0][i];
+  st.tlo = S.X[1][i];
+  st.thi = S.X[2][i];
+  st.fprev = S.X[3][i];
+  st.org = (int)S.X[4][i];
+  const int code = (int)S.X[5][i];
+  st.it = code >> 1;
+  st.fixed_weight = code & 1;
+  if (!dc_secular_run(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], st, kDcMaxIter)) S.fail = 1;
+  dc_root_store(S, i, a, k, st);
 }
 
 // M4: z recomputed from the computed roots (Gu & Eisenstat), one thread per (merge, pole):
@@ -903,7 +1010,9 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
     __syncthreads();
     dc_phase_m2(S, l, tid);
     __syncthreads();
-    dc_phase_m3(S, l, n, tid);
+    dc_phase_m3a(S, l, n, tid);
+    __syncthreads();
+    dc_phase_m3b(S, l, tid);
     __syncthreads();
     dc_phase_m4(S, l, n, tid);
     __syncthreads();

@@ -18,7 +18,8 @@ extern "C" int dc_host_solve(const double* in, int n, double* out, int* status, 
   for (int l = S.nlev - 1; l >= 0; l--) {
     for (int t = 0; t < T; t++) dc_phase_m1(S, l, n, t);
     for (int t = 0; t < T; t++) dc_phase_m2(S, l, t);
-    for (int t = 0; t < T; t++) dc_phase_m3(S, l, n, t);
+    for (int t = 0; t < T; t++) dc_phase_m3a(S, l, n, t);
+    for (int t = 0; t < T; t++) dc_phase_m3b(S, l, t);
     for (int t = 0; t < T; t++) dc_phase_m4(S, l, n, t);
     for (int t = 0; t < T; t++) dc_phase_m5(S, l, n, t);
     if (l > 0) {
@@ -41,5 +42,25 @@ extern "C" int wigner_slot_mismatches(double a, double b, double g) {
   for (int t = 0; t < 34; t++) spk::wigner_coeff_slot(a, b, g, t, sre, sim);
   int bad = 0;
   for (int t = 0; t < 34; t++) bad += (cre[t] != sre[t]) + (cim[t] != sim[t]);
+  return bad;
+}
+
+// dc_select_bit against a plain scan over pseudo-random masks: returns the
+// number of disagreements
+extern "C" int select_bit_mismatches(int trials) {
+  unsigned long long x = 88172645463325252ull;
+  int bad = 0;
+  for (int t = 0; t < trials; t++) {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    const unsigned long long m = (t % 3 == 0) ? (x & (x >> 11)) : x;  // sparser masks too
+    int r = 0;
+    for (int b = 0; b < 64; b++)
+      if ((m >> b) & 1ull) {
+        if (spk::dc_select_bit(m, r) != b) bad++;
+        r++;
+      }
+  }
   return bad;
 }

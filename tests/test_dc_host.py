@@ -106,3 +106,10 @@ def test_wigner_slot_matches_whole_set(harness):
     rng = np.random.default_rng(11)
     for a, b, g in rng.uniform([0, 0, 0], [2 * np.pi, np.pi, 2 * np.pi], (50, 3)):
         assert harness.wigner_slot_mismatches(a, b, g) == 0
+
+
+def test_select_bit(harness):
+    """The second secular pass finds its root as the r-th set bit of a mask."""
+    harness.select_bit_mismatches.argtypes = [ctypes.c_int]
+    harness.select_bit_mismatches.restype = ctypes.c_int
+    assert harness.select_bit_mismatches(20000) == 0
