@@ -937,11 +937,20 @@ __host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
     const double tu = S.tau[a + slot], dorg = S.lorg[a + slot];
     double nrm = 0.0;
     for (int q = 0; q < 6; q++) out[q] = 0.0;
-    for (int i = 0; i < K; i++) {
-      const double u = S.zh[a + i] * dc_rcp((S.Dc[a + i] - dorg) - tu);
-      nrm = fma(u, u, nrm);
-      const int src = S.keep[a + i];
-      for (int q = 0; q < 6; q++) out[q] = fma(u, S.Y[q][src], out[q]);
+    if (l > 0) {
+      for (int i = 0; i < K; i++) {
+        const double u = S.zh[a + i] * dc_rcp((S.Dc[a + i] - dorg) - tu);
+        nrm = fma(u, u, nrm);
+        const int src = S.keep[a + i];
+        for (int q = 0; q < 6; q++) out[q] = fma(u, S.Y[q][src], out[q]);
+      }
+    } else {  // the top merge has no parent: its first and last eigenvector rows are not needed
+      for (int i = 0; i < K; i++) {
+        const double u = S.zh[a + i] * dc_rcp((S.Dc[a + i] - dorg) - tu);
+        nrm = fma(u, u, nrm);
+        const int src = S.keep[a + i];
+        for (int q = 0; q < 4; q++) out[q] = fma(u, S.Y[q][src], out[q]);
+      }
     }
     const double inv = 1.0 / sqrt(nrm);
     for (int q = 0; q < 6; q++) out[q] *= inv;
