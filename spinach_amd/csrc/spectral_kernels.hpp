@@ -297,7 +297,10 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         const double K = 0.5 * tau * qt;
         if (wave == own) {
           // look-ahead: row jn after this update, from its copy, and its reflection for the
-          // next step (the same operations as the register update below)
+          // next step (the same operations as the register update below); with its own update
+          // after it, the owner wave is the step's critical path, so its instructions go ahead
+          // of the partner wave's until the end of the update
+          __builtin_amdgcn_s_setprio(3);
           const double vn = v[jn], wn = sp[jn] - K * vn;
           double x[kTriNP / 64];
 #pragma unroll
@@ -315,6 +318,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           // the owners'): u <- u - tau*v*(v'u)
           const int q = (wave - own - 1) & 7;
           if (q < 4) {
+            __builtin_amdgcn_s_setprio(1);  // these waves carry more of the half-step than the rest
             double vv[kTriNP / 64], uu[kTriNP / 64];
             double s = 0.0;
 #pragma unroll
@@ -354,6 +358,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
                 R[a][4 * bc + q] = fma(-vi[a], wb[q], fma(-wi[a], vb[q], R[a][4 * bc + q]));
           }
         }
+        __builtin_amdgcn_s_setprio(0);
       } else {
         // no reflection at this step: row jn stands as it is in the registers
         householder_publish(R, jn, tr, tc, lane, wave, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
