@@ -1017,12 +1017,18 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
   __syncthreads();
   if (tid == 0) dc_phase_plan(S, n);
   __syncthreads();
+  // the leaves and the deflation scans are serial on a few lanes of wave 0 while the
+  // rest of the workgroup waits: their wave runs ahead of other workgroups' waves
+  if (tid < 64) __builtin_amdgcn_s_setprio(3);
   dc_phase_leaf(S, tid);
+  if (tid < 64) __builtin_amdgcn_s_setprio(0);
   __syncthreads();
   for (int l = S.nlev - 1; l >= 0; l--) {
     dc_phase_m1(S, l, n, tid);
     __syncthreads();
+    if (tid < 64) __builtin_amdgcn_s_setprio(3);
     dc_phase_m2(S, l, tid);
+    if (tid < 64) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     dc_phase_m3a(S, l, n, tid);
     __syncthreads();
