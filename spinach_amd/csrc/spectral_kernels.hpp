@@ -261,8 +261,12 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       const int jn = j + 1;         // the next row to reflect (jn + 1 <= n - 1 < kTriNP)
       const int own = row_owner_wave(jn);
       if (tau != 0.0) {  // uniform
-        // the owners of row jn copy it (as the previous step left it) for the look-ahead below
-        if (wave == own) copy_row(R, jn, tr, tc, srow);
+        // the owners of row jn copy it (as the previous step left it) for the look-ahead below;
+        // their wave carries the most work of the step and runs at raised priority throughout
+        if (wave == own) {
+          __builtin_amdgcn_s_setprio(3);
+          copy_row(R, jn, tr, tc, srow);
+        }
         // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread
         double part[kTriRowsPer] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -297,10 +301,7 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         const double K = 0.5 * tau * qt;
         if (wave == own) {
           // look-ahead: row jn after this update, from its copy, and its reflection for the
-          // next step (the same operations as the register update below); with its own update
-          // after it, the owner wave is the step's critical path, so its instructions go ahead
-          // of the partner wave's until the end of the update
-          __builtin_amdgcn_s_setprio(3);
+          // next step (the same operations as the register update below)
           const double vn = v[jn], wn = sp[jn] - K * vn;
           double x[kTriNP / 64];
 #pragma unroll
