@@ -174,22 +174,25 @@ __host__ __device__ inline void wigner_coeffs(double a, double b, double g, doub
 
 // coefficient of Q slot t (0..33) alone, for one thread per slot: the same
 // formulas, so the same values, as wigner_coeffs
+// (the element is picked by a select over constant indices, so that the small
+// matrices stay in registers)
 __host__ __device__ inline void wigner_coeff_slot(double a, double b, double g, int t, double* cre, double* cim) {
+  double dv = 0.0, ph;
   if (t < 9) {
     double d1[3][3];
     wigner_small_d1(b, d1);
     const int i = t / 3, j = t % 3;
-    const double ph = -(1 - i) * a - (1 - j) * g;
-    cre[t] = d1[i][j] * cos(ph);
-    cim[t] = d1[i][j] * sin(ph);
+    for (int q = 0; q < 9; q++) dv = (q == t) ? d1[q / 3][q % 3] : dv;
+    ph = -(1 - i) * a - (1 - j) * g;
   } else {
     double d2[5][5];
     wigner_small_d2(b, d2);
     const int i = (t - 9) / 5, j = (t - 9) % 5;
-    const double ph = -(2 - i) * a - (2 - j) * g;
-    cre[t] = d2[i][j] * cos(ph);
-    cim[t] = d2[i][j] * sin(ph);
+    for (int q = 0; q < 25; q++) dv = (q == t - 9) ? d2[q / 5][q % 5] : dv;
+    ph = -(2 - i) * a - (2 - j) * g;
   }
+  cre[t] = dv * cos(ph);
+  cim[t] = dv * sin(ph);
 }
 
 // -------------------------------------------------------------- MFMA traits
