@@ -257,6 +257,12 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     for (int j = 0; j < n - 2; j++) {
       const int cb = j & 1;
       const double* v = sv[cb];
+      // the last column chunk (columns 160..191) is live in every step (j <= n - 3 <= 189):
+      // its v loads are issued with the read of tau, ahead of the branch on it
+      constexpr int kLast = kTriColsPer / 4 - 1;
+      double vlast[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) vlast[q] = v[tc + kTriTC * (4 * kLast + q)];
       const double tau = stau[cb];  // reflection of row j (published by the owners of row j)
       const int jn = j + 1;         // the next row to reflect (jn + 1 <= n - 1 < kTriNP)
       const int own = row_owner_wave(jn);
@@ -267,14 +273,15 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
           __builtin_amdgcn_s_setprio(3);
           copy_row(R, jn, tr, tc, srow);
         }
-        // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread
+        // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread, from the last
+        // chunk down (the eliminated chunks are skipped)
         double part[kTriRowsPer] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int bc = 0; bc < kTriColsPer / 4; bc++) {
-          if (4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
+        for (int bc = kLast; bc >= 0; bc--) {
+          if (bc == kLast || 4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
             double vb[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) vb[q] = v[tc + kTriTC * (4 * bc + q)];
+            for (int q = 0; q < 4; q++) vb[q] = (bc == kLast) ? vlast[q] : v[tc + kTriTC * (4 * bc + q)];
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
