@@ -415,6 +415,7 @@ struct DcShared {
   double Dc[kTriNP], Zc[kTriNP], Z2c[kTriNP], zh[kTriNP], tau[kTriNP];
   double lorg[kTriNP];  // Dc[org] of each root: lambda = lorg + tau
   int org[kTriNP], keep[kTriNP], defl[kTriNP];
+  int seg[kTriNP];  // merge index of each element at the current level (set by M1)
   int bnd[kDcMaxLevels + 1][(1 << kDcMaxLevels) + 1];  // segment boundaries per level
   double beta[(1 << kDcMaxLevels)];                     // rho = 2|coupling| of the merges of a level
   double zsum[(1 << kDcMaxLevels)];                     // sum of the kept z^2 of each merge
@@ -755,6 +756,7 @@ __host__ __device__ inline void dc_phase_leaf(DcShared& S, int tid) {
 __host__ __device__ inline void dc_phase_m1(DcShared& S, int l, int n, int i) {
   if (i >= n) return;
   const int s2 = dc_segment_of(S, l, i);
+  S.seg[i] = s2;
   const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1], m = S.bnd[l + 1][2 * s2 + 1];
   const double bcoup = (m > a && m < b) ? S.e[m - 1] : 0.0;
   const double sg = (bcoup >= 0.0) ? 1.0 : -1.0;
@@ -866,7 +868,7 @@ __host__ __device__ inline void dc_root_store(DcShared& S, int i, int a, int k, 
 // iterations cost its slowest lane, and most roots converge in 2-4.
 __host__ __device__ inline void dc_phase_m3a(DcShared& S, int l, int n, int i) {
   if (i >= n) return;
-  const int s2 = dc_segment_of(S, l, i);
+  const int s2 = S.seg[i];
   const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
   if (k >= K) return;
   SecState st;
@@ -899,7 +901,7 @@ __host__ __device__ inline void dc_phase_m3b(DcShared& S, int l, int u) {
   }
   if (w == kDcThreads / 64) return;
   const int i = 64 * w + dc_select_bit(S.pend[w], u - base);
-  const int s2 = dc_segment_of(S, l, i);
+  const int s2 = S.seg[i];
   const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
   SecState st;
   st.tau = S.X[0][i];
@@ -919,7 +921,7 @@ __host__ __device__ inline void dc_phase_m3b(DcShared& S, int l, int u) {
 // taken four factors at a time with one reciprocal per group
 __host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
   if (t >= n) return;
-  const int s2 = dc_segment_of(S, l, t);
+  const int s2 = S.seg[t];
   const int a = S.bnd[l][s2], i = t - a, K = S.K[s2];
   if (i >= K) return;
   const double di = S.Dc[a + i];
@@ -942,7 +944,7 @@ __host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
 // M5: eigenpairs of each merge into d, X: roots in slots 0..K-1, deflated pairs after
 __host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
   if (t >= n) return;
-  const int s2 = dc_segment_of(S, l, t);
+  const int s2 = S.seg[t];
   const int a = S.bnd[l][s2], slot = t - a, K = S.K[s2];
   double out[6];
   double lam;
@@ -983,7 +985,7 @@ __host__ __device__ inline void dc_phase_m6a(DcShared& S, int l, int n, int t) {
   if (t >= kTriNP) return;
   S.org[t] = -1;
   if (t >= n) return;
-  const int s2 = dc_segment_of(S, l, t);
+  const int s2 = S.seg[t];
   const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1];
   const double lam = S.d[t];
   int r = 0;
