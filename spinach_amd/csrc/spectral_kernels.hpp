@@ -797,25 +797,35 @@ __host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
   const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, rho);
   int K = 0, nd = 0, prev = -1;
   double zs = 0.0;  // sum of the kept z^2 (a rotation keeps the sum of the pair's squares)
+  // the last kept element's z and D as they stand are carried in registers, and the next
+  // element's are read one iteration ahead (an iteration writes only elements <= i)
+  double zp = 0.0, dp = 0.0;
+  double zn = (a < b) ? S.z[a] : 0.0, dn = (a < b) ? S.D[a] : 0.0;
   for (int i = a; i < b; i++) {
-    const double zi = S.z[i];
+    const double zi = zn, di = dn;
+    if (i + 1 < b) {
+      zn = S.z[i + 1];
+      dn = S.D[i + 1];
+    }
     if (rho == 0.0 || rho * fabs(zi) <= tol) {
       S.defl[a + nd++] = i;
       continue;
     }
     zs = fma(zi, zi, zs);
+    double zk = zi, dk = di;
     if (prev >= 0) {
-      const double zp = S.z[prev], t = S.D[i] - S.D[prev];
+      const double t = di - dp;
       const double zz = fma(zp, zp, zi * zi);
       // |t c s| <= tol for the rotation c = zi/|z|, s = -zp/|z|, tested without the root
       if (fabs(t * zi * zp) <= tol * zz) {
         const double tz = sqrt(zz);
         const double c = zi / tz, s = -zp / tz;
+        zk = tz;
         S.z[i] = tz;
         S.z[prev] = 0.0;
-        const double dp = S.D[prev], di = S.D[i];
         S.D[prev] = dp * c * c + di * s * s;
-        S.D[i] = dp * s * s + di * c * c;
+        dk = dp * s * s + di * c * c;
+        S.D[i] = dk;
         for (int q = 0; q < 6; q++) {
           const double u0 = S.Y[q][prev], u1 = S.Y[q][i];
           S.Y[q][prev] = c * u0 + s * u1;
@@ -825,13 +835,14 @@ __host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
         K--;
       }
     }
-    const double zk = S.z[i];
     S.keep[a + K] = i;
-    S.Dc[a + K] = S.D[i];
+    S.Dc[a + K] = dk;
     S.Zc[a + K] = zk;
     S.Z2c[a + K] = zk * zk;
     K++;
     prev = i;
+    zp = zk;
+    dp = dk;
   }
   S.K[s2] = K;
   S.zsum[s2] = zs;
