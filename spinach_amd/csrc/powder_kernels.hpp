@@ -58,6 +58,8 @@ struct SysDesc {
   int64_t or_off;    // first orientation of this system in the global orientation arrays
   int64_t n_or;      // number of orientations
   int32_t real;      // 1: the generator is real at every orientation (real-product path)
+  int32_t rb64;      // first pattern entry of row 64 (the pattern is sorted by row, then column)
+  int32_t rb128;     // first pattern entry of row 128
   int32_t pad2;
 };
 

@@ -52,39 +52,28 @@ struct SpecParams {
   const double* gammas;
   double dt;
   int64_t nsteps;
-  double* work;          // per-workgroup workspace of k_tridiag: dense np x np, all zero between uses
-  int64_t work_stride;   // doubles per workgroup
   double* spec;          // kSpecStride doubles per spectral item
   int32_t* status;       // per item: most secular evaluations of a root, or -1 if a solve failed
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
 };
 
-// Scatter (sign = +1) or clear (sign = 0) the real symmetric generator
-// A = dt*(x + x^T)/2 of one orientation into a dense row-major np x np buffer.
-__device__ inline void scatter_real_generator(const SpecParams& p, const SysDesc& sd, const double* cre,
-                                              const double* cim, double* W, bool clear) {
-  const int32_t* prow = p.prow + sd.pat_off;
-  const int32_t* pcol = p.pcol + sd.pat_off;
-  const int32_t* ptp = p.ptpos + sd.pat_off;
+// Entry e (row prow[e], column pcol[e] of the reduced union pattern) of the real symmetric
+// generator A = dt*(x + x^T)/2 of one orientation, x = I + sum_q c_q Q_q with c_q the
+// orientation's Wigner coefficients (cre/cim, indexed by the slot of each Q).
+__device__ inline double real_generator_entry(const SpecParams& p, const SysDesc& sd, const double* cre,
+                                              const double* cim, int e) {
   const double* vals = p.vals + 2 * sd.val_off;
   const int32_t* slots = p.slots + sd.slot_off;
-  const int nnz = sd.nnz, np = sd.np;
-  for (int e = threadIdx.x; e < nnz; e += blockDim.x) {
-    const size_t off = (size_t)prow[e] * np + pcol[e];
-    if (clear) {
-      W[off] = 0.0;
-      continue;
-    }
-    const int et = ptp[e];
-    double xr = vals[2 * e], tr = vals[2 * et];
-    for (int q = 0; q < sd.nq; q++) {
-      const double* qv = vals + 2 * (size_t)(q + 1) * nnz;
-      const double cr = cre[slots[q]], ci = cim[slots[q]];
-      xr += cr * qv[2 * e] - ci * qv[2 * e + 1];
-      tr += cr * qv[2 * et] - ci * qv[2 * et + 1];
-    }
-    W[off] = p.dt * 0.5 * (xr + tr);
+  const int nnz = sd.nnz;
+  const int et = p.ptpos[sd.pat_off + e];
+  double xr = vals[2 * e], tr = vals[2 * et];
+  for (int q = 0; q < sd.nq; q++) {
+    const double* qv = vals + 2 * (size_t)(q + 1) * nnz;
+    const double cr = cre[slots[q]], ci = cim[slots[q]];
+    xr += cr * qv[2 * e] - ci * qv[2 * e + 1];
+    tr += cr * qv[2 * et] - ci * qv[2 * et + 1];
   }
+  return p.dt * 0.5 * (xr + tr);
 }
 
 // ------------------------------------------------------------ k_tridiag
@@ -201,10 +190,13 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
   __shared__ double sd[kTriNP], se[kTriNP];
   __shared__ double cre[34], cim[34];
   __shared__ int sitem;
+  // 64 rows of the generator under assembly (rows 64a..64a+63 in pass a); all zero between uses
+  constexpr int kAsmStride = kTriNP + 4;
+  __shared__ double sA[64 * kAsmStride];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int tr = tid / kTriTC, tc = tid % kTriTC;  // row group 0..63, column group
-  double* W = p.work + (size_t)blockIdx.x * p.work_stride;
+  for (int k = tid; k < 64 * kAsmStride; k += kTriThreads) sA[k] = 0.0;
 
   for (;;) {
     if (tid == 0) sitem = (int)atomicAdd(&p.queue[0], 1u);
@@ -217,8 +209,6 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
     const int n = sdesc.n, np = sdesc.np;
     const int64_t o = item.orient;
     if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
-    __syncthreads();
-    scatter_real_generator(p, sdesc, cre, cim, W, false);
     {
       const double* coilv = p.coil + 2 * sdesc.vec_off;
       const double* rhov = p.rho0 + 2 * sdesc.vec_off;
@@ -240,17 +230,27 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
       if (tid == 0) srow[kTriNP] = 0.0;
     }
     __syncthreads();
+    // the generator, 64 rows per pass: the pattern entries of the pass's rows (a contiguous
+    // range, the pattern being sorted by row) are scattered into sA, then every thread takes
+    // its 24 elements of row tr + 64a into registers and clears them for the next pass
     double R[kTriRowsPer][kTriColsPer];
 #pragma unroll
-    for (int a = 0; a < kTriRowsPer; a++)
+    for (int a = 0; a < kTriRowsPer; a++) {
+      const int lo = (a == 0) ? 0 : (a == 1) ? sdesc.rb64 : sdesc.rb128;
+      const int hi = (a == 0) ? sdesc.rb64 : (a == 1) ? sdesc.rb128 : sdesc.nnz;
+      const int32_t* prow = p.prow + sdesc.pat_off;
+      const int32_t* pcol = p.pcol + sdesc.pat_off;
+      for (int e = lo + tid; e < hi; e += kTriThreads)
+        sA[(prow[e] - 64 * a) * kAsmStride + pcol[e]] = real_generator_entry(p, sdesc, cre, cim, e);
+      __syncthreads();
 #pragma unroll
       for (int b = 0; b < kTriColsPer; b++) {
-        const int i = tr + 64 * a, j = tc + kTriTC * b;
-        R[a][b] = (i < np && j < np) ? W[(size_t)i * np + j] : 0.0;
+        double* s = &sA[tr * kAsmStride + tc + kTriTC * b];
+        R[a][b] = *s;
+        *s = 0.0;
       }
-    __syncthreads();
-    // restore the all-zero workspace for the next item
-    scatter_real_generator(p, sdesc, cre, cim, W, true);
+      if (a + 1 < kTriRowsPer) __syncthreads();
+    }
     householder_publish(R, 0, tr, tc, lane, wave, srow, sv[0], &stau[0], sd, se);
     __syncthreads();
 

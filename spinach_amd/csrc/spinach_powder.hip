@@ -366,8 +366,7 @@ struct sp_plan {
   int64_t n_prop_complex = 0;  // the first n_prop_complex propagator items have a complex generator
   int32_t spectral_systems = 0;
   int grid_tri = 0, grid_syn = 0;
-  int64_t tri_stride = 0;
-  DevBuf d_spec, d_status, d_work_tri;
+  DevBuf d_spec, d_status;
   // orientation sum: chunks of consecutive orientations, chunk ranges per system
   int64_t n_chunks = 0;
   DevBuf d_chunks, d_chunk_off, d_partial;
@@ -499,6 +498,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     d.or_off = or_base;
     d.n_or = (int64_t)sp.weights.size();
     d.real = (sp.real && !force_complex) ? 1 : 0;
+    d.rb64 = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 64) - sp.row.begin());
+    d.rb128 = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 128) - sp.row.begin());
     pl->real_systems += d.real;
     pl->spectral_systems += spectral[s];
     descs.push_back(d);
@@ -594,9 +595,6 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   if (pl->n_spec > 0) {
     // one 512-thread workgroup per CU holds a 192 x 192 matrix in registers
     pl->grid_tri = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->n_cu, pl->n_spec));
-    pl->tri_stride = ((int64_t)np_spec * np_spec + 63) / 64 * 64;
-    pl->d_work_tri.alloc((size_t)pl->grid_tri * pl->tri_stride * sizeof(double));
-    HIPCHK(hipMemsetAsync(pl->d_work_tri.p, 0, pl->d_work_tri.bytes, st));  // kept all-zero between items
     pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, pl->n_spec));
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
@@ -655,7 +653,7 @@ void launch(sp_plan* pl) {
   }
 }
 
-spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
+spk::SpecParams spec_params(sp_plan* pl) {
   spk::SpecParams sp{};
   sp.sys = pl->d_sys.as<spk::SysDesc>();
   sp.items = pl->d_items.as<spk::Item>() + pl->n_prop;
@@ -673,8 +671,6 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
   sp.gammas = pl->d_ga.as<double>();
   sp.dt = pl->dt;
   sp.nsteps = pl->nsteps;
-  sp.work = work;
-  sp.work_stride = stride;
   sp.spec = pl->d_spec.as<double>();
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
@@ -684,14 +680,14 @@ spk::SpecParams spec_params(sp_plan* pl, double* work, int64_t stride) {
 // The three spectral kernels; ev[0..2] are recorded after each of them.
 void launch_spectral(sp_plan* pl, hipEvent_t* ev) {  // ev == nullptr: no stage events (graph recording)
   hipStream_t st = pl->ctx->stream;
-  spk::SpecParams tri = spec_params(pl, pl->d_work_tri.as<double>(), pl->tri_stride);
+  spk::SpecParams tri = spec_params(pl);
   hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
-  spk::SpecParams syn = spec_params(pl, nullptr, 0);  // the synthesis needs no workspace
+  spk::SpecParams syn = spec_params(pl);
   hipLaunchKernelGGL(spk::k_synth, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
