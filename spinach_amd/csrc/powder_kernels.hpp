@@ -58,9 +58,7 @@ struct SysDesc {
   int64_t or_off;    // first orientation of this system in the global orientation arrays
   int64_t n_or;      // number of orientations
   int32_t real;      // 1: the generator is real at every orientation (real-product path)
-  int32_t rb64;      // first pattern entry of row 64 (the pattern is sorted by row, then column)
-  int32_t rb128;     // first pattern entry of row 128
-  int32_t pad2;
+  int32_t rb[13];    // rb[k]: first pattern entry of row 16k (the pattern is sorted by row, then column)
 };
 
 struct Item {
@@ -178,7 +176,7 @@ __host__ __device__ inline void wigner_coeffs(double a, double b, double g, doub
 // formulas, so the same values, as wigner_coeffs
 // (the element is picked by a select over constant indices, so that the small
 // matrices stay in registers)
-__host__ __device__ inline void wigner_coeff_slot(double a, double b, double g, int t, double* cre, double* cim) {
+__host__ __device__ __forceinline__ void wigner_coeff_slot(double a, double b, double g, int t, double* cre, double* cim) {
   double dv = 0.0, ph;
   if (t < 9) {
     double d1[3][3];

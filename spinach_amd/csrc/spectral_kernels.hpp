@@ -10,9 +10,9 @@
 // rounding in both is bounded by t*eps*|A|, so the two agree to ~1e-12 over
 // thousands of steps. Three kernels:
 //   k_tridiag  Householder reduction A = Q T Q^T, T tridiagonal, with Q^T
-//              applied on the fly to conj(coil) and rho0. The 192 x 192 FP64
-//              matrix lives in the registers of a 512-thread workgroup (each
-//              thread 3 rows x 24 columns, cyclic), so the symmetric
+//              applied on the fly to conj(coil) and rho0. The lower triangle
+//              of the 192 x 192 FP64 matrix lives in the registers of a
+//              256-thread workgroup (two per CU), so the symmetric
 //              matrix-vector products and rank-2 updates of every step run
 //              from registers with the Householder vectors broadcast from LDS.
 //   k_dceig    eigenvalues of T and the projections g_k by divide and
@@ -28,10 +28,6 @@
 namespace spk {
 
 constexpr int kTriNP = 192;      // largest padded dimension of the register layout
-constexpr int kTriTC = 8;                   // column groups of the register layout (8 measured faster than 16)
-constexpr int kTriThreads = 64 * kTriTC;    // 64 row groups x kTriTC column groups
-constexpr int kTriRowsPer = kTriNP / 64;    // rows per thread (3)
-constexpr int kTriColsPer = kTriNP / kTriTC;  // columns per thread
 constexpr int kSpecStride = 6 * kTriNP;  // per item: d | e | four vectors
 constexpr int kSynthB = 64;      // time block of the synthesis product
 
@@ -76,7 +72,6 @@ __device__ inline double real_generator_entry(const SpecParams& p, const SysDesc
   return p.dt * 0.5 * (xr + tr);
 }
 
-// ------------------------------------------------------------ k_tridiag
 // Cross-lane sums with DPP moves (VALU latency, no LDS round trip).
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -94,14 +89,6 @@ __device__ __forceinline__ double sum8(double v) {
   return v;
 }
 
-// sum over each aligned group of kTriTC lanes (one row group of the layout)
-__device__ __forceinline__ double sum_tc(double v) {
-  v = sum8(v);
-  if constexpr (kTriTC == 16) v += dpp_f64<0x140>(v);  // row_mirror: the other 8-group
-  static_assert(kTriTC == 8 || kTriTC == 16, "column groups: 8 or 16");
-  return v;
-}
-
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
@@ -113,22 +100,6 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   v = sum8(v);
   v += dpp_f64<0x140>(v);  // row_mirror: the other 8-group of the 16-lane row
   return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
-}
-
-// The wave of the owners of row jj: the kTriTC lanes of row group jj % 64.
-__device__ __forceinline__ int row_owner_wave(int jj) { return ((jj & 63) * kTriTC) >> 6; }
-
-// The owners of row jj copy it from their registers to srow (indices into R
-// are compile-time constants, so R stays in registers). Called by the owner wave.
-__device__ __forceinline__ void copy_row(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr, int tc,
-                                         double* srow) {
-#pragma unroll
-  for (int a = 0; a < kTriRowsPer; a++) {
-    if (a == (jj >> 6) && tr == (jj & 63)) {
-#pragma unroll
-      for (int b = 0; b < kTriColsPer; b++) srow[tc + kTriTC * b] = R[a][b];
-    }
-  }
 }
 
 // One wave forms the Householder reflection that clears row jj right of its
@@ -164,49 +135,140 @@ __device__ __forceinline__ void householder_form(const double (&x)[kTriNP / 64],
   }
 }
 
-// The reflection of row jj as it stands in the registers: the owners copy the
-// row to srow and their wave forms the reflection from the copy (read back by
-// the same wave, whose LDS operations complete in order). Every thread calls
-// it; the other waves return at once.
-__device__ __forceinline__ void householder_publish(const double (&R)[kTriRowsPer][kTriColsPer], int jj, int tr,
-                                                    int tc, int lane, int wave, double* srow, double* vb,
-                                                    double* ptau, double* sd, double* se) {
-  if (wave != row_owner_wave(jj)) return;
-  copy_row(R, jj, tr, tc, srow);
-  __builtin_amdgcn_wave_barrier();
-  double x[kTriNP / 64];
-#pragma unroll
-  for (int r = 0; r < kTriNP / 64; r++) x[r] = srow[lane + 64 * r];
-  householder_form(x, srow[jj], srow[jj + 1], jj, lane, vb, ptau, sd, se);  // srow[kTriNP] stays zero
+// ------------------------------------------------------------ k_tridiag
+// Householder tridiagonalisation with the lower triangle of A in registers: 148 KB
+// per orientation, so that two orientations (workgroups) share a CU and hide each
+// other's barrier and reduction latency (the whole matrix would fill the CU's
+// register file). 256 threads on a 16 x 16 grid, thread (r, c) = (tid / 16, tid % 16);
+// the thread holds A(r + 16a, c + 16b) for b <= a (78 slots). In the diagonal
+// blocks (a == b) the slot holds h * A with h = 1 (r > c), 1/2 (r == c, the
+// diagonal, so that the symmetric product counts it once) or 0 (r < c, an upper
+// entry, held by the thread (c, r) as its transpose).
+constexpr int kTriThreads = 256;
+constexpr int kTriBlk = kTriNP / 16;                      // blocks of 16 rows / columns
+constexpr int kTriSlots = kTriBlk * (kTriBlk + 1) / 2;  // 78
+__host__ __device__ constexpr int sym_slot(int a, int b) { return a * (a + 1) / 2 + b; }
+
+// sum over each aligned group of 16 lanes, returned in all 16; every lane active
+__device__ __forceinline__ double sum16(double v) {
+  v = sum8(v);
+  return v + dpp_f64<0x140>(v);  // row_mirror: the other 8-group
 }
 
-__global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
-  __shared__ double sv[2][kTriNP];  // Householder vectors of rows j, double-buffered
+// The owners of column jj (the threads with c == jj % 16) copy its entries at rows
+// >= jj (rows below jj in its block get 0 or a stale value, never read) to srow.
+__device__ __forceinline__ void sym_copy_col(const double (&R)[kTriSlots], int jj, int r, int c, double hd,
+                                             double* srow) {
+  if (c != (jj & 15)) return;
+  const int bj = jj >> 4;
+#pragma unroll
+  for (int b = 0; b < kTriBlk; b++) {
+    if (b == bj) {
+#pragma unroll
+      for (int a = b; a < kTriBlk; a++) srow[r + 16 * a] = (a == b) ? hd * R[sym_slot(a, b)] : R[sym_slot(a, b)];
+    }
+  }
+}
+
+// Column jj as it stands in the registers, reflected by wave 0 (every thread calls it).
+__device__ __forceinline__ void sym_publish(const double (&R)[kTriSlots], int jj, int r, int c, double hd,
+                                            int wave, int lane, double* srow, double* vb, double* ptau, double* sd,
+                                            double* se) {
+  sym_copy_col(R, jj, r, c, hd, srow);
+  __syncthreads();
+  if (wave == 0) {
+    double x[kTriNP / 64];
+#pragma unroll
+    for (int q = 0; q < kTriNP / 64; q++) x[q] = srow[lane + 64 * q];
+    householder_form(x, srow[jj], srow[jj + 1], jj, lane, vb, ptau, sd, se);  // srow[kTriNP] stays zero
+  }
+}
+
+__device__ __forceinline__ double uniform_f64(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// A*v over this thread's slots of the live column blocks (bmin <= b < bmax): row sums into
+// yr, column sums stored to spc[r][.]; returns this thread's part of v'A v from the column
+// sums (the row sums' part is added by the caller). The row blocks of a live column block
+// are not tested against bmax: rows at or beyond np hold zeros (a test there would be
+// compiled to a select per product).
+__device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const double (&vr)[kTriBlk],
+                                             double (&yr)[kTriBlk], const double* v, double (*spc)[kTriNP], int r,
+                                             int c, int bmin, int bmax) {
+  double s = 0.0;
+#pragma unroll
+  for (int b = 0; b < kTriBlk; b++) {
+    if (b >= bmin && b < bmax) {
+      const double vc = v[c + 16 * b];
+      double yc = 0.0;
+#pragma unroll
+      for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros
+        const double x = R[sym_slot(a, b)];
+        yr[a] = fma(x, vc, yr[a]);
+        yc = fma(x, vr[a], yc);
+      }
+      spc[r][c + 16 * b] = yc;
+      s = fma(vc, yc, s);
+    }
+  }
+  return s;
+}
+
+// A <- A - v w' - w v' over this thread's slots of the live column blocks (h weights the
+// diagonal blocks)
+__device__ __forceinline__ void sym_update(double (&R)[kTriSlots], const double (&vr)[kTriBlk],
+                                           const double (&wr)[kTriBlk], const double* v, const double* sp, double K,
+                                           double h, int c, int bmin, int bmax) {
+#pragma unroll
+  for (int b = 0; b < kTriBlk; b++) {
+    if (b >= bmin && b < bmax) {
+      const double vc = v[c + 16 * b], wc = sp[c + 16 * b] - K * vc;
+#pragma unroll
+      for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros and stay zero
+        const double va = (a == b) ? h * vr[a] : vr[a];
+        const double wa = (a == b) ? h * wr[a] : wr[a];
+        R[sym_slot(a, b)] = fma(-va, wc, fma(-wa, vc, R[sym_slot(a, b)]));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
+  __shared__ double sv[2][kTriNP];  // Householder vectors, double-buffered
   __shared__ double stau[2];
-  __shared__ double srow[kTriNP + 1];  // the row being published; srow[kTriNP] = 0
-  __shared__ double sp[kTriNP];     // p = tau*A*v
-  __shared__ double sq[kTriThreads / 64];  // per-wave parts of v'p
+  __shared__ double srow[kTriNP + 1];  // the column being reflected; srow[kTriNP] = 0
+  __shared__ double sp[kTriNP];        // p = tau*A*v
+  __shared__ double spr[kTriNP];       // row sums of A*v over the 16 column groups
+  __shared__ double spc[16][kTriNP];   // column sums of A*v, one row per row group r
+  __shared__ double sq[kTriThreads / 64];
   __shared__ double su[4][kTriNP];
   __shared__ double sd[kTriNP], se[kTriNP];
   __shared__ double cre[34], cim[34];
   __shared__ int sitem;
-  // 64 rows of the generator under assembly (rows 64a..64a+63 in pass a); all zero between uses
+  // 16 rows of the generator under assembly (rows 16a..16a+15 in pass a); all zero between uses
   constexpr int kAsmStride = kTriNP + 4;
-  __shared__ double sA[64 * kAsmStride];
+  __shared__ double sA[16 * kAsmStride];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int tr = tid / kTriTC, tc = tid % kTriTC;  // row group 0..63, column group
-  for (int k = tid; k < 64 * kAsmStride; k += kTriThreads) sA[k] = 0.0;
+  const int r = tid >> 4, c = tid & 15;
+  const double h = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
+  const double hd = (r == c) ? 2.0 : 1.0;
+  for (int k = tid; k < 16 * kAsmStride; k += kTriThreads) sA[k] = 0.0;
+  for (int k = tid; k <= kTriNP; k += kTriThreads) srow[k] = 0.0;
 
   for (;;) {
     if (tid == 0) sitem = (int)atomicAdd(&p.queue[0], 1u);
     __syncthreads();
-    const int64_t it = sitem;
+    const int64_t it = __builtin_amdgcn_readfirstlane(sitem);  // uniform (scalar branches below)
     __syncthreads();
     if (it >= p.n_items) break;
     const Item item = p.items[it];
     const SysDesc sdesc = p.sys[item.sys];
-    const int n = sdesc.n, np = sdesc.np;
+    const int n = __builtin_amdgcn_readfirstlane(sdesc.n), np = __builtin_amdgcn_readfirstlane(sdesc.np);
+    const int bmax = (np + 15) >> 4;  // blocks below the padded dimension
     const int64_t o = item.orient;
     if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
     {
@@ -227,154 +289,128 @@ __global__ void __launch_bounds__(kTriThreads, 1) k_tridiag(SpecParams p) {
         sd[i] = 0.0;
         se[i] = 0.0;
       }
-      if (tid == 0) srow[kTriNP] = 0.0;
     }
     __syncthreads();
-    // the generator, 64 rows per pass: the pattern entries of the pass's rows (a contiguous
-    // range, the pattern being sorted by row) are scattered into sA, then every thread takes
-    // its 24 elements of row tr + 64a into registers and clears them for the next pass
-    double R[kTriRowsPer][kTriColsPer];
+    // the lower triangle, 16 rows per pass: the pass's lower pattern entries are
+    // scattered into sA, then every thread takes its slots of row r + 16a and clears them
+    double R[kTriSlots];
+    const int32_t* prow = p.prow + sdesc.pat_off;
+    const int32_t* pcol = p.pcol + sdesc.pat_off;
 #pragma unroll
-    for (int a = 0; a < kTriRowsPer; a++) {
-      const int lo = (a == 0) ? 0 : (a == 1) ? sdesc.rb64 : sdesc.rb128;
-      const int hi = (a == 0) ? sdesc.rb64 : (a == 1) ? sdesc.rb128 : sdesc.nnz;
-      const int32_t* prow = p.prow + sdesc.pat_off;
-      const int32_t* pcol = p.pcol + sdesc.pat_off;
-      for (int e = lo + tid; e < hi; e += kTriThreads)
-        sA[(prow[e] - 64 * a) * kAsmStride + pcol[e]] = real_generator_entry(p, sdesc, cre, cim, e);
-      __syncthreads();
+    for (int a = 0; a < kTriBlk; a++) {
+      if (a < bmax) {
+        for (int e = sdesc.rb[a] + tid; e < sdesc.rb[a + 1]; e += kTriThreads) {
+          const int i = prow[e], jc = pcol[e];
+          if (jc <= i) sA[(i - 16 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
+        }
+        __syncthreads();
 #pragma unroll
-      for (int b = 0; b < kTriColsPer; b++) {
-        double* s = &sA[tr * kAsmStride + tc + kTriTC * b];
-        R[a][b] = *s;
-        *s = 0.0;
+        for (int b = 0; b <= a; b++) {
+          double* s = &sA[r * kAsmStride + c + 16 * b];
+          R[sym_slot(a, b)] = (b == a) ? h * *s : *s;
+          *s = 0.0;
+        }
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int b = 0; b <= a; b++) R[sym_slot(a, b)] = 0.0;
       }
-      if (a + 1 < kTriRowsPer) __syncthreads();
     }
-    householder_publish(R, 0, tr, tc, lane, wave, srow, sv[0], &stau[0], sd, se);
+    sym_publish(R, 0, r, c, hd, wave, lane, srow, sv[0], &stau[0], sd, se);
     __syncthreads();
 
     for (int j = 0; j < n - 2; j++) {
       const int cb = j & 1;
       const double* v = sv[cb];
-      // the last column chunk (columns 160..191) is live in every step (j <= n - 3 <= 189):
-      // its v loads are issued with the read of tau, ahead of the branch on it
-      constexpr int kLast = kTriColsPer / 4 - 1;
-      double vlast[4];
+      const double tau = uniform_f64(stau[cb]);  // reflection of column j
+      const int jn = j + 1;         // the next column to reflect
+      const int bmin = jn >> 4;     // blocks below bmin are eliminated
+      if (tau != 0.0) {             // uniform
+        // the wave that reflects column jn rotates with the step, so that the two workgroups
+        // sharing a CU rarely have their look-ahead waves on the same SIMD
+        const int la = jn & 3;
+        if (wave == la) __builtin_amdgcn_s_setprio(3);
+        sym_copy_col(R, jn, r, c, hd, srow);  // column jn as the previous step left it
+        // A*v: row sums yr over this thread's columns, column sums yc over its rows
+        double vr[kTriBlk], yr[kTriBlk];
 #pragma unroll
-      for (int q = 0; q < 4; q++) vlast[q] = v[tc + kTriTC * (4 * kLast + q)];
-      const double tau = stau[cb];  // reflection of row j (published by the owners of row j)
-      const int jn = j + 1;         // the next row to reflect (jn + 1 <= n - 1 < kTriNP)
-      const int own = row_owner_wave(jn);
-      if (tau != 0.0) {  // uniform
-        // the owners of row jn copy it (as the previous step left it) for the look-ahead below;
-        // their wave carries the most work of the step and runs at raised priority throughout
-        if (wave == own) {
-          __builtin_amdgcn_s_setprio(3);
-          copy_row(R, jn, tr, tc, srow);
+        for (int a = 0; a < kTriBlk; a++) {
+          vr[a] = v[r + 16 * a];
+          yr[a] = 0.0;
         }
-        // p = tau*A*v on the trailing rows, in chunks of 4 columns per thread, from the last
-        // chunk down (the eliminated chunks are skipped)
-        double part[kTriRowsPer] = {0.0, 0.0, 0.0};
+        // this thread's part of v'A v
+        double s = sym_matvec(R, vr, yr, v, spc, r, c, bmin, bmax);
 #pragma unroll
-        for (int bc = kLast; bc >= 0; bc--) {
-          if (bc == kLast || 4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
-            double vb[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) vb[q] = (bc == kLast) ? vlast[q] : v[tc + kTriTC * (4 * bc + q)];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-              for (int a = 0; a < kTriRowsPer; a++) part[a] = fma(R[a][4 * bc + q], vb[q], part[a]);
+        for (int a = 0; a < kTriBlk; a++) {
+          if (a >= bmin) {  // (rows at or beyond np: zero sums)
+            s = fma(vr[a], yr[a], s);
+            const double t = sum16(yr[a]);
+            if (c == 0) spr[r + 16 * a] = t;
           }
         }
-        double qsum = 0.0;
-#pragma unroll
-        for (int a = 0; a < kTriRowsPer; a++) {
-          const int i = tr + 64 * a;
-          double s = sum_tc(part[a]) * tau;
-          if (i <= j) s = 0.0;
-          if (tc == 0) sp[i] = s;
-          qsum = fma(v[i], s, qsum);
-        }
-        // the kTriTC lanes of a group hold the same qsum
-        const double qw = wave_sum_dpp(qsum) * (1.0 / kTriTC);
+        const double qw = wave_sum_dpp(s) * tau;
         if (lane == 0) sq[wave] = qw;
         __syncthreads();
-        // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
-        double qt = 0.0;
+        if (tid < kTriNP) {
+          double t4[4];
 #pragma unroll
-        for (int w8 = 0; w8 < kTriThreads / 64; w8++) qt += sq[w8];
-        const double K = 0.5 * tau * qt;
-        if (wave == own) {
-          // look-ahead: row jn after this update, from its copy, and its reflection for the
-          // next step (the same operations as the register update below)
+          for (int q = 0; q < 4; q++) t4[q] = (spc[q][tid] + spc[q + 4][tid]) + (spc[q + 8][tid] + spc[q + 12][tid]);
+          const double t = spr[tid] + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
+          sp[tid] = (tid > j && tid < np) ? t * tau : 0.0;
+        }
+        __syncthreads();
+        // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
+        const double K = 0.5 * tau * ((sq[0] + sq[1]) + (sq[2] + sq[3]));
+        if (wave == la) {
+          // column jn after this update, from its copy, and its reflection for the next step
           const double vn = v[jn], wn = sp[jn] - K * vn;
           double x[kTriNP / 64];
 #pragma unroll
-          for (int r = 0; r < kTriNP / 64; r++) {
-            const int c = lane + 64 * r;
-            const double vc = v[c], wc = sp[c] - K * vc;
-            x[r] = fma(-vn, wc, fma(-wn, vc, srow[c]));
+          for (int q = 0; q < kTriNP / 64; q++) {
+            const int i = lane + 64 * q;
+            const double vi = v[i], wi = sp[i] - K * vi;
+            x[q] = fma(-vn, wi, fma(-wn, vi, srow[i]));
           }
           const double xd = fma(-vn, wn, fma(-wn, vn, srow[jn]));
           const double v1 = v[jn + 1], w1 = sp[jn + 1] - K * v1;
           const double x1 = fma(-vn, w1, fma(-wn, v1, srow[jn + 1]));
           householder_form(x, xd, x1, jn, lane, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
         } else {
-          // Q^T applied to conj(coil) and rho0 (four vectors, one per wave other than
-          // the owners'): u <- u - tau*v*(v'u)
-          const int q = (wave - own - 1) & 7;
-          if (q < 4) {
-            __builtin_amdgcn_s_setprio(1);  // these waves carry more of the half-step than the rest
-            double vv[kTriNP / 64], uu[kTriNP / 64];
-            double s = 0.0;
+          // Q^T applied to conj(coil) and rho0: the first wave after la takes vectors 0 and 3,
+          // the other two one each
+          const int qw0 = (wave - la - 1) & 3;
 #pragma unroll
-            for (int r = 0; r < kTriNP / 64; r++) {
-              const int i = lane + 64 * r;
-              vv[r] = v[i];
-              uu[r] = su[q][i];
-              s = fma(vv[r], uu[r], s);
+          for (int pass = 0; pass < 2; pass++) {
+            const int q = (pass == 0) ? qw0 : 3;
+            if (pass == 0 || qw0 == 0) {
+              double vv[kTriNP / 64], uu[kTriNP / 64];
+              double t = 0.0;
+#pragma unroll
+              for (int k = 0; k < kTriNP / 64; k++) {
+                const int i = lane + 64 * k;
+                vv[k] = v[i];
+                uu[k] = su[q][i];
+                t = fma(vv[k], uu[k], t);
+              }
+              t = wave_sum_dpp(t) * tau;
+#pragma unroll
+              for (int k = 0; k < kTriNP / 64; k++)
+                if (vv[k] != 0.0) su[q][lane + 64 * k] = uu[k] - t * vv[k];
             }
-            s = wave_sum_dpp(s) * tau;
-#pragma unroll
-            for (int r = 0; r < kTriNP / 64; r++)
-              if (vv[r] != 0.0) su[q][lane + 64 * r] = uu[r] - s * vv[r];
           }
         }
-        double vi[kTriRowsPer], wi[kTriRowsPer];
+        double wr[kTriBlk];
 #pragma unroll
-        for (int a = 0; a < kTriRowsPer; a++) {
-          const int i = tr + 64 * a;
-          vi[a] = v[i];
-          wi[a] = sp[i] - K * vi[a];
-        }
-#pragma unroll
-        for (int bc = 0; bc < kTriColsPer / 4; bc++) {
-          if (4 * kTriTC * bc + 4 * kTriTC - 1 > j) {
-            double vb[4], wb[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int c = tc + kTriTC * (4 * bc + q);
-              vb[q] = v[c];
-              wb[q] = sp[c] - K * vb[q];
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-              for (int a = 0; a < kTriRowsPer; a++)
-                R[a][4 * bc + q] = fma(-vi[a], wb[q], fma(-wi[a], vb[q], R[a][4 * bc + q]));
-          }
-        }
+        for (int a = 0; a < kTriBlk; a++) wr[a] = sp[r + 16 * a] - K * vr[a];
+        sym_update(R, vr, wr, v, sp, K, h, c, bmin, bmax);
         __builtin_amdgcn_s_setprio(0);
       } else {
-        // no reflection at this step: row jn stands as it is in the registers
-        householder_publish(R, jn, tr, tc, lane, wave, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+        // no reflection at this step: column jn stands as it is in the registers
+        sym_publish(R, jn, r, c, hd, wave, lane, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
       }
       __syncthreads();
     }
-    // the last row (d_{n-1}; rows up to n-2 are published by now)
-    if (n >= 2) householder_publish(R, n - 1, tr, tc, lane, wave, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
+    if (n >= 2) sym_publish(R, n - 1, r, c, hd, wave, lane, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
     __syncthreads();
     double* out = p.spec + (size_t)it * kSpecStride;
     for (int i = tid; i < kTriNP; i += kTriThreads) {

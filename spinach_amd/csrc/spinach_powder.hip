@@ -498,8 +498,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     d.or_off = or_base;
     d.n_or = (int64_t)sp.weights.size();
     d.real = (sp.real && !force_complex) ? 1 : 0;
-    d.rb64 = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 64) - sp.row.begin());
-    d.rb128 = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 128) - sp.row.begin());
+    for (int k = 0; k < 13; k++)
+      d.rb[k] = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 16 * k) - sp.row.begin());
     pl->real_systems += d.real;
     pl->spectral_systems += spectral[s];
     descs.push_back(d);
@@ -593,8 +593,9 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
       if (atoi(e) > 0) pl->d_phase.alloc((size_t)pl->grid * spk::kPhases * sizeof(unsigned long long));
   }
   if (pl->n_spec > 0) {
-    // one 512-thread workgroup per CU holds a 192 x 192 matrix in registers
-    pl->grid_tri = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->n_cu, pl->n_spec));
+    // two 256-thread workgroups per CU, each holding the lower triangle of a 192 x 192
+    // matrix in registers
+    pl->grid_tri = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)ctx->n_cu, pl->n_spec));
     pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, pl->n_spec));
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
