@@ -149,12 +149,6 @@ constexpr int kTriBlk = kTriNP / 16;                      // blocks of 16 rows /
 constexpr int kTriSlots = kTriBlk * (kTriBlk + 1) / 2;  // 78
 __host__ __device__ constexpr int sym_slot(int a, int b) { return a * (a + 1) / 2 + b; }
 
-// sum over each aligned group of 16 lanes, returned in all 16; every lane active
-__device__ __forceinline__ double sum16(double v) {
-  v = sum8(v);
-  return v + dpp_f64<0x140>(v);  // row_mirror: the other 8-group
-}
-
 // The owners of column jj (the threads with c == jj % 16) copy its entries at rows
 // >= jj (rows below jj in its block get 0 or a stale value, never read) to srow.
 __device__ __forceinline__ void sym_copy_col(const double (&R)[kTriSlots], int jj, int r, int c, double hd,
@@ -241,7 +235,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
   __shared__ double stau[2];
   __shared__ double srow[kTriNP + 1];  // the column being reflected; srow[kTriNP] = 0
   __shared__ double sp[kTriNP];        // p = tau*A*v
-  __shared__ double spr[kTriNP];       // row sums of A*v over the 16 column groups
+  __shared__ double spr[2][kTriNP];    // row sums of A*v over column groups 0-7 and 8-15
   __shared__ double spc[16][kTriNP];   // column sums of A*v, one row per row group r
   __shared__ double sq[kTriThreads / 64];
   __shared__ double su[4][kTriNP];
@@ -344,8 +338,8 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         for (int a = 0; a < kTriBlk; a++) {
           if (a >= bmin) {  // (rows at or beyond np: zero sums)
             s = fma(vr[a], yr[a], s);
-            const double t = sum16(yr[a]);
-            if (c == 0) spr[r + 16 * a] = t;
+            const double t = sum8(yr[a]);  // (the last 8-group sum is left to the assembly)
+            if ((c & 7) == 0) spr[c >> 3][r + 16 * a] = t;
           }
         }
         const double qw = wave_sum_dpp(s) * tau;
@@ -355,7 +349,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
           double t4[4];
 #pragma unroll
           for (int q = 0; q < 4; q++) t4[q] = (spc[q][tid] + spc[q + 4][tid]) + (spc[q + 8][tid] + spc[q + 12][tid]);
-          const double t = spr[tid] + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
+          const double t = (spr[0][tid] + spr[1][tid]) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
           sp[tid] = (tid > j && tid < np) ? t * tau : 0.0;
         }
         __syncthreads();
