@@ -286,14 +286,33 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
     }
     __syncthreads();
     // the lower triangle, 16 rows per pass: the pass's lower pattern entries are
-    // scattered into sA, then every thread takes its slots of row r + 16a and clears them
+    // scattered into sA, then every thread takes its slots of row r + 16a and clears them.
+    // The values of the first kAsmCache * 256 pattern entries are computed up front (their
+    // loads all in flight at once), entry tid + 256k in this thread's registers with its
+    // position (row << 8 | column, -1 for an upper entry); later entries are read per pass.
+    constexpr int kAsmCache = 8;
     double R[kTriSlots];
     const int32_t* prow = p.prow + sdesc.pat_off;
     const int32_t* pcol = p.pcol + sdesc.pat_off;
+    const int nnz = sdesc.nnz;
+    double cval[kAsmCache];
+    int cpos[kAsmCache];
+#pragma unroll
+    for (int k = 0; k < kAsmCache; k++) {
+      const int e = tid + kTriThreads * k;
+      const int ee = (e < nnz) ? e : 0;
+      const int i = prow[ee], jc = pcol[ee];
+      cval[k] = (e < nnz) ? real_generator_entry(p, sdesc, cre, cim, ee) : 0.0;
+      cpos[k] = (e < nnz && jc <= i) ? ((i << 8) | jc) : -1;
+    }
 #pragma unroll
     for (int a = 0; a < kTriBlk; a++) {
       if (a < bmax) {
-        for (int e = sdesc.rb[a] + tid; e < sdesc.rb[a + 1]; e += kTriThreads) {
+#pragma unroll
+        for (int k = 0; k < kAsmCache; k++)
+          if (cpos[k] >= 0 && (cpos[k] >> 12) == a)
+            sA[((cpos[k] >> 8) - 16 * a) * kAsmStride + (cpos[k] & 255)] = cval[k];
+        for (int e = max(sdesc.rb[a], kAsmCache * kTriThreads) + tid; e < sdesc.rb[a + 1]; e += kTriThreads) {
           const int i = prow[e], jc = pcol[e];
           if (jc <= i) sA[(i - 16 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
         }

@@ -105,3 +105,27 @@ def test_fp32_real_path_error_reported():
     print(f'8-spin FP32 vs FP64: real path {e_r:.3e}, complex path {e_c:.3e}', flush=True)
     assert s32['real_systems'] == 1
     assert np.isfinite(e_r) and e_r < 1e-2
+
+
+def test_dense_real_generator_spectral_path():
+    # a dense real symmetric block on the 8-spin system's 185 coil-connected states: the
+    # union pattern (~34K entries) is far beyond the entries whose values the
+    # tridiagonalisation computes up front, so most are read during its assembly passes
+    w = workloads.static_8spin(n_orient=4000)
+    sub = grids.subset(w['grid'], np.arange(3))
+    idx = O.union_states(w['I'], w['Q'], w['coil'])
+    rng = np.random.default_rng(11)
+    m = len(idx)
+    K = rng.standard_normal((m, m))
+    K = (K + K.T) * (1e-3 * abs(w['I']).max())
+    I = w['I'].toarray()
+    I[np.ix_(idx, idx)] += K
+    y = dict(w, I=sps.csc_matrix(I))
+    assert _native.generator_is_real(y['I'], y['Q'])
+    f, _, st = _run(y, sub, 'fp64')
+    print('dense generator stats', st, flush=True)
+    assert st['spectral_systems'] == 1 and st['reduced_dim'] == m and st['reduced_nnz'] > 8 * 2048
+    ref = O.refcpu_powder(y['I'], y['Q'], sub, y['rho0'], y['coil'], y['dt'], y['nsteps'], variant=2)
+    e = O.rel_err(f, ref)
+    print(f'dense real generator, 185 states, 3 x 4096: vs oracle {e:.2e}', flush=True)
+    assert e < 1e-10
