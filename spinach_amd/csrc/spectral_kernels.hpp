@@ -489,10 +489,11 @@ __host__ __device__ inline double dc_rcp(double x) {
 }
 
 // 1/x to within 2^-48.8, about 18 ulp (one Newton step on v_rcp_f64's 2^-24.4): for the
-// term sums of the secular iteration, where it is below the rounding bound of the K-term
-// sums from K = 19 on; on the 8-spin workload the FIDs then differ from the oracle by
-// 9.0e-13 instead of 8.3e-13 (batch and dense-generator cases unchanged). Everything else
-// (the initial guesses, z recomputation, eigenvector formula) uses dc_rcp.
+// term sums of the secular iteration and of its starting point (the sign of f at the
+// midpoint), where it is below the rounding bound of the K-term sums from K = 19 on; on the
+// 8-spin workload the FIDs then differ from the oracle by 9.0e-13 instead of 8.3e-13 (batch
+// and dense-generator cases unchanged). Everything else (the model roots, z recomputation,
+// eigenvector formula) uses dc_rcp.
 __host__ __device__ inline double dc_rcp_iter(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const double r = __builtin_amdgcn_rcp(x);
@@ -623,7 +624,7 @@ __host__ __device__ __attribute__((always_inline)) inline bool dc_secular_start(
   const double dbase = Dc[base];
   double f = rinv;
 #pragma unroll 4
-  for (int i = 0; i < K; i++) f += Z2[i] * dc_rcp((Dc[i] - dbase) - x0);
+  for (int i = 0; i < K; i++) f += Z2[i] * dc_rcp_iter((Dc[i] - dbase) - x0);
   const double s2 = Z2[kk] + Z2[kk + 1];
   // the bracket and the origin as values (no branch assigns them, so they stay in registers)
   const bool left = outer ? true : (f >= 0.0);  // interior: root in (Dc[k], midpoint]
