@@ -454,7 +454,7 @@ constexpr int kDcThreads = 256;
 constexpr int kDcLeaf = 6;
 constexpr int kDcMaxLevels = 5;  // up to 32 leaves
 constexpr int kDcMaxIter = 100;  // secular-equation iterations per root (typically 3-8)
-constexpr int kDcPass1 = 4;      // evaluations per root before the open roots are packed (see M3a)
+constexpr int kDcPass1 = 3;      // evaluations per root before the open roots are packed (see M3a; 2 and 4-5 measured slower)
 
 struct DcShared {
   double d[kTriNP], e[kTriNP];
