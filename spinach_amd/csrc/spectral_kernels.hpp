@@ -214,12 +214,12 @@ __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const
 // A <- A - v w' - w v' over this thread's slots of the live column blocks (h weights the
 // diagonal blocks)
 __device__ __forceinline__ void sym_update(double (&R)[kTriSlots], const double (&vr)[kTriBlk],
-                                           const double (&wr)[kTriBlk], const double* v, const double* sp, double K,
-                                           double h, int c, int bmin, int bmax) {
+                                           const double (&wr)[kTriBlk], const double* v, const double* sw, double h,
+                                           int c, int bmin, int bmax) {
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
     if (b >= bmin && b < bmax) {
-      const double vc = v[c + 16 * b], wc = sp[c + 16 * b] - K * vc;
+      const double vc = v[c + 16 * b], wc = sw[c + 16 * b];
 #pragma unroll
       for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros and stay zero
         const double va = (a == b) ? h * vr[a] : vr[a];
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
   __shared__ double sv[2][kTriNP];  // Householder vectors, double-buffered
   __shared__ double stau[2];
   __shared__ double srow[kTriNP + 1];  // the column being reflected; srow[kTriNP] = 0
-  __shared__ double sp[kTriNP];        // p = tau*A*v
+  __shared__ double sw[kTriNP];        // w = p - (tau/2)(v'p) v, p = tau*A*v
   __shared__ double spr[2][kTriNP];    // row sums of A*v over column groups 0-7 and 8-15
   __shared__ double spc[16][kTriNP];   // column sums of A*v, one row per row group r
   __shared__ double sq[kTriThreads / 64];
@@ -364,28 +364,29 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         const double qw = wave_sum_dpp(s) * tau;
         if (lane == 0) sq[wave] = qw;
         __syncthreads();
+        // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v, assembled with p
         if (tid < kTriNP) {
+          const double K = 0.5 * tau * ((sq[0] + sq[1]) + (sq[2] + sq[3]));
           double t4[4];
 #pragma unroll
           for (int q = 0; q < 4; q++) t4[q] = (spc[q][tid] + spc[q + 4][tid]) + (spc[q + 8][tid] + spc[q + 12][tid]);
           const double t = (spr[0][tid] + spr[1][tid]) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
-          sp[tid] = (tid > j && tid < np) ? t * tau : 0.0;
+          const double pk = (tid > j && tid < np) ? t * tau : 0.0;
+          sw[tid] = pk - K * v[tid];
         }
         __syncthreads();
-        // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v
-        const double K = 0.5 * tau * ((sq[0] + sq[1]) + (sq[2] + sq[3]));
         if (wave == la) {
           // column jn after this update, from its copy, and its reflection for the next step
-          const double vn = v[jn], wn = sp[jn] - K * vn;
+          const double vn = v[jn], wn = sw[jn];
           double x[kTriNP / 64];
 #pragma unroll
           for (int q = 0; q < kTriNP / 64; q++) {
             const int i = lane + 64 * q;
-            const double vi = v[i], wi = sp[i] - K * vi;
+            const double vi = v[i], wi = sw[i];
             x[q] = fma(-vn, wi, fma(-wn, vi, srow[i]));
           }
           const double xd = fma(-vn, wn, fma(-wn, vn, srow[jn]));
-          const double v1 = v[jn + 1], w1 = sp[jn + 1] - K * v1;
+          const double v1 = v[jn + 1], w1 = sw[jn + 1];
           const double x1 = fma(-vn, w1, fma(-wn, v1, srow[jn + 1]));
           householder_form(x, xd, x1, jn, lane, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
         } else {
@@ -414,8 +415,8 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         }
         double wr[kTriBlk];
 #pragma unroll
-        for (int a = 0; a < kTriBlk; a++) wr[a] = sp[r + 16 * a] - K * vr[a];
-        sym_update(R, vr, wr, v, sp, K, h, c, bmin, bmax);
+        for (int a = 0; a < kTriBlk; a++) wr[a] = sw[r + 16 * a];
+        sym_update(R, vr, wr, v, sw, h, c, bmin, bmax);
         __builtin_amdgcn_s_setprio(0);
       } else {
         // no reflection at this step: column jn stands as it is in the registers
