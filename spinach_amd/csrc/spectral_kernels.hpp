@@ -192,11 +192,15 @@ __device__ __forceinline__ double uniform_f64(double v) {
 __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const double (&vr)[kTriBlk],
                                              double (&yr)[kTriBlk], const double* v, double (*spc)[kTriNP], int r,
                                              int c, int bmin, int bmax) {
+  // the live blocks are bmin..bmax-1, so each block loads the next block's v ahead of its
+  // own products
   double s = 0.0;
+  double vnext = v[c + 16 * bmin];
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
     if (b >= bmin && b < bmax) {
-      const double vc = v[c + 16 * b];
+      const double vc = vnext;
+      if (b + 1 < kTriBlk) vnext = v[c + 16 * (b + 1)];
       double yc = 0.0;
 #pragma unroll
       for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros
@@ -216,10 +220,15 @@ __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const
 __device__ __forceinline__ void sym_update(double (&R)[kTriSlots], const double (&vr)[kTriBlk],
                                            const double (&wr)[kTriBlk], const double* v, const double* sw, double h,
                                            int c, int bmin, int bmax) {
+  double vnext = v[c + 16 * bmin], wnext = sw[c + 16 * bmin];  // (next block's loads ahead, as above)
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
     if (b >= bmin && b < bmax) {
-      const double vc = v[c + 16 * b], wc = sw[c + 16 * b];
+      const double vc = vnext, wc = wnext;
+      if (b + 1 < kTriBlk) {
+        vnext = v[c + 16 * (b + 1)];
+        wnext = sw[c + 16 * (b + 1)];
+      }
 #pragma unroll
       for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros and stay zero
         const double va = (a == b) ? h * vr[a] : vr[a];
