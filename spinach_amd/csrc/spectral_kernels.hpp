@@ -178,6 +178,7 @@ __device__ __forceinline__ void sym_publish(const double (&R)[kTriSlots], int jj
   }
 }
 
+// v (the same in every lane) as a wave-uniform value, so that branches on it are scalar
 __device__ __forceinline__ double uniform_f64(double v) {
   const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
   const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
