@@ -373,6 +373,9 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         const double qw = wave_sum_dpp(s) * tau;
         if (lane == 0) sq[wave] = qw;
         __syncthreads();
+        // the assembly, a short latency-bound phase, goes ahead of the other workgroup's waves
+        // (4.64 -> 4.61 ms)
+        __builtin_amdgcn_s_setprio(2);
         // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v, assembled with p
         if (tid < kTriNP) {
           const double K = 0.5 * tau * ((sq[0] + sq[1]) + (sq[2] + sq[3]));
