@@ -362,6 +362,8 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         }
         // this thread's part of v'A v
         double s = sym_matvec(R, vr, yr, v, spc, r, c, bmin, bmax);
+        // the reductions that end the phase are latency-bound: raised priority (4.61 -> 4.56 ms)
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int a = 0; a < kTriBlk; a++) {
           if (a >= bmin) {  // (rows at or beyond np: zero sums)
