@@ -185,21 +185,22 @@ __device__ __forceinline__ double uniform_f64(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// A*v over this thread's slots of the live column blocks (bmin <= b < bmax): row sums into
+// A*v over this thread's slots of the live column blocks (bmin <= b < bmax, the set bits of
+// live): row sums into
 // yr, column sums stored to spc[r][.]; returns this thread's part of v'A v from the column
 // sums (the row sums' part is added by the caller). The row blocks of a live column block
 // are not tested against bmax: rows at or beyond np hold zeros (a test there would be
 // compiled to a select per product).
 __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const double (&vr)[kTriBlk],
                                              double (&yr)[kTriBlk], const double* v, double (*spc)[kTriNP], int r,
-                                             int c, int bmin, int bmax) {
+                                             int c, int bmin, unsigned live) {
   // the live blocks are bmin..bmax-1, so each block loads the next block's v ahead of its
   // own products
   double s = 0.0;
   double vnext = v[c + 16 * bmin];
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
-    if (b >= bmin && b < bmax) {
+    if ((live >> b) & 1u) {
       const double vc = vnext;
       if (b + 1 < kTriBlk) vnext = v[c + 16 * (b + 1)];
       double yc = 0.0;
@@ -220,11 +221,11 @@ __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const
 // diagonal blocks)
 __device__ __forceinline__ void sym_update(double (&R)[kTriSlots], const double (&vr)[kTriBlk],
                                            const double (&wr)[kTriBlk], const double* v, const double* sw, double h,
-                                           int c, int bmin, int bmax) {
+                                           int c, int bmin, unsigned live) {
   double vnext = v[c + 16 * bmin], wnext = sw[c + 16 * bmin];  // (next block's loads ahead, as above)
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
-    if (b >= bmin && b < bmax) {
+    if ((live >> b) & 1u) {
       const double vc = vnext, wc = wnext;
       if (b + 1 < kTriBlk) {
         vnext = v[c + 16 * (b + 1)];
@@ -348,6 +349,8 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
       const double tau = uniform_f64(stau[cb]);  // reflection of column j
       const int jn = j + 1;         // the next column to reflect
       const int bmin = jn >> 4;     // blocks below bmin are eliminated
+      // the live column blocks bmin..bmax-1 as a bit mask (one bit test per block)
+      const unsigned live = ((1u << bmax) - 1u) & ~((1u << bmin) - 1u);
       if (tau != 0.0) {             // uniform
         // the wave that reflects column jn rotates with the step, so that the two workgroups
         // sharing a CU rarely have their look-ahead waves on the same SIMD
@@ -361,12 +364,12 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
           yr[a] = 0.0;
         }
         // this thread's part of v'A v
-        double s = sym_matvec(R, vr, yr, v, spc, r, c, bmin, bmax);
+        double s = sym_matvec(R, vr, yr, v, spc, r, c, bmin, live);
         // the reductions that end the phase are latency-bound: raised priority (4.61 -> 4.56 ms)
         __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int a = 0; a < kTriBlk; a++) {
-          if (a >= bmin) {  // (rows at or beyond np: zero sums)
+          if ((live >> a) & 1u) {
             s = fma(vr[a], yr[a], s);
             const double t = sum8(yr[a]);  // (the last 8-group sum is left to the assembly)
             if ((c & 7) == 0) spr[c >> 3][r + 16 * a] = t;
@@ -436,7 +439,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         double wr[kTriBlk];
 #pragma unroll
         for (int a = 0; a < kTriBlk; a++) wr[a] = sw[r + 16 * a];
-        sym_update(R, vr, wr, v, sw, h, c, bmin, bmax);
+        sym_update(R, vr, wr, v, sw, h, c, bmin, live);
         __builtin_amdgcn_s_setprio(0);
       } else {
         // no reflection at this step: column jn stands as it is in the registers
