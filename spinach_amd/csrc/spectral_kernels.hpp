@@ -186,11 +186,10 @@ __device__ __forceinline__ double uniform_f64(double v) {
 }
 
 // A*v over this thread's slots of the live column blocks (bmin <= b < bmax, the set bits of
-// live): row sums into
-// yr, column sums stored to spc[r][.]; returns this thread's part of v'A v from the column
-// sums (the row sums' part is added by the caller). The row blocks of a live column block
-// are not tested against bmax: rows at or beyond np hold zeros (a test there would be
-// compiled to a select per product).
+// live): row sums into yr, column sums stored to spc[r][.]; returns this thread's part of
+// v'A v from the column sums (the row sums' part is added by the caller). The row blocks of
+// a live column block are not tested against bmax: rows at or beyond np hold zeros (a test
+// there would be compiled to a select per product).
 __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const double (&vr)[kTriBlk],
                                              double (&yr)[kTriBlk], const double* v, double (*spc)[kTriNP], int r,
                                              int c, int bmin, unsigned live) {
