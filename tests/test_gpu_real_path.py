@@ -129,3 +129,30 @@ def test_dense_real_generator_spectral_path():
     e = O.rel_err(f, ref)
     print(f'dense real generator, 185 states, 3 x 4096: vs oracle {e:.2e}', flush=True)
     assert e < 1e-10
+
+
+def test_tridiagonal_generator_takes_no_reflection():
+    # a generator that is already tridiagonal on the 8-spin system's 185 coil-connected states
+    # (kept in index order by the reduction) and orientation-independent (no Q): every
+    # Householder step finds nothing below the subdiagonal and publishes the column as it is
+    w = workloads.static_8spin(n_orient=4000)
+    sub = grids.subset(w['grid'], np.arange(2))
+    idx = O.union_states(w['I'], w['Q'], w['coil'])
+    rng = np.random.default_rng(12)
+    m = len(idx)
+    scale = abs(w['I']).max()
+    I = sps.lil_matrix(w['I'].shape)
+    for k in range(m):
+        I[idx[k], idx[k]] = scale * rng.uniform(-1, 1)
+        if k + 1 < m:
+            e = 0.1 * scale * rng.uniform(0.5, 1.0)
+            I[idx[k], idx[k + 1]] = e
+            I[idx[k + 1], idx[k]] = e
+    y = dict(w, I=sps.csc_matrix(I), Q=[None] * len(w['Q']))
+    f, _, st = _run(y, sub, 'fp64')
+    print('tridiagonal generator stats', st, flush=True)
+    assert st['spectral_systems'] == 1 and st['reduced_dim'] == m
+    ref = O.refcpu_powder(y['I'], y['Q'], sub, y['rho0'], y['coil'], y['dt'], y['nsteps'], variant=2)
+    e = O.rel_err(f, ref)
+    print(f'tridiagonal generator, 185 states, 2 x 4096: vs oracle {e:.2e}', flush=True)
+    assert e < 1e-10
