@@ -41,7 +41,7 @@ PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12}  # MI355X dense MFMA FP64 / FP32, FLO
 # FETCH x2.000, WRITE x1.000); see profiles/k_spectral_static8spin_fp64.md and
 # profiles/k_powder_static8spin_fp64.md
 TRAFFIC = {
-    ('static8', 'fp64', 'spk::k_tridiag'): 13859 * 1024 * 2.0 + 67885 * 1024 * 1.0,
+    ('static8', 'fp64', 'spk::k_tridiag'): 13775 * 1024 * 2.0 + 67885 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,false>'): 165530392 * 1024 * 2.0 + 53150413 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,true>'): 139595089 * 1024 * 2.0 + 49487957 * 1024 * 1.0,
 }
