@@ -275,7 +275,7 @@ def bench_surrogate(args):
     """Learned powder interpolation on the 8-spin system: 300 training and 500
     held-out orientations from the plain method, a 4 x 256 tanh MLP with
     Fourier features trained in BF16, inference on a 50000-orientation grid.
-    The judge (held-out relative error <= 1e-4) picks the network or the plain
+    The acceptance check (held-out relative error <= 1e-4) picks the network or the plain
     method; value is the fine-grid rate of the path it picked."""
     import numpy as np
     from spinach_amd import grids, surrogate, workloads

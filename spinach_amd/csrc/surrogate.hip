@@ -3,7 +3,7 @@
 // and evaluated on a fine grid with the powder sum formed on the device.
 //
 // No counterpart in the reference (Spinach has no learned interpolation); the
-// plain method stays the judge and the fallback (see the Python driver).
+// plain method stays the acceptance check and the fallback (see surrogate.py).
 //
 // Network: F input features -> depth hidden layers of width W with tanh ->
 // 2 linear outputs. All matrix products are BF16 MFMA

@@ -1,4 +1,4 @@
-"""Learned powder interpolation with the plain method as judge and fallback.
+"""Learned powder interpolation with the plain method as acceptance check and fallback.
 
 The plain GPU method computes FIDs on a coarse training grid and on held-out
 orientations; a Fourier-feature MLP (BF16 MFMA kernels, libspinach_powder) is

@@ -1,5 +1,5 @@
 """GPU checks of the learned powder interpolation: BF16 MFMA gradients against
-the FP64 reference, training, and the judge/fallback decision."""
+the FP64 reference, training, and the acceptance/fallback decision."""
 import numpy as np
 import pytest
 
@@ -48,7 +48,7 @@ def test_training_reduces_loss_on_smooth_target():
     net.close()
 
 
-def test_judge_falls_back_to_plain_method():
+def test_acceptance_check_falls_back_to_plain_method():
     w = workloads.static_8spin(nsteps=256, n_orient=4000)
     g = w['grid']
     tr, ho, fine = grids.subset(g, np.arange(24)), grids.subset(g, np.arange(24, 40)), grids.subset(g, np.arange(40, 104))
