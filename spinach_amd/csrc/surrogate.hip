@@ -184,7 +184,7 @@ __global__ void k_loss_grad(const float* Y, const float* target, int64_t n, floa
   }
   red[threadIdx.x] = l;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (unsigned s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
