@@ -347,7 +347,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print('bench.py measures one GPU: every box in this setup has a single MI355X, and the multi-GPU path '
+        print('bench.py measures one GPU: the multi-GPU path '
               '(orientation sharding + one FID all-reduce) is a design, not a build. Run with --gpus 1.',
               file=sys.stderr)
         sys.exit(2)
