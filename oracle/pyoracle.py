@@ -34,8 +34,34 @@ _dp = C.POINTER(C.c_double)
 _lib = None
 
 # default tolerance vector: liouv_zero, zte_tol, prop_chop, small_matrix,
-# krylov_tol, subs_drop, zte_maxden, zte_nsteps (kernel/utilities/tolerances.m)
-DEFAULT_TOLS = (1e-10, 1e-24, 1e-10, 200.0, 10000.0, 1e-10, 0.5, 32.0)
+# krylov_tol, subs_drop, zte_maxden, zte_nsteps, merge_dim
+# (kernel/utilities/tolerances.m)
+TOL_NAMES = ('liouv_zero', 'zte_tol', 'prop_chop', 'small_matrix', 'krylov_tol', 'subs_drop', 'zte_maxden',
+             'zte_nsteps', 'merge_dim')
+DEFAULT_TOLS = (1e-10, 1e-24, 1e-10, 200.0, 10000.0, 1e-10, 0.5, 32.0, 1000.0)
+
+# Planted mistakes of the C++ restatement (refcpu.cpp, enum Plant): the tests
+# use them to show that each check of the oracle fails on a plausible mistake.
+PLANT_STEP_SIGN, PLANT_STEP_TOL, PLANT_DROP_BIN, PLANT_DUP_BOX, PLANT_KRYLOV_SHIFT = 1, 2, 3, 4, 5
+
+
+def make_tols(**over):
+    """DEFAULT_TOLS with named entries replaced, e.g. make_tols(krylov_tol=2)."""
+    t = dict(zip(TOL_NAMES, DEFAULT_TOLS))
+    for k, v in over.items():
+        if k not in t:
+            raise KeyError(k)
+        t[k] = float(v)
+    return tuple(t[k] for k in TOL_NAMES)
+
+
+def _tol_vector(tols, plant):
+    t = list(tols)
+    if len(t) == 8:  # the pre-merge_dim layout
+        t.append(DEFAULT_TOLS[8])
+    if len(t) != 9:
+        raise ValueError('tolerance vector must have 9 entries')
+    return np.array(t + [float(plant)], np.float64)
 
 
 def build():
@@ -60,17 +86,25 @@ def load():
         lib.refcpu_wigner.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp]
         lib.refcpu_propagator.restype = C.c_int
         lib.refcpu_propagator.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, _dp]
+        _ip = C.POINTER(C.c_int64)
+        lib.refcpu_step.restype = C.c_int
+        lib.refcpu_step.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, C.c_double, C.c_int]
+        lib.refcpu_reduce.restype = C.c_int64
+        lib.refcpu_reduce.argtypes = [C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p), C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                      _ip, C.c_int64, _ip]
         _lib = lib
     return _lib
 
 
-def refcpu_powder(I, qlist, grid, rho0, coil, dt, nsteps, variant=0, nthreads=0, tols=DEFAULT_TOLS,
-                  per_orientation=False):
-    """variant 0: faithful per-orientation reduce + reference propagator
-    branches; 1: same reduction with exact expm at every size; 2: single
-    union-pattern reduction with exact expm."""
-    lib = load()
-    dim = I.shape[0]
+def _csr_arrays(m):
+    m = sps.csr_matrix(m, dtype=np.complex128)
+    m.sum_duplicates()
+    return (np.ascontiguousarray(m.indptr, np.int64), np.ascontiguousarray(m.indices, np.int64),
+            np.ascontiguousarray(m.data, np.complex128))
+
+
+def _slot_arrays(I, qlist):
     keep = []
     ptrs = (C.c_void_p * 35)()
     idxs = (C.c_void_p * 35)()
@@ -79,20 +113,29 @@ def refcpu_powder(I, qlist, grid, rho0, coil, dt, nsteps, variant=0, nthreads=0,
         if m is None or m.nnz == 0:
             ptrs[t] = None
             continue
-        m = sps.csr_matrix(m, dtype=np.complex128)
-        m.sum_duplicates()
-        p = np.ascontiguousarray(m.indptr, np.int64)
-        i = np.ascontiguousarray(m.indices, np.int64)
-        v = np.ascontiguousarray(m.data, np.complex128)
+        p, i, v = _csr_arrays(m)
         keep += [p, i, v]
         ptrs[t] = p.ctypes.data
         idxs[t] = i.ctypes.data
         vals[t] = v.ctypes.data
+    return keep, ptrs, idxs, vals
+
+
+def refcpu_powder(I, qlist, grid, rho0, coil, dt, nsteps, variant=0, nthreads=0, tols=DEFAULT_TOLS,
+                  per_orientation=False, plant=0):
+    """variant 0: faithful per-orientation reduce + reference propagator
+    branches; 1: same reduction with exact expm at every size; 2: single
+    union-pattern reduction with exact expm. Subspaces of dimension
+    >= krylov_tol are propagated by step() per time step (krylov.m) in every
+    variant. plant: a planted mistake (PLANT_*; tests only)."""
+    lib = load()
+    dim = I.shape[0]
+    keep, ptrs, idxs, vals = _slot_arrays(I, qlist)
     g = {k: np.ascontiguousarray(grid[k], np.float64) for k in ('alphas', 'betas', 'gammas', 'weights')}
     n_or = len(g['weights'])
     r = np.ascontiguousarray(rho0, np.complex128)
     c = np.ascontiguousarray(coil, np.complex128)
-    tl = np.array(tols, np.float64)
+    tl = _tol_vector(tols, plant)
     npts = int(nsteps) + 1
     fid = np.zeros(npts, np.complex128)
     per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
@@ -105,6 +148,42 @@ def refcpu_powder(I, qlist, grid, rho0, coil, dt, nsteps, variant=0, nthreads=0,
     if rc:
         raise RuntimeError('refcpu_powder: ' + lib.refcpu_last_error().decode())
     return (fid, per) if per_orientation else fid
+
+
+def refcpu_step(L, rho, dt, plant=0):
+    """step(L, rho, dt) of kernel/step.m:204-375 (reordered Taylor with
+    substeps) for a sparse generator L; returns the new state."""
+    lib = load()
+    p, i, v = _csr_arrays(L)
+    r = np.array(rho, np.complex128).ravel().copy()
+    rc = lib.refcpu_step(L.shape[0], p.ctypes.data_as(C.POINTER(C.c_int64)), i.ctypes.data_as(C.POINTER(C.c_int64)),
+                         v.view(np.float64).ctypes.data_as(_dp), r.view(np.float64).ctypes.data_as(_dp), float(dt),
+                         int(plant))
+    if rc:
+        raise RuntimeError('refcpu_step: ' + lib.refcpu_last_error().decode())
+    return r
+
+
+def refcpu_reduce(I, qlist, euler, coil, tols=DEFAULT_TOLS, plant=0):
+    """The subspaces reduce(L',coil) (ZTE, path tracing, binpack) keeps at one
+    orientation, as lists of state indices in propagation order."""
+    lib = load()
+    dim = I.shape[0]
+    keep, ptrs, idxs, vals = _slot_arrays(I, qlist)
+    c = np.ascontiguousarray(coil, np.complex128)
+    tl = _tol_vector(tols, plant)
+    dims = np.zeros(dim, np.int64)
+    states = np.zeros(dim, np.int64)
+    ns = lib.refcpu_reduce(dim, ptrs, idxs, vals, float(euler[0]), float(euler[1]), float(euler[2]),
+                           c.view(np.float64).ctypes.data_as(_dp), tl.ctypes.data_as(_dp),
+                           dims.ctypes.data_as(C.POINTER(C.c_int64)), dim, states.ctypes.data_as(C.POINTER(C.c_int64)))
+    if ns < 0:
+        raise RuntimeError('refcpu_reduce: ' + lib.refcpu_last_error().decode())
+    out, pos = [], 0
+    for k in range(ns):
+        out.append(states[pos:pos + dims[k]].copy())
+        pos += dims[k]
+    return out
 
 
 def refcpu_expm(a):

@@ -317,6 +317,19 @@ static double norm1(const Csr& m) {
   return r;
 }
 
+// Planted mistakes, selected by the last entry of the tolerance vector. They
+// exist only so that the tests can show that each check of the oracle fails
+// on a plausible mistake (0 = none, the faithful restatement).
+enum Plant {
+  kPlantNone = 0,
+  kPlantStepSign = 1,    // step(): the Taylor term with +i instead of -i
+  kPlantStepTol = 2,     // step(): the series stopped at 1e-8 instead of eps (dropped terms)
+  kPlantDropBin = 3,     // path_trace(): the last bin of binpack() lost
+  kPlantDupBox = 4,      // binpack(): each bin after the first also gets the states of the bin before
+  kPlantKrylovShift = 5  // krylov(): detection after the step instead of before it
+};
+static int g_plant = kPlantNone;
+
 // step(L,rho,dt) Liouville branch with reordered Taylor (step.m:204-267,328-375)
 static void step_liouv(const Csr& L, std::vector<cd>& rho, double dt) {
   double scaling = 0;
@@ -328,7 +341,8 @@ static void step_liouv(const Csr& L, std::vector<cd>& rho, double dt) {
   if (nsteps > 10000) throw std::runtime_error("either dt is too long, or L is too big: |L*dt|>1e4");
   if (nsteps < 1) nsteps = 1;  // ceil(0/2)=0 substeps leaves rho unchanged in the reference
   if (norm_mat == 0) nsteps = 0;
-  const double tol = 2.220446049250313e-16;
+  const double tol = (g_plant == kPlantStepTol) ? 1e-8 : 2.220446049250313e-16;
+  const double isign = (g_plant == kPlantStepSign) ? 1.0 : -1.0;
   std::vector<cd> next(rho.size()), tmp(rho.size());
   for (i64 s = 0; s < nsteps; s++) {
     next = rho;
@@ -341,7 +355,7 @@ static void step_liouv(const Csr& L, std::vector<cd>& rho, double dt) {
         }
       if (!any) break;
       spmv(L, next.data(), tmp.data());
-      cd f = cd(0, -1.0 / k) * (dt / (double)nsteps);
+      cd f = cd(0, isign / k) * (dt / (double)nsteps);
       for (size_t i = 0; i < next.size(); i++) {
         next[i] = f * tmp[i];
         rho[i] += next[i];
@@ -627,6 +641,9 @@ static std::vector<std::vector<i64>> path_trace(const Csr& L, const std::vector<
     bins.push_back(bin);
     rest = left;
   }
+  if (g_plant == kPlantDropBin && bins.size() > 1) bins.pop_back();
+  if (g_plant == kPlantDupBox)
+    for (size_t k = bins.size(); k-- > 1;) bins[k].insert(bins[k].end(), bins[k - 1].begin(), bins[k - 1].end());
   return bins;
 }
 
@@ -682,11 +699,12 @@ static void evolve_sub(const Csr& Lsub, std::vector<cd> rho, std::vector<cd> coi
   } else {
     // krylov(...,'observable') (krylov.m:139-163)
     for (i64 t = 0; t <= nsteps; t++) {
+      if (g_plant == kPlantKrylovShift) step_liouv(L, rho, dt);
       cd s(0);
       for (int i = 0; i < n; i++) s += std::conj(coil[i]) * rho[i];
       out[t] += s;
       if (t == nsteps) break;
-      step_liouv(L, rho, dt);
+      if (g_plant != kPlantKrylovShift) step_liouv(L, rho, dt);
     }
   }
 }
@@ -700,6 +718,58 @@ static Csr from_arrays(i64 n, const i64* ptr, const i64* idx, const double* val)
   m.val.resize(nnz);
   for (i64 p = 0; p < nnz; p++) m.val[p] = cd(val[2 * p], val[2 * p + 1]);
   return m;
+}
+
+// Tolerance vector (kernel/utilities/tolerances.m names): liouv_zero, zte_tol,
+// prop_chop, small_matrix, krylov_tol, subs_drop, zte_maxden, zte_nsteps,
+// merge_dim, then the planted-mistake code (0 in every use but the tests).
+static Tols read_tols(const double* tols) {
+  Tols tl;
+  tl.liouv_zero = tols[0];
+  tl.zte_tol = tols[1];
+  tl.prop_chop = tols[2];
+  tl.small_matrix = tols[3];
+  tl.krylov_tol = tols[4];
+  tl.subs_drop = tols[5];
+  tl.zte_maxden = tols[6];
+  tl.zte_nsteps = (int)tols[7];
+  tl.merge_dim = (int)tols[8];
+  g_plant = (int)tols[9];
+  return tl;
+}
+
+static Problem make_problem(i64 dim, const i64* const* ptrs, const i64* const* idxs, const double* const* vals,
+                            std::vector<Csr>& store) {
+  Problem pr;
+  pr.dim = dim;
+  pr.mats.assign(35, nullptr);
+  for (int t = 0; t < 35; t++)
+    if (ptrs[t]) {
+      store[t] = from_arrays(dim, ptrs[t], idxs[t], vals[t]);
+      pr.mats[t] = &store[t];
+    }
+  build_pattern(pr);
+  return pr;
+}
+
+// The per-orientation reduction of evolution.m:141-145 (reduce(L',coil):
+// zte() then path_trace() with binpack()): the state lists of the subspaces,
+// in the order the reference propagates them.
+static std::vector<std::vector<i64>> reduce_orientation(const Problem& pr, double a, double b, double g,
+                                                        const std::vector<cd>& c0, const Tols& tl) {
+  Csr H = assemble(pr, a, b, g);
+  Csr Lt = ctranspose(H, pr.tpos);
+  std::vector<i64> kz = zte(Lt, c0, tl);
+  Csr Lz = restrict_(Lt, kz);
+  std::vector<cd> cz(kz.size());
+  for (size_t k = 0; k < kz.size(); k++) cz[k] = c0[kz[k]];
+  std::vector<std::vector<i64>> subs;
+  for (auto& s : path_trace(Lz, cz, tl)) {
+    std::vector<i64> glob(s.size());
+    for (size_t k = 0; k < s.size(); k++) glob[k] = kz[s[k]];
+    subs.push_back(glob);
+  }
+  return subs;
 }
 
 }  // namespace
@@ -720,29 +790,12 @@ const char* refcpu_last_error() { return g_err.c_str(); }
 // empty. Complex values are interleaved (re, im).
 int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const double* const* vals,
                   const double* alphas, const double* betas, const double* gammas, const double* weights,
-                  i64 n_or, const double* rho0, const double* coil, double dt, i64 nsteps, const double* tols8,
+                  i64 n_or, const double* rho0, const double* coil, double dt, i64 nsteps, const double* tols,
                   int variant, int nthreads, double* fid_out, double* per_orient_out) {
   try {
-    Tols tl;
-    tl.liouv_zero = tols8[0];
-    tl.zte_tol = tols8[1];
-    tl.prop_chop = tols8[2];
-    tl.small_matrix = tols8[3];
-    tl.krylov_tol = tols8[4];
-    tl.subs_drop = tols8[5];
-    tl.zte_maxden = tols8[6];
-    tl.zte_nsteps = (int)tols8[7];
-    tl.merge_dim = 1000;
+    Tols tl = read_tols(tols);
     std::vector<Csr> store(35);
-    Problem pr;
-    pr.dim = dim;
-    pr.mats.assign(35, nullptr);
-    for (int t = 0; t < 35; t++)
-      if (ptrs[t]) {
-        store[t] = from_arrays(dim, ptrs[t], idxs[t], vals[t]);
-        pr.mats[t] = &store[t];
-      }
-    build_pattern(pr);
+    Problem pr = make_problem(dim, ptrs, idxs, vals, store);
     std::vector<cd> r0(dim), c0(dim);
     for (i64 i = 0; i < dim; i++) {
       r0[i] = cd(rho0[2 * i], rho0[2 * i + 1]);
@@ -780,21 +833,10 @@ int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const
         Csr H = assemble(pr, alphas[o], betas[o], gammas[o]);
         cd* out = &per[(size_t)o * npts];
         std::vector<std::vector<i64>> subs;
-        if (variant == 2) {
+        if (variant == 2)
           subs.push_back(ustates);
-        } else {
-          Csr Lt = ctranspose(H, pr.tpos);  // reduce(L',coil)
-          std::vector<i64> kz = zte(Lt, c0, tl);
-          Csr Lz = restrict_(Lt, kz);
-          std::vector<cd> cz(kz.size());
-          for (size_t k = 0; k < kz.size(); k++) cz[k] = c0[kz[k]];
-          auto pt = path_trace(Lz, cz, tl);
-          for (auto& s : pt) {
-            std::vector<i64> g(s.size());
-            for (size_t k = 0; k < s.size(); k++) g[k] = kz[s[k]];
-            subs.push_back(g);
-          }
-        }
+        else
+          subs = reduce_orientation(pr, alphas[o], betas[o], gammas[o], c0, tl);
         for (auto& s : subs) {
           Csr Ls = restrict_(H, s);
           std::vector<cd> rs(s.size()), cs(s.size());
@@ -819,6 +861,7 @@ int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const
       fid_out[2 * t] = acc.real();
       fid_out[2 * t + 1] = acc.imag();
     }
+    g_plant = kPlantNone;
     if (per_orient_out)
       for (size_t k = 0; k < per.size(); k++) {
         per_orient_out[2 * k] = per[k].real();
@@ -826,6 +869,7 @@ int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const
       }
     return 0;
   } catch (std::exception& e) {
+    g_plant = kPlantNone;
     g_err = e.what();
     return 1;
   }
@@ -878,6 +922,59 @@ int refcpu_propagator(int n, const double* L, double dt, double small_matrix, do
   } catch (std::exception& e) {
     g_err = e.what();
     return 1;
+  }
+}
+
+// step(L,rho,dt) (kernel/step.m:204-267, reordered_taylor :328-375) for one
+// CSR generator; rho (n interleaved complex) is overwritten. plant selects a
+// planted mistake (0 = none).
+int refcpu_step(i64 n, const i64* ptr, const i64* idx, const double* val, double* rho, double dt, int plant) {
+  try {
+    Csr L = from_arrays(n, ptr, idx, val);
+    std::vector<cd> r(n);
+    for (i64 i = 0; i < n; i++) r[i] = cd(rho[2 * i], rho[2 * i + 1]);
+    g_plant = plant;
+    step_liouv(L, r, dt);
+    g_plant = kPlantNone;
+    for (i64 i = 0; i < n; i++) {
+      rho[2 * i] = r[i].real();
+      rho[2 * i + 1] = r[i].imag();
+    }
+    return 0;
+  } catch (std::exception& e) {
+    g_plant = kPlantNone;
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// The subspaces reduce(L',coil) keeps at one orientation: their count is
+// returned (negative on failure) and their dimensions written to dims_out
+// (at most max_out entries); states_out (optional, at most dim entries)
+// receives the state indices of the subspaces one after another.
+i64 refcpu_reduce(i64 dim, const i64* const* ptrs, const i64* const* idxs, const double* const* vals, double alpha,
+                  double beta, double gamma, const double* coil, const double* tols, i64* dims_out, i64 max_out,
+                  i64* states_out) {
+  try {
+    Tols tl = read_tols(tols);
+    std::vector<Csr> store(35);
+    Problem pr = make_problem(dim, ptrs, idxs, vals, store);
+    std::vector<cd> c0(dim);
+    for (i64 i = 0; i < dim; i++) c0[i] = cd(coil[2 * i], coil[2 * i + 1]);
+    auto subs = reduce_orientation(pr, alpha, beta, gamma, c0, tl);
+    g_plant = kPlantNone;
+    i64 pos = 0;
+    for (size_t k = 0; k < subs.size(); k++) {
+      if ((i64)k < max_out) dims_out[k] = (i64)subs[k].size();
+      if (states_out)
+        for (i64 v : subs[k])
+          if (pos < dim) states_out[pos++] = v;
+    }
+    return (i64)subs.size();
+  } catch (std::exception& e) {
+    g_plant = kPlantNone;
+    g_err = e.what();
+    return -1;
   }
 }
 

@@ -96,11 +96,7 @@ struct KParams {
   int32_t np_max;
   double* per_or;    // per-orientation FIDs: interleaved complex FP64, (nsteps+1) per orientation
   int32_t* sq_out;   // per orientation: scaling exponent used (diagnostics)
-  unsigned long long* phase_cycles;  // optional: per workgroup, kPhases clock64() totals
 };
-
-// phases timed by the optional per-workgroup cycle counters
-constexpr int kPhases = 5;  // assemble+norm, expm, powers+U block, V chain, detection
 
 // zeroes the kQueues work-queue heads of a plan: [0] propagator items with a
 // complex generator, [1] tridiagonalisation, [2] synthesis, [3] propagator
@@ -685,14 +681,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
     Buf<R> bf[6];
     for (int b = 0; b < 6; b++) bf[b] = Buf<R>{bufbase[b], pl, np};
 
-    long long t_mark = clock64();
-    auto mark = [&](int ph) {
-      if (p.phase_cycles && tid == 0) {
-        long long t = clock64();
-        p.phase_cycles[(size_t)blockIdx.x * kPhases + ph] += (unsigned long long)(t - t_mark);
-        t_mark = t;
-      }
-    };
     // 1. Wigner coefficients
     if (tid == 0) wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], sh.cre, sh.cim);
     // 2. zero X and the U block
@@ -760,7 +748,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
       for (size_t e = tid; e < 2 * pl; e += kThreads) X[e] *= scl;
       __syncthreads();
     }
-    mark(0);
     // 4. P = expm(X): Paterson-Stockmeyer Taylor polynomial and s squarings
     Buf<R> cur;
     if constexpr (REAL)
@@ -774,7 +761,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
         if (b2 != 2 && bf[b2].p != cur.p) spare[ns++] = bf[b2];
     }
     int sp_i = 0;
-    mark(1);
     // cur = P. 5a. U doubling with P^(2^k) and squarings up to Q = P^B (stored transposed in QT)
     const double* coilv = p.coil + 2 * sd.vec_off;
     const double* rhov = p.rho0 + 2 * sd.vec_off;
@@ -843,7 +829,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
       }
       __syncthreads();
     }
-    mark(2);
     // 5b. V chain: v_{k+1}[r] = sum_c Q[r][c] v_k[c] = sum_c QT[c][r] v_k[c].
     //     The vector lives in LDS (the GEMM staging area is idle here); each
     //     wave sums a quarter of the columns for 64 rows at a time with
@@ -901,7 +886,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
       }
       __syncthreads();
     }
-    mark(3);
     // 5c. F = U * V^T ; f(kB + j) = F[j][k]
     {
       const int64_t npts = p.nsteps + 1;
@@ -917,7 +901,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
                  },
                  sh.g);
     }
-    mark(4);
   }
 }
 

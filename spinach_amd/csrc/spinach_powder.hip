@@ -379,8 +379,6 @@ struct sp_plan {
   bool mas = false;
   int32_t ticks = 0, n_rho = 0, mas_np = 0, mas_nnz = 0, mas_nq = 0;
   DevBuf d_rot2lab, d_lab2rot;
-  DevBuf d_phase;  // optional per-workgroup phase cycle counters (SPINACH_PHASE_TIMING=1)
-  std::vector<double> phase_ms;
 };
 
 namespace {
@@ -589,8 +587,6 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->work_stride = 6 * mat + 2 * (int64_t)(pl->block + 16) * np_prop + 2 * (int64_t)pl->nblk * np_prop;
     pl->work_stride = (pl->work_stride + 63) / 64 * 64;
     pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
-    if (const char* e = getenv("SPINACH_PHASE_TIMING"))
-      if (atoi(e) > 0) pl->d_phase.alloc((size_t)pl->grid * spk::kPhases * sizeof(unsigned long long));
   }
   if (pl->n_spec > 0) {
     // two 256-thread workgroups per CU, each holding the lower triangle of a 192 x 192
@@ -635,7 +631,6 @@ void launch(sp_plan* pl) {
   kp.np_max = pl->np_prop;
   kp.per_or = pl->d_per.as<double>();
   kp.sq_out = pl->d_sq.as<int32_t>();
-  kp.phase_cycles = pl->d_phase.p ? pl->d_phase.as<unsigned long long>() : nullptr;
   // complex-generator items first (queue 0), then real-generator items (queue 3)
   const int64_t n_cplx = pl->n_prop_complex, n_real = pl->n_prop - pl->n_prop_complex;
   if (n_cplx > 0) {
@@ -893,7 +888,6 @@ void enqueue_pass(sp_plan* pl, bool events) {
   const int64_t npts = pl->nsteps + 1;
   hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, pl->d_queue.as<unsigned int>());
   HIPCHK(hipGetLastError());
-  if (events && pl->d_phase.p) HIPCHK(hipMemsetAsync(pl->d_phase.p, 0, pl->d_phase.bytes, st));
   if (events) HIPCHK(hipEventRecord(ctx->ev1, st));
   if (pl->mas) {
     if (pl->mode == SP_MODE_FP64)
@@ -951,8 +945,7 @@ int run(sp_plan* pl, sp_stats* stats) {
   sp_ctx* ctx = pl->ctx;
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  // phase timing (a debugging aid) clears its counters with a memset: eager only
-  const bool graph = pl->graph && !pl->d_phase.p;
+  const bool graph = pl->graph;
   if (graph && !pl->gexec) record_pass(pl);
   HIPCHK(hipEventRecord(ctx->ev0, st));
   if (graph)
@@ -1010,17 +1003,6 @@ int run(sp_plan* pl, sp_stats* stats) {
   s.spectral_systems = pl->spectral_systems;
   s.eig_max_sweeps = max_sweeps;
   for (int k = 0; k < 4; k++) s.stage_ms[k] = ms_stage[k];
-  if (pl->d_phase.p) {
-    std::vector<unsigned long long> ph((size_t)pl->grid * spk::kPhases);
-    HIPCHK(hipMemcpy(ph.data(), pl->d_phase.p, ph.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    fprintf(stderr, "[spinach_powder] phase cycles summed over %d workgroups:", pl->grid);
-    for (int k = 0; k < spk::kPhases; k++) {
-      unsigned long long t = 0;
-      for (int g = 0; g < pl->grid; g++) t += ph[(size_t)g * spk::kPhases + k];
-      fprintf(stderr, " %llu", t);
-    }
-    fprintf(stderr, "\n");
-  }
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   pl->last = s;
   if (stats) *stats = s;

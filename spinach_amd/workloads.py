@@ -108,6 +108,12 @@ def _batch_member(args):
     return w
 
 
+def batch_member(seed, s, min_or=100, max_or=1000, min_spins=2, max_spins=10):
+    """Member s of batch_uneven(seed, ...) on its own (same draws, so member
+    363 of the 512-system batch can be rebuilt without the other 511)."""
+    return _batch_member((seed, s, min_spins, max_spins, min_or, max_or))
+
+
 def batch_uneven(seed=2, n_systems=512, nsteps=1024, min_or=100, max_or=1000, min_spins=2, max_spins=10,
                  workers=None):
     """Every member system s is drawn from its own generator seeded with

@@ -1,0 +1,180 @@
+"""Pins of the oracle's step(), krylov() and path_trace()/binpack()
+restatements (oracle/refcpu.cpp), each with a planted mistake that must fail
+the same check.
+
+Reference checks restated:
+  tests/kernel/test_dynamic_propagation_frontends.m:39-58  step() vs expm, 1e-12
+  tests/kernel/test_dynamic_propagation_frontends.m:146-175 krylov() observable vs P^n*rho, 1e-12
+  tests/kernel/test_step_matches_expm.m:39-45               step() vs expm
+  tests/lib/test_close.m:51-53   |obs - ref|_2 <= abs_tol + rel_tol * max(1, |ref|_2)
+The reduction pin compares the faithful per-orientation reduce(L',coil)
+(kernel/reduce.m, zte.m, path_trace.m, binpack.m) with merge_dim lowered so
+that path tracing and bin packing run, against propagation without any
+reduction (numpy/scipy, an independent restatement).
+"""
+import functools
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sps
+
+import pyoracle as O
+from spinach_amd import grids, workloads
+from spinach_amd import spinsys as S
+
+
+def close(obs, ref, abs_tol, rel_tol):
+    """test_close.m:51-53."""
+    obs, ref = np.ravel(obs), np.ravel(ref)
+    return np.linalg.norm(obs - ref) <= abs_tol + rel_tol * max(1.0, np.linalg.norm(ref))
+
+
+def _one_spin():
+    ss = S.create({'magnet': 14.1, 'isotopes': ['1H']}, {'zeeman': {'scalar': [0.0]}})
+    return S.basis(ss, {'formalism': 'sphten-liouv', 'approximation': 'none'})
+
+
+# ------------------------------------------------------------------- step()
+
+def test_step_one_spin_matches_expm():
+    """test_dynamic_propagation_frontends.m:39-58: one 1H spin, L = Lz, rho =
+    Lx + 0.25 Ly, dt = 2.5e-4, step() = expm(-1i*L*dt)*rho to 1e-12."""
+    ss = _one_spin()
+    L = sps.csr_matrix(S.operator(ss, 'Lz', '1H'))
+    rho = S.state(ss, 'Lx', '1H') + 0.25 * S.state(ss, 'Ly', '1H')
+    dt = 2.5e-4
+    ref = sla.expm(-1j * L.toarray() * dt) @ rho
+    assert close(O.refcpu_step(L, rho, dt), ref, 1e-12, 1e-12)
+    # zero time step returns the state (test_step_zero_time.m)
+    assert close(O.refcpu_step(L, rho, 0.0), rho, 1e-14, 1e-14)
+    # the Spinach sign convention: the planted +i step fails
+    assert not close(O.refcpu_step(L, rho, dt, plant=O.PLANT_STEP_SIGN), ref, 1e-12, 1e-12)
+
+
+def _eight_spin_generator():
+    w = workloads.static_8spin(n_orient=4)
+    keep = O.union_states(w['I'], w['Q'], w['coil'])
+    H = O.assemble_dense(w['I'], w['Q'], (w['grid']['alphas'][0], w['grid']['betas'][0], w['grid']['gammas'][0]))
+    H = H[np.ix_(keep, keep)]
+    return w, keep, H
+
+
+@pytest.mark.parametrize('mult', [1, 7])
+def test_step_eight_spin_generator_matches_expm(mult):
+    """The 185-state coil-connected generator of the 8-spin workload at one
+    orientation; mult = 7 makes |L dt|_1 large enough for several substeps
+    (nsteps = ceil(|L|_1 dt / 2), step.m:222-227)."""
+    w, keep, H = _eight_spin_generator()
+    dt = w['dt'] * mult
+    L = sps.csr_matrix(H)
+    nsub = int(np.ceil(np.abs(H).sum(axis=0).max() * dt / 2))
+    assert nsub >= (3 if mult > 1 else 1)
+    rng = np.random.default_rng(4)
+    rho = rng.normal(size=len(keep)) + 1j * rng.normal(size=len(keep))
+    ref = sla.expm(-1j * H * dt) @ rho
+    obs = O.refcpu_step(L, rho, dt)
+    err = np.linalg.norm(obs - ref) / np.linalg.norm(ref)
+    print(f'step vs expm, |L dt|_1 / 2 -> {nsub} substeps: rel err {err:.2e}')
+    assert close(obs, ref, 1e-12, 1e-12)
+    for plant in (O.PLANT_STEP_SIGN, O.PLANT_STEP_TOL):
+        assert not close(O.refcpu_step(L, rho, dt, plant=plant), ref, 1e-12, 1e-12), plant
+
+
+# ----------------------------------------------------------------- krylov()
+
+def _small_system(seed, nspins, n_or, nsteps):
+    rng = np.random.default_rng(seed)
+    y = workloads.random_system(rng, nspins)
+    y['grid'] = grids.haar(n_or, rng)
+    return y, 5e-6, nsteps
+
+
+@pytest.mark.parametrize('case', ['2spin', '5spin'])
+def test_krylov_branch_matches_propagator_loop(case):
+    """With krylov_tol lowered to 2 every subspace takes krylov() (evolution.m:
+    574): f(n) = coil'*rho after n step() calls. It must equal the propagator
+    loop f(n) = coil'*P^n*rho (evolution.m:562-566) to 1e-12, as
+    test_dynamic_propagation_frontends.m:146-175 requires; the planted detection
+    shift (one step late) must fail."""
+    if case == '2spin':
+        w = workloads.static_2spin(npoints=96, n_orient=6)
+        args = (w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'])
+    else:
+        y, dt, nsteps = _small_system(11, 5, 3, 48)
+        args = (y['I'], y['Q'], y['grid'], y['rho0'], y['coil'], dt, nsteps)
+    kry = O.make_tols(krylov_tol=2)
+    f_kry = O.refcpu_powder(*args, variant=1, tols=kry)
+    f_prop = O.refcpu_powder(*args, variant=1)
+    f_np = O.np_powder(*args, reduce=False)
+    e1, e2 = O.rel_err(f_kry, f_prop), O.rel_err(f_kry, f_np)
+    print(f'{case}: krylov vs propagator loop {e1:.2e}, vs numpy expm (no reduction) {e2:.2e}')
+    assert close(f_kry, f_prop, 1e-12, 1e-12)
+    assert close(f_kry, f_np, 1e-12, 1e-12)
+    f_bad = O.refcpu_powder(*args, variant=1, tols=kry, plant=O.PLANT_KRYLOV_SHIFT)
+    assert not close(f_bad, f_prop, 1e-12, 1e-12)
+
+
+# ---------------------------------------------------- path_trace + binpack
+
+@functools.lru_cache(maxsize=1)
+def _three_groups(seed=7):
+    """Three J-coupled groups (1H-1H-13C, 1H-13C, 1H-1H-15N) with no coupling
+    between the groups (no coordinates, so no dipolar terms): the states
+    connected to the 1H L+ coil fall into three independent subspaces, of 8, 2
+    and 8 states."""
+    rng = np.random.default_rng(seed)
+    iso = ['1H', '1H', '13C', '1H', '13C', '1H', '1H', '15N']
+    inter = {'zeeman': {'eigs': [list(rng.uniform(-100, 100, 3)) for _ in iso],
+                        'euler': [list(rng.uniform(0, 3, 3)) for _ in iso]},
+             'coupling': {'scalar': {(0, 1): 12.0, (1, 2): 140.0, (0, 2): 5.0, (3, 4): 150.0, (5, 6): 7.0,
+                                     (6, 7): -90.0}}}
+    w = workloads.build({'magnet': 14.1, 'isotopes': iso}, inter, {'formalism': 'sphten-liouv',
+                                                                  'approximation': 'IK-0', 'level': 3})
+    w['grid'] = grids.haar(4, rng)
+    return w
+
+
+@functools.lru_cache(maxsize=1)
+def _three_groups_unreduced():
+    w = _three_groups()
+    return O.np_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], 5e-6, 64, reduce=False)
+
+
+# merge_dim -> bin dimensions binpack.m gives for component sizes [8, 2, 8]
+# (components in order of their first state): boxes above merge_dim go alone
+# and first, the rest are cut into greedy prefixes whose cumsum <= merge_dim.
+BINS = {1: [8, 2, 8], 2: [8, 8, 2], 8: [8, 2, 8], 10: [10, 8], 20: [18]}
+
+
+@pytest.mark.parametrize('merge_dim', sorted(BINS))
+def test_path_trace_binpack_vs_no_reduction(merge_dim):
+    w = _three_groups()
+    args = (w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], 5e-6, 64)
+    tl = O.make_tols(merge_dim=merge_dim)
+    g = w['grid']
+    subs = O.refcpu_reduce(w['I'], w['Q'], (g['alphas'][0], g['betas'][0], g['gammas'][0]), w['coil'], tols=tl)
+    dims = [len(s) for s in subs]
+    allst = np.concatenate(subs)
+    print(f'merge_dim {merge_dim}: {len(subs)} bins of dims {dims}')
+    assert dims == BINS[merge_dim]
+    # a partition of the coil-connected states
+    assert len(np.unique(allst)) == len(allst)
+    assert set(allst) == set(O.union_states(w['I'], w['Q'], w['coil']))
+    f_red = O.refcpu_powder(*args, variant=1, tols=tl)
+    f_none = _three_groups_unreduced()
+    e = O.rel_err(f_red, f_none)
+    print(f'merge_dim {merge_dim}: reduced vs unreduced {e:.2e}')
+    assert close(f_red, f_none, 1e-12, 1e-12)
+    for plant in (O.PLANT_DROP_BIN, O.PLANT_DUP_BOX):
+        if len(subs) >= 2:
+            f_bad = O.refcpu_powder(*args, variant=1, tols=tl, plant=plant)
+            assert not close(f_bad, f_none, 1e-12, 1e-12), plant
+
+
+def test_default_merge_dim_skips_path_tracing():
+    """Below merge_dim (1000) path_trace.m:49-51 returns one projector."""
+    w = _three_groups()
+    g = w['grid']
+    subs = O.refcpu_reduce(w['I'], w['Q'], (g['alphas'][0], g['betas'][0], g['gammas'][0]), w['coil'])
+    assert [len(s) for s in subs] == [18]
