@@ -63,6 +63,15 @@ extern "C" {
  * Replaces: nothing in the reference (an execution option for callers that
  * replay a plan). */
 #define SP_MODE_GRAPH 64
+/* Flag OR-ed into the mode: propagate every system by time stepping, one
+ * step(L,rho,dt) per time step with detection after each (kernel/krylov.m:
+ * 139-163 'observable', kernel/step.m:204-375), as evolution() does when
+ * spin_system.sys.enable contains 'krylov' (kernel/evolution.m:531-533,574).
+ * Without the flag, systems whose coil-connected dimension is at least
+ * tols->krylov_tol, or above the dense limit of 1024, are stepped; the rest
+ * take the spectral or propagator path. The stepping path runs in FP64 in
+ * both precision modes. */
+#define SP_MODE_KRYLOV 128
 
 #define SP_NQ 34 /* Q slots: rank 1 (k,m) row-major 3x3, then rank 2 (k,m) row-major 5x5 */
 
@@ -79,9 +88,12 @@ typedef struct sp_csc {
   int32_t is_complex;
 } sp_csc;
 
-/* Tolerances from spin_system.tols (kernel/utilities/tolerances.m). The
- * library uses liouv_zero for the structural reduction; the others are
- * accepted for interface compatibility and reported back in sp_stats. */
+/* Tolerances from spin_system.tols (kernel/utilities/tolerances.m). NULL
+ * means the reference defaults below. The library uses krylov_tol for the path
+ * choice (see SP_MODE_KRYLOV); the coil-connected reduction is exact on the
+ * union pattern and the dense paths compute the exact exponential, so
+ * liouv_zero, zte_tol, prop_chop and small_matrix are validated (finite,
+ * non-negative) but change nothing. */
 typedef struct sp_tols {
   double liouv_zero;   /* 1e-10 */
   double zte_tol;      /* 1e-24 */
@@ -106,8 +118,9 @@ typedef struct sp_stats {
   int32_t real_systems;      /* systems whose generator is real at every orientation */
   int32_t spectral_systems;  /* systems evaluated on the spectral path */
   int32_t eig_max_sweeps;    /* most secular-equation iterations any root of the spectral path needed */
-  int32_t reserved;
-  double stage_ms[4];        /* propagator kernel, tridiagonalisation, tridiagonal eigensolver, synthesis */
+  int32_t krylov_systems;    /* systems propagated by time stepping (SP_MODE_KRYLOV) */
+  double stage_ms[4];        /* propagator and stepping kernels, tridiagonalisation, tridiagonal
+                                eigensolver, synthesis */
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */
@@ -180,6 +193,13 @@ int sp_execute(sp_plan* plan, sp_stats* stats);
 int sp_plan_result(sp_plan* plan, double* fid_out, double* per_orient_out);
 
 void sp_plan_destroy(sp_plan* plan);
+
+/* Sets the dead time of a plan (seconds, >= 0; 0 removes it): every
+ * orientation's initial state is first propagated by step(L,rho0,dead_time)
+ * and the acquisition starts from the result. Replaces: experiments/acquire.m:
+ * 65-67 (parameters.dead_time). Takes effect from the next sp_execute; applies
+ * to every system of a batch plan. Not available on MAS plans. */
+int sp_plan_set_dead_time(sp_plan* plan, double dead_time);
 
 /* ---- batched uneven workload ------------------------------------------ */
 

@@ -55,13 +55,13 @@ def make_tols(**over):
     return tuple(t[k] for k in TOL_NAMES)
 
 
-def _tol_vector(tols, plant):
+def _tol_vector(tols, plant, dead_time=0.0):
     t = list(tols)
     if len(t) == 8:  # the pre-merge_dim layout
         t.append(DEFAULT_TOLS[8])
     if len(t) != 9:
         raise ValueError('tolerance vector must have 9 entries')
-    return np.array(t + [float(plant)], np.float64)
+    return np.array(t + [float(plant), float(dead_time)], np.float64)
 
 
 def build():
@@ -122,12 +122,14 @@ def _slot_arrays(I, qlist):
 
 
 def refcpu_powder(I, qlist, grid, rho0, coil, dt, nsteps, variant=0, nthreads=0, tols=DEFAULT_TOLS,
-                  per_orientation=False, plant=0):
+                  per_orientation=False, plant=0, dead_time=0.0):
     """variant 0: faithful per-orientation reduce + reference propagator
     branches; 1: same reduction with exact expm at every size; 2: single
     union-pattern reduction with exact expm. Subspaces of dimension
     >= krylov_tol are propagated by step() per time step (krylov.m) in every
-    variant. plant: a planted mistake (PLANT_*; tests only)."""
+    variant. dead_time: rho0 = step(L, rho0, dead_time) at every orientation
+    before the acquisition (acquire.m:65-67). plant: a planted mistake
+    (PLANT_*; tests only)."""
     lib = load()
     dim = I.shape[0]
     keep, ptrs, idxs, vals = _slot_arrays(I, qlist)
@@ -135,7 +137,7 @@ def refcpu_powder(I, qlist, grid, rho0, coil, dt, nsteps, variant=0, nthreads=0,
     n_or = len(g['weights'])
     r = np.ascontiguousarray(rho0, np.complex128)
     c = np.ascontiguousarray(coil, np.complex128)
-    tl = _tol_vector(tols, plant)
+    tl = _tol_vector(tols, plant, dead_time)
     npts = int(nsteps) + 1
     fid = np.zeros(npts, np.complex128)
     per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
@@ -279,15 +281,16 @@ def np_powder(I, qlist, grid, rho0, coil, dt, nsteps, reduce=True, per_orientati
     return (fid, per) if per_orientation else fid
 
 
-def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False):
+def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False, t0=0.0):
     """Exact FID of the secular heteronuclear two-spin system with rho0 = Lx
     and coil = L+ on spin 0: f(t) = 1/4 exp(-i w0 t) cos(a t / 2), with
     w0 = tr(Z)/3 - basefrq + (R (Z - tr(Z)/3) R^T)_zz (offset chemical shift of spin 0) and
     a = (R C R^T)_zz summed over both stored coupling directions, where
-    R = euler2dcm(alpha, beta, gamma)."""
+    R = euler2dcm(alpha, beta, gamma). t0 shifts the time origin (a dead time:
+    the acquisition starts from the state at t0)."""
     from spinach_amd.spinsys import euler2dcm  # host setup data only
     n_or = len(grid['weights'])
-    t = np.arange(int(nsteps) + 1) * dt
+    t = t0 + np.arange(int(nsteps) + 1) * dt
     per = np.zeros((n_or, len(t)), complex)
     for o in range(n_or):
         R = euler2dcm(grid['alphas'][o], grid['betas'][o], grid['gammas'][o])

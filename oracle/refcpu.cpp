@@ -722,7 +722,9 @@ static Csr from_arrays(i64 n, const i64* ptr, const i64* idx, const double* val)
 
 // Tolerance vector (kernel/utilities/tolerances.m names): liouv_zero, zte_tol,
 // prop_chop, small_matrix, krylov_tol, subs_drop, zte_maxden, zte_nsteps,
-// merge_dim, then the planted-mistake code (0 in every use but the tests).
+// merge_dim, then the planted-mistake code (0 in every use but the tests);
+// refcpu_powder reads one more entry, the acquisition's dead time
+// (parameters.dead_time, 0 = none).
 static Tols read_tols(const double* tols) {
   Tols tl;
   tl.liouv_zero = tols[0];
@@ -794,6 +796,7 @@ int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const
                   int variant, int nthreads, double* fid_out, double* per_orient_out) {
   try {
     Tols tl = read_tols(tols);
+    const double dead_time = tols[10];
     std::vector<Csr> store(35);
     Problem pr = make_problem(dim, ptrs, idxs, vals, store);
     std::vector<cd> r0(dim), c0(dim);
@@ -832,6 +835,9 @@ int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const
       try {
         Csr H = assemble(pr, alphas[o], betas[o], gammas[o]);
         cd* out = &per[(size_t)o * npts];
+        // dead time: rho0 = step(L, rho0, dead_time) on the whole generator (acquire.m:65-67)
+        std::vector<cd> r0o = r0;
+        if (dead_time > 0) step_liouv(H, r0o, dead_time);
         std::vector<std::vector<i64>> subs;
         if (variant == 2)
           subs.push_back(ustates);
@@ -841,7 +847,7 @@ int refcpu_powder(i64 dim, const i64* const* ptrs, const i64* const* idxs, const
           Csr Ls = restrict_(H, s);
           std::vector<cd> rs(s.size()), cs(s.size());
           for (size_t k = 0; k < s.size(); k++) {
-            rs[k] = r0[s[k]];
+            rs[k] = r0o[s[k]];
             cs[k] = c0[s[k]];
           }
           evolve_sub(Ls, rs, cs, dt, nsteps, tl, variant == 0 ? 0 : 1, out);

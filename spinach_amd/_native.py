@@ -20,11 +20,12 @@ SP_MODE_FP32 = 1
 SP_MODE_COMPLEX = 16  # flag: keep complex arithmetic for a real generator
 SP_MODE_PROPAGATOR = 32  # flag: propagator power scheme for every system (no spectral path)
 SP_MODE_GRAPH = 64  # flag: record the pass once as a HIP graph and replay it
+SP_MODE_KRYLOV = 128  # flag: time stepping (step() per time step) for every system
 SP_NQ = 34
 
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
-           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real')
+           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real', 'sp_plan_set_dead_time')
 SP_MAS_FRAME_MAGNET = 0
 SP_MAS_FRAME_ROTOR = 1
 
@@ -45,7 +46,7 @@ class SpStats(C.Structure):
                 ('padded_dim', C.c_int32), ('block', C.c_int32), ('taylor_terms', C.c_int32),
                 ('max_squarings', C.c_int32), ('ps_block', C.c_int32), ('sum_squarings_lo', C.c_int32),
                 ('real_systems', C.c_int32), ('spectral_systems', C.c_int32), ('eig_max_sweeps', C.c_int32),
-                ('reserved', C.c_int32), ('stage_ms', C.c_double * 4)]
+                ('krylov_systems', C.c_int32), ('stage_ms', C.c_double * 4)]
 
     def as_dict(self):
         out = {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,6 +109,8 @@ def load():
     lib.sp_mas_fid.restype = C.c_int
     lib.sp_mas_fid.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
                                _dp, _dp, C.POINTER(SpRotor), C.c_int64, C.c_int, _dp, _dp, C.POINTER(SpStats)]
+    lib.sp_plan_set_dead_time.restype = C.c_int
+    lib.sp_plan_set_dead_time.argtypes = [C.c_void_p, C.c_double]
     lib.sp_generator_is_real.restype = C.c_int
     lib.sp_generator_is_real.argtypes = [C.POINTER(SpCsc), C.POINTER(SpCsc), C.POINTER(C.c_int32)]
     _lib = lib
@@ -207,9 +210,11 @@ def _tols(t):
 def mode_id(mode):
     """'fp64' / 'fp32' (or SP_MODE_*), with optional '+complex' (SP_MODE_COMPLEX:
     complex arithmetic even for a real generator), '+propagator'
-    (SP_MODE_PROPAGATOR: no spectral path) and '+graph' (SP_MODE_GRAPH: the pass
-    recorded once as a HIP graph and replayed) flags."""
-    flags = {'complex': SP_MODE_COMPLEX, 'propagator': SP_MODE_PROPAGATOR, 'graph': SP_MODE_GRAPH}
+    (SP_MODE_PROPAGATOR: no spectral path), '+graph' (SP_MODE_GRAPH: the pass
+    recorded once as a HIP graph and replayed) and '+krylov' (SP_MODE_KRYLOV: time
+    stepping for every system, sys.enable={'krylov'}) flags."""
+    flags = {'complex': SP_MODE_COMPLEX, 'propagator': SP_MODE_PROPAGATOR, 'graph': SP_MODE_GRAPH,
+             'krylov': SP_MODE_KRYLOV}
     if isinstance(mode, str) and '+' in mode:
         base, *extra = mode.split('+')
         out = mode_id(base)
@@ -218,7 +223,7 @@ def mode_id(mode):
                 raise ValueError(f'unknown mode flag {f!r}')
             out |= flags[f]
         return out
-    mask = SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH
+    mask = SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH | SP_MODE_KRYLOV
     if isinstance(mode, (int, np.integer)) and not isinstance(mode, bool) and int(mode) & mask:
         return mode_id(int(mode) & ~mask) | (int(mode) & mask)
     if mode in (SP_MODE_FP64, 'fp64', 'c128'):
@@ -247,6 +252,13 @@ class Plan:
             _err(self.ctx, rc, 'sp_execute')
         self.stats = st.as_dict()
         return self.stats
+
+    def set_dead_time(self, dead_time):
+        """sp_plan_set_dead_time: step(L, rho0, dead_time) before the acquisition
+        (experiments/acquire.m:65-67); 0 removes it."""
+        rc = load().sp_plan_set_dead_time(self.h, float(dead_time))
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_set_dead_time')
 
     def result(self, per_orientation=False):
         fid = np.zeros(self.npts, np.complex128)

@@ -79,6 +79,8 @@ struct KParams {
   const double* vals;    // interleaved complex FP64
   const int32_t* slots;  // Q slot ids 0..33
   const double* rho0;    // interleaved complex
+  const double* rho_or;  // optional per-orientation initial states (after a dead time), stride rho_or_stride
+  int64_t rho_or_stride;
   const double* coil;    // interleaved complex
   const double* alphas;
   const double* betas;
@@ -100,8 +102,9 @@ struct KParams {
 
 // zeroes the kQueues work-queue heads of a plan: [0] propagator items with a
 // complex generator, [1] tridiagonalisation, [2] synthesis, [3] propagator
-// items with a real generator
-constexpr int kQueues = 4;
+// items with a real generator, [4] stepping items with a complex generator,
+// [5] stepping items with a real generator, [6] the dead-time step of every item
+constexpr int kQueues = 8;
 __global__ void k_reset(unsigned int* q) {
   if (blockIdx.x == 0 && threadIdx.x < kQueues) q[threadIdx.x] = 0u;
 }
@@ -763,7 +766,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
     int sp_i = 0;
     // cur = P. 5a. U doubling with P^(2^k) and squarings up to Q = P^B (stored transposed in QT)
     const double* coilv = p.coil + 2 * sd.vec_off;
-    const double* rhov = p.rho0 + 2 * sd.vec_off;
+    const double* rhov = p.rho_or ? p.rho_or + 2 * (size_t)o * p.rho_or_stride : p.rho0 + 2 * sd.vec_off;
     R* Ure = Ubase;
     R* Uim = Ubase + ub;
     const size_t vb = (size_t)p.nblk * np;

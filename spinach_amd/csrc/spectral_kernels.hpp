@@ -42,6 +42,8 @@ struct SpecParams {
   const double* vals;
   const int32_t* slots;
   const double* rho0;
+  const double* rho_or;  // optional per-orientation initial states (after a dead time), stride rho_or_stride
+  int64_t rho_or_stride;
   const double* coil;
   const double* alphas;
   const double* betas;
@@ -277,7 +279,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
     if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
     {
       const double* coilv = p.coil + 2 * sdesc.vec_off;
-      const double* rhov = p.rho0 + 2 * sdesc.vec_off;
+      const double* rhov = p.rho_or ? p.rho_or + 2 * (size_t)o * p.rho_or_stride : p.rho0 + 2 * sdesc.vec_off;
       for (int i = tid; i < kTriNP; i += kTriThreads) {
         double c0 = 0.0, c1 = 0.0, r0 = 0.0, r1 = 0.0;
         if (i < n) {

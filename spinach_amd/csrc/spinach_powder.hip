@@ -29,6 +29,7 @@
 #include "powder_kernels.hpp"
 #include "mas_kernels.hpp"
 #include "spectral_kernels.hpp"
+#include "step_kernels.hpp"
 
 namespace {
 
@@ -116,6 +117,31 @@ Triplets read_csc(const sp_csc* m, int64_t dim, const char* what) {
   }
   return t;
 }
+
+// spin_system.tols fields the library reads (kernel/utilities/tolerances.m defaults).
+struct Tols {
+  double liouv_zero = 1e-10, zte_tol = 1e-24, prop_chop = 1e-10, small_matrix = 200, krylov_tol = 10000;
+};
+
+Tols read_tols(const sp_tols* t) {
+  Tols r;
+  if (!t) return r;
+  const double v[5] = {t->liouv_zero, t->zte_tol, t->prop_chop, t->small_matrix, t->krylov_tol};
+  for (double x : v)
+    if (!std::isfinite(x) || x < 0) throw SpError(SP_EINVAL, "tolerances must be finite and non-negative");
+  if (t->krylov_tol < 1) throw SpError(SP_EINVAL, "krylov_tol must be at least 1");
+  r.liouv_zero = t->liouv_zero;
+  r.zte_tol = t->zte_tol;
+  r.prop_chop = t->prop_chop;
+  r.small_matrix = t->small_matrix;
+  r.krylov_tol = t->krylov_tol;
+  return r;
+}
+
+// Largest coil-connected dimension of the dense paths, and of the stepping
+// path (four state vectors of the padded dimension in LDS, 64 B per row).
+constexpr int32_t kDenseMaxDim = 1024;
+constexpr int32_t kStepMaxDim = 2304;
 
 // Reduced, device-ready description of one spin system.
 struct SysPlan {
@@ -227,9 +253,10 @@ SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, const do
     if (seen[i]) map[i] = n++;
   sp.n = n;
   sp.np = std::max<int32_t>(16, (n + 15) / 16 * 16);
-  if (sp.np > 1024)
+  if (n > kStepMaxDim)
     throw SpError(SP_ERANGE, "the coil-connected subspace has dimension " + std::to_string(n) +
-                                 "; the dense propagation path supports up to 1024");
+                                 "; the dense paths support up to " + std::to_string(kDenseMaxDim) +
+                                 " and the stepping path up to " + std::to_string(kStepMaxDim));
   // reduced union pattern
   std::vector<std::pair<int64_t, int64_t>> ent;
   for (auto& m : mats)
@@ -365,6 +392,19 @@ struct sp_plan {
   int64_t n_prop = 0, n_spec = 0;
   int64_t n_prop_complex = 0;  // the first n_prop_complex propagator items have a complex generator
   int32_t spectral_systems = 0;
+  // stepping path (SP_MODE_KRYLOV, dimensions >= krylov_tol or above the dense limit)
+  int64_t n_kry = 0, n_kry_complex = 0;  // the first n_kry_complex stepping items have a complex generator
+  int32_t krylov_systems = 0;
+  int32_t step_vec_words = 0;            // LDS doubles of the four state vectors
+  int32_t step_max_sell = 0;             // largest slice layout over the plan's systems
+  bool step_vl[2] = {false, false};      // [REAL] values staged in LDS
+  size_t step_lds[2] = {0, 0};           // [REAL] dynamic LDS bytes
+  int grid_step = 0;
+  DevBuf d_sdesc, d_sell_start, d_sell_col, d_sell_pos, d_nsub, d_step_work;
+  // dead time (acquire.m:65-67): per-orientation initial states
+  double dead_time = 0.0;
+  int64_t rho_or_stride = 0;
+  DevBuf d_rho_or;
   int grid_tri = 0, grid_syn = 0;
   DevBuf d_spec, d_status;
   // orientation sum: chunks of consecutive orientations, chunk ranges per system
@@ -433,11 +473,77 @@ int choose_log2_block(int64_t npts, int np) {
   return best;
 }
 
-sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode) {
+// Sliced-ELL layout of one system's reduced union pattern (rows in slices of
+// 64, entry k of row r of slice s at start[s] + 64k + r % 64; padding entries
+// point at their own row and hold zero), appended to the plan's arrays.
+void sell_layout(const SysPlan& sp, std::vector<int32_t>& start, std::vector<int32_t>& colv,
+                 std::vector<int32_t>& pos, spk::StepDesc& d) {
+  const int n = sp.n;
+  const int ns = (n + spk::kSell - 1) / spk::kSell;
+  std::vector<int32_t> rowlen(n, 0), first(n + 1, 0);
+  for (int32_t e = 0; e < sp.nnz; e++) rowlen[sp.row[e]]++;
+  for (int i = 0; i < n; i++) first[i + 1] = first[i] + rowlen[i];
+  d.nslices = ns;
+  d.slice_off = (int64_t)start.size();
+  d.sell_off = (int64_t)colv.size();
+  std::vector<int32_t> st(ns + 1, 0);
+  for (int s2 = 0; s2 < ns; s2++) {
+    int w = 0;
+    for (int r = s2 * spk::kSell; r < std::min(n, (s2 + 1) * spk::kSell); r++) w = std::max(w, rowlen[r]);
+    st[s2 + 1] = st[s2] + spk::kSell * w;
+  }
+  d.sell_nnz = st[ns];
+  start.insert(start.end(), st.begin(), st.end());
+  const size_t base = colv.size();
+  colv.resize(base + (size_t)d.sell_nnz);
+  for (int s2 = 0; s2 < ns; s2++)
+    for (int k = st[s2]; k < st[s2 + 1]; k++) colv[base + k] = s2 * spk::kSell + (k - st[s2]) % spk::kSell;
+  for (int32_t e = 0; e < sp.nnz; e++) {
+    const int r = sp.row[e], k = e - first[r];
+    const int p2 = st[r / spk::kSell] + k * spk::kSell + r % spk::kSell;
+    colv[base + p2] = sp.col[e];
+    pos.push_back(p2);
+  }
+}
+
+// LDS budget of the stepping kernel per variant: the four state vectors, and the
+// slice-layout values and columns when the plan's largest layout fits beside
+// them; otherwise the values go to a global slot per workgroup.
+constexpr size_t kStepLdsMax = 156 * 1024;
+
+template <bool REAL, bool VL>
+void step_attr(size_t bytes) {
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&spk::k_step<REAL, VL>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+void setup_step(sp_ctx* ctx, sp_plan* pl) {
+  const size_t vec_bytes = (size_t)pl->step_vec_words * sizeof(double);
+  if (vec_bytes > kStepLdsMax) throw SpError(SP_ERANGE, "state vectors exceed the stepping path's LDS budget");
+  size_t work_words = 0;
+  for (int real = 0; real < 2; real++) {
+    const size_t vw = real ? 1 : 2;
+    const size_t with_vals = vec_bytes + (size_t)pl->step_max_sell * (vw * sizeof(double) + sizeof(int32_t));
+    pl->step_vl[real] = with_vals <= kStepLdsMax;
+    pl->step_lds[real] = pl->step_vl[real] ? with_vals : vec_bytes;
+    if (!pl->step_vl[real]) work_words = std::max(work_words, vw * (size_t)pl->step_max_sell);
+  }
+  // both variants of each kind may run in one pass
+  if (pl->step_vl[0]) step_attr<false, true>(pl->step_lds[0]); else step_attr<false, false>(pl->step_lds[0]);
+  if (pl->step_vl[1]) step_attr<true, true>(pl->step_lds[1]); else step_attr<true, false>(pl->step_lds[1]);
+  // persistent grid: as many workgroups as the LDS lets reside on each CU
+  const size_t lds_max = std::max(pl->step_lds[0], pl->step_lds[1]);
+  const int per_cu = std::max<int>(1, std::min<int>(4, (int)((160 * 1024) / (lds_max + 1024))));
+  pl->grid_step = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * per_cu, pl->n_items));
+  pl->d_step_work.alloc(std::max<size_t>(1, work_words) * pl->grid_step * sizeof(double));
+}
+
+sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode, const Tols& tl) {
   const bool force_complex = (mode & SP_MODE_COMPLEX) != 0;
   const bool force_prop = force_complex || (mode & SP_MODE_PROPAGATOR) != 0;
+  const bool force_krylov = (mode & SP_MODE_KRYLOV) != 0;
   const bool graph = (mode & SP_MODE_GRAPH) != 0;
-  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH);
+  mode &= ~(SP_MODE_COMPLEX | SP_MODE_PROPAGATOR | SP_MODE_GRAPH | SP_MODE_KRYLOV);
   if (!(std::isfinite(dt)) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
   if (nsteps < 0) throw SpError(SP_EINVAL, "nsteps must be non-negative");
   if (nsteps > (int64_t)1 << 30) throw SpError(SP_ERANGE, "nsteps is too large");
@@ -453,13 +559,16 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   const int64_t npts = nsteps + 1;
   // path of each system: spectral (real generator, FP64, fits the register
   // layout of the tridiagonalisation) or the propagator power scheme
-  std::vector<char> spectral(sps.size(), 0);
+  // stepping: forced (sys.enable 'krylov'), at krylov_tol (evolution.m:531-533), or above the dense limit
+  std::vector<char> spectral(sps.size(), 0), krylov(sps.size(), 0);
   int np_prop = 16, np_spec = 16;
   for (size_t s = 0; s < sps.size(); s++) {
-    spectral[s] = (!force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriNP) ? 1 : 0;
+    krylov[s] = (force_krylov || (double)sps[s].n >= tl.krylov_tol || sps[s].np > kDenseMaxDim) ? 1 : 0;
+    spectral[s] =
+        (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriNP) ? 1 : 0;
     if (spectral[s])
       np_spec = std::max(np_spec, sps[s].np);
-    else
+    else if (!krylov[s])
       np_prop = std::max(np_prop, sps[s].np);
   }
   pl->log2b = choose_log2_block(npts, np_prop);
@@ -477,10 +586,13 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->ps_q = 3;  // X^2, X^3 + 3 Horner steps = 5 products
   }
   std::vector<spk::SysDesc> descs;
+  std::vector<spk::StepDesc> sdescs;
+  std::vector<int32_t> sell_start, sell_col, sell_pos;
   std::vector<int32_t> row, col, tpos, slots;
   std::vector<double> vals, rho, coil, al, be, ga, w;
   // queue order: longest-processing-time first within each path (stable within a system)
-  std::vector<std::pair<double, spk::Item>> prop_items, spec_items;
+  std::vector<std::pair<double, spk::Item>> prop_items, spec_items, kry_items;
+  int32_t nmax = 1;
   int64_t or_base = 0;
   for (size_t s = 0; s < sps.size(); s++) {
     SysPlan& sp = sps[s];
@@ -500,7 +612,13 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
       d.rb[k] = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 16 * k) - sp.row.begin());
     pl->real_systems += d.real;
     pl->spectral_systems += spectral[s];
+    pl->krylov_systems += krylov[s];
     descs.push_back(d);
+    spk::StepDesc sdd{};
+    sell_layout(sp, sell_start, sell_col, sell_pos, sdd);
+    sdescs.push_back(sdd);
+    pl->step_max_sell = std::max(pl->step_max_sell, sdd.sell_nnz);
+    nmax = std::max(nmax, sp.n);
     row.insert(row.end(), sp.row.begin(), sp.row.end());
     col.insert(col.end(), sp.col.begin(), sp.col.end());
     tpos.insert(tpos.end(), sp.tpos.begin(), sp.tpos.end());
@@ -514,8 +632,9 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     w.insert(w.end(), sp.weights.begin(), sp.weights.end());
     pl->or_off.push_back(or_base);
     pl->n_or.push_back(d.n_or);
-    const double c = (double)sp.np * sp.np * sp.np;  // dense-stage cost per orientation
-    auto& dst = spectral[s] ? spec_items : prop_items;
+    // cost per orientation: dense stages ~ np^3, stepping ~ slice-layout entries
+    const double c = krylov[s] ? (double)sdd.sell_nnz : (double)sp.np * sp.np * sp.np;
+    auto& dst = krylov[s] ? kry_items : (spectral[s] ? spec_items : prop_items);
     for (int64_t k = 0; k < d.n_or; k++) dst.emplace_back(c, spk::Item{(int32_t)s, 0, or_base + k});
     or_base += d.n_or;
     pl->np_max = std::max(pl->np_max, sp.np);
@@ -534,16 +653,28 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
                      return a.first > b.first;
                    });
   std::stable_sort(spec_items.begin(), spec_items.end(), by_cost);
+  // stepping items: complex generators first, then real ones (two launches)
+  std::stable_sort(kry_items.begin(), kry_items.end(),
+                   [&](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
+                     const int ra = descs[a.second.sys].real, rb = descs[b.second.sys].real;
+                     if (ra != rb) return ra < rb;
+                     return a.first > b.first;
+                   });
   pl->n_prop_complex = 0;
   for (auto& c : prop_items)
     if (!descs[c.second.sys].real) pl->n_prop_complex++;
+  pl->n_kry_complex = 0;
+  for (auto& c : kry_items)
+    if (!descs[c.second.sys].real) pl->n_kry_complex++;
   std::vector<spk::Item> ordered;
-  ordered.reserve(prop_items.size() + spec_items.size());
+  ordered.reserve(prop_items.size() + spec_items.size() + kry_items.size());
   for (auto& c : prop_items) ordered.push_back(c.second);
   for (auto& c : spec_items) ordered.push_back(c.second);
+  for (auto& c : kry_items) ordered.push_back(c.second);
   pl->n_prop = (int64_t)prop_items.size();
   pl->n_spec = (int64_t)spec_items.size();
-  pl->n_items = pl->n_prop + pl->n_spec;
+  pl->n_kry = (int64_t)kry_items.size();
+  pl->n_items = pl->n_prop + pl->n_spec + pl->n_kry;
   pl->total_or = or_base;
 
   hipStream_t st = ctx->stream;
@@ -560,6 +691,13 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   upload(pl->d_be, be, st);
   upload(pl->d_ga, ga, st);
   upload(pl->d_w, w, st);
+  upload(pl->d_sdesc, sdescs, st);
+  upload(pl->d_sell_start, sell_start, st);
+  upload(pl->d_sell_col, sell_col, st);
+  upload(pl->d_sell_pos, sell_pos, st);
+  pl->rho_or_stride = nmax;
+  pl->step_vec_words = 8 * ((nmax + spk::kSell - 1) / spk::kSell * spk::kSell);
+  pl->d_nsub.alloc((size_t)std::max<int64_t>(1, or_base) * sizeof(int32_t));
   pl->d_queue.alloc(spk::kQueues * sizeof(unsigned int));
   pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)pl->n_systems * npts * 2 * sizeof(double));
@@ -596,6 +734,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
+  if (pl->n_kry > 0) setup_step(ctx, pl.get());
   setup_sum(pl.get(), st);
   HIPCHK(hipStreamSynchronize(st));
   return pl.release();
@@ -614,6 +753,8 @@ void launch(sp_plan* pl) {
   kp.vals = pl->d_vals.as<double>();
   kp.slots = pl->d_slots.as<int32_t>();
   kp.rho0 = pl->d_rho.as<double>();
+  kp.rho_or = pl->dead_time > 0 ? pl->d_rho_or.as<double>() : nullptr;
+  kp.rho_or_stride = pl->rho_or_stride;
   kp.coil = pl->d_coil.as<double>();
   kp.alphas = pl->d_al.as<double>();
   kp.betas = pl->d_be.as<double>();
@@ -661,6 +802,8 @@ spk::SpecParams spec_params(sp_plan* pl) {
   sp.vals = pl->d_vals.as<double>();
   sp.slots = pl->d_slots.as<int32_t>();
   sp.rho0 = pl->d_rho.as<double>();
+  sp.rho_or = pl->dead_time > 0 ? pl->d_rho_or.as<double>() : nullptr;
+  sp.rho_or_stride = pl->rho_or_stride;
   sp.coil = pl->d_coil.as<double>();
   sp.alphas = pl->d_al.as<double>();
   sp.betas = pl->d_be.as<double>();
@@ -671,6 +814,64 @@ spk::SpecParams spec_params(sp_plan* pl) {
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
   return sp;
+}
+
+template <bool REAL, bool VL>
+void launch_step_v(sp_plan* pl, const spk::StepParams& sp, size_t lds, int grid) {
+  hipLaunchKernelGGL((spk::k_step<REAL, VL>), dim3(grid), dim3(spk::kStepThreads), lds, pl->ctx->stream, sp);
+  HIPCHK(hipGetLastError());
+}
+
+// n stepping items from items, on queue head q: the observable (final_state
+// false, from rho0 or the dead-time states) or the state after nsteps steps of
+// dt written to rho_out (final_state true, from rho0).
+void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n, int q, bool final_state, double dt,
+                       int64_t nsteps, double* rho_out) {
+  spk::StepParams sp{};
+  sp.sys = pl->d_sys.as<spk::SysDesc>();
+  sp.sdesc = pl->d_sdesc.as<spk::StepDesc>();
+  sp.items = items;
+  sp.n_items = n;
+  sp.queue = pl->d_queue.as<unsigned int>() + q;
+  sp.ptpos = pl->d_tpos.as<int32_t>();
+  sp.vals = pl->d_vals.as<double>();
+  sp.slots = pl->d_slots.as<int32_t>();
+  sp.sell_start = pl->d_sell_start.as<int32_t>();
+  sp.sell_col = pl->d_sell_col.as<int32_t>();
+  sp.sell_pos = pl->d_sell_pos.as<int32_t>();
+  sp.rho0 = pl->d_rho.as<double>();
+  sp.rho_or = (!final_state && pl->dead_time > 0) ? pl->d_rho_or.as<double>() : nullptr;
+  sp.rho_or_stride = pl->rho_or_stride;
+  sp.coil = pl->d_coil.as<double>();
+  sp.alphas = pl->d_al.as<double>();
+  sp.betas = pl->d_be.as<double>();
+  sp.gammas = pl->d_ga.as<double>();
+  sp.dt = dt;
+  sp.nsteps = nsteps;
+  sp.final_state = final_state ? 1 : 0;
+  sp.vec_words = pl->step_vec_words;
+  sp.work = pl->d_step_work.as<double>();
+  sp.work_stride = (int64_t)(pl->d_step_work.bytes / sizeof(double) / std::max(1, pl->grid_step));
+  sp.per_or = pl->d_per.as<double>();
+  sp.rho_out = rho_out;
+  sp.nsub_out = final_state ? nullptr : pl->d_nsub.as<int32_t>();
+  const int g = (int)std::min<int64_t>(pl->grid_step, n);
+  const int r = real ? 1 : 0;
+  if (real) {
+    if (pl->step_vl[r]) launch_step_v<true, true>(pl, sp, pl->step_lds[r], g);
+    else launch_step_v<true, false>(pl, sp, pl->step_lds[r], g);
+  } else {
+    if (pl->step_vl[r]) launch_step_v<false, true>(pl, sp, pl->step_lds[r], g);
+    else launch_step_v<false, false>(pl, sp, pl->step_lds[r], g);
+  }
+}
+
+// The stepping items of a plan: complex generators, then real ones.
+void launch_krylov(sp_plan* pl) {
+  const spk::Item* items = pl->d_items.as<spk::Item>() + pl->n_prop + pl->n_spec;
+  const int64_t n_c = pl->n_kry_complex, n_r = pl->n_kry - pl->n_kry_complex;
+  if (n_c > 0) launch_step_items(pl, false, items, n_c, 4, false, pl->dt, pl->nsteps, nullptr);
+  if (n_r > 0) launch_step_items(pl, true, items + n_c, n_r, 5, false, pl->dt, pl->nsteps, nullptr);
 }
 
 // The three spectral kernels; ev[0..2] are recorded after each of them.
@@ -897,12 +1098,17 @@ void enqueue_pass(sp_plan* pl, bool events) {
     if (events)
       for (auto& e : ctx->evs) HIPCHK(hipEventRecord(e, st));
   } else {
+    // dead time: rho0(Omega) = step(L(Omega), rho0, dead_time) for every item (acquire.m:65-67)
+    if (pl->dead_time > 0)
+      launch_step_items(pl, false, pl->d_items.as<spk::Item>(), pl->n_items, 6, true, pl->dead_time, 1,
+                        pl->d_rho_or.as<double>());
     if (pl->n_prop > 0) {
       if (pl->mode == SP_MODE_FP64)
         launch<double>(pl);
       else
         launch<float>(pl);
     }
+    if (pl->n_kry > 0) launch_krylov(pl);
     if (events) HIPCHK(hipEventRecord(ctx->evs[0], st));
     if (pl->n_spec > 0) {
       launch_spectral(pl, events ? ctx->evs + 1 : nullptr);
@@ -969,6 +1175,14 @@ int run(sp_plan* pl, sp_stats* stats) {
   }
   (void)hipEventDestroy(evend);
   HIPCHK(se);
+  int32_t max_nsub = 0;
+  if (pl->n_kry > 0) {
+    std::vector<int32_t> ns((size_t)pl->total_or);
+    HIPCHK(hipMemcpy(ns.data(), pl->d_nsub.p, ns.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int32_t v : ns) max_nsub = std::max(max_nsub, v);
+    if (max_nsub > spk::kStepMaxSub)
+      throw SpError(SP_ERANGE, "either dt is too long, or L is too big: |L*dt|>1e4, check both.");
+  }
   int32_t eig_failures = 0, max_sweeps = 0;
   if (pl->n_spec > 0) {
     std::vector<int32_t> stv((size_t)pl->n_spec);
@@ -978,7 +1192,7 @@ int run(sp_plan* pl, sp_stats* stats) {
       max_sweeps = std::max(max_sweeps, v);
     }
   }
-  pl->executed = true;
+  pl->executed = false;
   sp_stats s{};
   s.elapsed_s = ms_all * 1e-3;
   s.kernel_ms = ms_k;
@@ -1002,6 +1216,7 @@ int run(sp_plan* pl, sp_stats* stats) {
   s.real_systems = pl->real_systems;
   s.spectral_systems = pl->spectral_systems;
   s.eig_max_sweeps = max_sweeps;
+  s.krylov_systems = pl->krylov_systems;
   for (int k = 0; k < 4; k++) s.stage_ms[k] = ms_stage[k];
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   pl->last = s;
@@ -1009,6 +1224,7 @@ int run(sp_plan* pl, sp_stats* stats) {
   if (eig_failures > 0)
     throw SpError(SP_ERANGE, "the tridiagonal eigensolver did not converge for " +
                                  std::to_string(eig_failures) + " orientation(s)");
+  pl->executed = true;  // sp_plan_result refuses the results of a failed pass
   return SP_OK;
 }
 
@@ -1079,10 +1295,10 @@ sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
                         const double* coil, double dt, int64_t nsteps, const sp_tols* tols, int mode) {
   if (!ctx) return nullptr;
   try {
-    (void)tols;
+    const Tols tl = read_tols(tols);
     std::vector<SysPlan> sps;
     sps.push_back(plan_system(I, Q, alphas, betas, gammas, weights, n_or, rho0, coil));
-    return build_plan(ctx, sps, dt, nsteps, mode);
+    return build_plan(ctx, sps, dt, nsteps, mode, tl);
   } catch (SpError& e) {
     fail(ctx, e.code, e.what());
   } catch (std::bad_alloc&) {
@@ -1097,14 +1313,14 @@ sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_s
                               const sp_tols* tols, int mode) {
   if (!ctx) return nullptr;
   try {
-    (void)tols;
+    const Tols tl = read_tols(tols);
     if (!systems || n_systems <= 0) throw SpError(SP_EINVAL, "the batch must hold at least one system");
     std::vector<SysPlan> sps;
     for (int64_t s = 0; s < n_systems; s++) {
       const sp_system& y = systems[s];
       sps.push_back(plan_system(y.I, y.Q, y.alphas, y.betas, y.gammas, y.weights, y.n_or, y.rho0, y.coil));
     }
-    sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode);
+    sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode, tl);
     for (int64_t s = 0; s < n_systems; s++) pl->fid_targets.push_back(systems[s].fid_out);
     return pl;
   } catch (SpError& e) {
@@ -1159,6 +1375,31 @@ int sp_plan_result_batch(sp_plan* plan) {
     for (int64_t s = 0; s < plan->n_systems; s++)
       if (plan->fid_targets[s])
         std::memcpy(plan->fid_targets[s], all.data() + 2 * (size_t)s * npts, (size_t)npts * 2 * sizeof(double));
+    return SP_OK;
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_set_dead_time(sp_plan* plan, double dead_time) {
+  if (!plan) return SP_EINVAL;
+  try {
+    if (plan->mas) throw SpError(SP_EINVAL, "a dead time is not available on MAS plans");
+    if (!std::isfinite(dead_time) || dead_time < 0) throw SpError(SP_EINVAL, "dead_time must be finite and >= 0");
+    HIPCHK(hipSetDevice(plan->ctx->device));
+    if (dead_time > 0) {
+      if (plan->step_lds[0] == 0) setup_step(plan->ctx, plan);
+      if (!plan->d_rho_or.p)
+        plan->d_rho_or.alloc((size_t)plan->total_or * plan->rho_or_stride * 2 * sizeof(double));
+    }
+    if (dead_time != plan->dead_time && plan->gexec) {  // the recorded pass changes
+      (void)hipGraphExecDestroy(plan->gexec);
+      plan->gexec = nullptr;
+    }
+    plan->dead_time = dead_time;
+    plan->executed = false;
     return SP_OK;
   } catch (SpError& e) {
     return fail(plan->ctx, e.code, e.what());

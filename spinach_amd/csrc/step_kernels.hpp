@@ -1,0 +1,327 @@
+// Time stepping by the reordered Taylor series: the step() / krylov() path.
+//
+// kernel/krylov.m:139-163 ('observable') propagates each subspace one time step
+// at a time with step(L,rho,dt) (kernel/step.m:204-267, reordered_taylor
+// :328-375) and detects coil'*rho after every step; evolution.m:574 takes it
+// for subspaces of dimension >= krylov_tol or when sys.enable holds 'krylov'.
+// experiments/acquire.m:65-67 runs the same step() once over the dead time.
+//
+// One 256-thread workgroup per (system, orientation) item, pulled from a
+// persistent queue. Per item:
+//   1. the rank-1/rank-2 Wigner coefficients of the orientation,
+//   2. L = (x + x')/2, x = I + sum_q c_q Q_q on the reduced union pattern,
+//      written in sliced-ELL layout (slices of 64 rows, entry k of the slice's
+//      row r at slice_offset + 64k + r, so that the 64 lanes of a wave read
+//      one entry of 64 consecutive rows with one coalesced load); real values
+//      when the generator is real at every orientation, complex otherwise;
+//      staged in LDS when the plan's largest slice layout fits, else in the
+//      workgroup's slot of a global workspace,
+//   3. the 1-norm of L in FP64 (cheap_norm.m:50; L is Hermitian, so the
+//      column sums equal the row sums) and the substep count
+//      nsub = ceil(|L|_1 |dt| / 2) (step.m:222-227),
+//   4. per time step: scaling = max|rho| (a workgroup max reduction),
+//      rho /= scaling; nsub substeps of next = -(i/k)(dt/nsub) L next,
+//      rho += next, k = 1, 2, ... while any |next| > eps (one
+//      __syncthreads_or per term: the series' own convergence test);
+//      rho *= scaling; then f(t) = coil'*rho (a workgroup sum).
+// Row ownership follows the slices: row i belongs to lane i % 64 of wave
+// (i / 64) % 4, so every thread updates only the rows it owns and the state
+// vector it reads in the product is double-buffered in LDS (one barrier per
+// Taylor term).
+#pragma once
+
+#include "powder_kernels.hpp"
+
+namespace spk {
+
+constexpr int kStepThreads = 256;
+constexpr int kSell = 64;           // rows per slice (one wave)
+constexpr int kStepMaxSub = 10000;  // substeps per time step (step.m:230-233)
+constexpr int kStepMaxTerms = 256;  // Taylor terms per substep: |L dt/nsub|_1 <= 2 converges in ~30
+
+// Per-system layout of the sliced-ELL generator.
+struct StepDesc {
+  int32_t nslices;    // ceil(n / 64)
+  int32_t sell_nnz;   // entries of the slice layout, padding included (64 * sum of slice widths)
+  int64_t slice_off;  // offset into the slice-start array (nslices + 1 entries)
+  int64_t sell_off;   // offset into the slice-layout column array (sell_nnz entries)
+};
+
+struct StepParams {
+  const SysDesc* sys;
+  const StepDesc* sdesc;
+  const Item* items;
+  int64_t n_items;
+  unsigned int* queue;    // work-queue head, zeroed by k_reset
+  const int32_t* ptpos;   // transposed pattern entry (union pattern, row-sorted)
+  const double* vals;     // interleaved complex FP64 value planes: I, then each non-empty Q slot
+  const int32_t* slots;   // Q slot ids
+  const int32_t* sell_start;  // per system: start of each slice within the system's slice layout
+  const int32_t* sell_col;    // column of every slice-layout entry (padding: the entry's own row)
+  const int32_t* sell_pos;    // slice-layout position of every union-pattern entry
+  const double* rho0;     // interleaved complex, per system
+  const double* rho_or;   // optional per-orientation initial states (dead time), stride rho_or_stride
+  int64_t rho_or_stride;  // complex entries per orientation in rho_or / rho_out
+  const double* coil;
+  const double* alphas;
+  const double* betas;
+  const double* gammas;
+  double dt;              // time step
+  int64_t nsteps;         // steps; the observable has nsteps+1 points
+  int32_t final_state;    // 0: write f(t) to per_or; 1: write rho after nsteps steps to rho_out
+  int32_t vec_words;      // LDS doubles reserved for the four state vectors (8 * largest n)
+  double* work;           // per-workgroup value slots (values not staged in LDS)
+  int64_t work_stride;    // doubles per workgroup slot
+  double* per_or;         // per-orientation FIDs, interleaved complex, nsteps+1 per orientation
+  double* rho_out;        // final states (final_state == 1)
+  int32_t* nsub_out;      // per orientation: substeps per time step (diagnostics)
+};
+
+// Sum and max over the workgroup: DPP within each wave, then four partials in LDS.
+__device__ __forceinline__ double step_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double step_wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// VL: the slice-layout values and columns are staged in LDS (else read from the
+// workgroup's global slot and the system's column array).
+template <bool REAL, bool VL>
+__global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
+  extern __shared__ double lds[];
+  __shared__ double cre[34], cim[34];
+  __shared__ double red[2][kStepThreads / 64];
+  __shared__ int sitem;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  constexpr int kVW = REAL ? 1 : 2;  // doubles per generator value
+  double* va = lds;                  // next term, read buffer (interleaved complex)
+  double* vb = lds + p.vec_words / 4;  // next term, write buffer
+  double* vr = lds + p.vec_words / 2;  // rho (each thread touches only its own rows)
+  double* vc = lds + 3 * (p.vec_words / 4);  // conj-free coil copy
+  double* lval = lds + p.vec_words;  // staged values (VL)
+  double* wslot = p.work + (size_t)blockIdx.x * p.work_stride;
+  const double eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
+
+  for (;;) {
+    if (tid == 0) sitem = (int)atomicAdd(p.queue, 1u);
+    __syncthreads();
+    const int64_t it = __builtin_amdgcn_readfirstlane(sitem);
+    __syncthreads();
+    if (it >= p.n_items) break;
+    const Item item = p.items[it];
+    const SysDesc sd = p.sys[item.sys];
+    const StepDesc st = p.sdesc[item.sys];
+    const int n = __builtin_amdgcn_readfirstlane(sd.n);
+    const int nslices = __builtin_amdgcn_readfirstlane(st.nslices);
+    const int sell_nnz = __builtin_amdgcn_readfirstlane(st.sell_nnz);
+    const int32_t* sstart = p.sell_start + st.slice_off;
+    const int32_t* gcol = p.sell_col + st.sell_off;
+    const int32_t* spos = p.sell_pos + sd.pat_off;
+    const int64_t o = item.orient;
+    double* val = VL ? lval : wslot;
+    int32_t* lcol = reinterpret_cast<int32_t*>(lval + (size_t)kVW * sell_nnz);
+    const int32_t* col = VL ? lcol : gcol;
+
+    // 1. Wigner coefficients; clear the value layout (padding stays zero); stage columns
+    if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
+    for (int k = tid; k < kVW * sell_nnz; k += kStepThreads) val[k] = 0.0;
+    if (VL)
+      for (int k = tid; k < sell_nnz; k += kStepThreads) lcol[k] = gcol[k];
+    __syncthreads();
+    // 2. L = (x + x')/2 on the union pattern, scattered into the slice layout
+    {
+      const double* v0 = p.vals + 2 * sd.val_off;
+      const int32_t* slots = p.slots + sd.slot_off;
+      const int32_t* ptp = p.ptpos + sd.pat_off;
+      const int nnz = sd.nnz;
+      for (int e = tid; e < nnz; e += kStepThreads) {
+        const int et = ptp[e];
+        double xr = v0[2 * e], xi = v0[2 * e + 1];
+        double tr = v0[2 * et], ti = v0[2 * et + 1];
+        for (int q = 0; q < sd.nq; q++) {
+          const double* qv = v0 + 2 * (size_t)(q + 1) * nnz;
+          const double cr = cre[slots[q]], ci = cim[slots[q]];
+          xr += cr * qv[2 * e] - ci * qv[2 * e + 1];
+          xi += cr * qv[2 * e + 1] + ci * qv[2 * e];
+          tr += cr * qv[2 * et] - ci * qv[2 * et + 1];
+          ti += cr * qv[2 * et + 1] + ci * qv[2 * et];
+        }
+        const int pos = spos[e];
+        if (REAL) {
+          val[pos] = 0.5 * (xr + tr);
+        } else {
+          val[2 * pos] = 0.5 * (xr + tr);
+          val[2 * pos + 1] = 0.5 * (xi - ti);
+        }
+      }
+    }
+    // states: rho0 (per system, or per orientation after a dead-time step) and the coil
+    {
+      const double* r0 = p.rho_or ? p.rho_or + 2 * (size_t)o * p.rho_or_stride : p.rho0 + 2 * sd.vec_off;
+      const double* c0 = p.coil + 2 * sd.vec_off;
+      for (int i = tid; i < n; i += kStepThreads) {
+        vr[2 * i] = r0[2 * i];
+        vr[2 * i + 1] = r0[2 * i + 1];
+        vc[2 * i] = c0[2 * i];
+        vc[2 * i + 1] = c0[2 * i + 1];
+      }
+    }
+    __syncthreads();
+    // 3. |L|_1 = max row sum of |L_ij| (Hermitian L), substep count
+    int nsub;
+    {
+      double mx = 0.0;
+      for (int s = wave; s < nslices; s += 4) {
+        const int b = sstart[s], w = (sstart[s + 1] - b) >> 6;
+        double acc = 0.0;
+        for (int k = 0; k < w; k++) {
+          const int e = b + (k << 6) + lane;
+          acc += REAL ? fabs(val[e]) : hypot(val[2 * e], val[2 * e + 1]);
+        }
+        mx = fmax(mx, acc);
+      }
+      mx = step_wave_max(mx);
+      if (lane == 0) red[0][wave] = mx;
+      __syncthreads();
+      const double nrm = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3])) * fabs(p.dt);
+      // step.m:230-233 stops above 1e4 substeps; the host turns the recorded count into that error
+      nsub = (nrm <= 2.0 * kStepMaxSub) ? (int)ceil(nrm / 2.0) : kStepMaxSub + 1;
+      if (tid == 0 && p.nsub_out) p.nsub_out[o] = nsub;
+      __syncthreads();
+    }
+    if (nsub > kStepMaxSub) continue;
+    const int64_t npts = p.nsteps + 1;
+    double* out = p.per_or + 2 * (size_t)o * npts;
+    // detection f = coil'*rho = sum conj(c_i) rho_i over the owned rows, then the workgroup
+    auto detect = [&](int64_t t) {
+      double sr = 0.0, si = 0.0;
+      for (int s = wave; s < nslices; s += 4) {
+        const int i = (s << 6) + lane;
+        if (i < n) {
+          const double cr = vc[2 * i], ci = vc[2 * i + 1], rr = vr[2 * i], ri = vr[2 * i + 1];
+          sr += cr * rr + ci * ri;
+          si += cr * ri - ci * rr;
+        }
+      }
+      sr = step_wave_sum(sr);
+      si = step_wave_sum(si);
+      if (lane == 0) {
+        red[0][wave] = sr;
+        red[1][wave] = si;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        out[2 * t] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        out[2 * t + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+      }
+      __syncthreads();
+    };
+    if (!p.final_state) detect(0);
+    // 4. time steps
+    for (int64_t t = 1; t <= p.nsteps; t++) {
+      // scaling = max|rho| over the state (step.m:212-222)
+      double m = 0.0;
+      for (int s = wave; s < nslices; s += 4) {
+        const int i = (s << 6) + lane;
+        if (i < n) m = fmax(m, hypot(vr[2 * i], vr[2 * i + 1]));
+      }
+      m = step_wave_max(m);
+      if (lane == 0) red[0][wave] = m;
+      __syncthreads();
+      const double scaling = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+      __syncthreads();
+      if (scaling > 0.0 && nsub > 0) {
+        for (int s = wave; s < nslices; s += 4) {
+          const int i = (s << 6) + lane;
+          if (i < n) {
+            vr[2 * i] /= scaling;
+            vr[2 * i + 1] /= scaling;
+          }
+        }
+        const double h = p.dt / (double)nsub;
+        for (int sub = 0; sub < nsub; sub++) {
+          // next = rho on the owned rows; padding rows of the slices stay zero in both buffers
+          for (int s = wave; s < nslices; s += 4) {
+            const int i = (s << 6) + lane;
+            double a = 0.0, b = 0.0;
+            if (i < n) {
+              a = vr[2 * i];
+              b = vr[2 * i + 1];
+            }
+            va[2 * i] = a;
+            va[2 * i + 1] = b;
+            vb[2 * i] = 0.0;
+            vb[2 * i + 1] = 0.0;
+          }
+          __syncthreads();
+          double* x = va;
+          double* y = vb;
+          for (int k = 1;; k++) {
+            // next = -(1i/k)*(dt/nsub)*(L*next): with c = -(dt/nsub)/k, next = 1i*c*(L*next)
+            const double c = -h / (double)k;
+            int any = 0;
+            for (int s = wave; s < nslices; s += 4) {
+              const int b = sstart[s], w = (sstart[s + 1] - b) >> 6;
+              double tr = 0.0, ti = 0.0;
+              for (int kk = 0; kk < w; kk++) {
+                const int e = b + (kk << 6) + lane;
+                const int j = col[e];
+                const double xr = x[2 * j], xi = x[2 * j + 1];
+                if (REAL) {
+                  const double l = val[e];
+                  tr = fma(l, xr, tr);
+                  ti = fma(l, xi, ti);
+                } else {
+                  const double lr = val[2 * e], li = val[2 * e + 1];
+                  tr = fma(lr, xr, tr);
+                  tr = fma(-li, xi, tr);
+                  ti = fma(lr, xi, ti);
+                  ti = fma(li, xr, ti);
+                }
+              }
+              const int i = (s << 6) + lane;
+              const double nr = -c * ti, ni = c * tr;
+              y[2 * i] = nr;
+              y[2 * i + 1] = ni;
+              if (i < n) {
+                vr[2 * i] += nr;
+                vr[2 * i + 1] += ni;
+                any |= (nr * nr + ni * ni > eps2) ? 1 : 0;
+              }
+            }
+            any = __syncthreads_or(any);
+            double* tswap = x;
+            x = y;
+            y = tswap;
+            if (!any || k >= kStepMaxTerms) break;
+          }
+        }
+        for (int s = wave; s < nslices; s += 4) {
+          const int i = (s << 6) + lane;
+          if (i < n) {
+            vr[2 * i] *= scaling;
+            vr[2 * i + 1] *= scaling;
+          }
+        }
+        __syncthreads();
+      }
+      if (!p.final_state) detect(t);
+    }
+    if (p.final_state) {
+      double* ro = p.rho_out + 2 * (size_t)o * p.rho_or_stride;
+      for (int i = tid; i < n; i += kStepThreads) {
+        ro[2 * i] = vr[2 * i];
+        ro[2 * i + 1] = vr[2 * i + 1];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace spk

@@ -178,3 +178,23 @@ def test_default_merge_dim_skips_path_tracing():
     g = w['grid']
     subs = O.refcpu_reduce(w['I'], w['Q'], (g['alphas'][0], g['betas'][0], g['gammas'][0]), w['coil'])
     assert [len(s) for s in subs] == [18]
+
+
+# ---------------------------------------------------------------- dead time
+
+def test_dead_time_is_a_time_shift():
+    """acquire.m:65-67 propagates rho0 by step(L, rho0, dead_time) before the
+    acquisition; for the two-spin system that is the closed-form FID with its
+    time origin at the dead time. Dead times with one and with several
+    substeps; the planted +i step fails."""
+    w = workloads.static_2spin(npoints=128, n_orient=8)
+    args = (w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'])
+    for td in (3.7e-6, 61.3e-6):
+        f = O.refcpu_powder(*args, dead_time=td)
+        cf = O.closed_form_2spin(w['ss'], w['grid'], w['dt'], w['nsteps'], t0=td)
+        e = O.rel_err(f, cf)
+        print(f'dead time {td:.2e} s: oracle vs shifted closed form {e:.2e}')
+        assert e < 5e-9
+        assert O.rel_err(O.refcpu_powder(*args, dead_time=td, plant=O.PLANT_STEP_SIGN), cf) > 1e-3
+    assert O.rel_err(O.refcpu_powder(*args), O.closed_form_2spin(w['ss'], w['grid'], w['dt'], w['nsteps'],
+                                                                t0=3.7e-6)) > 1e-3
