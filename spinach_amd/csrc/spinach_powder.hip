@@ -698,6 +698,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   pl->rho_or_stride = nmax;
   pl->step_vec_words = 8 * ((nmax + spk::kSell - 1) / spk::kSell * spk::kSell);
   pl->d_nsub.alloc((size_t)std::max<int64_t>(1, or_base) * sizeof(int32_t));
+  HIPCHK(hipMemsetAsync(pl->d_nsub.p, 0, pl->d_nsub.bytes, st));  // only stepped orientations write it
   pl->d_queue.alloc(spk::kQueues * sizeof(unsigned int));
   pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)pl->n_systems * npts * 2 * sizeof(double));
