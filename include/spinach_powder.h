@@ -172,6 +172,25 @@ int sp_powder_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* a
                   const sp_tols* tols, int mode, double* fid_out, double* per_orient_out,
                   sp_stats* stats);
 
+/* Variant of sp_powder_fid that calls no allocator (no hipMalloc, hipFree or
+ * allocating library routine): every device buffer of the call - the staged
+ * inputs, per-orientation spectra and FIDs, orientation-sum partials and
+ * work-queue heads - is carved from a static device array of 64 MiB that each
+ * device holds from its first sp_create until the process ends. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid it). Same arguments and results as
+ * sp_powder_fid (results identical bit for bit), for the FP64 spectral path
+ * only: mode must be SP_MODE_FP64 and the system real with a coil-connected
+ * dimension of at most 192 (SP_EINVAL otherwise). Orientations run in blocks
+ * sized to the array; a block of 64 needs about 1 KiB per time point, so
+ * nsteps beyond ~60000 returns SP_ERANGE. Calls on one device share the array:
+ * calls that overlap on one device (through different contexts) race. */
+int sp_powder_fid_noalloc(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                          const double* betas, const double* gammas, const double* weights, int64_t n_or,
+                          const double* rho0, const double* coil, double dt, int64_t nsteps,
+                          const sp_tols* tols, int mode, double* fid_out, double* per_orient_out,
+                          sp_stats* stats);
+
 /* ---- plan / execute split (inputs resident on the device) -------------- */
 
 /* Replaces: the host-side part of powder.m:155-225 plus the per-orientation

@@ -25,7 +25,8 @@ SP_NQ = 34
 
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
-           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real', 'sp_plan_set_dead_time')
+           'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real', 'sp_plan_set_dead_time',
+           'sp_powder_fid_noalloc')
 SP_MAS_FRAME_MAGNET = 0
 SP_MAS_FRAME_ROTOR = 1
 
@@ -109,6 +110,8 @@ def load():
     lib.sp_mas_fid.restype = C.c_int
     lib.sp_mas_fid.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
                                _dp, _dp, C.POINTER(SpRotor), C.c_int64, C.c_int, _dp, _dp, C.POINTER(SpStats)]
+    lib.sp_powder_fid_noalloc.restype = C.c_int
+    lib.sp_powder_fid_noalloc.argtypes = lib.sp_powder_fid.argtypes
     lib.sp_plan_set_dead_time.restype = C.c_int
     lib.sp_plan_set_dead_time.argtypes = [C.c_void_p, C.c_double]
     lib.sp_generator_is_real.restype = C.c_int
@@ -367,6 +370,32 @@ def powder_fid(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', p
                            per.view(np.float64).ctypes.data_as(_dp) if per is not None else None, C.byref(st))
     if rc:
         _err(ctx, rc, 'sp_powder_fid')
+    return fid, per, st.as_dict()
+
+
+def powder_fid_noalloc(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', per_orientation=False,
+                       device=0):
+    """sp_powder_fid_noalloc: sp_powder_fid on the spectral path with all device
+    memory taken from the library's static scratch (no allocation in the call)."""
+    lib = load()
+    ctx = context(device)
+    k = _Keep()
+    dim = I.shape[0]
+    Ic = k.csc(I)
+    Qa = k.qarray(qlist)
+    n_or = len(grid['weights'])
+    npts = int(nsteps) + 1
+    fid = np.zeros(npts, np.complex128)
+    per = np.zeros((n_or, npts), np.complex128) if per_orientation else None
+    st = SpStats()
+    t = _tols(tols)
+    rc = lib.sp_powder_fid_noalloc(ctx, C.byref(Ic), Qa, *k.grid(grid), k.cvec(rho0, dim), k.cvec(coil, dim),
+                                   float(dt), int(nsteps), C.byref(t), mode_id(mode),
+                                   fid.view(np.float64).ctypes.data_as(_dp),
+                                   per.view(np.float64).ctypes.data_as(_dp) if per is not None else None,
+                                   C.byref(st))
+    if rc:
+        _err(ctx, rc, 'sp_powder_fid_noalloc')
     return fid, per, st.as_dict()
 
 

@@ -952,4 +952,26 @@ __global__ void k_system_sum(const double* __restrict__ partial, const int64_t* 
   fid[2 * ((size_t)s * npts + t) + 1] = si;
 }
 
+// Zero fill (a kernel, not a memset node, so that a captured pass replays it).
+__global__ void k_fill_zero(double* __restrict__ x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// fid += partial[0] + partial[1] + ... in chunk order, one addition at a time:
+// block after block this repeats k_system_sum's sequence of additions.
+__global__ void k_accumulate(const double* __restrict__ partial, int32_t n_chunks, int64_t npts,
+                             double* __restrict__ fid) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  double sr = fid[2 * t], si = fid[2 * t + 1];
+  for (int32_t c = 0; c < n_chunks; c++) {
+    const double* f = partial + 2 * ((size_t)c * npts + t);
+    sr += f[0];
+    si += f[1];
+  }
+  fid[2 * t] = sr;
+  fid[2 * t + 1] = si;
+}
+
 }  // namespace spk

@@ -31,6 +31,14 @@
 #include "spectral_kernels.hpp"
 #include "step_kernels.hpp"
 
+// Device memory of sp_powder_fid_noalloc, which calls no allocator: the staged
+// inputs, per-item spectra, per-orientation FIDs, orientation-sum partials and
+// work-queue heads of one call. Every device the library runs on holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the device (sp_create records it).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_err;
@@ -362,6 +370,8 @@ bool generator_is_real_full(const sp_csc* I, const sp_csc* Q) {
 
 struct sp_ctx {
   int device = 0;
+  void* scratch = nullptr;  // this device's copy of the static scratch (sp_powder_fid_noalloc)
+  std::string scratch_err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};  // stage boundaries inside the propagation
@@ -1229,6 +1239,208 @@ int run(sp_plan* pl, sp_stats* stats) {
   return SP_OK;
 }
 
+// Device address of the static scratch, looked up once per device (the device
+// must be current).
+std::mutex g_scratch_mu;
+void* g_scratch_dev[256] = {};
+
+void* scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= 256) throw SpError(SP_EINVAL, "device index out of range");
+  if (!g_scratch_dev[device]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = ptr;
+  }
+  return g_scratch_dev[device];
+}
+
+// Bump allocation inside the static scratch (256-byte aligned pieces).
+struct Arena {
+  unsigned char* base;
+  size_t cap;
+  size_t used = 0;
+  template <typename T>
+  T* take(size_t count) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (off > cap || bytes > cap - off)
+      throw SpError(SP_ERANGE, "the problem does not fit the " + std::to_string(kScratchBytes >> 20) +
+                                   " MiB static scratch of sp_powder_fid_noalloc");
+    used = off + bytes;
+    return reinterpret_cast<T*>(base + off);
+  }
+};
+
+template <typename T>
+void put(T* dst, const std::vector<T>& v, hipStream_t s) {
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// sp_powder_fid on the spectral path with every device buffer carved from the
+// static scratch, orientations in blocks of a multiple of 64 (so the chunk
+// partials add up in the same order as k_system_sum's).
+int noalloc_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al, const double* be, const double* ga,
+                const double* w, int64_t n_or, const double* rho0, const double* coil, double dt, int64_t nsteps,
+                const sp_tols* tols, int mode, double* fid_out, double* per_out, sp_stats* stats) {
+  if (!fid_out) throw SpError(SP_EINVAL, "fid_out must not be NULL");
+  if (!ctx->scratch) throw SpError(SP_EDEVICE, "the static scratch is not available: " + ctx->scratch_err);
+  const Tols tl = read_tols(tols);
+  if (mode != SP_MODE_FP64)
+    throw SpError(SP_EINVAL, "sp_powder_fid_noalloc runs the FP64 spectral path only (mode must be SP_MODE_FP64)");
+  if (!(std::isfinite(dt)) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
+  if (nsteps < 0 || nsteps > (int64_t)1 << 30) throw SpError(SP_EINVAL, "invalid nsteps");
+  SysPlan sp = plan_system(I, Q, al, be, ga, w, n_or, rho0, coil);
+  if (!sp.real || sp.np > spk::kTriNP || (double)sp.n >= tl.krylov_tol)
+    throw SpError(SP_EINVAL,
+                  "sp_powder_fid_noalloc runs the spectral path only: the generator must be real at every "
+                  "orientation and its coil-connected dimension at most " + std::to_string(spk::kTriNP) +
+                      " and below krylov_tol");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t npts = nsteps + 1;
+  Arena a{static_cast<unsigned char*>(ctx->scratch), kScratchBytes};
+  spk::SysDesc d{};
+  d.n = sp.n;
+  d.np = sp.np;
+  d.nnz = sp.nnz;
+  d.nq = (int32_t)sp.slots.size();
+  d.n_or = n_or;
+  d.real = 1;
+  for (int k = 0; k < 13; k++)
+    d.rb[k] = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 16 * k) - sp.row.begin());
+  std::vector<spk::SysDesc> dv(1, d);
+  auto* d_sys = a.take<spk::SysDesc>(1);
+  auto* d_row = a.take<int32_t>(sp.row.size());
+  auto* d_col = a.take<int32_t>(sp.col.size());
+  auto* d_tpos = a.take<int32_t>(sp.tpos.size());
+  auto* d_vals = a.take<double>(sp.vals.size());
+  auto* d_slots = a.take<int32_t>(sp.slots.size());
+  auto* d_rho = a.take<double>(sp.rho.size());
+  auto* d_coil = a.take<double>(sp.coil.size());
+  auto* d_al = a.take<double>((size_t)n_or);
+  auto* d_be = a.take<double>((size_t)n_or);
+  auto* d_ga = a.take<double>((size_t)n_or);
+  auto* d_w = a.take<double>((size_t)n_or);
+  auto* d_fid = a.take<double>(2 * (size_t)npts);
+  auto* d_queue = a.take<unsigned int>(spk::kQueues);
+  // block of C orientations: items, chunk table, spectra, status, FIDs, partials
+  const size_t per64 = 64 * (spk::kSpecStride * sizeof(double) + sizeof(int32_t) + sizeof(spk::Item) +
+                             (size_t)npts * 2 * sizeof(double)) +
+                       (size_t)npts * 2 * sizeof(double) + sizeof(spk::SumChunk);
+  const size_t left = a.used + 8 * 256 < kScratchBytes ? kScratchBytes - a.used - 8 * 256 : 0;
+  int64_t C = 64 * (int64_t)(left / per64);
+  if (C < 64)
+    throw SpError(SP_ERANGE, "nsteps is too large for the " + std::to_string(kScratchBytes >> 20) +
+                                 " MiB static scratch of sp_powder_fid_noalloc (" + std::to_string(per64) +
+                                 " bytes per 64 orientations)");
+  C = std::min<int64_t>(C, (n_or + 63) / 64 * 64);
+  auto* d_items = a.take<spk::Item>((size_t)C);
+  auto* d_chunks = a.take<spk::SumChunk>((size_t)C / 64);
+  auto* d_spec = a.take<double>((size_t)C * spk::kSpecStride);
+  auto* d_status = a.take<int32_t>((size_t)C);
+  auto* d_per = a.take<double>((size_t)C * npts * 2);
+  auto* d_partial = a.take<double>((size_t)C / 64 * npts * 2);
+  std::vector<double> alv(al, al + n_or), bev(be, be + n_or), gav(ga, ga + n_or), wv(w, w + n_or);
+  put(d_sys, dv, st);
+  put(d_row, sp.row, st);
+  put(d_col, sp.col, st);
+  put(d_tpos, sp.tpos, st);
+  put(d_vals, sp.vals, st);
+  put(d_slots, sp.slots, st);
+  put(d_rho, sp.rho, st);
+  put(d_coil, sp.coil, st);
+  put(d_al, alv, st);
+  put(d_be, bev, st);
+  put(d_ga, gav, st);
+  put(d_w, wv, st);
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  const unsigned nbt = (unsigned)((npts + 255) / 256);
+  hipLaunchKernelGGL(spk::k_fill_zero, dim3((unsigned)((2 * npts + 255) / 256)), dim3(256), 0, st, d_fid, 2 * npts);
+  HIPCHK(hipGetLastError());
+  std::vector<spk::Item> items;
+  std::vector<spk::SumChunk> chunks;
+  std::vector<int32_t> status;
+  int32_t max_sweeps = 0;
+  for (int64_t b0 = 0; b0 < n_or; b0 += C) {
+    const int64_t nb = std::min<int64_t>(C, n_or - b0);
+    items.clear();
+    chunks.clear();
+    for (int64_t k = 0; k < nb; k++) items.push_back(spk::Item{0, 0, k});
+    for (int64_t k = 0; k < nb; k += 64) chunks.push_back(spk::SumChunk{k, (int32_t)std::min<int64_t>(64, nb - k), 0});
+    put(d_items, items, st);
+    put(d_chunks, chunks, st);
+    hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, d_queue);
+    HIPCHK(hipGetLastError());
+    spk::SpecParams sp2{};
+    sp2.sys = d_sys;
+    sp2.items = d_items;
+    sp2.n_items = nb;
+    sp2.queue = d_queue + 1;
+    sp2.prow = d_row;
+    sp2.pcol = d_col;
+    sp2.ptpos = d_tpos;
+    sp2.vals = d_vals;
+    sp2.slots = d_slots;
+    sp2.rho0 = d_rho;
+    sp2.coil = d_coil;
+    sp2.alphas = d_al + b0;
+    sp2.betas = d_be + b0;
+    sp2.gammas = d_ga + b0;
+    sp2.dt = dt;
+    sp2.nsteps = nsteps;
+    sp2.spec = d_spec;
+    sp2.status = d_status;
+    sp2.per_or = d_per;
+    const int g_tri = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)ctx->n_cu, nb));
+    const int g_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, nb));
+    hipLaunchKernelGGL(spk::k_tridiag, dim3(g_tri), dim3(spk::kTriThreads), 0, st, sp2);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)nb), dim3(spk::kDcThreads), 0, st, sp2);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(spk::k_synth, dim3(g_syn), dim3(spk::kThreads), 0, st, sp2);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(spk::k_chunk_sum, dim3(nbt, (unsigned)chunks.size()), dim3(256), 0, st, d_per, d_w + b0,
+                       d_chunks, npts, d_partial);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(spk::k_accumulate, dim3(nbt), dim3(256), 0, st, d_partial, (int32_t)chunks.size(), npts,
+                       d_fid);
+    HIPCHK(hipGetLastError());
+    status.resize((size_t)nb);
+    HIPCHK(hipMemcpyAsync(status.data(), d_status, (size_t)nb * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (per_out)
+      HIPCHK(hipMemcpyAsync(per_out + 2 * (size_t)b0 * npts, d_per, (size_t)nb * npts * 2 * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));  // the next block reuses the block buffers
+    for (int32_t v : status) {
+      if (v < 0)
+        throw SpError(SP_ERANGE, "the tridiagonal eigensolver did not converge for an orientation");
+      max_sweeps = std::max(max_sweeps, v);
+    }
+  }
+  HIPCHK(hipEventRecord(ctx->ev2, st));
+  HIPCHK(hipMemcpyAsync(fid_out, d_fid, (size_t)npts * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (stats) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2);
+    sp_stats s{};
+    s.elapsed_s = ms * 1e-3;
+    s.kernel_ms = ms;
+    s.orient_steps_per_s = s.elapsed_s > 0 ? (double)n_or * (double)nsteps / s.elapsed_s : 0.0;
+    s.full_dim = sp.full_dim;
+    s.reduced_dim = sp.n;
+    s.reduced_nnz = sp.nnz;
+    s.padded_dim = sp.np;
+    s.real_systems = 1;
+    s.spectral_systems = 1;
+    s.eig_max_sweeps = max_sweeps;
+    *stats = s;
+  }
+  return SP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1242,6 +1454,11 @@ sp_ctx* sp_create(int device) {
     HIPCHK(hipSetDevice(device));
     std::unique_ptr<sp_ctx> c(new sp_ctx);
     c->device = device;
+    try {
+      c->scratch = scratch_address(device);  // once per device, with the device current
+    } catch (SpError& e) {
+      c->scratch_err = e.what();  // only sp_powder_fid_noalloc needs it
+    }
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -1460,6 +1677,23 @@ int sp_powder_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* a
   if (rc == SP_OK) rc = sp_plan_result(pl, fid_out, per_orient_out);
   sp_plan_destroy(pl);
   return rc;
+}
+
+int sp_powder_fid_noalloc(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                          const double* betas, const double* gammas, const double* weights, int64_t n_or,
+                          const double* rho0, const double* coil, double dt, int64_t nsteps, const sp_tols* tols,
+                          int mode, double* fid_out, double* per_orient_out, sp_stats* stats) {
+  if (!ctx) return SP_EINVAL;
+  try {
+    return noalloc_fid(ctx, I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, dt, nsteps, tols, mode, fid_out,
+                       per_orient_out, stats);
+  } catch (SpError& e) {
+    return fail(ctx, e.code, e.what());
+  } catch (std::bad_alloc&) {
+    return fail(ctx, SP_ENOMEM, "host allocation failed");
+  } catch (std::exception& e) {
+    return fail(ctx, SP_EDEVICE, e.what());
+  }
 }
 
 }  // extern "C"
