@@ -28,7 +28,9 @@ _REQUIRED = ('rho0', 'coil', 'sweep', 'npoints', 'grid')
 
 
 def _grumble(parameters):
-    """Input validation in the spirit of powder.m:340-428."""
+    """Input validation in the spirit of powder.m:340-428, plus the acquire.m
+    options this path does not run (decouple.m, homodecoupling), which must
+    not be dropped silently."""
     for f in _REQUIRED:
         if f not in parameters:
             raise ValueError(f'parameters.{f} subfield must be present.')
@@ -39,11 +41,17 @@ def _grumble(parameters):
     if callable(parameters['rho0']):
         raise NotImplementedError('orientation-dependent rho0 is not supported on this path')
     if parameters.get('decouple'):
-        raise NotImplementedError('decoupling is not supported on this path')
+        raise NotImplementedError('decoupling (acquire.m:51) is not supported on this path')
+    if 'homodec_oper' in parameters:
+        raise NotImplementedError('homodecoupling (acquire.m:54-62) is not supported on this path')
     if parameters['sweep'] <= 0:
         raise ValueError('parameters.sweep must be positive.')
     if int(parameters['npoints']) < 1:
         raise ValueError('parameters.npoints must be a positive integer.')
+    if 'dead_time' in parameters:
+        td = parameters['dead_time']
+        if not (np.isscalar(td) and np.isfinite(td) and td >= 0):
+            raise ValueError('parameters.dead_time must be a non-negative real scalar.')
 
 
 def _grid(g):
@@ -61,7 +69,10 @@ def powder(spin_system, pulse_sequence, parameters, assumptions):
     """Powder-averaged FID (kernel/contexts/powder.m:99-309) for
     pulse_sequence == acquire. parameters: rho0, coil, sweep (Hz), npoints,
     grid (dict or 'zcwn_<n>' / 'single_crystal'), optional offset (dict
-    isotope -> Hz), sum_up (default True), mode ('fp64' | 'fp32').
+    isotope -> Hz), dead_time (s, acquire.m:65-67), sum_up (default True),
+    mode ('fp64' | 'fp32'). spin_system.tols is passed to the library and
+    spin_system.sys.enable = ['krylov'] selects time stepping
+    (evolution.m:531-533).
 
     Returns (answer, grid) like the reference: answer is the weighted sum, or
     a list of per-orientation FIDs when sum_up is False."""
@@ -75,9 +86,21 @@ def powder(spin_system, pulse_sequence, parameters, assumptions):
     dt = 1.0 / parameters['sweep']                          # acquire.m:70-71
     nsteps = int(parameters['npoints']) - 1
     sum_up = parameters.get('sum_up', True)
-    fid, per, stats = _native.powder_fid(I, S.q_list(Q), grid, parameters['rho0'], parameters['coil'], dt, nsteps,
-                                         tols=ss['tols'], mode=parameters.get('mode', 'fp64'),
-                                         per_orientation=not sum_up)
+    mode = parameters.get('mode', 'fp64')
+    if 'krylov' in ss.get('enable', []):
+        mode = mode + '+krylov'
+    plan = _native.plan_powder(I, S.q_list(Q), grid, parameters['rho0'], parameters['coil'], dt, nsteps,
+                               tols=ss['tols'], mode=mode)
+    try:
+        if parameters.get('dead_time'):
+            plan.set_dead_time(parameters['dead_time'])     # acquire.m:65-67
+        plan.execute()
+        if sum_up:
+            fid = plan.result()
+        else:
+            fid, per = plan.result(per_orientation=True)
+    finally:
+        plan.close()
     if sum_up:
         return fid, grid
     return [per[k] for k in range(per.shape[0])], grid

@@ -176,7 +176,7 @@ _HUMAN = {
 # ------------------------------------------------------------------ create
 def create(sys, inter):
     """Build the spin-system structure (kernel/create.m). Supported fields:
-    sys.magnet, sys.isotopes; inter.zeeman.{eigs,euler,matrix,scalar} (ppm),
+    sys.magnet, sys.isotopes, sys.tols, sys.enable ('krylov', 'gpu'); inter.zeeman.{eigs,euler,matrix,scalar} (ppm),
     inter.coupling.{scalar (Hz), matrix (Hz)}, inter.coordinates (Angstrom).
     Every field uses 0-based spin numbers; couplings are dicts {(n,k): value}.
     """
@@ -243,7 +243,11 @@ def create(sys, inter):
                 d = d - np.eye(3) * np.trace(d) / 3
                 _add(n, k, d)
 
-    return dict(isotopes=isotopes, nspins=nspins, magnet=magnet, gammas=gammas,
+    enable = list(sys.get('enable', []))  # sys.enable switches (create.m:1556); 'krylov' is read by powder()
+    for e in enable:
+        if e not in ('krylov', 'gpu'):
+            raise ValueError(f'sys.enable option {e!r} is not supported on this path')
+    return dict(isotopes=isotopes, nspins=nspins, magnet=magnet, gammas=gammas, enable=enable,
                 basefrqs=basefrqs, zeeman=zmat, coupling=coup, tols=tols,
                 zeeman_strength=['full'] * nspins,
                 coupling_strength=[['strong'] * nspins for _ in range(nspins)])
