@@ -121,6 +121,9 @@ typedef struct sp_stats {
   int32_t krylov_systems;    /* systems propagated by time stepping (SP_MODE_KRYLOV) */
   double stage_ms[4];        /* propagator and stepping kernels, tridiagonalisation, tridiagonal
                                 eigensolver, synthesis */
+  int64_t step_products;     /* sparse matrix-vector products the stepping path made in the pass
+                                (Taylor terms over all substeps, steps and orientations) */
+  double plan_ms;            /* host planning (reduction, layouts) and upload of the plan */
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */

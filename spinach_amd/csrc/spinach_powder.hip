@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -136,7 +137,8 @@ struct sp_plan {
   bool step_vl[2] = {false, false};      // [REAL] values staged in LDS
   size_t step_lds[2] = {0, 0};           // [REAL] dynamic LDS bytes
   int grid_step = 0;
-  DevBuf d_sdesc, d_sell_start, d_sell_col, d_sell_pos, d_nsub, d_step_work;
+  DevBuf d_sdesc, d_sell_start, d_sell_col, d_sell_pos, d_nsub, d_terms, d_step_work;
+  double plan_ms = 0.0;  // host planning and upload (reported in every sp_stats)
   // dead time (acquire.m:65-67): per-orientation initial states
   double dead_time = 0.0;
   int64_t rho_or_stride = 0;
@@ -403,6 +405,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   pl->step_vec_words = 8 * ((nmax + spk::kSell - 1) / spk::kSell * spk::kSell);
   pl->d_nsub.alloc((size_t)std::max<int64_t>(1, or_base) * sizeof(int32_t));
   HIPCHK(hipMemsetAsync(pl->d_nsub.p, 0, pl->d_nsub.bytes, st));  // only stepped orientations write it
+  pl->d_terms.alloc((size_t)std::max<int64_t>(1, or_base) * sizeof(int32_t));
+  HIPCHK(hipMemsetAsync(pl->d_terms.p, 0, pl->d_terms.bytes, st));
   pl->d_queue.alloc(spk::kQueues * sizeof(unsigned int));
   pl->d_per.alloc((size_t)pl->total_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)pl->n_systems * npts * 2 * sizeof(double));
@@ -560,6 +564,7 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.per_or = pl->d_per.as<double>();
   sp.rho_out = rho_out;
   sp.nsub_out = final_state ? nullptr : pl->d_nsub.as<int32_t>();
+  sp.terms_out = final_state ? nullptr : pl->d_terms.as<int32_t>();
   const int g = (int)std::min<int64_t>(pl->grid_step, n);
   const int r = real ? 1 : 0;
   if (real) {
@@ -881,10 +886,13 @@ int run(sp_plan* pl, sp_stats* stats) {
   (void)hipEventDestroy(evend);
   HIPCHK(se);
   int32_t max_nsub = 0;
+  int64_t step_products = 0;
   if (pl->n_kry > 0) {
-    std::vector<int32_t> ns((size_t)pl->total_or);
+    std::vector<int32_t> ns((size_t)pl->total_or), tm((size_t)pl->total_or);
     HIPCHK(hipMemcpy(ns.data(), pl->d_nsub.p, ns.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tm.data(), pl->d_terms.p, tm.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int32_t v : ns) max_nsub = std::max(max_nsub, v);
+    for (int32_t v : tm) step_products += v;
     if (max_nsub > spk::kStepMaxSub)
       throw SpError(SP_ERANGE, "either dt is too long, or L is too big: |L*dt|>1e4, check both.");
   }
@@ -922,6 +930,8 @@ int run(sp_plan* pl, sp_stats* stats) {
   s.spectral_systems = pl->spectral_systems;
   s.eig_max_sweeps = max_sweeps;
   s.krylov_systems = pl->krylov_systems;
+  s.step_products = step_products;
+  s.plan_ms = pl->plan_ms;
   for (int k = 0; k < 4; k++) s.stage_ms[k] = ms_stage[k];
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   pl->last = s;
@@ -1207,10 +1217,13 @@ sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
                         const double* coil, double dt, int64_t nsteps, const sp_tols* tols, int mode) {
   if (!ctx) return nullptr;
   try {
+    const auto t0 = std::chrono::steady_clock::now();
     const Tols tl = read_tols(tols);
     std::vector<SysPlan> sps;
     sps.push_back(plan_system(I, Q, alphas, betas, gammas, weights, n_or, rho0, coil));
-    return build_plan(ctx, sps, dt, nsteps, mode, tl);
+    sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode, tl);
+    pl->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return pl;
   } catch (SpError& e) {
     fail(ctx, e.code, e.what());
   } catch (std::bad_alloc&) {
@@ -1225,6 +1238,7 @@ sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_s
                               const sp_tols* tols, int mode) {
   if (!ctx) return nullptr;
   try {
+    const auto t0 = std::chrono::steady_clock::now();
     const Tols tl = read_tols(tols);
     if (!systems || n_systems <= 0) throw SpError(SP_EINVAL, "the batch must hold at least one system");
     std::vector<SysPlan> sps;
@@ -1234,6 +1248,7 @@ sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_s
     }
     sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode, tl);
     for (int64_t s = 0; s < n_systems; s++) pl->fid_targets.push_back(systems[s].fid_out);
+    pl->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return pl;
   } catch (SpError& e) {
     fail(ctx, e.code, e.what());

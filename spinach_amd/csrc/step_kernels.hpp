@@ -75,6 +75,7 @@ struct StepParams {
   double* per_or;         // per-orientation FIDs, interleaved complex, nsteps+1 per orientation
   double* rho_out;        // final states (final_state == 1)
   int32_t* nsub_out;      // per orientation: substeps per time step (diagnostics)
+  int32_t* terms_out;     // per orientation: Taylor terms (matrix-vector products) over all steps
 };
 
 // Sum and max over the workgroup: DPP within each wave, then four partials in LDS.
@@ -223,6 +224,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
       __syncthreads();
     };
     if (!p.final_state) detect(0);
+    int32_t terms = 0;  // uniform: counted by every thread
     // 4. time steps
     for (int64_t t = 1; t <= p.nsteps; t++) {
       // scaling = max|rho| over the state (step.m:212-222)
@@ -296,6 +298,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
               }
             }
             any = __syncthreads_or(any);
+            terms++;
             double* tswap = x;
             x = y;
             y = tswap;
@@ -313,6 +316,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
       }
       if (!p.final_state) detect(t);
     }
+    if (tid == 0 && p.terms_out) p.terms_out[o] = terms;
     if (p.final_state) {
       double* ro = p.rho_out + 2 * (size_t)o * p.rho_or_stride;
       for (int i = tid; i < n; i += kStepThreads) {
