@@ -305,44 +305,60 @@ inline bool generator_is_real_full(const sp_csc* I, const sp_csc* Q) {
   return generator_is_real(slots, vals, nnz);
 }
 
-// Rows per slice of the stepping path's sliced-ELL layout (one wave).
-constexpr int kSellRows = 64;
+// Sliced-ELL layout of the stepping path. Each slice holds 64 / G rows and is
+// read by one wave: the G lanes of a row take its entries k = q, q+G, ... so
+// that entry k of row r (r' = r - first row of the slice) sits at
+//   start[s] + 64 * (k / G) + G * r' + k % G
+// and one load instruction of the wave reads 64 consecutive entries. Padding
+// entries point at their own row and hold zero.
+constexpr int kSellWave = 64;
 
-// Sliced-ELL layout of one system's reduced union pattern (rows in slices of
-// 64, entry k of row r of slice s at start[s] + 64k + r % 64; padding entries
-// point at their own row and hold zero), appended to the plan's arrays.
 struct SellDesc {
-  int32_t nslices;    // ceil(n / kSellRows)
+  int32_t nslices;    // ceil(n / (64 / G))
   int32_t sell_nnz;   // entries of the slice layout, padding included
   int64_t slice_off;  // offset of the system's slice starts in the plan's array
   int64_t sell_off;   // offset of the system's slice-layout columns in the plan's array
+  int32_t lanes;      // G: lanes per row (1, 2, 4 or 8)
+  int32_t pad;
 };
 
+// Lanes per row from the mean row length: enough lanes that a row's entries
+// take a few loads each, few enough that a slice still spans many rows.
+inline int sell_lanes(int n, int nnz) {
+  const double mean = n > 0 ? (double)nnz / n : 0.0;
+  return mean >= 16 ? 8 : (mean >= 8 ? 4 : (mean >= 3 ? 2 : 1));
+}
+
 inline void sell_layout(const SysPlan& sp, std::vector<int32_t>& start, std::vector<int32_t>& colv,
-                        std::vector<int32_t>& pos, SellDesc& d) {
+                        std::vector<int32_t>& pos, SellDesc& d, int lanes = 0) {
   const int n = sp.n;
-  const int ns = (n + kSellRows - 1) / kSellRows;
+  const int G = lanes > 0 ? lanes : sell_lanes(sp.n, sp.nnz);
+  const int H = kSellWave / G;  // rows per slice
+  const int ns = (n + H - 1) / H;
   std::vector<int32_t> rowlen(n, 0), first(n + 1, 0);
   for (int32_t e = 0; e < sp.nnz; e++) rowlen[sp.row[e]]++;
   for (int i = 0; i < n; i++) first[i + 1] = first[i] + rowlen[i];
   d.nslices = ns;
+  d.lanes = G;
+  d.pad = 0;
   d.slice_off = (int64_t)start.size();
   d.sell_off = (int64_t)colv.size();
   std::vector<int32_t> st(ns + 1, 0);
   for (int s2 = 0; s2 < ns; s2++) {
     int w = 0;
-    for (int r = s2 * kSellRows; r < std::min(n, (s2 + 1) * kSellRows); r++) w = std::max(w, rowlen[r]);
-    st[s2 + 1] = st[s2] + kSellRows * w;
+    for (int r = s2 * H; r < std::min(n, (s2 + 1) * H); r++) w = std::max(w, (rowlen[r] + G - 1) / G);
+    st[s2 + 1] = st[s2] + kSellWave * w;
   }
   d.sell_nnz = st[ns];
   start.insert(start.end(), st.begin(), st.end());
   const size_t base = colv.size();
   colv.resize(base + (size_t)d.sell_nnz);
   for (int s2 = 0; s2 < ns; s2++)
-    for (int k = st[s2]; k < st[s2 + 1]; k++) colv[base + k] = s2 * kSellRows + (k - st[s2]) % kSellRows;
+    for (int k = st[s2]; k < st[s2 + 1]; k++) colv[base + k] = s2 * H + ((k - st[s2]) % kSellWave) / G;
   for (int32_t e = 0; e < sp.nnz; e++) {
     const int r = sp.row[e], k = e - first[r];
-    const int p2 = st[r / kSellRows] + k * kSellRows + r % kSellRows;
+    const int s2 = r / H, rr = r - s2 * H;
+    const int p2 = st[s2] + kSellWave * (k / G) + G * rr + k % G;
     colv[base + p2] = sp.col[e];
     pos.push_back(p2);
   }

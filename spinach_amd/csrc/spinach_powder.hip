@@ -41,7 +41,8 @@
 constexpr size_t kScratchBytes = (size_t)64 << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
-static_assert(spk::kSell == spp::kSellRows, "slice height of the stepping kernel and the planner");
+static_assert(spk::kSell == spp::kSellWave, "slice row width of the stepping kernel and the planner");
+static_assert(spk::kStepMaxDim == spp::kStepMaxDim, "largest dimension of the stepping path");
 
 namespace {
 
@@ -231,7 +232,7 @@ void setup_step(sp_ctx* ctx, sp_plan* pl) {
     const size_t with_vals = vec_bytes + (size_t)pl->step_max_sell * (vw * sizeof(double) + sizeof(int32_t));
     pl->step_vl[real] = with_vals <= kStepLdsMax;
     pl->step_lds[real] = pl->step_vl[real] ? with_vals : vec_bytes;
-    if (!pl->step_vl[real]) work_words = std::max(work_words, vw * (size_t)pl->step_max_sell);
+    if (!pl->step_vl[real]) work_words = std::max(work_words, (vw * (size_t)pl->step_max_sell + 1) / 2 * 2);
   }
   // both variants of each kind may run in one pass
   if (pl->step_vl[0]) step_attr<false, true>(pl->step_lds[0]); else step_attr<false, false>(pl->step_lds[0]);
@@ -321,7 +322,10 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     descs.push_back(d);
     SellDesc sl{};
     sell_layout(sp, sell_start, sell_col, sell_pos, sl);
-    spk::StepDesc sdd{sl.nslices, sl.sell_nnz, sl.slice_off, sl.sell_off};
+    int lg = 0;
+    while ((1 << lg) < sl.lanes) lg++;
+    if (sl.nslices > spk::kStepMaxSlices) throw SpError(SP_ERANGE, "too many slices for the stepping path");
+    spk::StepDesc sdd{sl.nslices, sl.sell_nnz, sl.slice_off, sl.sell_off, sl.lanes, lg};
     sdescs.push_back(sdd);
     pl->step_max_sell = std::max(pl->step_max_sell, sdd.sell_nnz);
     nmax = std::max(nmax, sp.n);

@@ -10,9 +10,10 @@
 // persistent queue. Per item:
 //   1. the rank-1/rank-2 Wigner coefficients of the orientation,
 //   2. L = (x + x')/2, x = I + sum_q c_q Q_q on the reduced union pattern,
-//      written in sliced-ELL layout (slices of 64 rows, entry k of the slice's
-//      row r at slice_offset + 64k + r, so that the 64 lanes of a wave read
-//      one entry of 64 consecutive rows with one coalesced load); real values
+//      written in sliced-ELL layout: a slice of 64/G rows is read by one wave,
+//      G lanes per row (G = 1, 2, 4 or 8 from the mean row length), entry k of
+//      the slice's row r at slice_offset + 64(k/G) + G r + k%G, so that one
+//      load instruction reads 64 consecutive entries; real values
 //      when the generator is real at every orientation, complex otherwise;
 //      staged in LDS when the plan's largest slice layout fits, else in the
 //      workgroup's slot of a global workspace,
@@ -24,8 +25,9 @@
 //      rho += next, k = 1, 2, ... while any |next| > eps (one
 //      __syncthreads_or per term: the series' own convergence test);
 //      rho *= scaling; then f(t) = coil'*rho (a workgroup sum).
-// Row ownership follows the slices: row i belongs to lane i % 64 of wave
-// (i / 64) % 4, so every thread updates only the rows it owns and the state
+// Row ownership follows the slices: slice s belongs to wave s % 4 and its row
+// r' to lane G r' of that wave (the row's partial sums meet there by cross-lane
+// shuffles), so every thread updates only the rows it owns and the state
 // vector it reads in the product is double-buffered in LDS (one barrier per
 // Taylor term).
 #pragma once
@@ -35,16 +37,20 @@
 namespace spk {
 
 constexpr int kStepThreads = 256;
-constexpr int kSell = 64;           // rows per slice (one wave)
+constexpr int kSell = 64;           // entries per slice row of the layout (one wave)
 constexpr int kStepMaxSub = 10000;  // substeps per time step (step.m:230-233)
 constexpr int kStepMaxTerms = 256;  // Taylor terms per substep: |L dt/nsub|_1 <= 2 converges in ~30
+constexpr int kStepMaxDim = 2304;                  // largest coil-connected dimension of the path
+constexpr int kStepMaxSlices = kStepMaxDim / 8;     // slices hold at least 8 rows (at most 8 lanes per row)
 
 // Per-system layout of the sliced-ELL generator.
 struct StepDesc {
-  int32_t nslices;    // ceil(n / 64)
+  int32_t nslices;    // ceil(n / (64 / G))
   int32_t sell_nnz;   // entries of the slice layout, padding included (64 * sum of slice widths)
   int64_t slice_off;  // offset into the slice-start array (nslices + 1 entries)
   int64_t sell_off;   // offset into the slice-layout column array (sell_nnz entries)
+  int32_t lanes;      // G, lanes per row
+  int32_t log2_lanes;
 };
 
 struct StepParams {
@@ -94,9 +100,12 @@ __device__ __forceinline__ double step_wave_max(double v) {
 // workgroup's global slot and the system's column array).
 template <bool REAL, bool VL>
 __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) double lds[];  // every region starts 16-byte aligned (double2 reads)
   __shared__ double cre[34], cim[34];
   __shared__ double red[2][kStepThreads / 64];
+  __shared__ double rk[kStepMaxTerms + 1];  // 1/k
+  __shared__ int sany[2][kStepThreads / 64];  // per-wave "a term entry exceeds eps", by term parity
+  __shared__ int sst[kStepMaxSlices + 1];     // slice starts of the current item
   __shared__ int sitem;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -108,6 +117,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
   double* lval = lds + p.vec_words;  // staged values (VL)
   double* wslot = p.work + (size_t)blockIdx.x * p.work_stride;
   const double eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
+  for (int k = tid; k <= kStepMaxTerms; k += kStepThreads) rk[k] = k > 0 ? 1.0 / (double)k : 0.0;
 
   for (;;) {
     if (tid == 0) sitem = (int)atomicAdd(p.queue, 1u);
@@ -121,6 +131,10 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
     const int n = __builtin_amdgcn_readfirstlane(sd.n);
     const int nslices = __builtin_amdgcn_readfirstlane(st.nslices);
     const int sell_nnz = __builtin_amdgcn_readfirstlane(st.sell_nnz);
+    const int lg = __builtin_amdgcn_readfirstlane(st.log2_lanes);
+    const int G = 1 << lg, H = kSell >> lg;  // lanes per row, rows per slice
+    const int rsub = lane >> lg;             // the lane's row within a slice
+    const bool owner = (lane & (G - 1)) == 0;
     const int32_t* sstart = p.sell_start + st.slice_off;
     const int32_t* gcol = p.sell_col + st.sell_off;
     const int32_t* spos = p.sell_pos + sd.pat_off;
@@ -134,6 +148,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
     for (int k = tid; k < kVW * sell_nnz; k += kStepThreads) val[k] = 0.0;
     if (VL)
       for (int k = tid; k < sell_nnz; k += kStepThreads) lcol[k] = gcol[k];
+    for (int k = tid; k <= nslices; k += kStepThreads) sst[k] = sstart[k];
     __syncthreads();
     // 2. L = (x + x')/2 on the union pattern, scattered into the slice layout
     {
@@ -179,12 +194,13 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
     {
       double mx = 0.0;
       for (int s = wave; s < nslices; s += 4) {
-        const int b = sstart[s], w = (sstart[s + 1] - b) >> 6;
+        const int b = sst[s], w = (sst[s + 1] - b) >> 6;
         double acc = 0.0;
         for (int k = 0; k < w; k++) {
           const int e = b + (k << 6) + lane;
           acc += REAL ? fabs(val[e]) : hypot(val[2 * e], val[2 * e + 1]);
         }
+        for (int off = G >> 1; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);  // the row's G lanes
         mx = fmax(mx, acc);
       }
       mx = step_wave_max(mx);
@@ -203,8 +219,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
     auto detect = [&](int64_t t) {
       double sr = 0.0, si = 0.0;
       for (int s = wave; s < nslices; s += 4) {
-        const int i = (s << 6) + lane;
-        if (i < n) {
+        const int i = s * H + rsub;
+        if (owner && i < n) {
           const double cr = vc[2 * i], ci = vc[2 * i + 1], rr = vr[2 * i], ri = vr[2 * i + 1];
           sr += cr * rr + ci * ri;
           si += cr * ri - ci * rr;
@@ -230,8 +246,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
       // scaling = max|rho| over the state (step.m:212-222)
       double m = 0.0;
       for (int s = wave; s < nslices; s += 4) {
-        const int i = (s << 6) + lane;
-        if (i < n) m = fmax(m, hypot(vr[2 * i], vr[2 * i + 1]));
+        const int i = s * H + rsub;
+        if (owner && i < n) m = fmax(m, hypot(vr[2 * i], vr[2 * i + 1]));
       }
       m = step_wave_max(m);
       if (lane == 0) red[0][wave] = m;
@@ -240,8 +256,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
       __syncthreads();
       if (scaling > 0.0 && nsub > 0) {
         for (int s = wave; s < nslices; s += 4) {
-          const int i = (s << 6) + lane;
-          if (i < n) {
+          const int i = s * H + rsub;
+          if (owner && i < n) {
             vr[2 * i] /= scaling;
             vr[2 * i + 1] /= scaling;
           }
@@ -250,54 +266,94 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
         for (int sub = 0; sub < nsub; sub++) {
           // next = rho on the owned rows; padding rows of the slices stay zero in both buffers
           for (int s = wave; s < nslices; s += 4) {
-            const int i = (s << 6) + lane;
-            double a = 0.0, b = 0.0;
-            if (i < n) {
-              a = vr[2 * i];
-              b = vr[2 * i + 1];
+            const int i = s * H + rsub;
+            if (owner) {
+              double a = 0.0, b = 0.0;
+              if (i < n) {
+                a = vr[2 * i];
+                b = vr[2 * i + 1];
+              }
+              va[2 * i] = a;
+              va[2 * i + 1] = b;
+              vb[2 * i] = 0.0;
+              vb[2 * i + 1] = 0.0;
             }
-            va[2 * i] = a;
-            va[2 * i + 1] = b;
-            vb[2 * i] = 0.0;
-            vb[2 * i + 1] = 0.0;
           }
           __syncthreads();
           double* x = va;
           double* y = vb;
           for (int k = 1;; k++) {
             // next = -(1i/k)*(dt/nsub)*(L*next): with c = -(dt/nsub)/k, next = 1i*c*(L*next)
-            const double c = -h / (double)k;
+            const double c = -h * rk[k];
             int any = 0;
+            const double2* x2 = reinterpret_cast<const double2*>(x);
             for (int s = wave; s < nslices; s += 4) {
-              const int b = sstart[s], w = (sstart[s + 1] - b) >> 6;
-              double tr = 0.0, ti = 0.0;
-              for (int kk = 0; kk < w; kk++) {
-                const int e = b + (kk << 6) + lane;
-                const int j = col[e];
-                const double xr = x[2 * j], xi = x[2 * j + 1];
+              const int b = sst[s], w = (sst[s + 1] - b) >> 6;
+              double tr = 0.0, ti = 0.0, ur = 0.0, ui = 0.0;  // two chains of the row sum
+              int kk = 0;
+              for (; kk + 1 < w; kk += 2) {
+                const int e0 = b + (kk << 6) + lane, e1 = e0 + 64;
+                const double2 p0 = x2[col[e0]], p1 = x2[col[e1]];
                 if (REAL) {
-                  const double l = val[e];
-                  tr = fma(l, xr, tr);
-                  ti = fma(l, xi, ti);
+                  const double l0 = val[e0], l1 = val[e1];
+                  tr = fma(l0, p0.x, tr);
+                  ti = fma(l0, p0.y, ti);
+                  ur = fma(l1, p1.x, ur);
+                  ui = fma(l1, p1.y, ui);
                 } else {
-                  const double lr = val[2 * e], li = val[2 * e + 1];
-                  tr = fma(lr, xr, tr);
-                  tr = fma(-li, xi, tr);
-                  ti = fma(lr, xi, ti);
-                  ti = fma(li, xr, ti);
+                  const double2 l0 = reinterpret_cast<const double2*>(val)[e0];
+                  const double2 l1 = reinterpret_cast<const double2*>(val)[e1];
+                  tr = fma(l0.x, p0.x, tr);
+                  tr = fma(-l0.y, p0.y, tr);
+                  ti = fma(l0.x, p0.y, ti);
+                  ti = fma(l0.y, p0.x, ti);
+                  ur = fma(l1.x, p1.x, ur);
+                  ur = fma(-l1.y, p1.y, ur);
+                  ui = fma(l1.x, p1.y, ui);
+                  ui = fma(l1.y, p1.x, ui);
                 }
               }
-              const int i = (s << 6) + lane;
+              if (kk < w) {
+                const int e0 = b + (kk << 6) + lane;
+                const double2 p0 = x2[col[e0]];
+                if (REAL) {
+                  const double l0 = val[e0];
+                  tr = fma(l0, p0.x, tr);
+                  ti = fma(l0, p0.y, ti);
+                } else {
+                  const double2 l0 = reinterpret_cast<const double2*>(val)[e0];
+                  tr = fma(l0.x, p0.x, tr);
+                  tr = fma(-l0.y, p0.y, tr);
+                  ti = fma(l0.x, p0.y, ti);
+                  ti = fma(l0.y, p0.x, ti);
+                }
+              }
+              tr += ur;
+              ti += ui;
+              for (int off = G >> 1; off >= 1; off >>= 1) {  // the row's G lanes
+                tr += __shfl_xor(tr, off, 64);
+                ti += __shfl_xor(ti, off, 64);
+              }
+              const int i = s * H + rsub;
               const double nr = -c * ti, ni = c * tr;
-              y[2 * i] = nr;
-              y[2 * i + 1] = ni;
-              if (i < n) {
-                vr[2 * i] += nr;
-                vr[2 * i + 1] += ni;
-                any |= (nr * nr + ni * ni > eps2) ? 1 : 0;
+              if (owner) {
+                y[2 * i] = nr;
+                y[2 * i + 1] = ni;
+                if (i < n) {
+                  vr[2 * i] += nr;
+                  vr[2 * i + 1] += ni;
+                  any |= (nr * nr + ni * ni > eps2) ? 1 : 0;
+                }
               }
             }
-            any = __syncthreads_or(any);
+            // the series' convergence test over the workgroup with one barrier: each wave
+            // posts its ballot in this term's parity slot, which is next written two terms
+            // later, after every thread has passed the barrier of the term in between
+            const int par = k & 1;
+            const bool wave_any = __any(any);
+            if (lane == 0) sany[par][wave] = wave_any ? 1 : 0;
+            __syncthreads();
+            any = sany[par][0] | sany[par][1] | sany[par][2] | sany[par][3];
             terms++;
             double* tswap = x;
             x = y;
@@ -306,8 +362,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
           }
         }
         for (int s = wave; s < nslices; s += 4) {
-          const int i = (s << 6) + lane;
-          if (i < n) {
+          const int i = s * H + rsub;
+          if (owner && i < n) {
             vr[2 * i] *= scaling;
             vr[2 * i + 1] *= scaling;
           }

@@ -20,8 +20,8 @@ int run(const Problem& p, std::vector<sp_csc>& m) {
     spp::sell_layout(sp, start, col, pos, d);
     int32_t is_real = spp::generator_is_real_full(&m[0], &m[1]) ? 1 : 0;
     long long colsum = std::accumulate(col.begin(), col.end(), 0LL);
-    std::printf("PLAN n=%d np=%d nnz=%d real=%d nslices=%d sell_nnz=%d colsum=%lld\n", sp.n, sp.np, sp.nnz, is_real,
-                d.nslices, d.sell_nnz, colsum);
+    std::printf("PLAN n=%d np=%d nnz=%d real=%d nslices=%d lanes=%d sell_nnz=%d colsum=%lld\n", sp.n, sp.np,
+                sp.nnz, is_real, d.nslices, d.lanes, d.sell_nnz, colsum);
     return 0;
   } catch (spp::SpError& e) {
     std::printf("REJECT %d %s\n", e.code, e.what());

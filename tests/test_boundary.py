@@ -195,7 +195,8 @@ def test_planner_under_sanitizers(native, tmp_path, which):
     n = len(O.union_states(w['I'], w['Q'], w['coil']))
     assert int(head['n']) == n and int(head['np']) == max(16, (n + 15) // 16 * 16)
     assert int(head['real']) == 1 and int(head['sell_nnz']) >= int(head['nnz'])
-    assert int(head['nslices']) == (n + 63) // 64
+    G = int(head['lanes'])
+    assert G in (1, 2, 4, 8) and int(head['nslices']) == -(-n // (64 // G))
     for line in lines[1:]:
         assert line.startswith('REJECT 1 '), line
 
