@@ -176,7 +176,7 @@ _HUMAN = {
 # ------------------------------------------------------------------ create
 def create(sys, inter):
     """Build the spin-system structure (kernel/create.m). Supported fields:
-    sys.magnet, sys.isotopes, sys.tols, sys.enable ('krylov', 'gpu'); inter.zeeman.{eigs,euler,matrix,scalar} (ppm),
+    sys.magnet, sys.isotopes, sys.tols, sys.enable ('krylov', 'gpu', 'ham_cache'); inter.zeeman.{eigs,euler,matrix,scalar} (ppm),
     inter.coupling.{scalar (Hz), matrix (Hz)}, inter.coordinates (Angstrom).
     Every field uses 0-based spin numbers; couplings are dicts {(n,k): value}.
     """
@@ -245,7 +245,7 @@ def create(sys, inter):
 
     enable = list(sys.get('enable', []))  # sys.enable switches (create.m:1556); 'krylov' is read by powder()
     for e in enable:
-        if e not in ('krylov', 'gpu'):
+        if e not in ('krylov', 'gpu', 'ham_cache'):
             raise ValueError(f'sys.enable option {e!r} is not supported on this path')
     return dict(isotopes=isotopes, nspins=nspins, magnet=magnet, gammas=gammas, enable=enable,
                 basefrqs=basefrqs, zeeman=zmat, coupling=coup, tols=tols,
@@ -470,11 +470,46 @@ def _clean_up(m, tol):
     return m
 
 
+_HAM_CACHE = {}
+
+
+def _ham_key(ss):
+    """Digest of everything I and Q depend on: isotopes, magnet, carrier
+    frequencies, interaction tensors, interaction strengths (the 'nmr'
+    assumptions), the basis and the tolerances that drop terms (the cache key
+    of hamiltonian.m:600-611 hashes the descriptor, isotopes and basis)."""
+    import hashlib
+    h = hashlib.sha256()
+    h.update(repr((ss['isotopes'], ss['magnet'], ss.get('formalism'), ss['zeeman_strength'],
+                   ss['coupling_strength'], ss['tols']['liouv_zero'], ss['tols']['inter_cutoff'])).encode())
+    h.update(np.asarray(ss['basefrqs'], float).tobytes())
+    for z in ss['zeeman']:
+        h.update(np.asarray(z if z is not None else np.zeros((3, 3)), float).tobytes())
+    for row in ss['coupling']:
+        for c in row:
+            h.update(b'-' if c is None else np.asarray(c, float).tobytes())
+    b = ss.get('basis')
+    h.update(b'hilbert' if b is None else np.ascontiguousarray(np.asarray(b, np.int64)).tobytes())
+    return h.hexdigest()
+
+
 def hamiltonian(ss):
     """Isotropic part I and irreducible components Q[r][k][m] (r=1,2 as
     Q[0], Q[1]; k,m descending projection) of the Liouvillian commutation
     superoperator (kernel/hamiltonian.m:44-792). Returns (I, Q) with scipy
-    CSR complex128 matrices; empty components are None."""
+    CSR complex128 matrices; empty components are None. With 'ham_cache' in
+    sys.enable the result is kept per process under a digest of its inputs
+    and returned from there on a repeated call (hamiltonian.m:600-640)."""
+    if 'ham_cache' in ss.get('enable', []):
+        key = _ham_key(ss)
+        if key not in _HAM_CACHE:
+            _HAM_CACHE[key] = _hamiltonian(ss)
+        I, Q = _HAM_CACHE[key]
+        return I.copy(), [[[q.copy() if q is not None else None for q in row] for row in rank] for rank in Q]
+    return _hamiltonian(ss)
+
+
+def _hamiltonian(ss):
     n = ss['nspins']
     tol = ss['tols']['liouv_zero']
     cut = ss['tols']['inter_cutoff']

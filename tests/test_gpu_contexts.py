@@ -1,0 +1,130 @@
+"""crystal(), singlerot() (Fokker-Planck MAS), hp_acquire and Spinach grid
+files on the GPU, through the host mirrors (spinach_amd/contexts.py,
+powder.py) and the C-ABI, against the oracle and closed forms."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.io as sio
+
+import pyoracle as O
+from spinach_amd import contexts as Cx
+from spinach_amd import grids, workloads
+from spinach_amd import powder as P
+from spinach_amd import spinsys as S
+from test_contexts import fp_one_spin_case
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _ref_one_spin():
+    """The system of test_ctx_powder_crystal.m / test_ctx_singlerot_acquire.m:
+    one 1H, CSA eigenvalues [-2 -2 4] ppm, projections +1, rho0 = coil = L+."""
+    ss = S.create({'magnet': 14.1, 'isotopes': ['1H']}, {'zeeman': {'eigs': [[-2, -2, 4]], 'euler': [[0, 0, 0]]}})
+    ss = S.basis(ss, {'formalism': 'sphten-liouv', 'approximation': 'none', 'projections': [1]})
+    return ss, S.state(ss, 'L+', '1H'), S.state(ss, 'L+', '1H')
+
+
+def test_crystal_equals_single_crystal_powder():
+    """test_ctx_powder_crystal.m:43-54: powder() on the single_crystal grid
+    equals crystal() at zero Euler angles (1e-12)."""
+    ss, rho0, coil = _ref_one_spin()
+    params = {'rho0': rho0, 'coil': coil, 'sweep': 2000, 'npoints': 4, 'grid': 'single_crystal',
+              'orientation': [0, 0, 0]}
+    fp, _ = P.powder(ss, P.acquire, params, 'nmr')
+    fc = Cx.crystal(ss, P.acquire, params, 'nmr')
+    assert np.linalg.norm(fp - fc) <= 1e-12 + 1e-12 * max(1, np.linalg.norm(fc))
+
+
+def test_crystal_at_an_orientation_vs_oracle():
+    w = workloads.static_8spin(n_orient=4)
+    eul = [0.7, 1.9, -0.4]
+    params = {'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 1025, 'orientation': eul}
+    f = Cx.crystal(w['ss'], P.acquire, params, 'nmr')
+    g = {'alphas': np.array([eul[0]]), 'betas': np.array([eul[1]]), 'gammas': np.array([eul[2]]),
+         'weights': np.ones(1)}
+    ref = O.refcpu_powder(w['I'], w['Q'], g, w['rho0'], w['coil'], w['dt'], 1024, variant=0)
+    e = O.rel_err(f, ref)
+    print(f'crystal(), 8-spin, one orientation x 1024 steps: vs oracle {e:.2e}', flush=True)
+    assert e < 1e-10
+
+
+def test_singlerot_reference_case():
+    """test_ctx_singlerot_acquire.m: one 1H, 1 kHz at the magic angle,
+    max_rank 1, single crystal, 3 points: finite, f(0) = coil'*rho0."""
+    ss, rho0, coil = _ref_one_spin()
+    params = {'rho0': rho0, 'coil': coil, 'sweep': 2000, 'npoints': 3, 'rate': 1000, 'axis': [1, 1, 1],
+              'max_rank': 1, 'grid': 'single_crystal'}
+    fid, _ = Cx.singlerot(ss, P.acquire, params, 'nmr')
+    assert fid.shape == (3,) and np.all(np.isfinite(fid))
+    f0 = np.vdot(coil, rho0)
+    assert abs(fid[0] - f0) <= 1e-12 + 1e-12 * max(1, abs(f0))
+
+
+def test_singlerot_one_spin_vs_exact_mas_signal():
+    """The Fokker-Planck generator (N = 33 rotor phases) on the GPU against the
+    closed-form rotor-phase-averaged MAS signal of one CSA-coupled spin."""
+    c = fp_one_spin_case(16)
+    fids = []
+    for o in range(3):
+        g = grids.subset(c['grid'], [o])
+        g['weights'] = np.ones(1)  # the orientation's own FID
+        params = {'rho0': S.state(c['ss'], 'L+', '1H'), 'coil': S.state(c['ss'], 'L+', '1H'),
+                  'sweep': 1 / c['dt'], 'npoints': c['nsteps'] + 1, 'rate': c['rate'], 'axis': list(c['axis']),
+                  'max_rank': 16, 'grid': g}
+        fid, _ = Cx.singlerot(c['ss'], P.acquire, params, 'nmr')
+        fids.append(fid)
+    e = max(O.rel_err(fids[o], c['exact'][o]) for o in range(3))
+    print(f'singlerot one spin, 33 rotor phases: vs exact MAS signal {e:.2e}', flush=True)
+    assert e < 1e-9
+
+
+def test_singlerot_two_spin_powder_vs_oracle():
+    """Two-spin 1H-13C at 12.5 kHz, max_rank 3 (7 rotor phases), 12
+    orientations x 512 steps: the GPU (complex generator) against the oracle
+    on the same Fokker-Planck matrices."""
+    w = workloads.static_2spin(npoints=8, n_orient=2)
+    g = grids.haar(12, np.random.default_rng(9))
+    params = {'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1e5, 'npoints': 513, 'rate': 12500.0,
+              'axis': [1, 1, 1], 'max_rank': 3, 'grid': g}
+    fid, _ = Cx.singlerot(w['ss'], P.acquire, params, 'nmr')
+    I, Q = S.hamiltonian(S.assume(w['ss'], 'nmr'))
+    Ifp, qfp = Cx.fokker_planck(I, Q, params['axis'], params['rate'], 3)
+    r0 = np.kron(np.ones(7) / 7, w['rho0'])
+    c0 = np.kron(np.ones(7), w['coil'])
+    ref = O.refcpu_powder(Ifp, qfp, g, r0, c0, 1e-5, 512, variant=0)
+    e = O.rel_err(fid, ref)
+    print(f'singlerot 2-spin, 7 rotor phases, 12 x 512: vs oracle {e:.2e}', flush=True)
+    assert e < 1e-10
+
+
+def test_hp_acquire_pulse_then_acquire_vs_closed_form():
+    """hp_acquire: rho0 = Lz, pi/2 pulse about y, then the two-spin FID of
+    rho0 = Lx (closed form)."""
+    w = workloads.static_2spin(npoints=256, n_orient=16)
+    ss = w['ss']
+    params = {'rho0': S.state(ss, 'Lz', '1H'), 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 256,
+              'grid': w['grid'], 'pulse_op': S.operator(ss, 'Ly', '1H'), 'pulse_angle': math.pi / 2}
+    fid, grid = P.powder(ss, P.hp_acquire, params, 'nmr')
+    cf = O.closed_form_2spin(ss, grid, w['dt'], w['nsteps'])
+    e = O.rel_err(fid, cf)
+    print(f'hp_acquire (pi/2 pulse on Lz): vs closed form {e:.2e}', flush=True)
+    assert e < 5e-9
+
+
+def test_powder_on_a_spinach_grid_file(tmp_path, monkeypatch):
+    """parameters.grid names a Spinach grid file (powder.m:173-179): the
+    fixture copy of leb_2ang_rank_5 written as .mat, against the oracle."""
+    fix = np.load(os.path.join(HERE, 'golden', 'grids.npz'))
+    gdir = tmp_path / 'kernel' / 'grids'
+    gdir.mkdir(parents=True)
+    sio.savemat(str(gdir / 'leb_2ang_rank_5.mat'), {k: fix[f'leb_2ang_rank_5__{k}'].reshape(-1, 1)
+                                                     for k in ('alphas', 'betas', 'gammas', 'weights')})
+    monkeypatch.setenv('SPINACH_ROOT', str(tmp_path))
+    w = workloads.static_2spin(npoints=128, n_orient=2)
+    params = {'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 128, 'grid': 'leb_2ang_rank_5'}
+    fid, grid = P.powder(w['ss'], P.acquire, params, 'nmr')
+    ref = O.refcpu_powder(w['I'], w['Q'], grid, w['rho0'], w['coil'], w['dt'], 127, variant=0)
+    assert len(grid['weights']) == 14 and O.rel_err(fid, ref) < 1e-10
