@@ -13,11 +13,14 @@ import numpy as np
 
 
 def n_features(ka, kt):
-    nf = 6 * ka + 2 * kt + 2
+    nf = 6 * ka + 5 + 2 * kt + 2
     return (nf + 31) // 32 * 32
 
 
-def features(alphas, betas, gammas, tidx, dt, t_max, ka, kt):
+def features(alphas, betas, gammas, tidx, dt, t_max, ka, kt, t_feat=None):
+    """t_max scales the linear time feature, t_feat (default t_max) the
+    harmonics cos/sin(pi m t / t_feat)."""
+    t_feat = t_max if t_feat is None else t_feat
     F = n_features(ka, kt)
     n = len(alphas)
     x = np.zeros((n, F))
@@ -27,10 +30,16 @@ def features(alphas, betas, gammas, tidx, dt, t_max, ka, kt):
             x[:, c] = np.cos(k * ang)
             x[:, c + 1] = np.sin(k * ang)
             c += 2
+    cb, sb, g = np.cos(betas), np.sin(betas), np.asarray(gammas)
+    for v in (0.5 * (3 * cb * cb - 1), sb * cb * np.cos(g), sb * cb * np.sin(g), sb * sb * np.cos(2 * g),
+              sb * sb * np.sin(2 * g)):
+        x[:, c] = v
+        c += 1
     tau = np.asarray(tidx) * dt / t_max
+    sf = np.asarray(tidx) * dt / t_feat
     for m in range(1, kt + 1):
-        x[:, c] = np.cos(math.pi * m * tau)
-        x[:, c + 1] = np.sin(math.pi * m * tau)
+        x[:, c] = np.cos(math.pi * m * sf)
+        x[:, c + 1] = np.sin(math.pi * m * sf)
         c += 2
     x[:, c] = tau
     x[:, c + 1] = 1.0

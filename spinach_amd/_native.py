@@ -464,7 +464,8 @@ def _load_surrogate():
     if getattr(lib, '_sg_ready', False):
         return lib
     lib.sp_surrogate_create.restype = C.c_void_p
-    lib.sp_surrogate_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64]
+    lib.sp_surrogate_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_uint64]
     lib.sp_surrogate_destroy.restype = None
     lib.sp_surrogate_destroy.argtypes = [C.c_void_p]
     lib.sp_surrogate_last_error.restype = C.c_char_p
@@ -492,10 +493,15 @@ def _load_surrogate():
 class SurrogateNet:
     """Device-resident Fourier-feature MLP (sp_surrogate_*)."""
 
-    def __init__(self, t_max, width=256, depth=4, ka=3, kt=22, seed=0, device=0):
+    def __init__(self, t_max, width=256, depth=4, ka=3, kt=22, seed=0, device=0, t_feat=None):
+        """t_max: record length (the linear time feature t/t_max); t_feat: time
+        scale of the harmonics cos/sin(pi m t/t_feat), m = 1..kt (default
+        t_max; kt*dt spans the band up to the Nyquist frequency)."""
         lib = _load_surrogate()
         context(device)
-        self.h = lib.sp_surrogate_create(device, width, depth, ka, kt, float(t_max), int(seed))
+        t_feat = float(t_max if t_feat is None else t_feat)
+        self.h = lib.sp_surrogate_create(device, width, depth, ka, kt, float(t_max), t_feat, int(seed))
+        self.t_feat = t_feat
         if not self.h:
             raise SpinachPowderError('sp_surrogate_create failed')
         self.width, self.depth, self.ka, self.kt, self.t_max = width, depth, ka, kt, float(t_max)

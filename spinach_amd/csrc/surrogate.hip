@@ -10,10 +10,17 @@
 // (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; master weights, Adam
 // moments and the loss are FP32; the powder sum is FP64.
 //
-// Features of a point (orientation o, time index j, t = j*dt, tau = t/T):
+// Features of a point (orientation o, time index j, t = j*dt, tau = t/T with
+// T the record length, s = t/Tf with Tf the time scale of the harmonics):
 //   cos(k a), sin(k a), cos(k b), sin(k b), cos(k g), sin(k g),  k = 1..KA
-//   cos(pi m tau), sin(pi m tau),                                  m = 1..KT
+//   (3 cos^2 b - 1)/2, sin b cos b cos g, sin b cos b sin g, sin^2 b cos 2g, sin^2 b sin 2g
+//       (up to constant factors the real and imaginary parts of the Wigner
+//       row D2(0,m)(a,b,g), on which a secular rank-2 Hamiltonian depends)
+//   cos(pi m s), sin(pi m s),                                      m = 1..KT
 //   tau, 1
+// With Tf = KT*dt the harmonics reach the Nyquist frequency 1/(2 dt) in steps
+// of 1/(2 KT dt), spanning the band of the FID; with Tf = T they span only
+// KT/(2T).
 // zero-padded to F (a multiple of 32).
 
 #include <hip/hip_runtime.h>
@@ -62,6 +69,7 @@ struct FeatArgs {
   int64_t npts;
   double dt;
   double T;
+  double Tf;
   int ka, kt, F;
   bf16* X;  // n_points x F
 };
@@ -74,6 +82,7 @@ __global__ void k_features(FeatArgs a) {
   int64_t j = a.pt_t ? a.pt_t[g] : g % a.npts;
   double ang[3] = {a.alphas[o], a.betas[o], a.gammas[o]};
   double tau = (double)j * a.dt / a.T;
+  double sf = (double)j * a.dt / a.Tf;
   bf16* x = a.X + p * a.F;
   int c = 0;
   for (int d = 0; d < 3; d++)
@@ -81,8 +90,16 @@ __global__ void k_features(FeatArgs a) {
       x[c++] = f2bf((float)cos(k * ang[d]));
       x[c++] = f2bf((float)sin(k * ang[d]));
     }
+  {
+    const double cb = cos(ang[1]), sb = sin(ang[1]);
+    x[c++] = f2bf((float)(0.5 * (3.0 * cb * cb - 1.0)));
+    x[c++] = f2bf((float)(sb * cb * cos(ang[2])));
+    x[c++] = f2bf((float)(sb * cb * sin(ang[2])));
+    x[c++] = f2bf((float)(sb * sb * cos(2.0 * ang[2])));
+    x[c++] = f2bf((float)(sb * sb * sin(2.0 * ang[2])));
+  }
   for (int m = 1; m <= a.kt; m++) {
-    double ph = 3.14159265358979323846 * m * tau;
+    double ph = 3.14159265358979323846 * m * sf;
     x[c++] = f2bf((float)cos(ph));
     x[c++] = f2bf((float)sin(ph));
   }
@@ -287,7 +304,7 @@ struct sp_surrogate {
   int device = 0;
   hipStream_t stream = nullptr;
   int ka = 3, kt = 22, F = 64, W = 256, depth = 4;
-  double T = 1.0;
+  double T = 1.0, Tf = 1.0;
   std::string err;
   int code = 0;
   // parameters: layer l maps fan_in(l) -> fan_out(l); W_l stored out x in (row-major)
@@ -405,7 +422,7 @@ float backward(sp_surrogate* s, int64_t n) {
 
 void features(sp_surrogate* s, const double* a, const double* b, const double* gm, const int32_t* por,
               const int32_t* pt, int64_t p0, int64_t n, int64_t npts, double dt) {
-  sgk::FeatArgs fa{a, b, gm, por, pt, p0, n, npts, dt, s->T, s->ka, s->kt, s->F, s->X.as<sgk::bf16>()};
+  sgk::FeatArgs fa{a, b, gm, por, pt, p0, n, npts, dt, s->T, s->Tf, s->ka, s->kt, s->F, s->X.as<sgk::bf16>()};
   hipLaunchKernelGGL(sgk::k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, fa);
   SGCHK(hipGetLastError());
 }
@@ -414,10 +431,11 @@ void features(sp_surrogate* s, const double* a, const double* b, const double* g
 
 extern "C" {
 
-sp_surrogate* sp_surrogate_create(int device, int width, int depth, int ka, int kt, double t_max, uint64_t seed) {
+sp_surrogate* sp_surrogate_create(int device, int width, int depth, int ka, int kt, double t_max, double t_feat,
+                                  uint64_t seed) {
   std::unique_ptr<sp_surrogate> s(new sp_surrogate);
   try {
-    if (width < 32 || width % 32 || depth < 1 || depth > 16 || ka < 0 || kt < 0 || !(t_max > 0))
+    if (width < 32 || width % 32 || depth < 1 || depth > 16 || ka < 0 || kt < 0 || !(t_max > 0) || !(t_feat > 0))
       throw SgErr(SP_EINVAL, "invalid surrogate shape");
     SGCHK(hipSetDevice(device));
     s->device = device;
@@ -427,7 +445,8 @@ sp_surrogate* sp_surrogate_create(int device, int width, int depth, int ka, int 
     s->ka = ka;
     s->kt = kt;
     s->T = t_max;
-    int nf = 6 * ka + 2 * kt + 2;
+    s->Tf = t_feat;
+    int nf = 6 * ka + 5 + 2 * kt + 2;
     s->F = (nf + 31) / 32 * 32;
     size_t off = 0;
     for (int l = 0; l <= depth; l++) {

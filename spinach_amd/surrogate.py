@@ -39,7 +39,8 @@ def learned_powder(w, train_grid, heldout_grid, fine_grid, tol=1e-4, steps=3000,
     held_fids, _ = plain_per_orientation(w, heldout_grid)
     # the network sees FIDs scaled to unit peak; the scale is restored after inference
     scale = float(np.abs(train_fids).max())
-    net = _native.SurrogateNet(t_max, width=width, depth=depth, ka=ka, kt=kt, seed=seed)
+    # time harmonics in steps of 1/(2 kt dt) up to the Nyquist frequency 1/(2 dt): the FID's band
+    net = _native.SurrogateNet(t_max, width=width, depth=depth, ka=ka, kt=kt, seed=seed, t_feat=kt * w['dt'])
     t1 = time.perf_counter()
     trace = net.train(train_grid, train_fids / scale, w['dt'], steps, batch=batch, lr=lr)
     t_train = time.perf_counter() - t1

@@ -33,9 +33,14 @@ def test_features_layout():
     x = M.features(np.array([0.3]), np.array([1.1]), np.array([2.0]), np.array([50]), 1e-5, 1e-3, 2, 3)
     assert x.shape[1] == 32
     assert abs(x[0, 0] - np.cos(0.3)) < 1e-15 and abs(x[0, 3] - np.sin(0.6)) < 1e-15
+    cb, sb = np.cos(1.1), np.sin(1.1)
+    assert abs(x[0, 12] - 0.5 * (3 * cb * cb - 1)) < 1e-15 and abs(x[0, 16] - sb * sb * np.sin(4.0)) < 1e-15
     tau = 50 * 1e-5 / 1e-3
-    assert abs(x[0, 12] - np.cos(np.pi * tau)) < 1e-15
-    assert x[0, 18] == tau and x[0, 19] == 1.0 and not x[0, 20:].any()
+    assert abs(x[0, 17] - np.cos(np.pi * tau)) < 1e-15
+    assert x[0, 23] == tau and x[0, 24] == 1.0 and not x[0, 25:].any()
+    # harmonics on their own time scale
+    xf = M.features(np.array([0.3]), np.array([1.1]), np.array([2.0]), np.array([50]), 1e-5, 1e-3, 2, 3, t_feat=3e-5)
+    assert abs(xf[0, 17] - np.cos(np.pi * 50 / 3)) < 1e-13 and xf[0, 23] == tau
 
 
 def test_planted_backward_mistake_is_caught():
