@@ -115,8 +115,13 @@ def ncores():
     return max(1, min(c, 64))
 
 
-def time_plan(plan, steps, warmup):
+def time_plan(plan, steps, warmup, autotune=0):
+    """Untimed: autotune passes per candidate (sp_plan_autotune, if > 0) and the
+    warm-up; timed: `steps` passes."""
     import numpy as np
+    plan.tuning_chosen = plan.autotune(passes=autotune) if autotune > 0 else None
+    if plan.tuning_chosen:
+        log(f'autotune: {plan.tuning_chosen}')
     for _ in range(warmup):
         plan.execute()
     kms, stages = [], []
@@ -160,7 +165,7 @@ def bench_static(args, two_spin=False):
     nsteps = w['nsteps']
     plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], nsteps,
                                mode=args.mode + args.path_flag)
-    sec, kms, st = time_plan(plan, args.steps, args.warmup)
+    sec, kms, st = time_plan(plan, args.steps, args.warmup, args.autotune)
     fid, per = plan.result(per_orientation=True)
     log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernels {kms:.2f} ms, stages {st["stage_ms"]}')
     n = st['reduced_dim']
@@ -211,6 +216,9 @@ def bench_static(args, two_spin=False):
         cfg['step_products_per_orientation'] = round(st['step_products'] / n_or, 1)
     if not st['spectral_systems'] and not krylov:
         cfg['time_block'] = st['block']
+    if plan.tuning_chosen:
+        cfg['tuning'] = {k: plan.tuning_chosen[k] for k in ('log2_block', 'prop_wg_per_cu', 'tri_wg_per_cu',
+                                                            'syn_wg_per_cu', 'step_wg_per_cu', 'candidates')}
     return n_or * nsteps / sec, sec, kms, stages, cfg, cpu, err
 
 
@@ -222,7 +230,7 @@ def bench_batch(args):
     log(f'{len(b["systems"])} systems built in {time.time() - t0:.1f}s')
     n_or_tot = sum(len(y['grid']['weights']) for y in b['systems'])
     plan = _native.plan_powder_batch(b['systems'], b['dt'], b['nsteps'], mode=args.mode + args.path_flag)
-    sec, kms, st = time_plan(plan, args.steps, args.warmup)
+    sec, kms, st = time_plan(plan, args.steps, args.warmup, args.autotune)
     fids = plan.result_batch()
     log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms, {n_or_tot} orientations')
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
@@ -271,6 +279,9 @@ def bench_batch(args):
            'mode': args.mode, 'spectral_systems': st['spectral_systems'], 'plan_ms': round(st['plan_ms'], 1)}
     if prop_flops:
         cfg['time_block'] = st['block']
+    if plan.tuning_chosen:
+        cfg['tuning'] = {k: plan.tuning_chosen[k] for k in ('log2_block', 'prop_wg_per_cu', 'tri_wg_per_cu',
+                                                            'syn_wg_per_cu', 'step_wg_per_cu', 'candidates')}
     return n_or_tot * b['nsteps'] / sec, sec, kms, stages, cfg, cpu, err
 
 
@@ -380,6 +391,9 @@ def main():
                     help='auto: spectral path for real generators (FP64, dim <= 192); propagator: the power '
                          'scheme with real products; complex: the general complex power scheme; krylov: '
                          'step() per time step (sys.enable krylov)')
+    ap.add_argument('--autotune', type=int, default=2, metavar='PASSES',
+                    help='timed passes per candidate of sp_plan_autotune before the warm-up (0: the planner\'s '
+                         'launch configuration)')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a device copy of MIB MiB (counter calibration) and exit')
     args = ap.parse_args()

@@ -223,6 +223,37 @@ void sp_plan_destroy(sp_plan* plan);
  * to every system of a batch plan. Not available on MAS plans. */
 int sp_plan_set_dead_time(sp_plan* plan, double dead_time);
 
+/* ---- launch configuration and autotuning ------------------------------ */
+
+/* Launch configuration of a plan's kernels. Replaces: nothing in the
+ * reference (an execution option). Fields that a plan's paths do not use are
+ * ignored by sp_plan_set_tuning and returned as the planner's defaults. */
+typedef struct sp_tuning {
+  int32_t log2_block;      /* propagator path: time block B = 2^log2_block of the power scheme (0..10) */
+  int32_t prop_wg_per_cu;  /* propagator path: resident workgroups per CU (1..8) */
+  int32_t tri_wg_per_cu;   /* spectral path: tridiagonalisation workgroups per CU (1..4) */
+  int32_t syn_wg_per_cu;   /* spectral path: synthesis workgroups per CU (1..8) */
+  int32_t step_wg_per_cu;  /* stepping path: workgroups per CU (1..4, capped by its LDS) */
+  int32_t candidates;      /* sp_plan_autotune: configurations timed (0: the cached choice was applied) */
+  double default_ms;       /* sp_plan_autotune: pass time of the configuration the plan had before */
+  double best_ms;          /* sp_plan_autotune: pass time of the chosen configuration */
+} sp_tuning;
+
+/* Times the plan's pass over candidate launch configurations (coordinate
+ * descent: time block, then workgroups per CU of each path, passes >= 1
+ * timed passes per candidate after one warm-up), keeps the fastest and
+ * records it in a per-process cache keyed by the plan's shape (dimensions,
+ * step count, item counts per path, precision); plans built later with the
+ * same shape start from the cached choice, and a call on such a plan applies
+ * it without timing (candidates = 0) unless passes < 0 forces a new search.
+ * The per-orientation arithmetic does not depend on the workgroup counts; the
+ * time block changes the propagator path's rounding only (~1e-15). */
+int sp_plan_autotune(sp_plan* plan, int32_t passes, sp_tuning* chosen);
+
+/* The plan's current launch configuration / apply one (validated, clamped). */
+int sp_plan_get_tuning(const sp_plan* plan, sp_tuning* out);
+int sp_plan_set_tuning(sp_plan* plan, const sp_tuning* in);
+
 /* ---- batched uneven workload ------------------------------------------ */
 
 /* One spin system of a batch: the arguments of sp_powder_fid for that system. */

@@ -26,7 +26,7 @@ SP_NQ = 34
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
            'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real', 'sp_plan_set_dead_time',
-           'sp_powder_fid_noalloc')
+           'sp_powder_fid_noalloc', 'sp_plan_autotune', 'sp_plan_get_tuning', 'sp_plan_set_tuning')
 SP_MAS_FRAME_MAGNET = 0
 SP_MAS_FRAME_ROTOR = 1
 
@@ -54,6 +54,15 @@ class SpStats(C.Structure):
         out = {k: getattr(self, k) for k, _ in self._fields_}
         out['stage_ms'] = list(self.stage_ms)
         return out
+
+
+class SpTuning(C.Structure):
+    _fields_ = [('log2_block', C.c_int32), ('prop_wg_per_cu', C.c_int32), ('tri_wg_per_cu', C.c_int32),
+                ('syn_wg_per_cu', C.c_int32), ('step_wg_per_cu', C.c_int32), ('candidates', C.c_int32),
+                ('default_ms', C.c_double), ('best_ms', C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SpRotor(C.Structure):
@@ -111,6 +120,12 @@ def load():
     lib.sp_mas_fid.restype = C.c_int
     lib.sp_mas_fid.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
                                _dp, _dp, C.POINTER(SpRotor), C.c_int64, C.c_int, _dp, _dp, C.POINTER(SpStats)]
+    lib.sp_plan_autotune.restype = C.c_int
+    lib.sp_plan_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SpTuning)]
+    lib.sp_plan_get_tuning.restype = C.c_int
+    lib.sp_plan_get_tuning.argtypes = [C.c_void_p, C.POINTER(SpTuning)]
+    lib.sp_plan_set_tuning.restype = C.c_int
+    lib.sp_plan_set_tuning.argtypes = [C.c_void_p, C.POINTER(SpTuning)]
     lib.sp_powder_fid_noalloc.restype = C.c_int
     lib.sp_powder_fid_noalloc.argtypes = lib.sp_powder_fid.argtypes
     lib.sp_plan_set_dead_time.restype = C.c_int
@@ -256,6 +271,28 @@ class Plan:
             _err(self.ctx, rc, 'sp_execute')
         self.stats = st.as_dict()
         return self.stats
+
+    def autotune(self, passes=2):
+        """sp_plan_autotune: time candidate launch configurations, keep the
+        fastest (cached per plan shape); returns the chosen tuning."""
+        t = SpTuning()
+        rc = load().sp_plan_autotune(self.h, int(passes), C.byref(t))
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_autotune')
+        return t.as_dict()
+
+    def tuning(self):
+        t = SpTuning()
+        rc = load().sp_plan_get_tuning(self.h, C.byref(t))
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_get_tuning')
+        return t.as_dict()
+
+    def set_tuning(self, tuning):
+        t = SpTuning(**{k: v for k, v in tuning.items() if k in dict(SpTuning._fields_)})
+        rc = load().sp_plan_set_tuning(self.h, C.byref(t))
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_set_tuning')
 
     def set_dead_time(self, dead_time):
         """sp_plan_set_dead_time: step(L, rho0, dead_time) before the acquisition

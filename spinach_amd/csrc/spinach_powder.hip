@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -137,7 +138,9 @@ struct sp_plan {
   int32_t step_max_sell = 0;             // largest slice layout over the plan's systems
   bool step_vl[2] = {false, false};      // [REAL] values staged in LDS
   size_t step_lds[2] = {0, 0};           // [REAL] dynamic LDS bytes
-  int grid_step = 0;
+  int grid_step = 0, step_per_cu = 0;
+  size_t step_work_words = 0;
+  int tri_per_cu = 2, syn_per_cu = 3;  // spectral path: resident workgroups per CU
   DevBuf d_sdesc, d_sell_start, d_sell_col, d_sell_pos, d_nsub, d_terms, d_step_work;
   double plan_ms = 0.0;  // host planning and upload (reported in every sp_stats)
   // dead time (acquire.m:65-67): per-orientation initial states
@@ -212,6 +215,33 @@ int choose_log2_block(int64_t npts, int np) {
   return best;
 }
 
+void apply_cached_tuning(sp_plan* pl);
+
+// Propagator path: time block B = 2^log2b of the power scheme and resident
+// workgroups per CU (the workspace holds one slot per workgroup).
+void configure_prop(sp_ctx* ctx, sp_plan* pl, int log2b, int per_cu) {
+  const int64_t npts = pl->nsteps + 1;
+  pl->log2b = std::max(0, std::min(10, log2b));
+  while (pl->log2b > 0 && ((int64_t)1 << pl->log2b) > npts) pl->log2b--;
+  pl->block = 1 << pl->log2b;
+  pl->nblk = (int32_t)((npts + pl->block - 1) / pl->block);
+  pl->per_cu = std::max(1, std::min(8, per_cu));
+  pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * pl->per_cu, pl->n_prop));
+  const size_t esz = (pl->mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
+  const int64_t np = pl->np_prop, mat = 2 * np * np;
+  pl->work_stride = 6 * mat + 2 * (int64_t)(pl->block + 16) * np + 2 * (int64_t)pl->nblk * np;
+  pl->work_stride = (pl->work_stride + 63) / 64 * 64;
+  pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+}
+
+// Spectral path: resident tridiagonalisation and synthesis workgroups per CU.
+void configure_spec(sp_ctx* ctx, sp_plan* pl, int tri_per_cu, int syn_per_cu) {
+  pl->tri_per_cu = std::max(1, std::min(4, tri_per_cu));
+  pl->syn_per_cu = std::max(1, std::min(8, syn_per_cu));
+  pl->grid_tri = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)pl->tri_per_cu * ctx->n_cu, pl->n_spec));
+  pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)pl->syn_per_cu * ctx->n_cu, pl->n_spec));
+}
+
 // LDS budget of the stepping kernel per variant: the four state vectors, and the
 // slice-layout values and columns when the plan's largest layout fits beside
 // them; otherwise the values go to a global slot per workgroup.
@@ -222,6 +252,8 @@ void step_attr(size_t bytes) {
   HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&spk::k_step<REAL, VL>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
+
+void configure_step(sp_ctx* ctx, sp_plan* pl, int per_cu);
 
 void setup_step(sp_ctx* ctx, sp_plan* pl) {
   const size_t vec_bytes = (size_t)pl->step_vec_words * sizeof(double);
@@ -237,11 +269,18 @@ void setup_step(sp_ctx* ctx, sp_plan* pl) {
   // both variants of each kind may run in one pass
   if (pl->step_vl[0]) step_attr<false, true>(pl->step_lds[0]); else step_attr<false, false>(pl->step_lds[0]);
   if (pl->step_vl[1]) step_attr<true, true>(pl->step_lds[1]); else step_attr<true, false>(pl->step_lds[1]);
-  // persistent grid: as many workgroups as the LDS lets reside on each CU
+  pl->step_work_words = work_words;
+  configure_step(ctx, pl, 0);
+}
+
+// Resident stepping workgroups per CU (0: as many as the LDS holds, at most 4).
+void configure_step(sp_ctx* ctx, sp_plan* pl, int per_cu) {
   const size_t lds_max = std::max(pl->step_lds[0], pl->step_lds[1]);
-  const int per_cu = std::max<int>(1, std::min<int>(4, (int)((160 * 1024) / (lds_max + 1024))));
-  pl->grid_step = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * per_cu, pl->n_items));
-  pl->d_step_work.alloc(std::max<size_t>(1, work_words) * pl->grid_step * sizeof(double));
+  const int fit = std::max<int>(1, std::min<int>(4, (int)((160 * 1024) / (lds_max + 1024))));
+  if (per_cu <= 0) per_cu = fit;
+  pl->step_per_cu = std::min(per_cu, fit);
+  pl->grid_step = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * pl->step_per_cu, pl->n_items));
+  pl->d_step_work.alloc(std::max<size_t>(1, pl->step_work_words) * pl->grid_step * sizeof(double));
 }
 
 sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t nsteps, int mode, const Tols& tl) {
@@ -281,6 +320,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   if (const char* e = getenv("SPINACH_LOG2_BLOCK")) pl->log2b = std::max(0, std::min(10, atoi(e)));
   pl->block = 1 << pl->log2b;
   pl->nblk = (int32_t)((npts + pl->block - 1) / pl->block);
+  pl->np_prop = np_prop;
   // Taylor degree K for |X|_1 <= theta = 1: the remainder is below
   // 1.05/(K+1)!, i.e. 8.6e-18 for K=18 (FP64) and 2.6e-8 for K=10 (FP32).
   pl->theta = 1.0;
@@ -429,27 +469,19 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     // SPINACH_WG_PER_CU). Workgroups beyond residency find the queue drained.
     per_cu = std::max(2, per_cu);
     if (const char* e = getenv("SPINACH_WG_PER_CU")) per_cu = std::max(1, atoi(e));
-    pl->per_cu = per_cu;
-    int64_t g = (int64_t)ctx->n_cu * per_cu;
-    pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, pl->n_prop));
-    const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
-    const int64_t mat = 2 * (int64_t)np_prop * np_prop;
-    pl->np_prop = np_prop;
-    pl->work_stride = 6 * mat + 2 * (int64_t)(pl->block + 16) * np_prop + 2 * (int64_t)pl->nblk * np_prop;
-    pl->work_stride = (pl->work_stride + 63) / 64 * 64;
-    pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+    configure_prop(ctx, pl.get(), pl->log2b, per_cu);
   }
   if (pl->n_spec > 0) {
     // two 256-thread workgroups per CU, each holding the lower triangle of a 192 x 192
-    // matrix in registers
-    pl->grid_tri = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)ctx->n_cu, pl->n_spec));
-    pl->grid_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, pl->n_spec));
+    // matrix in registers; three synthesis workgroups per CU
+    configure_spec(ctx, pl.get(), 2, 3);
     pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
   if (pl->n_kry > 0) setup_step(ctx, pl.get());
   setup_sum(pl.get(), st);
   HIPCHK(hipStreamSynchronize(st));
+  apply_cached_tuning(pl.get());
   return pl.release();
 }
 
@@ -1149,6 +1181,146 @@ int noalloc_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al,
   return SP_OK;
 }
 
+// ------------------------------------------------------------- autotuning
+std::mutex g_tune_mu;
+std::map<std::string, sp_tuning> g_tune_cache;
+
+std::string tune_key(const sp_plan* pl) {
+  char b[256];
+  snprintf(b, sizeof(b), "m%d s%lld p%lld/%lld k%lld/%lld t%lld np%d/%d o%lld", pl->mode, (long long)pl->nsteps,
+           (long long)pl->n_prop, (long long)pl->n_prop_complex, (long long)pl->n_spec, (long long)pl->n_kry,
+           (long long)pl->n_systems, pl->np_prop, pl->np_max, (long long)pl->total_or);
+  return b;
+}
+
+sp_tuning get_tuning(const sp_plan* pl) {
+  sp_tuning t{};
+  t.log2_block = pl->log2b;
+  t.prop_wg_per_cu = pl->per_cu;
+  t.tri_wg_per_cu = pl->tri_per_cu;
+  t.syn_wg_per_cu = pl->syn_per_cu;
+  t.step_wg_per_cu = pl->step_per_cu;
+  return t;
+}
+
+void apply_tuning(sp_plan* pl, const sp_tuning& t) {
+  sp_ctx* ctx = pl->ctx;
+  if (pl->n_prop > 0 && (t.log2_block != pl->log2b || t.prop_wg_per_cu != pl->per_cu))
+    configure_prop(ctx, pl, t.log2_block, t.prop_wg_per_cu);
+  if (pl->n_spec > 0) configure_spec(ctx, pl, t.tri_wg_per_cu, t.syn_wg_per_cu);
+  if (pl->step_lds[0] > 0 && t.step_wg_per_cu != pl->step_per_cu) configure_step(ctx, pl, t.step_wg_per_cu);
+  if (pl->gexec) {  // the recorded pass holds the old grids and workspaces
+    (void)hipGraphExecDestroy(pl->gexec);
+    pl->gexec = nullptr;
+  }
+  pl->executed = false;
+}
+
+// Mean kernel time of `passes` eager passes after one warm-up.
+double time_passes(sp_plan* pl, int passes) {
+  const bool graph = pl->graph;
+  pl->graph = false;
+  sp_stats st{};
+  double tot = 0.0;
+  try {
+    run(pl, &st);
+    for (int k = 0; k < passes; k++) {
+      run(pl, &st);
+      tot += st.kernel_ms;
+    }
+  } catch (...) {
+    pl->graph = graph;
+    throw;
+  }
+  pl->graph = graph;
+  return tot / passes;
+}
+
+int autotune(sp_plan* pl, int passes, sp_tuning* out) {
+  if (pl->mas) throw SpError(SP_EINVAL, "MAS plans have no tunable launch configuration");
+  HIPCHK(hipSetDevice(pl->ctx->device));
+  const std::string key = tune_key(pl);
+  if (passes >= 0) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      apply_tuning(pl, it->second);
+      if (out) *out = it->second, out->candidates = 0;
+      return SP_OK;
+    }
+  }
+  const int np = std::max(1, passes < 0 ? -passes : passes);
+  sp_tuning best = get_tuning(pl);
+  double best_ms = time_passes(pl, np);
+  const double default_ms = best_ms;
+  int cand = 1;
+  auto trial = [&](sp_tuning t) {
+    apply_tuning(pl, t);
+    const double ms = time_passes(pl, np);
+    cand++;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = get_tuning(pl);
+    }
+  };
+  const int64_t npts = pl->nsteps + 1;
+  if (pl->n_prop > 0) {
+    const int l0 = best.log2_block;
+    for (int l = std::max(0, l0 - 2); l <= std::min(10, l0 + 2); l++)
+      if (l != l0 && ((int64_t)1 << l) <= std::max<int64_t>(1, npts)) {
+        sp_tuning t = best;
+        t.log2_block = l;
+        trial(t);
+      }
+    const int p0 = best.prop_wg_per_cu;
+    for (int pc = 1; pc <= 3; pc++)
+      if (pc != p0) {
+        sp_tuning t = best;
+        t.prop_wg_per_cu = pc;
+        trial(t);
+      }
+  }
+  if (pl->n_spec > 0) {
+    for (int tc = 1; tc <= 2; tc++)
+      if (tc != best.tri_wg_per_cu) {
+        sp_tuning t = best;
+        t.tri_wg_per_cu = tc;
+        trial(t);
+      }
+    for (int sc = 1; sc <= 4; sc++)
+      if (sc != best.syn_wg_per_cu) {
+        sp_tuning t = best;
+        t.syn_wg_per_cu = sc;
+        trial(t);
+      }
+  }
+  if (pl->n_kry > 0)
+    for (int kc = 1; kc <= 4; kc++)
+      if (kc != best.step_wg_per_cu) {
+        sp_tuning t = best;
+        t.step_wg_per_cu = kc;
+        trial(t);
+      }
+  apply_tuning(pl, best);
+  best = get_tuning(pl);
+  best.candidates = cand;
+  best.default_ms = default_ms;
+  best.best_ms = best_ms;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  if (out) *out = best;
+  return SP_OK;
+}
+
+// A cached tuning for a newly built plan of the same shape, if any.
+void apply_cached_tuning(sp_plan* pl) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cache.find(tune_key(pl));
+  if (it != g_tune_cache.end()) apply_tuning(pl, it->second);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1331,6 +1503,37 @@ int sp_plan_set_dead_time(sp_plan* plan, double dead_time) {
     }
     plan->dead_time = dead_time;
     plan->executed = false;
+    return SP_OK;
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_autotune(sp_plan* plan, int32_t passes, sp_tuning* chosen) {
+  if (!plan) return SP_EINVAL;
+  try {
+    return autotune(plan, passes, chosen);
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_get_tuning(const sp_plan* plan, sp_tuning* out) {
+  if (!plan || !out) return SP_EINVAL;
+  *out = get_tuning(plan);
+  return SP_OK;
+}
+
+int sp_plan_set_tuning(sp_plan* plan, const sp_tuning* in) {
+  if (!plan || !in) return SP_EINVAL;
+  try {
+    if (plan->mas) throw SpError(SP_EINVAL, "MAS plans have no tunable launch configuration");
+    HIPCHK(hipSetDevice(plan->ctx->device));
+    apply_tuning(plan, *in);
     return SP_OK;
   } catch (SpError& e) {
     return fail(plan->ctx, e.code, e.what());
