@@ -281,6 +281,61 @@ def np_powder(I, qlist, grid, rho0, coil, dt, nsteps, reduce=True, per_orientati
     return (fid, per) if per_orientation else fid
 
 
+def _chop(x, tol):
+    """clean_up(): tol*round(x/tol) on the real and imaginary parts (clean_up.m:70)."""
+    if tol <= 0:
+        return x
+    return tol * np.round(x.real / tol) + 1j * tol * np.round(x.imag / tol)
+
+
+def chop_propagator(L, dt, prop_chop=1e-10):
+    """The chop-Taylor branch of propagator.m:64-201 in numpy, a second
+    restatement beside refcpu's (different summation order): A =
+    clean_up(-1i*dt*L); scale by 2^ceil(log2|A|_1); P = sum (1/n) A*term with
+    clean_up after every term until a term is all zero; clean_up(P); square
+    with clean_up after every squaring (eps instead of prop_chop when
+    |A|_1 > 1024)."""
+    A = _chop(-1j * dt * np.asarray(L, complex), prop_chop)
+    nrm = np.abs(A).sum(axis=0).max()
+    if nrm > 1e9:
+        raise ValueError('norm of -i*L*dt exceeds 1e9, check your L and your dt.')
+    tol = 2.220446049250313e-16 if nrm > 1024 else prop_chop
+    ns = max(0, int(math.ceil(math.log2(nrm)))) if nrm > 0 else 0
+    A = A / 2 ** ns
+    P = np.eye(len(A), dtype=complex)
+    term = P.copy()
+    k = 1
+    while np.any(term != 0):
+        term = _chop(((1.0 / k) * A) @ term, tol)
+        P = P + term
+        k += 1
+    P = _chop(P, tol)
+    for _ in range(ns):
+        P = _chop(P @ P, tol)
+    return P
+
+
+def np_powder_chop(I, qlist, grid, rho0, coil, dt, nsteps, tols=DEFAULT_TOLS):
+    """Powder FID on the union-pattern subspace with the reference's
+    propagator branch choice (expm below small_matrix, chop_propagator at and
+    above it) and the clean_up of L, rho and coil of evolution.m:526-528."""
+    liouv_zero, zte_tol, prop_chop, small_matrix = tols[0], tols[1], tols[2], tols[3]
+    keep = union_states(I, qlist, coil)
+    n_or = len(grid['weights'])
+    fid = np.zeros(int(nsteps) + 1, complex)
+    r0 = _chop(np.asarray(rho0, complex)[keep], zte_tol)
+    c0 = _chop(np.asarray(coil, complex)[keep], zte_tol)
+    for o in range(n_or):
+        H = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))[np.ix_(keep, keep)]
+        H = _chop(H, liouv_zero)
+        P = sla.expm(-1j * dt * H) if len(keep) < small_matrix else chop_propagator(H, dt, prop_chop)
+        r = r0.copy()
+        for t in range(int(nsteps) + 1):
+            fid[t] += grid['weights'][o] * np.vdot(c0, r)
+            r = P @ r
+    return fid
+
+
 def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False, t0=0.0):
     """Exact FID of the secular heteronuclear two-spin system with rho0 = Lx
     and coil = L+ on spin 0: f(t) = 1/4 exp(-i w0 t) cos(a t / 2), with
