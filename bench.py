@@ -44,8 +44,9 @@ BOUND = {'spk::k_tridiag': 'valu', 'spk::k_dceig': 'valu', 'spk::k_synth': 'mfma
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs of this script (counters scaled by a device copy of known size:
 # FETCH x2.000, WRITE x1.000); see profiles/k_spectral_static8spin_fp64.md and
-# profiles/k_powder_static8spin_fp64.md
+# profiles/k_powder_static8spin_fp64.md and profiles/k_powder_batch512_fp64.md
 TRAFFIC = {
+    ('batch', 'fp64', 'spk::k_powder<double,true>'): 2534736904 * 1024 * 2.0 + 528656886 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_tridiag'): 13775 * 1024 * 2.0 + 67885 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,false>'): 165530392 * 1024 * 2.0 + 53150413 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,true>'): 139595089 * 1024 * 2.0 + 49487957 * 1024 * 1.0,
