@@ -1,7 +1,7 @@
 """Committed fixtures (tests/golden/make_golden.py): Spinach's own grid files,
 and oracle FIDs of seeded workloads. The oracle and the host-side builder
 must keep reproducing them; the GPU path is compared with them in
-test_gpu_parity.py."""
+test_gpu_golden.py and test_gpu_contexts.py."""
 import os
 
 import numpy as np
