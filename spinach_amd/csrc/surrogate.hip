@@ -113,9 +113,13 @@ __global__ void k_features(FeatArgs a) {
 //   TA = false: A stored M x K (row-major, lda); true: stored K x M.
 //   TB = false: B stored K x N (row-major, ldb); true: stored N x K.
 // Epilogue EPI: 0 = store C (fp32, ldc); 1 = +bias[n], tanh, store fp32 and a
-// BF16 copy; 2 = +bias[n], store fp32; 3 = accumulate into C (C += A*B).
+// BF16 copy; 2 = +bias[n], store fp32; 3 = accumulate into C (C += A*B);
+// 4 = +bias[n], tanh, store the BF16 copy only (inference: no backward pass
+// reads the FP32 activations).
 // Tile 64x64 per workgroup (4 waves, 32x32 each as 2x2 MFMA tiles), k-panels
-// of 32. M, N and K arbitrary (masked).
+// of 32. M, N and K arbitrary (masked). Split K (EPI 0 only): workgroup z of
+// gridDim.z sums k in [z*kchunk, (z+1)*kchunk) and stores its partial product
+// at C + z*zstride; k_sum_parts adds the partials in z order.
 struct GemmArgs {
   const bf16* A;
   const bf16* B;
@@ -123,7 +127,42 @@ struct GemmArgs {
   bf16* Cb;
   const float* bias;
   int M, N, K, lda, ldb, ldc;
+  int kchunk;       // k range of one split (a multiple of 32), K when not split
+  int64_t zstride;  // floats between the partial products of two splits
 };
+
+// stage one 64 x 32 operand panel into LDS: rows of the panel are output rows
+// (A: i, B: j), columns are k. CONTIG: the 32 k-values of a row are contiguous
+// in memory (A row-major M x K, or B stored N x K), read 8 at a time (16 bytes)
+// when the row stride and the panel start allow it.
+template <bool CONTIG>
+__device__ inline void stage_panel(bf16 (*S)[40], const bf16* X, int ld, int r0, int rmax, int k0, int kend,
+                                   bool vec) {
+  constexpr int KP = 32;
+  const int tid = threadIdx.x;
+  if (CONTIG && vec) {
+    // 64 rows x 4 groups of 8: one 16-byte load per thread
+    const int r = tid >> 2, c = 8 * (tid & 3);
+    const int gr = r0 + r;
+    s8v v = s8v{0, 0, 0, 0, 0, 0, 0, 0};
+    if (gr < rmax && k0 + c + 8 <= kend) {
+      v = *reinterpret_cast<const s8v*>(X + (size_t)gr * ld + k0 + c);
+    } else if (gr < rmax) {
+      for (int u = 0; u < 8; u++)
+        if (k0 + c + u < kend) v[u] = (short)X[(size_t)gr * ld + k0 + c + u];
+    }
+    *reinterpret_cast<s8v*>(&S[r][c]) = v;
+    return;
+  }
+  for (int e = tid; e < 64 * KP; e += 256) {
+    int r, c;
+    if (CONTIG) { r = e / KP; c = e % KP; } else { c = e / 64; r = e % 64; }
+    const int gr = r0 + r;
+    bf16 v = 0;
+    if (gr < rmax && k0 + c < kend) v = CONTIG ? X[(size_t)gr * ld + k0 + c] : X[(size_t)(k0 + c) * ld + gr];
+    S[r][c] = v;
+  }
+}
 
 template <bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
@@ -133,26 +172,16 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  const int kbeg = blockIdx.z * g.kchunk, kend = min(g.K, kbeg + g.kchunk);
+  float* C = g.C + (size_t)blockIdx.z * g.zstride;
+  // 16-byte staging needs 8-element aligned rows
+  const bool va = !TA && (g.lda % 8) == 0, vb = TB && (g.ldb % 8) == 0;
   f4v acc[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) acc[a][b] = f4v{0, 0, 0, 0};
-  for (int k0 = 0; k0 < g.K; k0 += KP) {
-    // stage: 64 x 32 elements of each operand, 8 per thread
-    for (int e = tid; e < 64 * KP; e += 256) {
-      int r, c;
-      if (!TA) { r = e / KP; c = e % KP; } else { c = e / 64; r = e % 64; }
-      int gi = i0 + r;
-      bf16 v = 0;
-      if (gi < g.M && k0 + c < g.K) v = TA ? g.A[(size_t)(k0 + c) * g.lda + gi] : g.A[(size_t)gi * g.lda + k0 + c];
-      As[r][c] = v;
-      int rb, cb;  // Bs[n][k]
-      if (TB) { rb = e / KP; cb = e % KP; } else { cb = e / 64; rb = e % 64; }
-      int gj = j0 + rb;
-      bf16 w = 0;
-      if (gj < g.N && k0 + cb < g.K)
-        w = TB ? g.B[(size_t)gj * g.ldb + k0 + cb] : g.B[(size_t)(k0 + cb) * g.ldb + gj];
-      Bs[rb][cb] = w;
-    }
+  for (int k0 = kbeg; k0 < kend; k0 += KP) {
+    stage_panel<!TA>(As, g.A, g.lda, i0, g.M, k0, kend, va);
+    stage_panel<TB>(Bs, g.B, g.ldb, j0, g.N, k0, kend, vb);
     __syncthreads();
     s8v xa[2], xb[2];
     for (int t = 0; t < 2; t++) {
@@ -173,15 +202,17 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
         float v = acc[a][b][e];
         size_t o = (size_t)i * g.ldc + j;
         if (EPI == 0) {
-          g.C[o] = v;
+          C[o] = v;
         } else if (EPI == 1) {
           v = tanhf(v + g.bias[j]);
-          g.C[o] = v;
+          C[o] = v;
           g.Cb[o] = f2bf(v);
         } else if (EPI == 2) {
-          g.C[o] = v + g.bias[j];
+          C[o] = v + g.bias[j];
+        } else if (EPI == 4) {
+          g.Cb[o] = f2bf(tanhf(v + g.bias[j]));
         } else {
-          g.C[o] += v;
+          C[o] += v;
         }
       }
 }
@@ -218,13 +249,28 @@ __global__ void k_tanh_bwd(const float* dH, const float* H, int64_t n, float* dZ
   dZb[i] = f2bf(v);
 }
 
-// column sums (bias gradient): db[j] = sum_i dZ[i][j]
-__global__ void k_colsum(const float* dZ, int64_t rows, int cols, float* db) {
-  int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= cols) return;
+// column sums (bias gradient), stage 1: workgroup (x, y) sums rows
+// y, y + gridDim.y, ... of columns [64x, 64x + 64) into part[y][j]; the 256
+// threads are 64 columns x 4 row phases, combined in LDS in a fixed order
+__global__ void __launch_bounds__(256) k_colsum_part(const float* dZ, int64_t rows, int cols, float* part) {
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + c;
   float s = 0.f;
-  for (int64_t i = 0; i < rows; i++) s += dZ[i * cols + j];
-  db[j] = s;
+  if (j < cols)
+    for (int64_t i = (int64_t)blockIdx.y * 4 + ph; i < rows; i += (int64_t)gridDim.y * 4) s += dZ[i * cols + j];
+  red[ph][c] = s;
+  __syncthreads();
+  if (ph == 0 && j < cols) part[(size_t)blockIdx.y * cols + j] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+}
+
+// out[i] = sum_z part[z * n + i], z in order (split-K products, column-sum partials)
+__global__ void k_sum_parts(const float* part, int nz, int64_t n, float* out) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < nz; z++) s += part[(size_t)z * n + i];
+  out[i] = s;
 }
 
 // Adam on FP32 master weights, BF16 copy refreshed
@@ -316,6 +362,7 @@ struct sp_surrogate {
   // activations for a batch of up to `cap` points
   int64_t cap = 0;
   Dev X, Hf, Hb, dHf, dHb, Y, dY, dYb, tgt, lossp, or_idx, t_idx, grid;
+  Dev part;  // split-K partial products and column-sum partials of the backward pass
 };
 
 namespace {
@@ -327,6 +374,8 @@ int sg_fail(sp_surrogate* s, int code, const std::string& m) {
   }
   return code;
 }
+
+constexpr int kMaxSplit = 32;  // split-K factor and column-sum partial rows of the backward pass
 
 void ensure_batch(sp_surrogate* s, int64_t n) {
   if (n <= s->cap) return;
@@ -342,19 +391,56 @@ void ensure_batch(sp_surrogate* s, int64_t n) {
   s->dYb.alloc((size_t)n * 2 * sizeof(sgk::bf16));
   s->tgt.alloc((size_t)n * 2 * sizeof(float));
   s->lossp.alloc((size_t)((2 * n + 255) / 256) * sizeof(float));
+  s->part.alloc((size_t)kMaxSplit * wmax * wmax * sizeof(float));
 }
 
 template <bool TA, bool TB, int EPI>
 void gemm(sp_surrogate* s, const sgk::bf16* A, int lda, const sgk::bf16* B, int ldb, float* C, int ldc,
           sgk::bf16* Cb, const float* bias, int M, int N, int K) {
-  sgk::GemmArgs a{A, B, C, Cb, bias, M, N, K, lda, ldb, ldc};
+  sgk::GemmArgs a{A, B, C, Cb, bias, M, N, K, lda, ldb, ldc, K, 0};
   dim3 grid((N + 63) / 64, (M + 63) / 64);
   hipLaunchKernelGGL((sgk::k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s->stream, a);
   SGCHK(hipGetLastError());
 }
 
-// forward pass of n points whose features are in s->X; output in s->Y (n x 2)
-void forward(sp_surrogate* s, int64_t n) {
+// C (M x N, ldc = N) = op(A) op(B) with the K dimension split so that a few
+// output tiles still fill the GPU (the weight gradients: M, N <= 256, K = the
+// batch); partials in s->part, summed in a fixed order
+template <bool TA, bool TB>
+void gemm_splitk(sp_surrogate* s, const sgk::bf16* A, int lda, const sgk::bf16* B, int ldb, float* C, int M, int N,
+                 int K) {
+  const int tiles = ((N + 63) / 64) * ((M + 63) / 64);
+  int nz = std::max(1, std::min(kMaxSplit, std::min(K / 512, 256 / std::max(1, tiles))));
+  const int kchunk = ((K + nz - 1) / nz + 31) / 32 * 32;
+  nz = (K + kchunk - 1) / kchunk;
+  if (nz <= 1) {
+    gemm<TA, TB, 0>(s, A, lda, B, ldb, C, N, nullptr, nullptr, M, N, K);
+    return;
+  }
+  const int64_t mn = (int64_t)M * N;
+  sgk::GemmArgs a{A, B, s->part.as<float>(), nullptr, nullptr, M, N, K, lda, ldb, N, kchunk, mn};
+  dim3 grid((N + 63) / 64, (M + 63) / 64, nz);
+  hipLaunchKernelGGL((sgk::k_gemm_bf16<TA, TB, 0>), grid, dim3(256), 0, s->stream, a);
+  SGCHK(hipGetLastError());
+  hipLaunchKernelGGL(sgk::k_sum_parts, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s->stream,
+                     s->part.as<float>(), nz, mn, C);
+  SGCHK(hipGetLastError());
+}
+
+// db[j] = sum_i dZ[i][j] over n rows of `cols` columns
+void colsum(sp_surrogate* s, const float* dz, int64_t n, int cols, float* db) {
+  const int ny = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxSplit, (n + 255) / 256));
+  hipLaunchKernelGGL(sgk::k_colsum_part, dim3((cols + 63) / 64, ny), dim3(256), 0, s->stream, dz, n, cols,
+                     s->part.as<float>());
+  SGCHK(hipGetLastError());
+  hipLaunchKernelGGL(sgk::k_sum_parts, dim3((cols + 255) / 256), dim3(256), 0, s->stream, s->part.as<float>(), ny,
+                     (int64_t)cols, db);
+  SGCHK(hipGetLastError());
+}
+
+// forward pass of n points whose features are in s->X; output in s->Y (n x 2);
+// train: keep the FP32 activations for backward()
+void forward(sp_surrogate* s, int64_t n, bool train = true) {
   const sgk::bf16* in = s->X.as<sgk::bf16>();
   int ld_in = s->F;
   float* Hf = s->Hf.as<float>();
@@ -364,8 +450,12 @@ void forward(sp_surrogate* s, int64_t n) {
   for (int l = 0; l < s->depth; l++) {
     float* h = Hf + (size_t)l * s->cap * s->W;
     sgk::bf16* hb = Hb + (size_t)l * s->cap * s->W;
-    gemm<false, true, 1>(s, in, ld_in, wb + s->woff[l], s->fin[l], h, s->W, hb, w32 + s->boff[l], (int)n, s->W,
-                         s->fin[l]);
+    if (train)
+      gemm<false, true, 1>(s, in, ld_in, wb + s->woff[l], s->fin[l], h, s->W, hb, w32 + s->boff[l], (int)n, s->W,
+                           s->fin[l]);
+    else
+      gemm<false, true, 4>(s, in, ld_in, wb + s->woff[l], s->fin[l], nullptr, s->W, hb, w32 + s->boff[l], (int)n,
+                           s->W, s->fin[l]);
     in = hb;
     ld_in = s->W;
   }
@@ -394,10 +484,8 @@ float backward(sp_surrogate* s, int64_t n) {
     const sgk::bf16* inb = (l == 0) ? s->X.as<sgk::bf16>() : Hb + (size_t)(l - 1) * s->cap * s->W;
     const int fin = s->fin[l];
     // dW_l = dZ^T (nout x n) * In (n x fin)
-    gemm<true, false, 0>(s, dzb, nout, inb, fin, gw + s->woff[l], fin, nullptr, nullptr, nout, fin, (int)n);
-    hipLaunchKernelGGL(sgk::k_colsum, dim3((nout + 255) / 256), dim3(256), 0, s->stream, dz, n, nout,
-                       gw + s->boff[l]);
-    SGCHK(hipGetLastError());
+    gemm_splitk<true, false>(s, dzb, nout, inb, fin, gw + s->woff[l], nout, fin, (int)n);
+    colsum(s, dz, n, nout, gw + s->boff[l]);
     if (l == 0) break;
     // dH_{l-1} = dZ (n x nout) * W_l (nout x fin)
     float* dh = s->dHf.as<float>() + ((l & 1) ? 0 : (size_t)n * wmax);
@@ -635,7 +723,7 @@ int sp_surrogate_powder(sp_surrogate* s, const double* a, const double* b, const
       int64_t no = std::min(chunk_or, n_or - o0);
       int64_t n = no * npts;
       features(s, da.as<double>(), dbv.as<double>(), dg.as<double>(), nullptr, nullptr, o0 * npts, n, npts, dt);
-      forward(s, n);
+      forward(s, n, false);
       hipLaunchKernelGGL(sgk::k_powder_accum, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s->stream,
                          s->Y.as<float>(), dw.as<double>(), o0, no, npts, dfid.as<double>());
       SGCHK(hipGetLastError());

@@ -56,6 +56,7 @@ def learned_powder(w, train_grid, heldout_grid, fine_grid, tol=1e-4, steps=3000,
     fid_p = plan.result()
     plan.close()
     info['plain_fine_ms'] = st['elapsed_s'] * 1e3
+    info['plain_stats'] = st
     info['speedup_surrogate_vs_plain'] = info['plain_fine_ms'] / ms_s if ms_s > 0 else None
     info['surrogate_fine_rel_err_vs_plain'] = rel_err(fid_s * scale, fid_p)
     if held_err <= tol:
