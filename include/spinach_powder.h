@@ -205,6 +205,36 @@ sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
                         int64_t n_or, const double* rho0, const double* coil, double dt,
                         int64_t nsteps, const sp_tols* tols, int mode);
 
+/* Relaxation superoperator R(Omega) of one spin system, so that the generator
+ * of each orientation is L = H + i*R (kernel/evolution.m: L = H + 1i*R + 1i*K;
+ * R from relaxation(spin_system,[alpha beta gamma]) at kernel/contexts/
+ * powder.m:263). Orientation-dependent terms are those of kernel/relaxation.m
+ * whose rates are quadratic forms ort*T*ort' of the orientation vector
+ * ort = [0 0 1]*euler2dcm(alpha,beta,gamma) = (-sin b cos g, sin b sin g, cos b):
+ * the extended T1/T2 model with 3x3 rate tensors (kernel/utilities/
+ * rlx_t1_t2.m) and anisotropic damping (relaxation.m, 'damp'); both are
+ * diagonal. Per orientation
+ *   R(Omega) = R0 + diag(g0 + sum_p q_p(Omega) g_p),
+ *   q = (ox^2, oy^2, oz^2, 2 ox oy, 2 ox oz, 2 oy oz),
+ * with R0 (any sparse matrix, e.g. orientation-independent theories) and the
+ * seven vectors g0..g6 of the full dimension in `diag` (7*dim doubles, g_p at
+ * p*dim). Either member may be NULL. */
+typedef struct sp_relax {
+  const sp_csc* R0;
+  const double* diag;
+} sp_relax;
+
+/* sp_plan_powder with a relaxation superoperator (rlx may be NULL: then the
+ * same as sp_plan_powder). The generator is not Hermitian, so such systems take
+ * the propagator path in complex arithmetic (reduced dimension <= 1024,
+ * SP_EINVAL with SP_MODE_KRYLOV); R0's pattern joins the union pattern of the
+ * coil-connected reduction. Replaces: the relaxation part of the parfor body
+ * of powder.m:262-269. */
+sp_plan* sp_plan_powder_relax(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                              const double* betas, const double* gammas, const double* weights,
+                              int64_t n_or, const double* rho0, const double* coil, double dt,
+                              int64_t nsteps, const sp_tols* tols, int mode, const sp_relax* rlx);
+
 /* Runs the propagation kernels and the orientation sum on the device and
  * waits for them. Replaces: the parfor of powder.m:226-274 and the sum of
  * powder.m:280-286. */

@@ -11,7 +11,11 @@ refcpu_powder  C++ FP64 (oracle/refcpu.cpp): faithful per-orientation
 np_powder      numpy/scipy FP64: dense assembly, union reduction,
                scipy.linalg.expm (Al-Mohy & Higham 2009), sequential loop.
 closed_form_2spin  exact FID of the secular two-spin heteronuclear system
-               (C1): f(t) = 1/4 exp(-i w_I t) cos(a t / 2) per orientation.
+               (C1): f(t) = 1/4 exp(-i w_I t) cos(a t / 2) per orientation,
+               and its form under T1/T2 relaxation and damping.
+np_powder_relax  numpy/scipy FP64 with an orientation-dependent relaxation
+               superoperator (relax_diag_at: the t1_t2 and damp theories of
+               kernel/relaxation.m), L = H + iR.
 
 Reference files: kernel/contexts/powder.m:226-286, kernel/orientation.m:29-59,
 kernel/utilities/wigner.m:39-89, experiments/acquire.m:42-73,
@@ -43,6 +47,9 @@ DEFAULT_TOLS = (1e-10, 1e-24, 1e-10, 200.0, 10000.0, 1e-10, 0.5, 32.0, 1000.0)
 # Planted mistakes of the C++ restatement (refcpu.cpp, enum Plant): the tests
 # use them to show that each check of the oracle fails on a plausible mistake.
 PLANT_STEP_SIGN, PLANT_STEP_TOL, PLANT_DROP_BIN, PLANT_DUP_BOX, PLANT_KRYLOV_SHIFT = 1, 2, 3, 4, 5
+# planted mistakes of the relaxation restatement: R with the wrong sign (growth), and the
+# orientation vector taken as the third column of the rotation instead of its third row
+PLANT_RLX_SIGN, PLANT_RLX_ORT = 11, 12
 
 
 def make_tols(**over):
@@ -281,6 +288,79 @@ def np_powder(I, qlist, grid, rho0, coil, dt, nsteps, reduce=True, per_orientati
     return (fid, per) if per_orientation else fid
 
 
+def _dcm(a, b, g):
+    """euler2dcm.m:40-58: Rz(a) Ry(b) Rz(g), active ZYZ."""
+    ca, sa, cb, sb, cg, sg = math.cos(a), math.sin(a), math.cos(b), math.sin(b), math.cos(g), math.sin(g)
+    ra = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1.0]])
+    rb = np.array([[cb, 0, sb], [0, 1.0, 0], [-sb, 0, cb]])
+    rg = np.array([[cg, -sg, 0], [sg, cg, 0], [0, 0, 1.0]])
+    return ra @ rb @ rg
+
+
+def _rate(spec, euler, plant):
+    if np.isscalar(spec):
+        return float(spec)
+    R = _dcm(*euler)
+    ort = R[:, 2] if plant == PLANT_RLX_ORT else R[2, :]   # [0 0 1]*R: the third row
+    return float(ort @ np.asarray(spec) @ ort)
+
+
+def relax_diag_at(spec, states, euler, plant=0):
+    """Diagonal of R(Omega) in the sphten-liouv basis `states` (per state and
+    spin: 0 unit, 1/3 transverse T1,+-1, 2 longitudinal T1,0) for the t1_t2
+    theory (kernel/utilities/rlx_t1_t2.m: each non-unit spin adds its R1 when
+    longitudinal and its R2 when transverse, rate = ort*T*ort' for a tensor,
+    R = -diag(rates)) and the damp theory (kernel/relaxation.m: -rate on every
+    state but the unit state). spec: {'r1': [...], 'r2': [...], 'damp': x}
+    with scalars or 3x3 tensors (None where absent)."""
+    states = np.asarray(states)
+    rates = np.zeros(len(states))
+    if spec.get('r1') is not None:
+        for n in range(states.shape[1]):
+            r1 = _rate(spec['r1'][n], euler, plant)
+            r2 = _rate(spec['r2'][n], euler, plant)
+            for i in range(len(states)):
+                if states[i, n] == 2:
+                    rates[i] += r1
+                elif states[i, n] in (1, 3):
+                    rates[i] += r2
+    if spec.get('damp') is not None:
+        d = _rate(spec['damp'], euler, plant)
+        for i in range(len(states)):
+            if np.any(states[i] != 0):
+                rates[i] += d
+    return rates if plant == PLANT_RLX_SIGN else -rates
+
+
+def np_powder_relax(I, qlist, grid, rho0, coil, dt, nsteps, spec, states, reduce=True, per_orientation=False,
+                    plant=0):
+    """np_powder with L = H + iR(Omega) at every orientation (kernel/contexts/
+    powder.m:263, kernel/evolution.m: L = H + 1i*R), R from relax_diag_at:
+    P = expm(-i dt L), f(t) = coil' P^t rho0, weighted sum in grid order.
+    R is diagonal, so the union reduction of np_powder stays exact."""
+    st = union_states(I, qlist, coil) if reduce else np.arange(I.shape[0])
+    n_or = len(grid['weights'])
+    npts = int(nsteps) + 1
+    per = np.zeros((n_or, npts), complex)
+    Ir = I[st][:, st]
+    qr = [None if q is None else q[st][:, st] for q in qlist]
+    for o in range(n_or):
+        eul = (grid['alphas'][o], grid['betas'][o], grid['gammas'][o])
+        h = assemble_dense(Ir, qr, eul)
+        r_diag = relax_diag_at(spec, np.asarray(states)[st], eul, plant)
+        P = sla.expm(-1j * dt * (h + 1j * np.diag(r_diag)))
+        r = np.asarray(rho0)[st].astype(complex)
+        c = np.asarray(coil)[st].astype(complex)
+        for t in range(npts):
+            per[o, t] = np.vdot(c, r)
+            r = P @ r
+    fid = np.zeros(npts, complex)
+    fid[:] = grid['weights'][0] * per[0]
+    for o in range(1, n_or):
+        fid = fid + grid['weights'][o] * per[o]
+    return (fid, per) if per_orientation else fid
+
+
 def _chop(x, tol):
     """clean_up(): tol*round(x/tol) on the real and imaginary parts (clean_up.m:70)."""
     if tol <= 0:
@@ -336,13 +416,19 @@ def np_powder_chop(I, qlist, grid, rho0, coil, dt, nsteps, tols=DEFAULT_TOLS):
     return fid
 
 
-def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False, t0=0.0):
+def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False, t0=0.0, relax=None):
     """Exact FID of the secular heteronuclear two-spin system with rho0 = Lx
     and coil = L+ on spin 0: f(t) = 1/4 exp(-i w0 t) cos(a t / 2), with
     w0 = tr(Z)/3 - basefrq + (R (Z - tr(Z)/3) R^T)_zz (offset chemical shift of spin 0) and
     a = (R C R^T)_zz summed over both stored coupling directions, where
     R = euler2dcm(alpha, beta, gamma). t0 shifts the time origin (a dead time:
-    the acquisition starts from the state at t0)."""
+    the acquisition starts from the state at t0).
+
+    relax: {'r1': [..], 'r2': [..], 'damp': x} (relax_diag_at's spec): the
+    detected coherence I+ (rate rA = R2 of spin 0, plus damping) exchanges with
+    2 I+ Sz (rate rB = rA + R1 of spin 1) at a/2, so the amplitude is
+    [exp(t N)]_11 e^{-(rA+rB)t/2} with N = [[-d, -ia/2], [-ia/2, d]],
+    d = (rA - rB)/2: cosh(k t) - d sinh(k t)/k, k = sqrt(d^2 - a^2/4)."""
     from spinach_amd.spinsys import euler2dcm  # host setup data only
     n_or = len(grid['weights'])
     t = t0 + np.arange(int(nsteps) + 1) * dt
@@ -359,7 +445,19 @@ def closed_form_2spin(ss, grid, dt, nsteps, per_orientation=False, t0=0.0):
             cm = ss['coupling'][l][s]
             if cm is not None:
                 a += (R @ cm @ R.T)[2, 2]
-        per[o] = 0.25 * np.exp(-1j * w0 * t) * np.cos(a * t / 2)
+        if relax is None:
+            per[o] = 0.25 * np.exp(-1j * w0 * t) * np.cos(a * t / 2)
+        else:
+            eul = (grid['alphas'][o], grid['betas'][o], grid['gammas'][o])
+            damp = _rate(relax['damp'], eul, 0) if relax.get('damp') is not None else 0.0
+            r2 = _rate(relax['r2'][0], eul, 0) if relax.get('r2') is not None else 0.0
+            r1 = _rate(relax['r1'][1], eul, 0) if relax.get('r1') is not None else 0.0
+            ra = r2 + damp
+            rb = ra + r1
+            d = 0.5 * (ra - rb)
+            k = np.sqrt(complex(d * d - a * a / 4))
+            amp = np.cosh(k * t) - (d * np.sinh(k * t) / k if abs(k) > 0 else d * t)
+            per[o] = 0.25 * np.exp(-1j * w0 * t) * np.exp(-0.5 * (ra + rb) * t) * amp
     fid = (np.asarray(grid['weights'])[:, None] * per).sum(axis=0)
     return (fid, per) if per_orientation else fid
 

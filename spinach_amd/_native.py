@@ -26,7 +26,8 @@ SP_NQ = 34
 EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan_powder',
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
            'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real', 'sp_plan_set_dead_time',
-           'sp_powder_fid_noalloc', 'sp_plan_autotune', 'sp_plan_get_tuning', 'sp_plan_set_tuning')
+           'sp_powder_fid_noalloc', 'sp_plan_autotune', 'sp_plan_get_tuning', 'sp_plan_set_tuning',
+           'sp_plan_powder_relax')
 SP_MAS_FRAME_MAGNET = 0
 SP_MAS_FRAME_ROTOR = 1
 
@@ -34,6 +35,10 @@ SP_MAS_FRAME_ROTOR = 1
 class SpCsc(C.Structure):
     _fields_ = [('n', C.c_int64), ('jc', C.POINTER(C.c_uint64)), ('ir', C.POINTER(C.c_uint64)),
                 ('pr', C.POINTER(C.c_double)), ('is_complex', C.c_int32)]
+
+
+class SpRelax(C.Structure):
+    _fields_ = [('R0', C.POINTER(SpCsc)), ('diag', C.POINTER(C.c_double))]
 
 
 class SpTols(C.Structure):
@@ -103,6 +108,8 @@ def load():
     lib.sp_plan_powder.restype = C.c_void_p
     lib.sp_plan_powder.argtypes = [C.c_void_p, C.POINTER(SpCsc), C.POINTER(SpCsc), _dp, _dp, _dp, _dp, C.c_int64,
                                    _dp, _dp, C.c_double, C.c_int64, C.POINTER(SpTols), C.c_int]
+    lib.sp_plan_powder_relax.restype = C.c_void_p
+    lib.sp_plan_powder_relax.argtypes = lib.sp_plan_powder.argtypes + [C.POINTER(SpRelax)]
     lib.sp_execute.restype = C.c_int
     lib.sp_execute.argtypes = [C.c_void_p, C.POINTER(SpStats)]
     lib.sp_plan_result.restype = C.c_int
@@ -341,7 +348,9 @@ def generator_is_real(I, qlist):
     return bool(out.value)
 
 
-def plan_powder(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', device=0):
+def plan_powder(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', device=0, relax=None):
+    """relax: None, or a dict with 'R0' (sparse dim x dim, or None) and 'diag'
+    (7 x dim array, or None): the relaxation superoperator of sp_plan_powder_relax."""
     lib = load()
     ctx = context(device)
     k = _Keep()
@@ -350,10 +359,24 @@ def plan_powder(I, qlist, grid, rho0, coil, dt, nsteps, tols=None, mode='fp64', 
     Qa = k.qarray(qlist)
     n_or = len(grid['weights'])
     t = _tols(tols)
-    h = lib.sp_plan_powder(ctx, C.byref(Ic), Qa, *k.grid(grid), k.cvec(rho0, dim), k.cvec(coil, dim),
-                           float(dt), int(nsteps), C.byref(t), mode_id(mode))
+    args = (ctx, C.byref(Ic), Qa, *k.grid(grid), k.cvec(rho0, dim), k.cvec(coil, dim), float(dt), int(nsteps),
+            C.byref(t), mode_id(mode))
+    if relax is None:
+        h = lib.sp_plan_powder(*args)
+    else:
+        r = SpRelax()
+        if relax.get('R0') is not None:
+            r0 = k.csc(relax['R0'])
+            k.refs.append(r0)
+            r.R0 = C.pointer(r0)
+        if relax.get('diag') is not None:
+            g = np.asarray(relax['diag'], np.float64)
+            if g.shape != (7, dim):
+                raise ValueError(f'relaxation diagonals must have shape (7, {dim}), got {g.shape}')
+            r.diag = k.dptr(g)
+        h = lib.sp_plan_powder_relax(*args, C.byref(r))
     if not h:
-        _err(ctx, -1, 'sp_plan_powder')
+        _err(ctx, -1, 'sp_plan_powder_relax' if relax is not None else 'sp_plan_powder')
     return Plan(h, ctx, int(nsteps) + 1, n_or)
 
 

@@ -122,6 +122,8 @@ def singlerot(spin_system, pulse_sequence, parameters, assumptions):
     if not np.isscalar(parameters['rate']) or not np.isreal(parameters['rate']):
         raise ValueError('parameters.rate must be a real number.')
     P._grumble(parameters)
+    if spin_system.get('rlx') is not None:
+        raise NotImplementedError('relaxation in the Fokker-Planck MAS context is not supported on this path')
     ss = S.assume(spin_system, assumptions)
     I, Q = S.hamiltonian(ss)
     I = S.frqoffset(ss, I, parameters.get('offset'))

@@ -99,6 +99,11 @@ struct SysPlan {
   std::vector<double> rho, coil;
   std::vector<double> alphas, betas, gammas, weights;
   bool real = false;  // generator real at every orientation
+  // relaxation superoperator (sp_relax): R0 on the union pattern (interleaved complex,
+  // nnz entries; empty without R0) and the seven diagonal planes on the reduced
+  // states (7*n; empty without them)
+  bool relax = false;
+  std::vector<double> rvals, rdiag;
 };
 
 // Partner slot and sign of the conjugation symmetry of the Wigner matrices:
@@ -146,7 +151,8 @@ inline bool generator_is_real(const std::vector<int32_t>& slots, const std::vect
 }
 
 inline SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, const double* be, const double* ga,
-                    const double* w, int64_t n_or, const double* rho0, const double* coil) {
+                    const double* w, int64_t n_or, const double* rho0, const double* coil,
+                    const sp_relax* rlx = nullptr) {
   if (!I) throw SpError(SP_EINVAL, "I is NULL");
   if (!Q) throw SpError(SP_EINVAL, "Q is NULL");
   if (!al || !be || !ga || !w) throw SpError(SP_EINVAL, "grid arrays must not be NULL");
@@ -167,6 +173,14 @@ inline SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, c
       throw SpError(SP_EINVAL, "non-finite grid entry");
   for (int64_t i = 0; i < 2 * dim; i++)
     if (!std::isfinite(rho0[i]) || !std::isfinite(coil[i])) throw SpError(SP_EINVAL, "non-finite state vector");
+  // relaxation: R0 joins the union pattern (mats[1 + SP_NQ]) without a value plane of its own in vals
+  Triplets r0;
+  const bool have_diag = rlx && rlx->diag;
+  if (rlx && rlx->R0) r0 = read_csc(rlx->R0, dim, "R0");
+  if (have_diag)
+    for (int64_t i = 0; i < 7 * dim; i++)
+      if (!std::isfinite(rlx->diag[i])) throw SpError(SP_EINVAL, "non-finite relaxation diagonal");
+  mats.push_back(r0);
 
   // union adjacency (symmetrised) and the coil-connected component
   std::vector<std::vector<int64_t>> adj(dim);
@@ -251,6 +265,14 @@ inline SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, c
       sp.slots.push_back(q);
       sp.vals.insert(sp.vals.end(), tmp.begin(), tmp.end());
     }
+  if (scatter(mats[1 + SP_NQ], tmp)) sp.rvals = tmp;
+  if (have_diag) {
+    sp.rdiag.assign(7 * (size_t)n, 0.0);
+    for (int64_t i = 0; i < dim; i++)
+      if (map[i] >= 0)
+        for (int p = 0; p < 7; p++) sp.rdiag[(size_t)p * n + map[i]] = rlx->diag[(size_t)p * dim + i];
+  }
+  sp.relax = !sp.rvals.empty() || have_diag;
   sp.rho.assign(2 * (size_t)n, 0.0);
   sp.coil.assign(2 * (size_t)n, 0.0);
   for (int64_t i = 0; i < dim; i++)
@@ -260,7 +282,8 @@ inline SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, c
       sp.coil[2 * map[i]] = coil[2 * i];
       sp.coil[2 * map[i] + 1] = coil[2 * i + 1];
     }
-  sp.real = generator_is_real(sp.slots, sp.vals, (size_t)sp.nnz);
+  // L = H + iR is not Hermitian: no real-generator (spectral, real-product) path
+  sp.real = !sp.relax && generator_is_real(sp.slots, sp.vals, (size_t)sp.nnz);
   sp.alphas.assign(al, al + n_or);
   sp.betas.assign(be, be + n_or);
   sp.gammas.assign(ga, ga + n_or);

@@ -59,6 +59,8 @@ struct SysDesc {
   int64_t n_or;      // number of orientations
   int32_t real;      // 1: the generator is real at every orientation (real-product path)
   int32_t rb[13];    // rb[k]: first pattern entry of row 16k (the pattern is sorted by row, then column)
+  int64_t rlx_off;   // relaxation R0 on the union pattern: offset (complex elements) into rvals, -1: none
+  int64_t rdiag_off; // relaxation diagonal planes g0..g6 (7*n doubles) in rdiag, -1: none
 };
 
 struct Item {
@@ -98,7 +100,24 @@ struct KParams {
   int32_t np_max;
   double* per_or;    // per-orientation FIDs: interleaved complex FP64, (nsteps+1) per orientation
   int32_t* sq_out;   // per orientation: scaling exponent used (diagnostics)
+  const double* rvals;  // relaxation R0 values (interleaved complex), see SysDesc::rlx_off
+  const double* rdiag;  // relaxation diagonal planes, see SysDesc::rdiag_off
 };
+
+// q_p(Omega) of the relaxation rate tensors: the quadratic monomials (ox^2, oy^2,
+// oz^2, 2 ox oy, 2 ox oz, 2 oy oz) of ort = [0 0 1]*euler2dcm(alpha, beta, gamma)
+// = (-sin(beta) cos(gamma), sin(beta) sin(gamma), cos(beta)) (kernel/utilities/
+// rlx_t1_t2.m, kernel/relaxation.m 'damp': rate = ort*T*ort')
+__host__ __device__ inline void relax_monomials(double beta, double gamma, double q[6]) {
+  const double sb = sin(beta), cb = cos(beta), sg = sin(gamma), cg = cos(gamma);
+  const double ox = -sb * cg, oy = sb * sg, oz = cb;
+  q[0] = ox * ox;
+  q[1] = oy * oy;
+  q[2] = oz * oz;
+  q[3] = 2.0 * ox * oy;
+  q[4] = 2.0 * ox * oz;
+  q[5] = 2.0 * oy * oz;
+}
 
 // zeroes the kQueues work-queue heads of a plan: [0] propagator items with a
 // complex generator, [1] tridiagonalisation, [2] synthesis, [3] propagator
@@ -719,12 +738,33 @@ __global__ void __launch_bounds__(kThreads, 2) k_powder(KParams<R> p) {
         if constexpr (REAL) {
           X[off] = (R)(p.dt * hr);  // A = dt*H; the imaginary parts cancel pairwise
         } else {
-          X[off] = (R)(p.dt * hi);
-          X[pl + off] = (R)(-p.dt * hr);
+          // L = H + i R: X = -i*dt*L = -i*dt*H + dt*R (R is not symmetrised)
+          double rr = 0.0, ri = 0.0;
+          if (sd.rlx_off >= 0) {
+            rr = p.rvals[2 * (sd.rlx_off + e)];
+            ri = p.rvals[2 * (sd.rlx_off + e) + 1];
+          }
+          X[off] = (R)(p.dt * (hi + rr));
+          X[pl + off] = (R)(p.dt * (ri - hr));
         }
       }
     }
     __syncthreads();
+    if constexpr (!REAL) {
+      // the diagonal relaxation terms of this orientation: R_ii += g0 + sum_p q_p g_p
+      if (sd.rdiag_off >= 0) {
+        double q[6];
+        relax_monomials(p.betas[o], p.gammas[o], q);
+        const double* g = p.rdiag + sd.rdiag_off;
+        for (int i = tid; i < sd.n; i += kThreads) {
+          double r = g[i];
+          for (int k = 0; k < 6; k++) r = fma(q[k], g[(size_t)(k + 1) * sd.n + i], r);
+          const size_t off = (size_t)i * np + i;
+          X[off] = (R)((double)X[off] + p.dt * r);
+        }
+        __syncthreads();
+      }
+    }
     // 3. FP64 1-norm (max column sum) and scaling exponent
     {
       double colmax = 0.0;

@@ -153,6 +153,8 @@ struct sp_plan {
   int64_t n_chunks = 0;
   DevBuf d_chunks, d_chunk_off, d_partial;
   DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
+  int32_t relax_systems = 0;  // systems with a relaxation superoperator (sp_plan_powder_relax)
+  DevBuf d_rvals, d_rdiag;
   DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
   bool executed = false;
   bool graph = false;                // SP_MODE_GRAPH: the pass is recorded once and replayed
@@ -309,6 +311,9 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   int np_prop = 16, np_spec = 16;
   for (size_t s = 0; s < sps.size(); s++) {
     krylov[s] = (force_krylov || (double)sps[s].n >= tl.krylov_tol || sps[s].np > kDenseMaxDim) ? 1 : 0;
+    if (sps[s].relax && krylov[s])
+      throw SpError(SP_EINVAL, "a relaxation superoperator runs on the propagator path only (reduced dimension <= " +
+                                   std::to_string(kDenseMaxDim) + ", below krylov_tol, no SP_MODE_KRYLOV)");
     spectral[s] =
         (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriNP) ? 1 : 0;
     if (spectral[s])
@@ -335,7 +340,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   std::vector<spk::StepDesc> sdescs;
   std::vector<int32_t> sell_start, sell_col, sell_pos;
   std::vector<int32_t> row, col, tpos, slots;
-  std::vector<double> vals, rho, coil, al, be, ga, w;
+  std::vector<double> vals, rho, coil, al, be, ga, w, rvals, rdiag;
   // queue order: longest-processing-time first within each path (stable within a system)
   std::vector<std::pair<double, spk::Item>> prop_items, spec_items, kry_items;
   int32_t nmax = 1;
@@ -354,6 +359,17 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     d.or_off = or_base;
     d.n_or = (int64_t)sp.weights.size();
     d.real = (sp.real && !force_complex) ? 1 : 0;
+    d.rlx_off = -1;
+    d.rdiag_off = -1;
+    if (!sp.rvals.empty()) {
+      d.rlx_off = (int64_t)rvals.size() / 2;
+      rvals.insert(rvals.end(), sp.rvals.begin(), sp.rvals.end());
+    }
+    if (!sp.rdiag.empty()) {
+      d.rdiag_off = (int64_t)rdiag.size();
+      rdiag.insert(rdiag.end(), sp.rdiag.begin(), sp.rdiag.end());
+    }
+    pl->relax_systems += sp.relax ? 1 : 0;
     for (int k = 0; k < 13; k++)
       d.rb[k] = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 16 * k) - sp.row.begin());
     pl->real_systems += d.real;
@@ -445,6 +461,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   upload(pl->d_sell_start, sell_start, st);
   upload(pl->d_sell_col, sell_col, st);
   upload(pl->d_sell_pos, sell_pos, st);
+  if (!rvals.empty()) upload(pl->d_rvals, rvals, st);
+  if (!rdiag.empty()) upload(pl->d_rdiag, rdiag, st);
   pl->rho_or_stride = nmax;
   pl->step_vec_words = 8 * ((nmax + spk::kSell - 1) / spk::kSell * spk::kSell);
   pl->d_nsub.alloc((size_t)std::max<int64_t>(1, or_base) * sizeof(int32_t));
@@ -517,6 +535,8 @@ void launch(sp_plan* pl) {
   kp.np_max = pl->np_prop;
   kp.per_or = pl->d_per.as<double>();
   kp.sq_out = pl->d_sq.as<int32_t>();
+  kp.rvals = pl->d_rvals.as<double>();
+  kp.rdiag = pl->d_rdiag.as<double>();
   // complex-generator items first (queue 0), then real-generator items (queue 3)
   const int64_t n_cplx = pl->n_prop_complex, n_real = pl->n_prop - pl->n_prop_complex;
   if (n_cplx > 0) {
@@ -1041,6 +1061,8 @@ int noalloc_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al,
   const int64_t npts = nsteps + 1;
   Arena a{static_cast<unsigned char*>(ctx->scratch), kScratchBytes};
   spk::SysDesc d{};
+  d.rlx_off = -1;
+  d.rdiag_off = -1;
   d.n = sp.n;
   d.np = sp.np;
   d.nnz = sp.nnz;
@@ -1410,6 +1432,29 @@ sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   return nullptr;
 }
 
+sp_plan* sp_plan_powder_relax(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                              const double* betas, const double* gammas, const double* weights, int64_t n_or,
+                              const double* rho0, const double* coil, double dt, int64_t nsteps, const sp_tols* tols,
+                              int mode, const sp_relax* rlx) {
+  if (!ctx) return nullptr;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    const Tols tl = read_tols(tols);
+    std::vector<SysPlan> sps;
+    sps.push_back(plan_system(I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, rlx));
+    sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode, tl);
+    pl->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return pl;
+  } catch (SpError& e) {
+    fail(ctx, e.code, e.what());
+  } catch (std::bad_alloc&) {
+    fail(ctx, SP_ENOMEM, "host allocation failed");
+  } catch (std::exception& e) {
+    fail(ctx, SP_EDEVICE, e.what());
+  }
+  return nullptr;
+}
+
 sp_plan* sp_plan_powder_batch(sp_ctx* ctx, const sp_system* systems, int64_t n_systems, double dt, int64_t nsteps,
                               const sp_tols* tols, int mode) {
   if (!ctx) return nullptr;
@@ -1490,6 +1535,8 @@ int sp_plan_set_dead_time(sp_plan* plan, double dead_time) {
   if (!plan) return SP_EINVAL;
   try {
     if (plan->mas) throw SpError(SP_EINVAL, "a dead time is not available on MAS plans");
+    if (plan->relax_systems > 0 && dead_time > 0)
+      throw SpError(SP_EINVAL, "a dead time is not available with a relaxation superoperator");
     if (!std::isfinite(dead_time) || dead_time < 0) throw SpError(SP_EINVAL, "dead_time must be finite and >= 0");
     HIPCHK(hipSetDevice(plan->ctx->device));
     if (dead_time > 0) {

@@ -108,8 +108,10 @@ def _acquire_on_grid(ss, I, qlist, parameters):
     mode = parameters.get('mode', 'fp64')
     if 'krylov' in ss.get('enable', []):
         mode = mode + '+krylov'
+    # R(Omega) at each orientation (powder.m:263) in the library's form
+    relax = S.relaxation_terms(ss)
     plan = _native.plan_powder(I, qlist, grid, parameters['rho0'], parameters['coil'], dt, nsteps,
-                               tols=ss['tols'], mode=mode)
+                               tols=ss['tols'], mode=mode, relax=relax)
     try:
         if parameters.get('dead_time'):
             plan.set_dead_time(parameters['dead_time'])

@@ -243,14 +243,135 @@ def create(sys, inter):
                 d = d - np.eye(3) * np.trace(d) / 3
                 _add(n, k, d)
 
+    rlx = _relaxation_spec(inter, nspins)
+
     enable = list(sys.get('enable', []))  # sys.enable switches (create.m:1556); 'krylov' is read by powder()
     for e in enable:
         if e not in ('krylov', 'gpu', 'ham_cache'):
             raise ValueError(f'sys.enable option {e!r} is not supported on this path')
     return dict(isotopes=isotopes, nspins=nspins, magnet=magnet, gammas=gammas, enable=enable,
-                basefrqs=basefrqs, zeeman=zmat, coupling=coup, tols=tols,
+                basefrqs=basefrqs, zeeman=zmat, coupling=coup, tols=tols, rlx=rlx,
                 zeeman_strength=['full'] * nspins,
                 coupling_strength=[['strong'] * nspins for _ in range(nspins)])
+
+
+# -------------------------------------------------------------- relaxation
+_RLX_THEORIES = ('t1_t2', 'damp')
+_RLX_KEEP = ('diagonal', 'kite', 'secular', 'labframe')
+
+
+def _rate_spec(v, what):
+    """A relaxation rate: a real scalar, or a real 3x3 tensor whose rate at an
+    orientation is ort*T*ort' (rlx_t1_t2.m, relaxation.m 'damp')."""
+    a = np.asarray(v, dtype=complex)
+    if a.shape == () and np.isfinite(a) and a.imag == 0:
+        return float(a.real)
+    if a.shape == (3, 3) and np.all(np.isfinite(a)) and np.all(a.imag == 0):
+        return a.real.copy()
+    raise ValueError(f'{what} must be a real scalar or a real 3x3 matrix.')
+
+
+def _relaxation_spec(inter, nspins):
+    """inter.relaxation, inter.r1_rates, inter.r2_rates, inter.damp_rate,
+    inter.rlx_keep, inter.equilibrium (kernel/create.m:1134-1300). Returns None
+    without relaxation theories."""
+    theories = list(inter.get('relaxation', []))
+    if not theories:
+        return None
+    for t in theories:
+        if t not in _RLX_THEORIES:
+            raise NotImplementedError(f'relaxation theory {t!r} is not supported on this path')
+    keep = inter.get('rlx_keep')
+    if keep not in _RLX_KEEP:
+        raise ValueError('inter.rlx_keep must be one of ' + ', '.join(_RLX_KEEP) + '.')
+    eq = inter.get('equilibrium', 'zero')
+    if eq != 'zero':
+        raise NotImplementedError("only inter.equilibrium = 'zero' (relaxation to the unit state) is supported")
+    spec = dict(theories=theories, keep=keep, equilibrium=eq, r1=None, r2=None, damp=None)
+    if 't1_t2' in theories:
+        for f in ('r1_rates', 'r2_rates'):
+            if f not in inter or len(inter[f]) != nspins:
+                raise ValueError(f'inter.{f} must have one entry per spin for the t1_t2 theory.')
+        spec['r1'] = [_rate_spec(v, f'R1 rate of spin {n}') for n, v in enumerate(inter['r1_rates'])]
+        spec['r2'] = [_rate_spec(v, f'R2 rate of spin {n}') for n, v in enumerate(inter['r2_rates'])]
+    if 'damp' in theories:
+        if 'damp_rate' not in inter:
+            raise ValueError('inter.damp_rate must be specified for the damp theory.')
+        spec['damp'] = _rate_spec(inter['damp_rate'], 'inter.damp_rate')
+    return spec
+
+
+def _ort(euler):
+    """ort = [0 0 1]*euler2dcm(alpha, beta, gamma) (rlx_t1_t2.m, relaxation.m)."""
+    return np.array([0.0, 0.0, 1.0]) @ euler2dcm(*euler)
+
+
+def _rate_at(spec, euler):
+    if np.isscalar(spec):
+        return spec
+    if euler is None:
+        raise ValueError('Euler angles must be specified with anisotropic relaxation rates.')
+    o = _ort(euler)
+    return float(o @ spec @ o)
+
+
+def relaxation(ss, euler=None):
+    """The relaxation superoperator R at one orientation (kernel/relaxation.m
+    for the t1_t2 and damp theories, sphten-liouv; zero without theories), as
+    a sparse diagonal matrix: state i decays at the sum over its non-unit spins
+    of R1 (longitudinal, T_{1,0}) or R2 (transverse, T_{1,+-1}) of that spin
+    (rlx_t1_t2.m), plus the damping rate on every state but the unit state."""
+    states = ss['basis']
+    if states is None:
+        raise NotImplementedError('relaxation is only available in the sphten-liouv formalism')
+    dim = len(states)
+    rates = np.zeros(dim)
+    spec = ss.get('rlx')
+    if spec is not None:
+        if spec['r1'] is not None:
+            r1 = np.array([_rate_at(v, euler) for v in spec['r1']])
+            r2 = np.array([_rate_at(v, euler) for v in spec['r2']])
+            lon = states == 2
+            tra = (states == 1) | (states == 3)
+            rates += lon.astype(float) @ r1 + tra.astype(float) @ r2
+        if spec['damp'] is not None:
+            unit = np.all(states == 0, axis=1)
+            rates += np.where(unit, 0.0, _rate_at(spec['damp'], euler))
+    return sps.diags(-rates, 0, shape=(dim, dim), format='csc')
+
+
+def relaxation_terms(ss):
+    """The library's form of relaxation(ss, euler) for every orientation
+    (sp_relax): R(Omega) = diag(g0 + sum_p q_p(Omega) g_p) with the monomials
+    q = (ox^2, oy^2, oz^2, 2 ox oy, 2 ox oz, 2 oy oz) of ort; a tensor rate T
+    contributes (Txx, Tyy, Tzz, (Txy+Tyx)/2, (Txz+Tzx)/2, (Tyz+Tzy)/2) to
+    g1..g6, a scalar rate to g0. Returns None without relaxation theories."""
+    spec = ss.get('rlx')
+    if spec is None:
+        return None
+    states = ss['basis']
+    if states is None:
+        raise NotImplementedError('relaxation is only available in the sphten-liouv formalism')
+    dim = len(states)
+    g = np.zeros((7, dim))
+
+    def add(mask, rate):
+        if np.isscalar(rate):
+            g[0, mask] -= rate
+        else:
+            t = rate
+            comps = (t[0, 0], t[1, 1], t[2, 2], 0.5 * (t[0, 1] + t[1, 0]), 0.5 * (t[0, 2] + t[2, 0]),
+                     0.5 * (t[1, 2] + t[2, 1]))
+            for p, cval in enumerate(comps):
+                g[1 + p, mask] -= cval
+
+    if spec['r1'] is not None:
+        for n in range(ss['nspins']):
+            add(states[:, n] == 2, spec['r1'][n])
+            add((states[:, n] == 1) | (states[:, n] == 3), spec['r2'][n])
+    if spec['damp'] is not None:
+        add(~np.all(states == 0, axis=1), spec['damp'])
+    return {'R0': None, 'diag': g}
 
 
 def assume(ss, assumptions):

@@ -52,7 +52,9 @@ def build(sys, inter, bas, channel='1H'):
     return dict(ss=ss, I=I, Q=S.q_list(Q), Qnested=Q, rho0=rho0, coil=coil)
 
 
-def static_2spin(seed=0, npoints=2048, n_orient=256):
+def static_2spin(seed=0, npoints=2048, n_orient=256, relax=None):
+    """relax: optional relaxation fields of inter (relaxation, r1_rates,
+    r2_rates, damp_rate, rlx_keep, equilibrium), as in kernel/create.m."""
     rng = np.random.default_rng(seed)
     sys = {'magnet': 14.1, 'isotopes': ['1H', '13C']}
     d = rng.normal(size=3)
@@ -61,18 +63,20 @@ def static_2spin(seed=0, npoints=2048, n_orient=256):
                         'euler': [_euler(rng) for _ in range(2)]},
              'coupling': {'scalar': {(0, 1): 140.0}},
              'coordinates': [np.zeros(3), d]}
+    inter.update(relax or {})
     w = build(sys, inter, {'formalism': 'sphten-liouv', 'approximation': 'none'})
     w.update(grid=grids.zcwn(n_orient), dt=1e-5, nsteps=npoints - 1, name='static_2spin')
     return w
 
 
-def static_8spin(seed=1, nsteps=4096, n_orient=4000):
+def static_8spin(seed=1, nsteps=4096, n_orient=4000, relax=None):
     rng = np.random.default_rng(seed)
     iso = ['1H'] * 5 + ['13C'] * 2 + ['15N']
     sys = {'magnet': 14.1, 'isotopes': iso}
     inter = {'zeeman': {'eigs': [list(rng.uniform(-100, 100, 3)) for _ in iso],
                         'euler': [_euler(rng) for _ in iso]},
              'coordinates': _coords(rng, len(iso))}
+    inter.update(relax or {})
     w = build(sys, inter, {'formalism': 'sphten-liouv', 'approximation': 'IK-0', 'level': 3})
     w.update(grid=grids.haar(n_orient, rng), dt=5e-6, nsteps=nsteps, name='static_8spin')
     return w

@@ -44,7 +44,8 @@ def test_binding_declares_signatures():
 
 
 STRUCTS = {'sp_csc': _native.SpCsc, 'sp_tols': _native.SpTols, 'sp_stats': _native.SpStats,
-           'sp_system': _native.SpSystem, 'sp_rotor': _native.SpRotor, 'sp_tuning': _native.SpTuning}
+           'sp_system': _native.SpSystem, 'sp_rotor': _native.SpRotor, 'sp_tuning': _native.SpTuning,
+           'sp_relax': _native.SpRelax}
 
 
 def test_struct_layouts_match_header(tmp_path):

@@ -1,0 +1,94 @@
+"""Orientation-dependent relaxation on the GPU (sp_plan_powder_relax): the
+propagator path with L = H + iR(Omega) against the oracle's restatement
+(np_powder_relax), the closed form of the two-spin system, and through the
+host mirror powder()."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from spinach_amd import _native, grids, workloads
+from spinach_amd import powder as P
+from spinach_amd import spinsys as S
+
+pytestmark = pytest.mark.gpu
+
+T1 = np.array([[1.0, 0.3, -0.2], [0.3, 3.0, 0.5], [-0.2, 0.5, 5.0]]) * 40.0
+T2 = np.array([[2.0, -0.4, 0.6], [-0.4, 4.0, 0.1], [0.6, 0.1, 6.0]]) * 60.0
+TD = np.array([[1.0, 0.2, 0.0], [0.2, 2.0, -0.3], [0.0, -0.3, 1.5]]) * 25.0
+RELAX2 = {'relaxation': ['t1_t2', 'damp'], 'r1_rates': [4.0, T1], 'r2_rates': [T2, 9.0], 'damp_rate': TD,
+          'rlx_keep': 'secular'}
+
+
+def _run(w, grid, mode='fp64', per=False):
+    plan = _native.plan_powder(w['I'], w['Q'], grid, w['rho0'], w['coil'], w['dt'], w['nsteps'], mode=mode,
+                               relax=S.relaxation_terms(w['ss']))
+    try:
+        st = plan.execute()
+        out = plan.result(per_orientation=per)
+    finally:
+        plan.close()
+    return out, st
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'fp64+graph'])
+def test_two_spin_relaxation_vs_oracle_and_closed_form(mode):
+    w = workloads.static_2spin(npoints=1024, n_orient=64, relax=RELAX2)
+    g = grids.haar(64, np.random.default_rng(8))
+    (fid, per), st = _run(w, g, mode=mode, per=True)
+    print(f'[{mode}] stats', {k: st[k] for k in ('spectral_systems', 'real_systems', 'reduced_dim')}, flush=True)
+    assert st['spectral_systems'] == 0 and st['real_systems'] == 0
+    ref, rper = O.np_powder_relax(w['I'], w['Q'], g, w['rho0'], w['coil'], w['dt'], w['nsteps'], w['ss']['rlx'],
+                                  w['ss']['basis'], per_orientation=True)
+    cf = O.closed_form_2spin(w['ss'], g, w['dt'], w['nsteps'], relax=w['ss']['rlx'])
+    e, ec = O.rel_err(fid, ref), O.rel_err(fid, cf)
+    ep = max(O.rel_err(per[k], rper[k]) for k in range(0, 64, 7))
+    print(f'[{mode}] 2-spin relaxation 64 x 1023: vs oracle {e:.2e} (per orientation {ep:.2e}), '
+          f'vs closed form {ec:.2e}', flush=True)
+    assert e < 1e-10 and ep < 1e-10 and ec < 5e-9
+
+
+def test_eight_spin_relaxation_vs_oracle():
+    """185 coil-connected states, t1_t2 rates on every spin (tensors on the
+    protons), 4 orientations x 4096 steps; the oracle on the reduced matrices."""
+    iso = ['1H'] * 5 + ['13C'] * 2 + ['15N']
+    r1 = [T1 * 0.2 if i == '1H' else 0.7 for i in iso]
+    r2 = [T2 * 0.3 if i == '1H' else 15.0 for i in iso]
+    w = workloads.static_8spin(n_orient=4000, relax={'relaxation': ['t1_t2'], 'r1_rates': r1, 'r2_rates': r2,
+                                                     'rlx_keep': 'secular'})
+    sub = grids.subset(w['grid'], np.arange(4))
+    (fid, per), st = _run(w, sub, per=True)
+    assert st['reduced_dim'] == 185
+    ref, rper = O.np_powder_relax(w['I'], w['Q'], sub, w['rho0'], w['coil'], w['dt'], w['nsteps'], w['ss']['rlx'],
+                                  w['ss']['basis'], per_orientation=True)
+    f0 = O.np_powder(w['I'], w['Q'], sub, w['rho0'], w['coil'], w['dt'], w['nsteps'])
+    e = O.rel_err(fid, ref)
+    print(f'8-spin relaxation 4 x 4096: vs oracle {e:.2e}; relaxation changes the FID by {O.rel_err(f0, ref):.2f}',
+          flush=True)
+    assert e < 1e-10
+    assert O.rel_err(f0, ref) > 0.05
+    for k in range(4):
+        assert O.rel_err(per[k], rper[k]) < 1e-10
+
+
+def test_powder_host_mirror_applies_relaxation():
+    w = workloads.static_2spin(npoints=256, n_orient=16, relax=RELAX2)
+    params = {'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 256, 'grid': w['grid']}
+    fid, grid = P.powder(w['ss'], P.acquire, params, 'nmr')
+    cf = O.closed_form_2spin(w['ss'], grid, w['dt'], 255, relax=w['ss']['rlx'])
+    e = O.rel_err(fid, cf)
+    print(f'powder() with relaxation: vs closed form {e:.2e}', flush=True)
+    assert e < 5e-9
+
+
+def test_relaxation_refuses_the_paths_that_do_not_take_it():
+    w = workloads.static_2spin(npoints=64, n_orient=4, relax=RELAX2)
+    rl = S.relaxation_terms(w['ss'])
+    with pytest.raises(_native.SpinachPowderError, match='propagator path'):
+        _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'],
+                            mode='fp64+krylov', relax=rl)
+    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'], relax=rl)
+    try:
+        with pytest.raises(_native.SpinachPowderError, match='dead time'):
+            plan.set_dead_time(1e-5)
+    finally:
+        plan.close()
