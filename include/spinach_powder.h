@@ -253,6 +253,30 @@ void sp_plan_destroy(sp_plan* plan);
  * to every system of a batch plan. Not available on MAS plans. */
 int sp_plan_set_dead_time(sp_plan* plan, double dead_time);
 
+/* Per-orientation initial states of a single-system plan: rho_or holds n_or
+ * complex state vectors of the full dimension (interleaved, orientation-major);
+ * from the next sp_execute the acquisition of orientation o starts from its
+ * state instead of rho0 (NULL restores rho0). The plan's coil-connected
+ * reduction keeps what the coil can see: the parts of a state outside it never
+ * reach the coil under L. Replaces: the state handed from one stage of a pulse
+ * sequence to the next acquisition, as in experiments/sp_acquire.m (after the
+ * shaped pulse) and experiments/cpmg.m (after every refocusing pulse). Not
+ * with a dead time; not on MAS or batch plans. */
+int sp_plan_set_initial_states(sp_plan* plan, const double* rho_or);
+
+/* Propagates every orientation's state by nsteps steps of dt under its own
+ * generator, rho(Omega) <- step(L(Omega), rho, dt) nsteps times (kernel/step.m:
+ * 204-375, the reordered Taylor series with the substeps of its 1-norm; SP_ERANGE
+ * when one step needs more than 10000 substeps), from the plan's rho0 or the
+ * states of sp_plan_set_initial_states, and writes the results (n_or complex
+ * vectors of the full dimension, zero outside the reduced space) to rho_out.
+ * Reduction follows the plan's coil: a plan built to propagate states passes
+ * as its coil a vector whose support seeds every state the sequence can
+ * reach. Replaces: the per-orientation evolution(...,'final') and step() of
+ * experiments/cpmg.m:41-55 and kernel/pulses/shaped_pulse_af.m (its 'expv'
+ * branch). Not with a relaxation superoperator or a dead time. */
+int sp_plan_propagate(sp_plan* plan, double dt, int64_t nsteps, double* rho_out);
+
 /* ---- launch configuration and autotuning ------------------------------ */
 
 /* Launch configuration of a plan's kernels. Replaces: nothing in the

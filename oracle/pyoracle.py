@@ -13,6 +13,9 @@ np_powder      numpy/scipy FP64: dense assembly, union reduction,
 closed_form_2spin  exact FID of the secular two-spin heteronuclear system
                (C1): f(t) = 1/4 exp(-i w_I t) cos(a t / 2) per orientation,
                and its form under T1/T2 relaxation and damping.
+np_cpmg, np_sp_acquire  numpy/scipy FP64 restatements of experiments/cpmg.m
+               and experiments/sp_acquire.m (kernel/pulses/shaped_pulse_af.m)
+               at every orientation, dense exponentials, no reduction.
 np_powder_relax  numpy/scipy FP64 with an orientation-dependent relaxation
                superoperator (relax_diag_at: the t1_t2 and damp theories of
                kernel/relaxation.m), L = H + iR.
@@ -50,6 +53,8 @@ PLANT_STEP_SIGN, PLANT_STEP_TOL, PLANT_DROP_BIN, PLANT_DUP_BOX, PLANT_KRYLOV_SHI
 # planted mistakes of the relaxation restatement: R with the wrong sign (growth), and the
 # orientation vector taken as the third column of the rotation instead of its third row
 PLANT_RLX_SIGN, PLANT_RLX_ORT = 11, 12
+# planted mistake of the soft-pulse restatement: the first phase block kept instead of the sum
+PLANT_FOLD = 13
 
 
 def make_tols(**over):
@@ -358,6 +363,93 @@ def np_powder_relax(I, qlist, grid, rho0, coil, dt, nsteps, spec, states, reduce
     fid[:] = grid['weights'][0] * per[0]
     for o in range(1, n_or):
         fid = fid + grid['weights'][o] * per[o]
+    return (fid, per) if per_orientation else fid
+
+
+def fourdif1(N):
+    """First-derivative Fourier collocation matrix on x_j = 2*pi*j/N
+    (kernel/derivatives/fourdif.m, m = 1): D[j,k] = (-1)^(j-k)/2 cot((j-k)h/2)
+    for even N, (-1)^(j-k)/2 csc((j-k)h/2) for odd N, zero diagonal."""
+    h = 2 * math.pi / N
+    D = np.zeros((N, N))
+    for j in range(N):
+        for k in range(N):
+            if j != k:
+                t = (j - k) * h / 2
+                D[j, k] = 0.5 * (-1) ** (j - k) * ((1 / math.sin(t)) if N % 2 else (math.cos(t) / math.sin(t)))
+    return h * np.arange(N), D
+
+
+def _weighted(grid, per):
+    fid = np.zeros(per.shape[1], complex)
+    fid[:] = grid['weights'][0] * per[0]
+    for o in range(1, per.shape[0]):
+        fid = fid + grid['weights'][o] * per[o]
+    return fid
+
+
+def np_cpmg(I, qlist, grid, rho0, coil, pulse_op, dt, npoints, nloops, per_orientation=False, angle=math.pi / 2):
+    """experiments/cpmg.m:36-57 per orientation: rho = step(pulse_op, rho0, pi/2)
+    (`angle`: a planted mistake changes it); npoints detected points of
+    rho <- P rho, P = expm(-i dt H(Omega)); then nloops times the last state
+    takes step(pulse_op, ., pi) and 2*npoints points follow (the first being the
+    pulsed state)."""
+    pop = np.asarray(pulse_op.toarray() if sps.issparse(pulse_op) else pulse_op, complex)
+    U2 = sla.expm(-1j * angle * pop)
+    U1 = sla.expm(-1j * math.pi * pop)
+    n_or = len(grid['weights'])
+    c = np.asarray(coil, complex)
+    total = npoints + nloops * 2 * npoints
+    per = np.zeros((n_or, total), complex)
+    for o in range(n_or):
+        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        P = sla.expm(-1j * dt * h)
+        r = U2 @ np.asarray(rho0, complex)
+        col = 0
+        for seg in range(nloops + 1):
+            n = npoints if seg == 0 else 2 * npoints
+            if seg > 0:
+                r = U1 @ r
+            for t in range(n):
+                per[o, col] = np.vdot(c, r)
+                col += 1
+                if t < n - 1:
+                    r = P @ r
+    fid = _weighted(grid, per)
+    return (fid, per) if per_orientation else fid
+
+
+def np_sp_acquire(I, qlist, grid, rho0, coil, ex, ey, frq, pwr, dur, phi, rnk, dt, nsteps, per_orientation=False,
+                  plant=0):
+    """experiments/sp_acquire.m:38-54 per orientation with one constant slice of
+    kernel/pulses/shaped_pulse_af.m:46-110: N = 2*rnk+1 collocation phases
+    phi_n = 2*pi*n/N + phi; F = kron(1_N, H) + pwr*(kron(diag(cos phi_n), Ex) +
+    kron(diag(sin phi_n), Ey)) + 2i*pi*frq*kron(D, 1); rho_F = expm(-i*dur*F)
+    kron(e_1, rho0); rho = the sum of its N phase blocks; then acquire()."""
+    N = 2 * int(rnk) + 1
+    ph, D = fourdif1(N)
+    ph = ph + phi
+    dim = I.shape[0]
+    exd = np.asarray(ex.toarray() if sps.issparse(ex) else ex, complex)
+    eyd = np.asarray(ey.toarray() if sps.issparse(ey) else ey, complex)
+    F1 = np.kron(np.diag(np.cos(ph)), exd) + np.kron(np.diag(np.sin(ph)), eyd)
+    M = np.kron(D, np.eye(dim))
+    e1 = np.zeros(N)
+    e1[0] = 1.0
+    r0 = np.kron(e1, np.asarray(rho0, complex))
+    n_or = len(grid['weights'])
+    per = np.zeros((n_or, nsteps + 1), complex)
+    c = np.asarray(coil, complex)
+    for o in range(n_or):
+        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        F = np.kron(np.eye(N), h) + pwr * F1 + 2j * math.pi * frq * M
+        rf = (sla.expm(-1j * dur * F) @ r0).reshape(N, dim)
+        r = rf[0].copy() if plant == PLANT_FOLD else rf.sum(axis=0)
+        P = sla.expm(-1j * dt * h)
+        for t in range(nsteps + 1):
+            per[o, t] = np.vdot(c, r)
+            r = P @ r
+    fid = _weighted(grid, per)
     return (fid, per) if per_orientation else fid
 
 

@@ -27,7 +27,7 @@ EXPORTS = ('sp_create', 'sp_destroy', 'sp_last_error', 'sp_powder_fid', 'sp_plan
            'sp_execute', 'sp_plan_result', 'sp_plan_destroy', 'sp_plan_powder_batch',
            'sp_plan_result_batch', 'sp_plan_mas', 'sp_mas_fid', 'sp_generator_is_real', 'sp_plan_set_dead_time',
            'sp_powder_fid_noalloc', 'sp_plan_autotune', 'sp_plan_get_tuning', 'sp_plan_set_tuning',
-           'sp_plan_powder_relax')
+           'sp_plan_powder_relax', 'sp_plan_set_initial_states', 'sp_plan_propagate')
 SP_MAS_FRAME_MAGNET = 0
 SP_MAS_FRAME_ROTOR = 1
 
@@ -137,6 +137,10 @@ def load():
     lib.sp_powder_fid_noalloc.argtypes = lib.sp_powder_fid.argtypes
     lib.sp_plan_set_dead_time.restype = C.c_int
     lib.sp_plan_set_dead_time.argtypes = [C.c_void_p, C.c_double]
+    lib.sp_plan_set_initial_states.restype = C.c_int
+    lib.sp_plan_set_initial_states.argtypes = [C.c_void_p, _dp]
+    lib.sp_plan_propagate.restype = C.c_int
+    lib.sp_plan_propagate.argtypes = [C.c_void_p, C.c_double, C.c_int64, _dp]
     lib.sp_generator_is_real.restype = C.c_int
     lib.sp_generator_is_real.argtypes = [C.POINTER(SpCsc), C.POINTER(SpCsc), C.POINTER(C.c_int32)]
     _lib = lib
@@ -300,6 +304,29 @@ class Plan:
         rc = load().sp_plan_set_tuning(self.h, C.byref(t))
         if rc:
             _err(self.ctx, rc, 'sp_plan_set_tuning')
+
+    def set_initial_states(self, states):
+        """sp_plan_set_initial_states: states is (n_or, dim) complex (full
+        dimension), or None to restore rho0."""
+        if states is None:
+            rc = load().sp_plan_set_initial_states(self.h, None)
+        else:
+            a = np.ascontiguousarray(states, np.complex128)
+            if a.ndim != 2 or a.shape[0] != self.n_or:
+                raise ValueError(f'initial states must have shape ({self.n_or}, dim), got {a.shape}')
+            self._init = a
+            rc = load().sp_plan_set_initial_states(self.h, a.ctypes.data_as(_dp))
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_set_initial_states')
+
+    def propagate(self, dt, nsteps, dim):
+        """sp_plan_propagate: every orientation's state after nsteps steps of dt
+        under its own generator, as an (n_or, dim) complex array."""
+        out = np.zeros((self.n_or, int(dim)), np.complex128)
+        rc = load().sp_plan_propagate(self.h, float(dt), int(nsteps), out.ctypes.data_as(_dp))
+        if rc:
+            _err(self.ctx, rc, 'sp_plan_propagate')
+        return out
 
     def set_dead_time(self, dead_time):
         """sp_plan_set_dead_time: step(L, rho0, dead_time) before the acquisition

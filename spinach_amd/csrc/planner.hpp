@@ -104,6 +104,7 @@ struct SysPlan {
   // states (7*n; empty without them)
   bool relax = false;
   std::vector<double> rvals, rdiag;
+  std::vector<int64_t> states;  // full-space index of every reduced state
 };
 
 // Partner slot and sign of the conjugation symmetry of the Wigner matrices:
@@ -211,7 +212,10 @@ inline SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, c
   sp.full_dim = dim;
   int32_t n = 0;
   for (int64_t i = 0; i < dim; i++)
-    if (seen[i]) map[i] = n++;
+    if (seen[i]) {
+      map[i] = n++;
+      sp.states.push_back(i);
+    }
   sp.n = n;
   sp.np = std::max<int32_t>(16, (n + 15) / 16 * 16);
   if (n > kStepMaxDim)

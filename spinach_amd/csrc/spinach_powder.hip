@@ -147,6 +147,12 @@ struct sp_plan {
   double dead_time = 0.0;
   int64_t rho_or_stride = 0;
   DevBuf d_rho_or;
+  // per-orientation initial states set by the caller (sp_plan_set_initial_states), and the
+  // states written by sp_plan_propagate; single-system plans keep their reduced-state map
+  bool init_states = false;
+  std::vector<int64_t> red_states;
+  DevBuf d_prop_out, d_nsub_fs;  // propagated states; substeps of the final-state passes
+  bool use_rho_or() const { return dead_time > 0 || init_states; }
   int grid_tri = 0, grid_syn = 0;
   DevBuf d_spec, d_status;
   // orientation sum: chunks of consecutive orientations, chunk ranges per system
@@ -516,7 +522,7 @@ void launch(sp_plan* pl) {
   kp.vals = pl->d_vals.as<double>();
   kp.slots = pl->d_slots.as<int32_t>();
   kp.rho0 = pl->d_rho.as<double>();
-  kp.rho_or = pl->dead_time > 0 ? pl->d_rho_or.as<double>() : nullptr;
+  kp.rho_or = pl->use_rho_or() ? pl->d_rho_or.as<double>() : nullptr;
   kp.rho_or_stride = pl->rho_or_stride;
   kp.coil = pl->d_coil.as<double>();
   kp.alphas = pl->d_al.as<double>();
@@ -567,7 +573,7 @@ spk::SpecParams spec_params(sp_plan* pl) {
   sp.vals = pl->d_vals.as<double>();
   sp.slots = pl->d_slots.as<int32_t>();
   sp.rho0 = pl->d_rho.as<double>();
-  sp.rho_or = pl->dead_time > 0 ? pl->d_rho_or.as<double>() : nullptr;
+  sp.rho_or = pl->use_rho_or() ? pl->d_rho_or.as<double>() : nullptr;
   sp.rho_or_stride = pl->rho_or_stride;
   sp.coil = pl->d_coil.as<double>();
   sp.alphas = pl->d_al.as<double>();
@@ -591,7 +597,7 @@ void launch_step_v(sp_plan* pl, const spk::StepParams& sp, size_t lds, int grid)
 // false, from rho0 or the dead-time states) or the state after nsteps steps of
 // dt written to rho_out (final_state true, from rho0).
 void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n, int q, bool final_state, double dt,
-                       int64_t nsteps, double* rho_out) {
+                       int64_t nsteps, double* rho_out, const double* rho_in = nullptr) {
   spk::StepParams sp{};
   sp.sys = pl->d_sys.as<spk::SysDesc>();
   sp.sdesc = pl->d_sdesc.as<spk::StepDesc>();
@@ -605,7 +611,8 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.sell_col = pl->d_sell_col.as<int32_t>();
   sp.sell_pos = pl->d_sell_pos.as<int32_t>();
   sp.rho0 = pl->d_rho.as<double>();
-  sp.rho_or = (!final_state && pl->dead_time > 0) ? pl->d_rho_or.as<double>() : nullptr;
+  // the acquisition starts from the per-orientation states; a final-state pass from rho_in
+  sp.rho_or = final_state ? rho_in : (pl->use_rho_or() ? pl->d_rho_or.as<double>() : nullptr);
   sp.rho_or_stride = pl->rho_or_stride;
   sp.coil = pl->d_coil.as<double>();
   sp.alphas = pl->d_al.as<double>();
@@ -619,7 +626,7 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.work_stride = (int64_t)(pl->d_step_work.bytes / sizeof(double) / std::max(1, pl->grid_step));
   sp.per_or = pl->d_per.as<double>();
   sp.rho_out = rho_out;
-  sp.nsub_out = final_state ? nullptr : pl->d_nsub.as<int32_t>();
+  sp.nsub_out = final_state ? pl->d_nsub_fs.as<int32_t>() : pl->d_nsub.as<int32_t>();
   sp.terms_out = final_state ? nullptr : pl->d_terms.as<int32_t>();
   const int g = (int)std::min<int64_t>(pl->grid_step, n);
   const int r = real ? 1 : 0;
@@ -943,6 +950,13 @@ int run(sp_plan* pl, sp_stats* stats) {
   HIPCHK(se);
   int32_t max_nsub = 0;
   int64_t step_products = 0;
+  if (pl->dead_time > 0) {  // the dead-time step skips orientations whose substeps exceed the limit
+    std::vector<int32_t> ns((size_t)pl->total_or);
+    HIPCHK(hipMemcpy(ns.data(), pl->d_nsub_fs.p, ns.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int32_t v : ns)
+      if (v > spk::kStepMaxSub)
+        throw SpError(SP_ERANGE, "either dead_time is too long, or L is too big: |L*dead_time|>1e4, check both.");
+  }
   if (pl->n_kry > 0) {
     std::vector<int32_t> ns((size_t)pl->total_or), tm((size_t)pl->total_or);
     HIPCHK(hipMemcpy(ns.data(), pl->d_nsub.p, ns.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1420,6 +1434,7 @@ sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
     std::vector<SysPlan> sps;
     sps.push_back(plan_system(I, Q, alphas, betas, gammas, weights, n_or, rho0, coil));
     sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode, tl);
+    pl->red_states = sps[0].states;
     pl->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return pl;
   } catch (SpError& e) {
@@ -1443,6 +1458,7 @@ sp_plan* sp_plan_powder_relax(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, con
     std::vector<SysPlan> sps;
     sps.push_back(plan_system(I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, rlx));
     sp_plan* pl = build_plan(ctx, sps, dt, nsteps, mode, tl);
+    pl->red_states = sps[0].states;
     pl->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return pl;
   } catch (SpError& e) {
@@ -1539,10 +1555,13 @@ int sp_plan_set_dead_time(sp_plan* plan, double dead_time) {
       throw SpError(SP_EINVAL, "a dead time is not available with a relaxation superoperator");
     if (!std::isfinite(dead_time) || dead_time < 0) throw SpError(SP_EINVAL, "dead_time must be finite and >= 0");
     HIPCHK(hipSetDevice(plan->ctx->device));
+    if (dead_time > 0 && plan->init_states)
+      throw SpError(SP_EINVAL, "a dead time and per-orientation initial states cannot be combined");
     if (dead_time > 0) {
       if (plan->step_lds[0] == 0) setup_step(plan->ctx, plan);
       if (!plan->d_rho_or.p)
         plan->d_rho_or.alloc((size_t)plan->total_or * plan->rho_or_stride * 2 * sizeof(double));
+      if (!plan->d_nsub_fs.p) plan->d_nsub_fs.alloc((size_t)plan->total_or * sizeof(int32_t));
     }
     if (dead_time != plan->dead_time && plan->gexec) {  // the recorded pass changes
       (void)hipGraphExecDestroy(plan->gexec);
@@ -1553,6 +1572,96 @@ int sp_plan_set_dead_time(sp_plan* plan, double dead_time) {
     return SP_OK;
   } catch (SpError& e) {
     return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_set_initial_states(sp_plan* plan, const double* rho_or) {
+  if (!plan) return SP_EINVAL;
+  try {
+    if (plan->mas) throw SpError(SP_EINVAL, "initial states are not available on MAS plans");
+    if (plan->n_systems != 1 || plan->red_states.empty())
+      throw SpError(SP_EINVAL, "per-orientation initial states need a single-system plan");
+    HIPCHK(hipSetDevice(plan->ctx->device));
+    const bool was = plan->init_states;
+    if (!rho_or) {
+      plan->init_states = false;
+    } else {
+      if (plan->dead_time > 0)
+        throw SpError(SP_EINVAL, "a dead time and per-orientation initial states cannot be combined");
+      const int64_t dim = plan->full_dim, n = (int64_t)plan->red_states.size(), s = plan->rho_or_stride;
+      std::vector<double> red((size_t)plan->total_or * s * 2, 0.0);
+      for (int64_t o = 0; o < plan->total_or; o++)
+        for (int64_t k = 0; k < n; k++) {
+          const double re = rho_or[2 * (o * dim + plan->red_states[k])];
+          const double im = rho_or[2 * (o * dim + plan->red_states[k]) + 1];
+          if (!std::isfinite(re) || !std::isfinite(im)) throw SpError(SP_EINVAL, "non-finite initial state");
+          red[2 * (o * s + k)] = re;
+          red[2 * (o * s + k) + 1] = im;
+        }
+      if (!plan->d_rho_or.p) plan->d_rho_or.alloc(red.size() * sizeof(double));
+      HIPCHK(hipMemcpyAsync(plan->d_rho_or.p, red.data(), red.size() * sizeof(double), hipMemcpyHostToDevice,
+                            plan->ctx->stream));
+      HIPCHK(hipStreamSynchronize(plan->ctx->stream));
+      plan->init_states = true;
+    }
+    if (was != plan->init_states && plan->gexec) {  // the recorded pass reads other states
+      (void)hipGraphExecDestroy(plan->gexec);
+      plan->gexec = nullptr;
+    }
+    plan->executed = false;
+    return SP_OK;
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::exception& e) {
+    return fail(plan->ctx, SP_EDEVICE, e.what());
+  }
+}
+
+int sp_plan_propagate(sp_plan* plan, double dt, int64_t nsteps, double* rho_out) {
+  if (!plan) return SP_EINVAL;
+  try {
+    if (plan->mas) throw SpError(SP_EINVAL, "propagation of states is not available on MAS plans");
+    if (plan->n_systems != 1 || plan->red_states.empty())
+      throw SpError(SP_EINVAL, "propagation of states needs a single-system plan");
+    if (plan->relax_systems > 0)
+      throw SpError(SP_EINVAL, "propagation of states is not available with a relaxation superoperator");
+    if (plan->dead_time > 0) throw SpError(SP_EINVAL, "propagation of states does not apply the dead time");
+    if (!std::isfinite(dt) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
+    if (nsteps < 1 || nsteps > ((int64_t)1 << 30)) throw SpError(SP_EINVAL, "nsteps must be between 1 and 2^30");
+    if (!rho_out) throw SpError(SP_EINVAL, "rho_out is NULL");
+    sp_ctx* ctx = plan->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (plan->step_lds[0] == 0) setup_step(ctx, plan);
+    const int64_t s = plan->rho_or_stride;
+    if (!plan->d_prop_out.p) plan->d_prop_out.alloc((size_t)plan->total_or * s * 2 * sizeof(double));
+    if (!plan->d_nsub_fs.p) plan->d_nsub_fs.alloc((size_t)plan->total_or * sizeof(int32_t));
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, plan->d_queue.as<unsigned int>());
+    HIPCHK(hipGetLastError());
+    launch_step_items(plan, false, plan->d_items.as<spk::Item>(), plan->n_items, 6, true, dt, nsteps,
+                      plan->d_prop_out.as<double>(), plan->init_states ? plan->d_rho_or.as<double>() : nullptr);
+    std::vector<double> red((size_t)plan->total_or * s * 2);
+    std::vector<int32_t> ns((size_t)plan->total_or);
+    HIPCHK(hipMemcpyAsync(red.data(), plan->d_prop_out.p, red.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ns.data(), plan->d_nsub_fs.p, ns.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int32_t v : ns)
+      if (v > spk::kStepMaxSub)
+        throw SpError(SP_ERANGE, "either dt is too long, or L is too big: |L*dt|>1e4, check both.");
+    const int64_t dim = plan->full_dim, n = (int64_t)plan->red_states.size();
+    std::fill(rho_out, rho_out + 2 * plan->total_or * dim, 0.0);
+    for (int64_t o = 0; o < plan->total_or; o++)
+      for (int64_t k = 0; k < n; k++) {
+        rho_out[2 * (o * dim + plan->red_states[k])] = red[2 * (o * s + k)];
+        rho_out[2 * (o * dim + plan->red_states[k]) + 1] = red[2 * (o * s + k) + 1];
+      }
+    return SP_OK;
+  } catch (SpError& e) {
+    return fail(plan->ctx, e.code, e.what());
+  } catch (std::bad_alloc&) {
+    return fail(plan->ctx, SP_ENOMEM, "host allocation failed");
   } catch (std::exception& e) {
     return fail(plan->ctx, SP_EDEVICE, e.what());
   }

@@ -33,6 +33,24 @@ def hp_acquire(spin_system, parameters, H, R, K):
     raise NotImplementedError('hp_acquire() runs inside powder() on the GPU')
 
 
+def cpmg(spin_system, parameters, H, R, K):
+    """Marker for experiments/cpmg.m: an ideal pi/2 pulse (pulse_op), an
+    acquisition of npoints, then nloops of [ideal pi pulse, acquisition of
+    2*npoints]; parameters: rho0, coil, pulse_op, timestep, npoints, nloops,
+    grid. powder() runs the acquisitions and the between-pulse propagation of
+    every orientation's state on the GPU and the orientation-independent pulses
+    on the host."""
+    raise NotImplementedError('cpmg() runs inside powder() on the GPU')
+
+
+def sp_acquire(spin_system, parameters, H, R, K):
+    """Marker for experiments/sp_acquire.m: a soft pulse on parameters.spins[0]
+    (pulse_frq, pulse_pwr, pulse_dur, pulse_phi, pulse_rnk; the Fokker-Planck
+    phase-space formulation of kernel/pulses/shaped_pulse_af.m) under each
+    orientation's own Hamiltonian, then acquire()."""
+    raise NotImplementedError('sp_acquire() runs inside powder() on the GPU')
+
+
 _REQUIRED = ('rho0', 'coil', 'sweep', 'npoints', 'grid')
 
 
@@ -115,6 +133,8 @@ def _acquire_on_grid(ss, I, qlist, parameters):
     try:
         if parameters.get('dead_time'):
             plan.set_dead_time(parameters['dead_time'])
+        if parameters.get('initial_states') is not None:
+            plan.set_initial_states(parameters['initial_states'])
         plan.execute()
         if sum_up:
             fid = plan.result()
@@ -125,6 +145,146 @@ def _acquire_on_grid(ss, I, qlist, parameters):
     if sum_up:
         return fid, grid
     return [per[k] for k in range(per.shape[0])], grid
+
+
+def _closure(I, qlist, extra, vec):
+    """Full-space indices of the states reachable from the support of vec
+    under the union sparsity pattern of I, the Q slots and the extra operators."""
+    import scipy.sparse as sps
+    import scipy.sparse.csgraph as cg
+    u = abs(sps.csr_matrix(I))
+    for q in list(qlist) + list(extra):
+        if q is not None:
+            u = u + abs(sps.csr_matrix(q))
+    _, lab = cg.connected_components((u + u.T) > 0, directed=False)
+    hit = np.unique(lab[np.abs(np.asarray(vec).ravel()) > 0])
+    return np.nonzero(np.isin(lab, hit))[0]
+
+
+def _cpmg_on_grid(ss, I, qlist, parameters):
+    """experiments/cpmg.m:36-57 at every orientation: rho = step(pulse_op, rho0,
+    pi/2); fid = coil'*trajectory of npoints; then nloops times: the last state
+    takes step(pulse_op, ., pi) and a trajectory of 2*npoints follows. The
+    acquisitions run as plans with per-orientation initial states; the last
+    state of each segment comes from sp_plan_propagate on a plan whose
+    reduction is seeded by every state the pulses and L can reach."""
+    import scipy.linalg as sla
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+    for f in ('nloops', 'timestep', 'npoints', 'pulse_op', 'rho0', 'coil'):
+        if f not in parameters:
+            raise ValueError(f'parameters.{f} must be specified.')
+    nl, npts, dt = parameters['nloops'], parameters['npoints'], parameters['timestep']
+    if not (isinstance(nl, (int, np.integer)) and nl >= 1):
+        raise ValueError('parameters.nloops should be a positive integer.')
+    if not (isinstance(npts, (int, np.integer)) and npts >= 1):
+        raise ValueError('parameters.npoints should be a positive integer.')
+    if not (np.isscalar(dt) and np.isreal(dt) and dt > 0):
+        raise ValueError('parameters.timestep should be a positive real number.')
+    if ss.get('rlx') is not None:
+        raise NotImplementedError('cpmg with relaxation is not supported on this path')
+    grid = parameters['grid']
+    n_or = len(grid['weights'])
+    dim = I.shape[0]
+    pop = sps.csr_matrix(parameters['pulse_op'], dtype=complex)
+    rho = spl.expm_multiply(-1j * (np.pi / 2) * pop, np.asarray(parameters['rho0'], complex).ravel())
+    coil = np.asarray(parameters['coil'], complex).ravel()
+    st = _closure(I, qlist, [pop], np.abs(rho) + np.abs(coil))
+    seed = np.zeros(dim, complex)
+    seed[st] = 1.0
+    U = sla.expm(-1j * np.pi * pop[st][:, st].toarray())    # step(pulse_op, ., pi) on the reachable states
+    mode = parameters.get('mode', 'fp64')
+    prop = _native.plan_powder(I, qlist, grid, rho, seed, dt, 1, tols=ss['tols'], mode=mode)
+    plans = [_native.plan_powder(I, qlist, grid, rho, coil, dt, n - 1, tols=ss['tols'], mode=mode)
+             for n in (npts, 2 * npts)]
+    try:
+        states = np.tile(rho, (n_or, 1))
+        pers = []
+        for seg in range(nl + 1):
+            plan, n = (plans[0], npts) if seg == 0 else (plans[1], 2 * npts)
+            if seg > 0:
+                states[:, st] = states[:, st] @ U.T
+            plan.set_initial_states(states)
+            plan.execute()
+            pers.append(plan.result(per_orientation=True)[1])
+            if seg < nl:
+                prop.set_initial_states(states)
+                states = prop.propagate(dt, n - 1, dim)
+    finally:
+        for p in plans + [prop]:
+            p.close()
+    per = np.concatenate(pers, axis=1)
+    w = np.asarray(grid['weights'])
+    fid = w[0] * per[0]
+    for o in range(1, n_or):
+        fid = fid + w[o] * per[o]
+    if parameters.get('sum_up', True):
+        return fid, grid
+    return [per[k] for k in range(n_or)], grid
+
+
+def _sp_acquire_on_grid(ss, I, qlist, parameters):
+    """experiments/sp_acquire.m:38-54 at every orientation: the soft pulse of
+    kernel/pulses/shaped_pulse_af.m (one constant slice) in its Fokker-Planck
+    phase space of N = 2*pulse_rnk+1 phases phi_n = 2*pi*n/N + pulse_phi,
+        F(Omega) = kron(1_N, H(Omega)) + pwr*(kron(diag(cos phi), Ex) + kron(diag(sin phi), Ey))
+                   + 2i*pi*frq*kron(d/dphi, 1),
+    rho <- sum over the phase blocks of exp(-i F(Omega) dur) kron(e_1, rho0),
+    with Ep = operator(L+) on parameters.spins[0], Ex = (Ep+Ep')/2,
+    Ey = (Ep-Ep')/2i and frq = pulse_frq - offset; then acquire() from that
+    state. F(Omega) is an ordinary powder problem (the slots kron(1_N, Q)), so
+    the library propagates it (sp_plan_propagate) and the acquisition plan
+    starts from the folded states (sp_plan_set_initial_states)."""
+    import scipy.sparse as sps
+    from . import contexts as Cx
+    for f in ('spins', 'pulse_frq', 'pulse_pwr', 'pulse_dur', 'pulse_phi', 'pulse_rnk'):
+        if f not in parameters:
+            raise ValueError(f'parameters.{f} must be specified.')
+    for f in ('pulse_frq', 'pulse_pwr', 'pulse_dur', 'pulse_phi'):
+        v = parameters[f]
+        if not (np.isscalar(v) and np.isreal(v) and np.isfinite(v)):
+            raise ValueError(f'parameters.{f} must be a real scalar.')
+    if parameters['pulse_pwr'] < 0:
+        raise ValueError('parameters.pulse_pwr must be a non-negative real scalar.')
+    if parameters['pulse_dur'] <= 0:
+        raise ValueError('parameters.pulse_dur must be a positive real scalar.')
+    rnk = parameters['pulse_rnk']
+    if not (isinstance(rnk, (int, np.integer)) and rnk >= 0):
+        raise ValueError('parameters.pulse_rnk must be a non-negative integer.')
+    if ss.get('rlx') is not None:
+        raise NotImplementedError('sp_acquire with relaxation is not supported on this path')
+    spin = parameters['spins'][0]
+    off = (parameters.get('offset') or {}).get(spin, 0.0)
+    frq = float(parameters['pulse_frq']) - float(off)
+    ep = S.operator(ss, 'L+', spin)
+    ex, ey = (ep + ep.conj().T) / 2, (ep - ep.conj().T) / 2j
+    N = 2 * int(rnk) + 1
+    phases, d_dphi = Cx.fourdif(N)
+    phases = phases + float(parameters['pulse_phi'])
+    one = sps.identity(N, format='csr')
+    dim = I.shape[0]
+    I_fp = (sps.kron(one, sps.csr_matrix(I, dtype=complex))
+            + float(parameters['pulse_pwr']) * (sps.kron(sps.diags(np.cos(phases)), ex)
+                                               + sps.kron(sps.diags(np.sin(phases)), ey))
+            + 2j * np.pi * frq * sps.kron(sps.csr_matrix(d_dphi), sps.identity(dim, format='csr')))
+    q_fp = [None if q is None else sps.kron(one, sps.csr_matrix(q, dtype=complex), format='csr') for q in qlist]
+    proj = np.zeros(N)
+    proj[0] = 1.0
+    rho_fp = np.kron(proj, np.asarray(parameters['rho0'], complex).ravel())
+    seed = (np.abs(rho_fp) > 0).astype(complex)   # the reduction keeps what the pulse can reach from rho0
+    grid = parameters['grid']
+    n_or = len(grid['weights'])
+    dt_acq = 1.0 / parameters['sweep']
+    k = max(1, int(np.ceil(parameters['pulse_dur'] / dt_acq)))
+    dur = float(parameters['pulse_dur'])
+    mode = parameters.get('mode', 'fp64')
+    plan = _native.plan_powder(I_fp.tocsr(), q_fp, grid, rho_fp, seed, dur / k, k, tols=ss['tols'], mode=mode)
+    try:
+        st_fp = plan.propagate(dur / k, k, N * dim)
+    finally:
+        plan.close()
+    states = st_fp.reshape(n_or, N, dim).sum(axis=1)
+    return _acquire_on_grid(ss, I, qlist, dict(parameters, initial_states=states))
 
 
 def powder(spin_system, pulse_sequence, parameters, assumptions):
@@ -140,11 +300,19 @@ def powder(spin_system, pulse_sequence, parameters, assumptions):
     a list of per-orientation FIDs when sum_up is False."""
     if pulse_sequence is hp_acquire:
         parameters = dict(parameters, rho0=_hp_pulse(parameters))
-    elif pulse_sequence is not acquire:
-        raise NotImplementedError('only the acquire() and hp_acquire() sequences run on this path')
+    elif pulse_sequence is cpmg:
+        # cpmg takes timestep and npoints itself (checked there); sweep is not one of its parameters
+        parameters = dict(parameters, sweep=1.0)
+    elif pulse_sequence not in (acquire, sp_acquire):
+        raise NotImplementedError('only the acquire(), hp_acquire(), sp_acquire() and cpmg() sequences run on '
+                                  'this path')
     _grumble(parameters)
     ss = S.assume(spin_system, assumptions)                 # powder.m:151
     I, Q = S.hamiltonian(ss)                                # powder.m:155
     I = S.frqoffset(ss, I, parameters.get('offset'))        # powder.m:161
     params = dict(parameters, grid=_grid(parameters['grid']))  # powder.m:173-179
+    if pulse_sequence is cpmg:
+        return _cpmg_on_grid(ss, I, S.q_list(Q), params)
+    if pulse_sequence is sp_acquire:
+        return _sp_acquire_on_grid(ss, I, S.q_list(Q), params)
     return _acquire_on_grid(ss, I, S.q_list(Q), params)

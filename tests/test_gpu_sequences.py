@@ -1,0 +1,78 @@
+"""cpmg() and sp_acquire() through the host mirror powder() on the GPU
+(sp_plan_set_initial_states, sp_plan_propagate) against the oracle's
+restatements and the closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from spinach_amd import grids, workloads
+from spinach_amd import powder as P
+from spinach_amd import spinsys as S
+from test_sequences import cpmg_closed_form, one_spin_csa
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cpmg_one_spin_closed_form():
+    ss, I, q = one_spin_csa()
+    g = grids.haar(10, np.random.default_rng(4))
+    params = {'rho0': S.state(ss, 'Lz', '1H'), 'coil': S.state(ss, 'L+', '1H'), 'pulse_op': S.operator(ss, 'Ly', '1H'),
+              'timestep': 2e-5, 'npoints': 40, 'nloops': 3, 'grid': g}
+    fid, _ = P.powder(ss, P.cpmg, params, 'nmr')
+    f0 = np.vdot(params['coil'], S.state(ss, 'Lx', '1H'))
+    cf = cpmg_closed_form(ss, g, 2e-5, 40, 3, f0)
+    e = O.rel_err(fid, cf)
+    print(f'CPMG one spin on the GPU: vs closed form {e:.2e}', flush=True)
+    assert e < 1e-9
+
+
+def test_cpmg_two_spin_powder_vs_oracle():
+    w = workloads.static_2spin(npoints=64, n_orient=16)
+    ss = w['ss']
+    g = grids.haar(16, np.random.default_rng(6))
+    pop = S.operator(ss, 'Ly', '1H')
+    rho0 = S.state(ss, 'Lz', '1H')
+    params = {'rho0': rho0, 'coil': w['coil'], 'pulse_op': pop, 'timestep': w['dt'], 'npoints': 64, 'nloops': 3,
+              'grid': g, 'sum_up': False}
+    per, _ = P.powder(ss, P.cpmg, params, 'nmr')
+    fid = (np.asarray(g['weights'])[:, None] * np.array(per)).sum(axis=0)
+    ref, rper = O.np_cpmg(w['I'], w['Q'], g, rho0, w['coil'], pop, w['dt'], 64, 3, per_orientation=True)
+    e = O.rel_err(fid, ref)
+    ep = max(O.rel_err(per[k], rper[k]) for k in range(16))
+    print(f'CPMG 2-spin, 16 orientations, 3 loops: vs oracle {e:.2e} (per orientation {ep:.2e})', flush=True)
+    assert len(fid) == 64 + 3 * 128 and e < 1e-10 and ep < 1e-10
+
+
+def test_sp_acquire_two_spin_vs_oracle():
+    w = workloads.static_2spin(npoints=256, n_orient=8)
+    ss = w['ss']
+    g = grids.haar(8, np.random.default_rng(7))
+    ep = S.operator(ss, 'L+', '1H')
+    ex, ey = (ep + ep.conj().T) / 2, (ep - ep.conj().T) / 2j
+    rho0 = S.state(ss, 'Lz', '1H')
+    pulse = dict(spins=['1H'], pulse_frq=3.0e3, pulse_pwr=2 * math.pi * 2.0e4, pulse_dur=1.25e-5, pulse_phi=0.4,
+                 pulse_rnk=2)
+    params = dict(pulse, rho0=rho0, coil=w['coil'], sweep=1 / w['dt'], npoints=256, grid=g)
+    fid, _ = P.powder(ss, P.sp_acquire, params, 'nmr')
+    ref = O.np_sp_acquire(w['I'], w['Q'], g, rho0, w['coil'], ex, ey, 3.0e3, pulse['pulse_pwr'], 1.25e-5, 0.4, 2,
+                          w['dt'], 255)
+    e = O.rel_err(fid, ref)
+    print(f'sp_acquire 2-spin (Fokker-Planck rank 2): vs oracle {e:.2e}', flush=True)
+    assert e < 1e-10
+
+
+def test_sp_acquire_reference_known_answer():
+    sys = {'magnet': 14.1, 'isotopes': ['1H']}
+    ss = S.basis(S.create(sys, {'zeeman': {'scalar': [0.0]}}), {'formalism': 'sphten-liouv', 'approximation': 'none'})
+    ep = S.operator(ss, 'L+', '1H')
+    ex, ey = (ep + ep.conj().T) / 2, (ep - ep.conj().T) / 2j
+    rho0, coil = S.state(ss, 'Lz', '1H'), S.state(ss, 'Lx', '1H')
+    amp, phi, dur = 2 * math.pi * 9e3, math.pi / 7, 1.8e-5
+    params = dict(spins=['1H'], pulse_frq=0.0, pulse_pwr=amp, pulse_dur=dur, pulse_phi=phi, pulse_rnk=1, rho0=rho0,
+                  coil=coil, sweep=1e5, npoints=4, grid='single_crystal')
+    fid, _ = P.powder(ss, P.sp_acquire, params, 'nmr')
+    import scipy.linalg as sla
+    ref = np.vdot(coil, sla.expm(-1j * dur * amp * (math.cos(phi) * ex + math.sin(phi) * ey).toarray()) @ rho0)
+    assert np.allclose(fid, ref, rtol=1e-10, atol=1e-13)
