@@ -5,8 +5,10 @@
  * The library replaces the orientation loop and the weighted sum of
  * Spinach's static powder context for the case
  *   powder(spin_system, @acquire, parameters, 'nmr')
- * in the sphten-liouv formalism with no relaxation, no kinetics and no
- * rotating frames. It receives the isotropic generator I and the 34
+ * in the sphten-liouv formalism with no kinetics and no rotating frames
+ * (orientation-dependent relaxation: sp_plan_powder_relax; the states handed
+ * between the stages of a pulse sequence: sp_plan_set_initial_states and
+ * sp_plan_propagate). It receives the isotropic generator I and the 34
  * anisotropic components Q{rank}{k,m} that hamiltonian() returns, the
  * orientation grid, the initial state and the coil, and returns the
  * powder-averaged free induction decay.
