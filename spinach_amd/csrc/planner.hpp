@@ -356,6 +356,28 @@ inline int sell_lanes(int n, int nnz) {
   return mean >= 16 ? 8 : (mean >= 8 ? 4 : (mean >= 3 ? 2 : 1));
 }
 
+// Slices and widest slice (in 64-entry columns) of the layout with G lanes per row.
+inline void sell_shape(const SysPlan& sp, int G, int& nslices, int& wmax) {
+  const int n = sp.n, H = kSellWave / G;
+  std::vector<int32_t> rowlen(n, 0);
+  for (int32_t e = 0; e < sp.nnz; e++) rowlen[sp.row[e]]++;
+  nslices = (n + H - 1) / H;
+  wmax = 0;
+  for (int r = 0; r < n; r++) wmax = std::max(wmax, (rowlen[r] + G - 1) / G);
+}
+
+// Lanes per row for the register-cached product of the stepping kernel with rcs (1 or 2)
+// slices per wave (slice s goes to wave s % 4) of at most entries / rcs columns: the fewest
+// lanes per row that fit (no cross-lane sums with one lane per row); 0 when none fits.
+inline int sell_lanes_cached(const SysPlan& sp, int rcs, int entries) {
+  for (int G = 1; G <= 8; G <<= 1) {
+    int ns = 0, w = 0;
+    sell_shape(sp, G, ns, w);
+    if ((ns + 3) / 4 <= rcs && w <= entries / rcs) return G;
+  }
+  return 0;
+}
+
 inline void sell_layout(const SysPlan& sp, std::vector<int32_t>& start, std::vector<int32_t>& colv,
                         std::vector<int32_t>& pos, SellDesc& d, int lanes = 0) {
   const int n = sp.n;

@@ -137,6 +137,7 @@ struct sp_plan {
   int32_t step_vec_words = 0;            // LDS doubles of the four state vectors
   int32_t step_max_sell = 0;             // largest slice layout over the plan's systems
   bool step_vl[2] = {false, false};      // [REAL] values staged in LDS
+  int step_rcs = 0;                      // register-cached product: slices per wave (1, 2; 0: off)
   size_t step_lds[2] = {0, 0};           // [REAL] dynamic LDS bytes
   int grid_step = 0, step_per_cu = 0;
   size_t step_work_words = 0;
@@ -255,9 +256,9 @@ void configure_spec(sp_ctx* ctx, sp_plan* pl, int tri_per_cu, int syn_per_cu) {
 // them; otherwise the values go to a global slot per workgroup.
 constexpr size_t kStepLdsMax = 156 * 1024;
 
-template <bool REAL, bool VL>
+template <bool REAL, bool VL, int RCS = 0>
 void step_attr(size_t bytes) {
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&spk::k_step<REAL, VL>),
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&spk::k_step<REAL, VL, RCS>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
@@ -277,6 +278,13 @@ void setup_step(sp_ctx* ctx, sp_plan* pl) {
   // both variants of each kind may run in one pass
   if (pl->step_vl[0]) step_attr<false, true>(pl->step_lds[0]); else step_attr<false, false>(pl->step_lds[0]);
   if (pl->step_vl[1]) step_attr<true, true>(pl->step_lds[1]); else step_attr<true, false>(pl->step_lds[1]);
+  if (pl->step_rcs == 1) {
+    if (pl->step_vl[0]) step_attr<false, true, 1>(pl->step_lds[0]);
+    if (pl->step_vl[1]) step_attr<true, true, 1>(pl->step_lds[1]);
+  } else if (pl->step_rcs == 2) {
+    if (pl->step_vl[0]) step_attr<false, true, 2>(pl->step_lds[0]);
+    if (pl->step_vl[1]) step_attr<true, true, 2>(pl->step_lds[1]);
+  }
   pl->step_work_words = work_words;
   configure_step(ctx, pl, 0);
 }
@@ -342,6 +350,13 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->taylor_k = 10;
     pl->ps_q = 3;  // X^2, X^3 + 3 Horner steps = 5 products
   }
+  // the register-cached stepping product: one configuration (slices per wave) that every
+  // system of the plan fits, the fewest slices per wave first
+  for (int rcs = 1; rcs <= 2 && pl->step_rcs == 0; rcs++) {
+    bool all = true;
+    for (auto& sp : sps) all = all && sell_lanes_cached(sp, rcs, spk::kRcEntries) > 0;
+    if (all) pl->step_rcs = rcs;
+  }
   std::vector<spk::SysDesc> descs;
   std::vector<spk::StepDesc> sdescs;
   std::vector<int32_t> sell_start, sell_col, sell_pos;
@@ -383,11 +398,16 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->krylov_systems += krylov[s];
     descs.push_back(d);
     SellDesc sl{};
-    sell_layout(sp, sell_start, sell_col, sell_pos, sl);
+    // lanes per row: the register-cached product's layout when the plan uses it
+    sell_layout(sp, sell_start, sell_col, sell_pos, sl,
+                pl->step_rcs > 0 ? sell_lanes_cached(sp, pl->step_rcs, spk::kRcEntries) : 0);
     int lg = 0;
     while ((1 << lg) < sl.lanes) lg++;
     if (sl.nslices > spk::kStepMaxSlices) throw SpError(SP_ERANGE, "too many slices for the stepping path");
-    spk::StepDesc sdd{sl.nslices, sl.sell_nnz, sl.slice_off, sl.sell_off, sl.lanes, lg};
+    int32_t wmax = 0;
+    for (int32_t q = 0; q < sl.nslices; q++)
+      wmax = std::max(wmax, (sell_start[sl.slice_off + q + 1] - sell_start[sl.slice_off + q]) >> 6);
+    spk::StepDesc sdd{sl.nslices, sl.sell_nnz, sl.slice_off, sl.sell_off, sl.lanes, lg, wmax, 0};
     sdescs.push_back(sdd);
     pl->step_max_sell = std::max(pl->step_max_sell, sdd.sell_nnz);
     nmax = std::max(nmax, sp.n);
@@ -587,10 +607,22 @@ spk::SpecParams spec_params(sp_plan* pl) {
   return sp;
 }
 
-template <bool REAL, bool VL>
+template <bool REAL, bool VL, int RCS = 0>
 void launch_step_v(sp_plan* pl, const spk::StepParams& sp, size_t lds, int grid) {
-  hipLaunchKernelGGL((spk::k_step<REAL, VL>), dim3(grid), dim3(spk::kStepThreads), lds, pl->ctx->stream, sp);
+  hipLaunchKernelGGL((spk::k_step<REAL, VL, RCS>), dim3(grid), dim3(spk::kStepThreads), lds, pl->ctx->stream, sp);
   HIPCHK(hipGetLastError());
+}
+
+template <bool REAL>
+void launch_step_kind(sp_plan* pl, const spk::StepParams& sp, size_t lds, int grid, bool vl) {
+  if (vl && pl->step_rcs == 1)
+    launch_step_v<REAL, true, 1>(pl, sp, lds, grid);
+  else if (vl && pl->step_rcs == 2)
+    launch_step_v<REAL, true, 2>(pl, sp, lds, grid);
+  else if (vl)
+    launch_step_v<REAL, true>(pl, sp, lds, grid);
+  else
+    launch_step_v<REAL, false>(pl, sp, lds, grid);
 }
 
 // n stepping items from items, on queue head q: the observable (final_state
@@ -630,13 +662,10 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.terms_out = final_state ? nullptr : pl->d_terms.as<int32_t>();
   const int g = (int)std::min<int64_t>(pl->grid_step, n);
   const int r = real ? 1 : 0;
-  if (real) {
-    if (pl->step_vl[r]) launch_step_v<true, true>(pl, sp, pl->step_lds[r], g);
-    else launch_step_v<true, false>(pl, sp, pl->step_lds[r], g);
-  } else {
-    if (pl->step_vl[r]) launch_step_v<false, true>(pl, sp, pl->step_lds[r], g);
-    else launch_step_v<false, false>(pl, sp, pl->step_lds[r], g);
-  }
+  if (real)
+    launch_step_kind<true>(pl, sp, pl->step_lds[r], g, pl->step_vl[r]);
+  else
+    launch_step_kind<false>(pl, sp, pl->step_lds[r], g, pl->step_vl[r]);
 }
 
 // The stepping items of a plan: complex generators, then real ones.

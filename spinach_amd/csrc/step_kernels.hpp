@@ -42,6 +42,10 @@ constexpr int kStepMaxSub = 10000;  // substeps per time step (step.m:230-233)
 constexpr int kStepMaxTerms = 256;  // Taylor terms per substep: |L dt/nsub|_1 <= 2 converges in ~30
 constexpr int kStepMaxDim = 2304;                  // largest coil-connected dimension of the path
 constexpr int kStepMaxSlices = kStepMaxDim / 8;     // slices hold at least 8 rows (at most 8 lanes per row)
+// register-cached product: a wave holds RCS (1 or 2) slices of at most kRcEntries / RCS
+// 64-entry columns each; its lanes keep their columns and values in registers for the item
+constexpr int kRcEntries = 18;
+__host__ __device__ constexpr int rc_width(int rcs) { return rcs > 0 ? kRcEntries / rcs : 1; }
 
 // Per-system layout of the sliced-ELL generator.
 struct StepDesc {
@@ -51,6 +55,8 @@ struct StepDesc {
   int64_t sell_off;   // offset into the slice-layout column array (sell_nnz entries)
   int32_t lanes;      // G, lanes per row
   int32_t log2_lanes;
+  int32_t wmax;       // widest slice, in 64-entry columns
+  int32_t pad;
 };
 
 struct StepParams {
@@ -97,9 +103,15 @@ __device__ __forceinline__ double step_wave_max(double v) {
 }
 
 // VL: the slice-layout values and columns are staged in LDS (else read from the
-// workgroup's global slot and the system's column array).
-template <bool REAL, bool VL>
-__global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
+// workgroup's global slot and the system's column array). RCS > 0 (with VL, for systems of
+// at most 4*RCS slices no wider than rc_width(RCS)): each lane copies its entries' columns
+// and values into registers once per item, so a term's product issues all its state gathers
+// at once instead of a column load before each gather.
+template <bool REAL, bool VL, int RCS = 0>
+__global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // two waves per SIMD: <= 256 VGPRs
+  constexpr bool RC = RCS > 0;
+  constexpr int kRcSlices = RC ? RCS : 1, kRcWidth = rc_width(RCS);
+  static_assert(!RC || VL, "the register-cached product reads the staged layout");
   extern __shared__ __align__(16) double lds[];  // every region starts 16-byte aligned (double2 reads)
   __shared__ double cre[34], cim[34];
   __shared__ double red[2][kStepThreads / 64];
@@ -189,6 +201,26 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
       }
     }
     __syncthreads();
+    // the lane's entries in registers: slice wave + 4j, column k (padding: value 0, own row)
+    const int nsw = (nslices - wave + 3) >> 2;  // slices of this wave (uniform)
+    const int wmx = __builtin_amdgcn_readfirstlane(st.wmax);
+    int rc_col[kRcSlices][kRcWidth];
+    double rc_val[kRcSlices][kRcWidth][REAL ? 1 : 2];
+    if constexpr (RC) {
+#pragma unroll
+      for (int j = 0; j < kRcSlices; j++) {
+        const int s = wave + 4 * j;
+        const int b = (j < nsw) ? sst[s] : 0, w = (j < nsw) ? (sst[s + 1] - sst[s]) >> 6 : 0;
+#pragma unroll
+        for (int k = 0; k < kRcWidth; k++) {
+          const bool in = k < w;
+          const int e = b + (k << 6) + lane;
+          rc_col[j][k] = in ? col[e] : 0;
+          rc_val[j][k][0] = in ? val[kVW * e] : 0.0;
+          if (!REAL) rc_val[j][k][REAL ? 0 : 1] = in ? val[kVW * e + 1] : 0.0;
+        }
+      }
+    }
     // 3. |L|_1 = max row sum of |L_ij| (Hermitian L), substep count
     int nsub;
     {
@@ -287,6 +319,57 @@ __global__ void __launch_bounds__(kStepThreads) k_step(StepParams p) {
             const double c = -h * rk[k];
             int any = 0;
             const double2* x2 = reinterpret_cast<const double2*>(x);
+            if constexpr (RC) {
+              // every gather of the term first, then the row sums of each slice
+              double2 g[kRcSlices][kRcWidth];
+#pragma unroll
+              for (int j = 0; j < kRcSlices; j++)
+#pragma unroll
+                for (int kk = 0; kk < kRcWidth; kk++)
+                  if (j < nsw && kk < wmx) g[j][kk] = x2[rc_col[j][kk]];
+#pragma unroll
+              for (int j = 0; j < kRcSlices; j++) {
+                if (j < nsw) {
+                  double tr = 0.0, ti = 0.0, ur = 0.0, ui = 0.0;  // two chains of the row sum
+#pragma unroll
+                  for (int kk = 0; kk < kRcWidth; kk++) {
+                    if (kk < wmx) {
+                      const double2 pv = g[j][kk];
+                      double& ar = (kk & 1) ? ur : tr;
+                      double& ai = (kk & 1) ? ui : ti;
+                      if (REAL) {
+                        const double l0 = rc_val[j][kk][0];
+                        ar = fma(l0, pv.x, ar);
+                        ai = fma(l0, pv.y, ai);
+                      } else {
+                        const double lr = rc_val[j][kk][0], li = rc_val[j][kk][REAL ? 0 : 1];
+                        ar = fma(lr, pv.x, ar);
+                        ar = fma(-li, pv.y, ar);
+                        ai = fma(lr, pv.y, ai);
+                        ai = fma(li, pv.x, ai);
+                      }
+                    }
+                  }
+                  tr += ur;
+                  ti += ui;
+                  for (int off = G >> 1; off >= 1; off >>= 1) {  // the row's G lanes
+                    tr += __shfl_xor(tr, off, 64);
+                    ti += __shfl_xor(ti, off, 64);
+                  }
+                  const int i = (wave + 4 * j) * H + rsub;
+                  const double nr = -c * ti, ni = c * tr;
+                  if (owner) {
+                    y[2 * i] = nr;
+                    y[2 * i + 1] = ni;
+                    if (i < n) {
+                      vr[2 * i] += nr;
+                      vr[2 * i + 1] += ni;
+                      any |= (nr * nr + ni * ni > eps2) ? 1 : 0;
+                    }
+                  }
+                }
+              }
+            } else
             for (int s = wave; s < nslices; s += 4) {
               const int b = sst[s], w = (sst[s + 1] - b) >> 6;
               double tr = 0.0, ti = 0.0, ur = 0.0, ui = 0.0;  // two chains of the row sum
