@@ -378,15 +378,71 @@ inline int sell_lanes_cached(const SysPlan& sp, int rcs, int entries) {
   return 0;
 }
 
+// Reverse Cuthill-McKee order of the reduced union pattern (order[k] = the state placed
+// at position k): breadth-first from a least-connected state, neighbours by increasing
+// degree, reversed. Rows that are neighbours in the order then have their k-th entries
+// in neighbouring columns, so a wave's gathers of the state hit neighbouring LDS words.
+inline std::vector<int32_t> rcm_order(const SysPlan& sp) {
+  const int n = sp.n;
+  std::vector<int32_t> first(n + 1, 0), deg(n, 0);
+  for (int32_t e = 0; e < sp.nnz; e++) deg[sp.row[e]]++;
+  for (int i = 0; i < n; i++) first[i + 1] = first[i] + deg[i];  // the pattern is sorted by row
+  std::vector<char> seen(n, 0);
+  std::vector<int32_t> order;
+  order.reserve(n);
+  std::vector<int32_t> nb;
+  for (int done = 0; done < n;) {
+    int start = -1;
+    for (int i = 0; i < n; i++)
+      if (!seen[i] && (start < 0 || deg[i] < deg[start])) start = i;
+    seen[start] = 1;
+    size_t head = order.size();
+    order.push_back(start);
+    done++;
+    while (head < order.size()) {
+      const int u = order[head++];
+      nb.clear();
+      for (int32_t e = first[u]; e < first[u + 1]; e++)
+        if (!seen[sp.col[e]]) {
+          seen[sp.col[e]] = 1;
+          nb.push_back(sp.col[e]);
+        }
+      std::stable_sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+      for (int32_t v : nb) order.push_back(v);
+      done += (int)nb.size();
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+// Sliced-ELL layout of the reduced union pattern in the state order `order` (position k
+// holds state order[k]; empty: the reduced order): row r' of slice s is position s*H + r',
+// its entries sorted by position of their column, entry k at start[s] + 64*(k/G) + G*r' +
+// k%G; colv holds positions, pos maps each union-pattern entry to its layout slot.
 inline void sell_layout(const SysPlan& sp, std::vector<int32_t>& start, std::vector<int32_t>& colv,
-                        std::vector<int32_t>& pos, SellDesc& d, int lanes = 0) {
+                        std::vector<int32_t>& pos, SellDesc& d, int lanes = 0,
+                        const std::vector<int32_t>& order = {}) {
   const int n = sp.n;
   const int G = lanes > 0 ? lanes : sell_lanes(sp.n, sp.nnz);
   const int H = kSellWave / G;  // rows per slice
   const int ns = (n + H - 1) / H;
+  std::vector<int32_t> at(n);  // position of every state
+  for (int i = 0; i < n; i++) at[i] = order.empty() ? i : -1;
+  for (int k = 0; k < (int)order.size(); k++) at[order[k]] = k;
   std::vector<int32_t> rowlen(n, 0), first(n + 1, 0);
-  for (int32_t e = 0; e < sp.nnz; e++) rowlen[sp.row[e]]++;
+  for (int32_t e = 0; e < sp.nnz; e++) rowlen[at[sp.row[e]]]++;
   for (int i = 0; i < n; i++) first[i + 1] = first[i] + rowlen[i];
+  // the rank of every entry within its row, by the position of its column
+  std::vector<int32_t> rank(sp.nnz);
+  {
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> rows(n);
+    for (int32_t e = 0; e < sp.nnz; e++) rows[at[sp.row[e]]].emplace_back(at[sp.col[e]], e);
+    for (auto& r : rows) {
+      std::sort(r.begin(), r.end());
+      for (int32_t k = 0; k < (int32_t)r.size(); k++) rank[r[k].second] = k;
+    }
+  }
   d.nslices = ns;
   d.lanes = G;
   d.pad = 0;
@@ -405,10 +461,10 @@ inline void sell_layout(const SysPlan& sp, std::vector<int32_t>& start, std::vec
   for (int s2 = 0; s2 < ns; s2++)
     for (int k = st[s2]; k < st[s2 + 1]; k++) colv[base + k] = s2 * H + ((k - st[s2]) % kSellWave) / G;
   for (int32_t e = 0; e < sp.nnz; e++) {
-    const int r = sp.row[e], k = e - first[r];
+    const int r = at[sp.row[e]], k = rank[e];
     const int s2 = r / H, rr = r - s2 * H;
     const int p2 = st[s2] + kSellWave * (k / G) + G * rr + k % G;
-    colv[base + p2] = sp.col[e];
+    colv[base + p2] = at[sp.col[e]];
     pos.push_back(p2);
   }
 }

@@ -142,7 +142,7 @@ struct sp_plan {
   int grid_step = 0, step_per_cu = 0;
   size_t step_work_words = 0;
   int tri_per_cu = 2, syn_per_cu = 3;  // spectral path: resident workgroups per CU
-  DevBuf d_sdesc, d_sell_start, d_sell_col, d_sell_pos, d_nsub, d_terms, d_step_work;
+  DevBuf d_sdesc, d_sell_start, d_sell_col, d_sell_pos, d_sell_perm, d_nsub, d_terms, d_step_work;
   double plan_ms = 0.0;  // host planning and upload (reported in every sp_stats)
   // dead time (acquire.m:65-67): per-orientation initial states
   double dead_time = 0.0;
@@ -359,7 +359,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   }
   std::vector<spk::SysDesc> descs;
   std::vector<spk::StepDesc> sdescs;
-  std::vector<int32_t> sell_start, sell_col, sell_pos;
+  std::vector<int32_t> sell_start, sell_col, sell_pos, sell_perm;
   std::vector<int32_t> row, col, tpos, slots;
   std::vector<double> vals, rho, coil, al, be, ga, w, rvals, rdiag;
   // queue order: longest-processing-time first within each path (stable within a system)
@@ -399,15 +399,19 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     descs.push_back(d);
     SellDesc sl{};
     // lanes per row: the register-cached product's layout when the plan uses it
+    // states in reverse Cuthill-McKee order: neighbouring rows gather neighbouring words
+    const std::vector<int32_t> order = rcm_order(sp);
+    const int64_t perm_off = (int64_t)sell_perm.size();
+    sell_perm.insert(sell_perm.end(), order.begin(), order.end());
     sell_layout(sp, sell_start, sell_col, sell_pos, sl,
-                pl->step_rcs > 0 ? sell_lanes_cached(sp, pl->step_rcs, spk::kRcEntries) : 0);
+                pl->step_rcs > 0 ? sell_lanes_cached(sp, pl->step_rcs, spk::kRcEntries) : 0, order);
     int lg = 0;
     while ((1 << lg) < sl.lanes) lg++;
     if (sl.nslices > spk::kStepMaxSlices) throw SpError(SP_ERANGE, "too many slices for the stepping path");
     int32_t wmax = 0;
     for (int32_t q = 0; q < sl.nslices; q++)
       wmax = std::max(wmax, (sell_start[sl.slice_off + q + 1] - sell_start[sl.slice_off + q]) >> 6);
-    spk::StepDesc sdd{sl.nslices, sl.sell_nnz, sl.slice_off, sl.sell_off, sl.lanes, lg, wmax, 0};
+    spk::StepDesc sdd{sl.nslices, sl.sell_nnz, sl.slice_off, sl.sell_off, sl.lanes, lg, wmax, 0, perm_off};
     sdescs.push_back(sdd);
     pl->step_max_sell = std::max(pl->step_max_sell, sdd.sell_nnz);
     nmax = std::max(nmax, sp.n);
@@ -487,6 +491,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   upload(pl->d_sell_start, sell_start, st);
   upload(pl->d_sell_col, sell_col, st);
   upload(pl->d_sell_pos, sell_pos, st);
+  upload(pl->d_sell_perm, sell_perm, st);
   if (!rvals.empty()) upload(pl->d_rvals, rvals, st);
   if (!rdiag.empty()) upload(pl->d_rdiag, rdiag, st);
   pl->rho_or_stride = nmax;
@@ -642,6 +647,7 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.sell_start = pl->d_sell_start.as<int32_t>();
   sp.sell_col = pl->d_sell_col.as<int32_t>();
   sp.sell_pos = pl->d_sell_pos.as<int32_t>();
+  sp.sell_perm = pl->d_sell_perm.as<int32_t>();
   sp.rho0 = pl->d_rho.as<double>();
   // the acquisition starts from the per-orientation states; a final-state pass from rho_in
   sp.rho_or = final_state ? rho_in : (pl->use_rho_or() ? pl->d_rho_or.as<double>() : nullptr);

@@ -57,6 +57,7 @@ struct StepDesc {
   int32_t log2_lanes;
   int32_t wmax;       // widest slice, in 64-entry columns
   int32_t pad;
+  int64_t perm_off;   // offset of the system's state order in sell_perm: layout row k is state sell_perm[k]
 };
 
 struct StepParams {
@@ -71,6 +72,7 @@ struct StepParams {
   const int32_t* sell_start;  // per system: start of each slice within the system's slice layout
   const int32_t* sell_col;    // column of every slice-layout entry (padding: the entry's own row)
   const int32_t* sell_pos;    // slice-layout position of every union-pattern entry
+  const int32_t* sell_perm;   // per system: the reduced state of every layout row
   const double* rho0;     // interleaved complex, per system
   const double* rho_or;   // optional per-orientation initial states (dead time), stride rho_or_stride
   int64_t rho_or_stride;  // complex entries per orientation in rho_or / rho_out
@@ -193,17 +195,18 @@ __global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // tw
     {
       const double* r0 = p.rho_or ? p.rho_or + 2 * (size_t)o * p.rho_or_stride : p.rho0 + 2 * sd.vec_off;
       const double* c0 = p.coil + 2 * sd.vec_off;
-      for (int i = tid; i < n; i += kStepThreads) {
-        vr[2 * i] = r0[2 * i];
-        vr[2 * i + 1] = r0[2 * i + 1];
-        vc[2 * i] = c0[2 * i];
-        vc[2 * i + 1] = c0[2 * i + 1];
+      const int32_t* perm = p.sell_perm + st.perm_off;
+      for (int i = tid; i < n; i += kStepThreads) {  // layout row i holds state perm[i]
+        const int q = perm[i];
+        vr[2 * i] = r0[2 * q];
+        vr[2 * i + 1] = r0[2 * q + 1];
+        vc[2 * i] = c0[2 * q];
+        vc[2 * i + 1] = c0[2 * q + 1];
       }
     }
     __syncthreads();
     // the lane's entries in registers: slice wave + 4j, column k (padding: value 0, own row)
     const int nsw = (nslices - wave + 3) >> 2;  // slices of this wave (uniform)
-    const int wmx = __builtin_amdgcn_readfirstlane(st.wmax);
     int rc_col[kRcSlices][kRcWidth];
     double rc_val[kRcSlices][kRcWidth][REAL ? 1 : 2];
     if constexpr (RC) {
@@ -326,14 +329,14 @@ __global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // tw
               for (int j = 0; j < kRcSlices; j++)
 #pragma unroll
                 for (int kk = 0; kk < kRcWidth; kk++)
-                  if (j < nsw && kk < wmx) g[j][kk] = x2[rc_col[j][kk]];
+                  if (j < nsw) g[j][kk] = x2[rc_col[j][kk]];  // padding: column 0, value 0
 #pragma unroll
               for (int j = 0; j < kRcSlices; j++) {
                 if (j < nsw) {
                   double tr = 0.0, ti = 0.0, ur = 0.0, ui = 0.0;  // two chains of the row sum
 #pragma unroll
                   for (int kk = 0; kk < kRcWidth; kk++) {
-                    if (kk < wmx) {
+                    {
                       const double2 pv = g[j][kk];
                       double& ar = (kk & 1) ? ur : tr;
                       double& ai = (kk & 1) ? ui : ti;
@@ -458,9 +461,10 @@ __global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // tw
     if (tid == 0 && p.terms_out) p.terms_out[o] = terms;
     if (p.final_state) {
       double* ro = p.rho_out + 2 * (size_t)o * p.rho_or_stride;
+      const int32_t* perm = p.sell_perm + st.perm_off;
       for (int i = tid; i < n; i += kStepThreads) {
-        ro[2 * i] = vr[2 * i];
-        ro[2 * i + 1] = vr[2 * i + 1];
+        ro[2 * perm[i]] = vr[2 * i];
+        ro[2 * perm[i] + 1] = vr[2 * i + 1];
       }
     }
     __syncthreads();
