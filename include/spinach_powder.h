@@ -126,6 +126,9 @@ typedef struct sp_stats {
   int64_t step_products;     /* sparse matrix-vector products the stepping path made in the pass
                                 (Taylor terms over all substeps, steps and orientations) */
   double plan_ms;            /* host planning (reduction, layouts) and upload of the plan */
+  int32_t eig_fallbacks;     /* spectral orientations whose tridiagonal eigensolution did not
+                                converge, re-run on the propagator path (real products) */
+  int32_t pad;
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */
@@ -188,7 +191,9 @@ int sp_powder_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* a
  * only: mode must be SP_MODE_FP64 and the system real with a coil-connected
  * dimension of at most 192 (SP_EINVAL otherwise). Orientations run in blocks
  * sized to the array; a block of 64 needs about 1 KiB per time point, so
- * nsteps beyond ~60000 returns SP_ERANGE. Calls on one device share the array:
+ * nsteps beyond ~60000 returns SP_ERANGE; an orientation whose tridiagonal
+ * eigensolution does not converge returns SP_ERANGE too (the plan path re-runs it on
+ * the propagator path, which needs workspace). Calls on one device share the array:
  * calls that overlap on one device (through different contexts) race. */
 int sp_powder_fid_noalloc(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
                           const double* betas, const double* gammas, const double* weights, int64_t n_or,

@@ -52,6 +52,7 @@ struct SpecParams {
   int64_t nsteps;
   double* spec;          // kSpecStride doubles per spectral item
   int32_t* status;       // per item: most secular evaluations of a root, or -1 if a solve failed
+  int32_t dc_max_iter;   // secular evaluations per root of the second pass (0: kDcMaxIter)
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
 };
 
@@ -498,6 +499,7 @@ struct DcShared {
   unsigned long long pend[kDcThreads / 64];  // roots the first secular pass left open, by thread
   int nlev;
   int fail;
+  int max_iter;  // secular-equation evaluations allowed per root in the second pass
 };
 
 // 1/x: on the device v_rcp_f64 refined by two Newton steps (full double
@@ -806,11 +808,12 @@ __host__ __device__ inline void dc_phase_load(DcShared& S, const double* gsrc, i
 }
 
 // phase 1 (one thread): the segment tree and the rank-one tears
-__host__ __device__ inline void dc_phase_plan(DcShared& S, int n) {
+__host__ __device__ inline void dc_phase_plan(DcShared& S, int n, int max_iter = kDcMaxIter) {
   int L = 0;
   while (L < kDcMaxLevels && ((n + (1 << L) - 1) >> L) > kDcLeaf) L++;
   S.nlev = L;
   S.fail = 0;
+  S.max_iter = max_iter;
   S.bnd[0][0] = 0;
   S.bnd[0][1] = n;
   for (int l = 0; l < L; l++) {
@@ -1014,7 +1017,7 @@ __host__ __device__ inline void dc_phase_m3b(DcShared& S, int l, int u) {
   const int code = (int)S.X[5][i];
   st.it = code >> 1;
   st.fixed_weight = code & 1;
-  if (!dc_secular_run(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], st, kDcMaxIter)) S.fail = 1;
+  if (!dc_secular_run(S.Dc + a, S.Z2c + a, K, k, S.beta[s2], st, S.max_iter)) S.fail = 1;
   dc_root_store(S, i, a, k, st);
 }
 
@@ -1127,7 +1130,7 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
   double* gsrc = p.spec + (size_t)it * kSpecStride;
   dc_phase_load(S, gsrc, tid, kDcThreads);
   __syncthreads();
-  if (tid == 0) dc_phase_plan(S, n);
+  if (tid == 0) dc_phase_plan(S, n, p.dc_max_iter > 0 ? p.dc_max_iter : kDcMaxIter);
   __syncthreads();
   // the leaves and the deflation scans are serial on a few lanes of wave 0 while the
   // rest of the workgroup waits: their wave runs ahead of other workgroups' waves

@@ -161,6 +161,10 @@ struct sp_plan {
   DevBuf d_chunks, d_chunk_off, d_partial;
   DevBuf d_sys, d_items, d_queue, d_row, d_col, d_tpos, d_vals, d_slots, d_rho, d_coil;
   int32_t relax_systems = 0;  // systems with a relaxation superoperator (sp_plan_powder_relax)
+  std::vector<spk::Item> h_items;  // the device item list, in queue order
+  std::vector<int32_t> h_np;       // padded dimension of every system
+  int dc_max_iter = 0;             // secular evaluations per root of the eigensolver's second pass (0: default)
+  int32_t eig_fallbacks = 0;       // spectral orientations of the last pass re-run on the propagator path
   DevBuf d_rvals, d_rdiag;
   DevBuf d_al, d_be, d_ga, d_w, d_work, d_per, d_fid, d_sq;
   bool executed = false;
@@ -397,6 +401,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     pl->spectral_systems += spectral[s];
     pl->krylov_systems += krylov[s];
     descs.push_back(d);
+    pl->h_np.push_back(sp.np);
     SellDesc sl{};
     // lanes per row: the register-cached product's layout when the plan uses it
     // states in reverse Cuthill-McKee order: neighbouring rows gather neighbouring words
@@ -467,6 +472,10 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   for (auto& c : prop_items) ordered.push_back(c.second);
   for (auto& c : spec_items) ordered.push_back(c.second);
   for (auto& c : kry_items) ordered.push_back(c.second);
+  pl->h_items = ordered;
+  // diagnostics: a smaller cap on the eigensolver's secular iterations (the propagator-path
+  // fallback for orientations it leaves unconverged is tested with it)
+  if (const char* e = getenv("SPINACH_DC_MAX_ITER")) pl->dc_max_iter = std::max(0, atoi(e));
   pl->n_prop = (int64_t)prop_items.size();
   pl->n_spec = (int64_t)spec_items.size();
   pl->n_kry = (int64_t)kry_items.size();
@@ -609,6 +618,7 @@ spk::SpecParams spec_params(sp_plan* pl) {
   sp.spec = pl->d_spec.as<double>();
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
+  sp.dc_max_iter = pl->dc_max_iter;
   return sp;
 }
 
@@ -955,6 +965,71 @@ void record_pass(sp_plan* pl) {
   }
 }
 
+// Spectral orientations whose eigensolution did not converge, re-run on the propagator path
+// (k_powder with real products: their generators are real) into the same per-orientation
+// FIDs, then the orientation sum again.
+void run_fallback(sp_plan* pl, const std::vector<spk::Item>& fb) {
+  sp_ctx* ctx = pl->ctx;
+  hipStream_t st = ctx->stream;
+  int np = 16;
+  for (const auto& it : fb) np = std::max(np, pl->h_np[it.sys]);
+  const int64_t npts = pl->nsteps + 1;
+  const int block = std::max(1, pl->block);
+  const int nblk = (int)((npts + block - 1) / block);
+  int lg = 0;
+  while ((1 << lg) < block) lg++;
+  const int64_t mat = 2 * (int64_t)np * np;
+  const int64_t stride = (6 * mat + 2 * (int64_t)(block + 16) * np + 2 * (int64_t)nblk * np + 63) / 64 * 64;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * 2, (int64_t)fb.size()));
+  DevBuf work, items;
+  work.alloc((size_t)grid * stride * sizeof(double));
+  upload(items, fb, st);
+  spk::KParams<double> kp{};
+  kp.sys = pl->d_sys.as<spk::SysDesc>();
+  kp.items = items.as<spk::Item>();
+  kp.n_items = (int64_t)fb.size();
+  kp.queue = pl->d_queue.as<unsigned int>() + 7;
+  kp.prow = pl->d_row.as<int32_t>();
+  kp.pcol = pl->d_col.as<int32_t>();
+  kp.ptpos = pl->d_tpos.as<int32_t>();
+  kp.vals = pl->d_vals.as<double>();
+  kp.slots = pl->d_slots.as<int32_t>();
+  kp.rho0 = pl->d_rho.as<double>();
+  kp.rho_or = pl->use_rho_or() ? pl->d_rho_or.as<double>() : nullptr;
+  kp.rho_or_stride = pl->rho_or_stride;
+  kp.coil = pl->d_coil.as<double>();
+  kp.alphas = pl->d_al.as<double>();
+  kp.betas = pl->d_be.as<double>();
+  kp.gammas = pl->d_ga.as<double>();
+  kp.dt = pl->dt;
+  kp.nsteps = pl->nsteps;
+  kp.block = block;
+  kp.nblk = nblk;
+  kp.log2b = lg;
+  kp.taylor_k = 18;  // the FP64 Taylor degree and block length of build_plan
+  kp.ps_q = 4;
+  kp.theta = 1.0;
+  kp.work = work.as<double>();
+  kp.work_stride = stride;
+  kp.np_max = np;
+  kp.per_or = pl->d_per.as<double>();
+  kp.sq_out = pl->d_sq.as<int32_t>();
+  kp.rvals = pl->d_rvals.as<double>();
+  kp.rdiag = pl->d_rdiag.as<double>();
+  hipLaunchKernelGGL(spk::k_reset, dim3(1), dim3(64), 0, st, pl->d_queue.as<unsigned int>());
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((spk::k_powder<double, true>), dim3(grid), dim3(spk::kThreads), 0, st, kp);
+  HIPCHK(hipGetLastError());
+  const unsigned nb = (unsigned)((npts + 255) / 256);
+  hipLaunchKernelGGL(spk::k_chunk_sum, dim3(nb, (unsigned)pl->n_chunks), dim3(256), 0, st, pl->d_per.as<double>(),
+                     pl->d_w.as<double>(), pl->d_chunks.as<spk::SumChunk>(), npts, pl->d_partial.as<double>());
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(spk::k_system_sum, dim3(nb, (unsigned)pl->n_systems), dim3(256), 0, st,
+                     pl->d_partial.as<double>(), pl->d_chunk_off.as<int64_t>(), npts, pl->d_fid.as<double>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+}
+
 int run(sp_plan* pl, sp_stats* stats) {
   sp_ctx* ctx = pl->ctx;
   HIPCHK(hipSetDevice(ctx->device));
@@ -1005,11 +1080,17 @@ int run(sp_plan* pl, sp_stats* stats) {
   if (pl->n_spec > 0) {
     std::vector<int32_t> stv((size_t)pl->n_spec);
     HIPCHK(hipMemcpy(stv.data(), pl->d_status.p, stv.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (int32_t v : stv) {
-      if (v < 0) eig_failures++;
-      max_sweeps = std::max(max_sweeps, v);
+    std::vector<spk::Item> fb;
+    for (size_t k = 0; k < stv.size(); k++) {
+      if (stv[k] < 0) {
+        eig_failures++;
+        fb.push_back(pl->h_items[(size_t)pl->n_prop + k]);
+      }
+      max_sweeps = std::max(max_sweeps, stv[k]);
     }
+    if (!fb.empty()) run_fallback(pl, fb);
   }
+  pl->eig_fallbacks = eig_failures;
   pl->executed = false;
   sp_stats s{};
   s.elapsed_s = ms_all * 1e-3;
@@ -1039,11 +1120,9 @@ int run(sp_plan* pl, sp_stats* stats) {
   s.plan_ms = pl->plan_ms;
   for (int k = 0; k < 4; k++) s.stage_ms[k] = ms_stage[k];
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
+  s.eig_fallbacks = eig_failures;
   pl->last = s;
   if (stats) *stats = s;
-  if (eig_failures > 0)
-    throw SpError(SP_ERANGE, "the tridiagonal eigensolver did not converge for " +
-                                 std::to_string(eig_failures) + " orientation(s)");
   pl->executed = true;  // sp_plan_result refuses the results of a failed pass
   return SP_OK;
 }
