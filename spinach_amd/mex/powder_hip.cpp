@@ -19,6 +19,15 @@
 //            sum_up     parameters.sum_up (powder.m:280-294), default true
 //            krylov     true when spin_system.sys.enable contains 'krylov'
 //            device     GPU index, default 0
+//            R0         sparse relaxation superoperator, orientation-independent
+//                       part (relaxation.m at any orientation, minus rlx_diag)
+//            rlx_diag   dim x 7 real matrix [g0 g1 ... g6]: the diagonal of
+//                       R(Omega) is g0 + sum_p q_p(Omega) g_p with
+//                       q = (ox^2, oy^2, oz^2, 2ox*oy, 2ox*oz, 2oy*oz) of
+//                       ort = [0 0 1]*euler2dcm(alpha,beta,gamma)
+//                       (rlx_t1_t2.m rate tensors, relaxation.m 'damp');
+//                       with R0 or rlx_diag the generator is L = H + 1i*R
+//                       (evolution.m), and dead_time and krylov are refused
 //   fid    npoints x 1 complex, the weighted sum (sum_up true)
 //   per    npoints x n_or complex, unweighted per-orientation FIDs (requested
 //          as a second output, or the first output when sum_up is false;
@@ -206,6 +215,19 @@ bool run_gateway(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[], Gat
     int device = 0;
     if (const mxArray* d = field(opts, "device")) device = (int)scalar(d, "opts.device");
     if (device < 0 || device >= 64) fail("device", "opts.device is out of range.");
+    sp_csc R0{};
+    if (const mxArray* r = field(opts, "R0")) {
+      R0 = as_csc(r, "opts.R0");
+      if (R0.n != 0 && R0.n != I.n) fail("input", "opts.R0 must be of the Liouville space dimension.");
+    }
+    const double* rlx_diag = nullptr;
+    if (const mxArray* d = field(opts, "rlx_diag")) {
+      if (!mxIsDouble(d) || mxIsComplex(d) || mxIsSparse(d) || mxGetM(d) != dim || mxGetN(d) != 7)
+        fail("input", "opts.rlx_diag must be a real full dim x 7 matrix [g0 g1 ... g6].");
+      rlx_diag = mxGetDoubles(d);  // column p = g_p, the library's p*dim layout
+    }
+    const sp_relax rlx{R0.n > 0 ? &R0 : nullptr, rlx_diag};
+    const bool relax = rlx.R0 || rlx.diag;
 
     if (!g_ctx[device]) {
       g_ctx[device] = sp_create(device);
@@ -214,8 +236,10 @@ bool run_gateway(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[], Gat
     }
     sp_ctx* ctx = g_ctx[device];
     PlanGuard plan;
-    plan.p = sp_plan_powder(ctx, &I, Q, al.data(), be.data(), ga.data(), w.data(), (int64_t)n_or, rho0.data(),
-                            coil.data(), dt, nsteps, &tols, mode);
+    plan.p = relax ? sp_plan_powder_relax(ctx, &I, Q, al.data(), be.data(), ga.data(), w.data(), (int64_t)n_or,
+                                          rho0.data(), coil.data(), dt, nsteps, &tols, mode, &rlx)
+                   : sp_plan_powder(ctx, &I, Q, al.data(), be.data(), ga.data(), w.data(), (int64_t)n_or,
+                                    rho0.data(), coil.data(), dt, nsteps, &tols, mode);
     if (!plan.p) fail("plan", sp_last_error(ctx));
     if (dead_time != 0.0 && sp_plan_set_dead_time(plan.p, dead_time) != SP_OK) fail("input", sp_last_error(ctx));
     if (sp_execute(plan.p, nullptr) != SP_OK) fail("run", sp_last_error(ctx));

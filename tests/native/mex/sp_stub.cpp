@@ -86,6 +86,26 @@ void sp_destroy(sp_ctx* ctx) { delete ctx; }
 
 const char* sp_last_error(const sp_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
+sp_plan* sp_plan_powder_relax(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
+                              const double* betas, const double* gammas, const double* weights, int64_t n_or,
+                              const double* rho0, const double* coil, double dt, int64_t nsteps,
+                              const sp_tols* tols, int mode, const sp_relax* rlx) {
+  sp_plan* p = sp_plan_powder(ctx, I, Q, alphas, betas, gammas, weights, n_or, rho0, coil, dt, nsteps, tols, mode);
+  if (!p) return p;
+  p->log += ",\"relax\":{\"R0\":";
+  if (rlx && rlx->R0)
+    csc(p->log, rlx->R0);
+  else
+    p->log += "null";
+  p->log += ",\"diag\":";
+  if (rlx && rlx->diag)
+    arr(p->log, rlx->diag, 7 * (size_t)I->n);
+  else
+    p->log += "null";
+  p->log += "}";
+  return p;
+}
+
 sp_plan* sp_plan_powder(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas, const double* betas,
                         const double* gammas, const double* weights, int64_t n_or, const double* rho0,
                         const double* coil, double dt, int64_t nsteps, const sp_tols* tols, int mode) {

@@ -4,7 +4,8 @@ UndefinedBehaviorSanitizer:
 * the MEX gateway (spinach_amd/mex/powder_hip.cpp) compiled against the
   mex.h stand-in and a recording stub of the C-ABI: argument marshalling
   (sparse and full states, the Q cell layout, spin_system.tols, dead_time,
-  sum_up, krylov, precision), output layout, and every error path with its
+  sum_up, krylov, precision, the relaxation planes and R0), output layout,
+  and every error path with its
   "Spinach:powder_hip:<id>" identifier (etc/mex/prune_cpu.cpp:199-223
   convention); a leak or invalid access fails the sanitizers;
 * the host planner (spinach_amd/csrc/planner.hpp) on workload systems and on
@@ -110,6 +111,39 @@ def test_gateway_marshalling(native, tmp_path):
     assert rec['dt'] == w['dt'] and rec['nsteps'] == w['nsteps']
 
 
+T1 = np.array([[1.0, 0.3, -0.2], [0.3, 3.0, 0.5], [-0.2, 0.5, 5.0]]) * 40.0
+RELAX2 = {'relaxation': ['t1_t2', 'damp'], 'r1_rates': [4.0, T1], 'r2_rates': [T1 * 1.5, 9.0], 'damp_rate': 7.0,
+          'rlx_keep': 'secular'}
+
+
+def test_gateway_relaxation_marshalling(native, tmp_path):
+    """opts.rlx_diag (dim x 7, column p = g_p) and opts.R0 reach
+    sp_plan_powder_relax as the library's 7*dim planes and CSC matrix; without
+    them the gateway calls sp_plan_powder."""
+    from spinach_amd import spinsys as S
+    w = workloads.static_2spin(npoints=16, n_orient=3, relax=RELAX2)
+    g = S.relaxation_terms(w['ss'])['diag']
+    dim = w['I'].shape[0]
+    R0 = sps.csc_matrix(([0.5 - 0.25j, 3.0], ([1, 2], [2, 1])), shape=(dim, dim))
+    for opts, want_r0 in (({'rlx_diag': B.mx_dense(g.T), 'R0': B.mx_sparse(R0)}, True),
+                          ({'rlx_diag': B.mx_dense(g.T)}, False)):
+        rhs = B.powder_hip_rhs(w['I'], w['Qnested'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'],
+                               opts=opts)
+        r, outs, rec = _stub(native, rhs, 1, str(tmp_path))
+        assert r.stdout.startswith('OK'), r.stdout
+        assert np.array_equal(np.array(rec['relax']['diag']), g.ravel())
+        if want_r0:
+            ent = np.array(rec['relax']['R0']['entries'])
+            assert rec['relax']['R0']['n'] == dim
+            assert np.array_equal(ent[:, 0], [2, 1]) and np.array_equal(ent[:, 1], [1, 2])
+            assert np.array_equal(ent[:, 2] + 1j * ent[:, 3], [3.0, 0.5 - 0.25j])
+        else:
+            assert rec['relax']['R0'] is None
+    rhs = B.powder_hip_rhs(w['I'], w['Qnested'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'])
+    r, outs, rec = _stub(native, rhs, 1, str(tmp_path))
+    assert r.stdout.startswith('OK') and 'relax' not in rec
+
+
 def test_gateway_defaults_full_states_single_and_sum_up_false(native, tmp_path):
     w = workloads.static_2spin(npoints=16, n_orient=3)
     rhs = B.powder_hip_rhs(w['I'], w['Qnested'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'],
@@ -138,6 +172,9 @@ def test_gateway_defaults_full_states_single_and_sum_up_false(native, tmp_path):
     ('fail_dead_time', 'Spinach:powder_hip:input'),
     ('fail_execute', 'Spinach:powder_hip:run'),
     ('fail_result', 'Spinach:powder_hip:run'),
+    ('rlx_diag_shape', 'Spinach:powder_hip:input'),
+    ('rlx_diag_complex', 'Spinach:powder_hip:input'),
+    ('r0_dim', 'Spinach:powder_hip:input'),
 ])
 def test_gateway_errors(native, tmp_path, case, ident):
     w = workloads.static_2spin(npoints=8, n_orient=2)
@@ -164,6 +201,13 @@ def test_gateway_errors(native, tmp_path, case, ident):
         rhs[0] = B.mx_dense(I.toarray())
     if case == 'q_shape':
         rhs[1] = B.mx_cell([rhs[1][1][0]], (1, 1))
+    dim = I.shape[0]
+    if case == 'rlx_diag_shape':
+        rhs[8][1]['rlx_diag'] = B.mx_dense(np.zeros((dim, 6)))
+    if case == 'rlx_diag_complex':
+        rhs[8][1]['rlx_diag'] = B.mx_dense(np.zeros((dim, 7)) + 1j)
+    if case == 'r0_dim':
+        rhs[8][1]['R0'] = B.mx_sparse(sps.eye(dim + 1, format='csc'))
     if case.startswith('fail_'):
         fail = case[5:]
     r, outs, rec = _stub(native, rhs, 1, str(tmp_path), fail=fail)

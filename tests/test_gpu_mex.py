@@ -51,3 +51,39 @@ def test_gateway_dead_time_krylov_sum_up_false(tmp_path):
           f'vs shifted closed form {O.rel_err(fid, cf):.2e}', flush=True)
     assert O.rel_err(fid, orc) < 1e-10
     assert O.rel_err(fid, cf) < 5e-9
+
+
+def test_gateway_relaxation_matches_ctypes_and_closed_form(tmp_path):
+    """opts.rlx_diag and opts.R0 through the gateway, with the scalar rates
+    moved from g0 into R0 = diag(g0): the same FID as the ctypes plan given
+    the same sp_relax (bit for bit), the ctypes plan with every rate in the
+    planes, and the two-spin closed form with relaxation."""
+    import scipy.sparse as sps
+    from spinach_amd import spinsys as S
+    t1 = np.array([[1.0, 0.3, -0.2], [0.3, 3.0, 0.5], [-0.2, 0.5, 5.0]]) * 40.0
+    relax = {'relaxation': ['t1_t2', 'damp'], 'r1_rates': [4.0, t1], 'r2_rates': [t1 * 1.5, 9.0],
+             'damp_rate': t1 * 0.5, 'rlx_keep': 'secular'}
+    w = workloads.static_2spin(npoints=512, n_orient=24, relax=relax)
+    g = S.relaxation_terms(w['ss'])['diag']
+    r0 = sps.csc_matrix(sps.diags(g[0]))
+    r0.eliminate_zeros()
+    split = {'R0': r0, 'diag': np.vstack([np.zeros_like(g[:1]), g[1:]])}
+    assert np.abs(g[0]).max() > 0
+    opts = {'rlx_diag': B.mx_dense(split['diag'].T), 'R0': B.mx_sparse(split['R0'])}
+    rhs = B.powder_hip_rhs(w['I'], w['Qnested'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'], opts=opts)
+    fid, = _call(rhs, 1, str(tmp_path))
+    refs = []
+    for rl in (split, {'R0': None, 'diag': g}):
+        plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'], relax=rl)
+        try:
+            plan.execute()
+            refs.append(plan.result())
+        finally:
+            plan.close()
+    cf = O.closed_form_2spin(w['ss'], w['grid'], w['dt'], w['nsteps'], relax=w['ss']['rlx'])
+    print(f'MEX gateway 2-spin relaxation 24 x 511: vs ctypes max |diff| {np.max(np.abs(fid[:, 0] - refs[0])):.2e}, '
+          f'vs planes only {O.rel_err(fid[:, 0], refs[1]):.2e}, vs closed form {O.rel_err(fid[:, 0], cf):.2e}',
+          flush=True)
+    assert np.array_equal(fid[:, 0], refs[0])
+    assert O.rel_err(fid[:, 0], refs[1]) < 1e-12
+    assert O.rel_err(fid[:, 0], cf) < 5e-9

@@ -132,3 +132,33 @@ def test_oracle_one_spin_anisotropic_t2_decay():
         R = O._dcm(g['alphas'][o], g['betas'][o], g['gammas'][o])
         r2 = R[2, :] @ T2 @ R[2, :]
         assert O.rel_err(per[o], per0[o] * np.exp(-r2 * t)) < 1e-11
+
+
+SIX_EULER = np.array([[0, np.pi / 2, np.pi], [0, np.pi / 2, np.pi / 2], [0, 0, 0],
+                      [0, np.pi / 2, 3 * np.pi / 4], [0, np.pi / 4, np.pi], [0, np.pi / 4, np.pi / 2]])
+
+
+def test_six_orientation_sampling_recovers_the_planes():
+    """The MATLAB caller's recipe for the gateway's opts.rlx_diag: the
+    diagonal of relaxation() at the six orientations whose ort is e_x, e_y,
+    e_z, (e_x+e_y)/sqrt2, (e_x+e_z)/sqrt2, (e_y+e_z)/sqrt2 gives g0 = 0,
+    g1..g3 = d1..d3, g4 = d4-(d1+d2)/2, g5 = d5-(d1+d3)/2, g6 = d6-(d2+d3)/2;
+    the planes so obtained must reproduce the oracle's rates at any
+    orientation (g0 folds into g1+g2+g3 because ox^2+oy^2+oz^2 = 1)."""
+    w = workloads.static_2spin(npoints=8, n_orient=2, relax={
+        'relaxation': ['t1_t2', 'damp'], 'r1_rates': [3.0, T1], 'r2_rates': [T2, 11.0], 'damp_rate': TD,
+        'rlx_keep': 'secular'})
+    ss = w['ss']
+    d = np.stack([S.relaxation(ss, tuple(e)).diagonal() for e in SIX_EULER], axis=1)
+    for k, e in enumerate(SIX_EULER):   # the angles give the stated orientation vectors
+        ort = O._dcm(*e)[2, :]
+        want = [[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, 0, 1], [0, 1, 1]][k]
+        assert np.allclose(ort, np.array(want) / np.linalg.norm(want), atol=1e-15)
+    g = np.stack([np.zeros(d.shape[0]), d[:, 0], d[:, 1], d[:, 2], d[:, 3] - (d[:, 0] + d[:, 1]) / 2,
+                  d[:, 4] - (d[:, 0] + d[:, 2]) / 2, d[:, 5] - (d[:, 1] + d[:, 2]) / 2])
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        eul = tuple(rng.uniform(-math.pi, math.pi, 3))
+        ox, oy, oz = O._dcm(*eul)[2, :]
+        q = np.array([ox * ox, oy * oy, oz * oz, 2 * ox * oy, 2 * ox * oz, 2 * oy * oz])
+        assert np.allclose(g[0] + q @ g[1:], O.relax_diag_at(ss['rlx'], ss['basis'], eul), rtol=1e-13, atol=1e-11)
