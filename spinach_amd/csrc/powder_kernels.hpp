@@ -46,6 +46,8 @@ namespace spk {
 constexpr int kThreads = 256;
 
 // Per-system descriptor, device resident.
+constexpr int kRowBlocks = 17;  // row-block boundaries rb[0..16] of a pattern: rows 16k up to 256
+
 struct SysDesc {
   int32_t n;         // reduced dimension
   int32_t np;        // padded dimension (multiple of 16)
@@ -58,7 +60,7 @@ struct SysDesc {
   int64_t or_off;    // first orientation of this system in the global orientation arrays
   int64_t n_or;      // number of orientations
   int32_t real;      // 1: the generator is real at every orientation (real-product path)
-  int32_t rb[13];    // rb[k]: first pattern entry of row 16k (the pattern is sorted by row, then column)
+  int32_t rb[kRowBlocks];  // rb[k]: first pattern entry of row 16k (the pattern is sorted by row, then column)
   int64_t rlx_off;   // relaxation R0 on the union pattern: offset (complex elements) into rvals, -1: none
   int64_t rdiag_off; // relaxation diagonal planes g0..g6 (7*n doubles) in rdiag, -1: none
 };
@@ -122,8 +124,9 @@ __host__ __device__ inline void relax_monomials(double beta, double gamma, doubl
 // zeroes the kQueues work-queue heads of a plan: [0] propagator items with a
 // complex generator, [1] tridiagonalisation, [2] synthesis, [3] propagator
 // items with a real generator, [4] stepping items with a complex generator,
-// [5] stepping items with a real generator, [6] the dead-time step of every item
-constexpr int kQueues = 8;
+// [5] stepping items with a real generator, [6] the dead-time step of every item,
+// [7] the eigensolver fallback, [8] tridiagonalisation above kTriNP (k_tridiag_w)
+constexpr int kQueues = 16;
 __global__ void k_reset(unsigned int* q) {
   if (blockIdx.x == 0 && threadIdx.x < kQueues) q[threadIdx.x] = 0u;
 }

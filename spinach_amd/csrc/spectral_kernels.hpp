@@ -27,8 +27,9 @@
 
 namespace spk {
 
-constexpr int kTriNP = 192;      // largest padded dimension of the register layout
-constexpr int kSpecStride = 6 * kTriNP;  // per item: d | e | four vectors
+constexpr int kTriNP = 192;      // largest padded dimension of the 256-thread register layout
+constexpr int kTriWNP = 256;     // largest padded dimension of the 1024-thread layout (k_tridiag_w)
+constexpr int kSpecStride = 6 * kTriNP;  // per item: d | e | four vectors (spec_nm = kTriNP)
 constexpr int kSynthB = 64;      // time block of the synthesis product
 
 struct SpecParams {
@@ -50,7 +51,8 @@ struct SpecParams {
   const double* gammas;
   double dt;
   int64_t nsteps;
-  double* spec;          // kSpecStride doubles per spectral item
+  double* spec;          // 6 * spec_nm doubles per spectral item: d | e | four vectors, each spec_nm long
+  int spec_nm;           // kTriNP, or kTriWNP in plans with systems above kTriNP
   int32_t* status;       // per item: most secular evaluations of a root, or -1 if a solve failed
   int32_t dc_max_iter;   // secular evaluations per root of the second pass (0: kDcMaxIter)
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
@@ -112,11 +114,12 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 //   v = (0, ..., 0, x1 - alpha, x_{jj+2}, ...),  tau = 2/(v'v)
 // (tau = 0, alpha = x1 when sigma = 0: no reflection), and writes v to vb,
 // tau to *ptau and T's d_jj = xd, e_jj = alpha. Every lane of the wave active.
-__device__ __forceinline__ void householder_form(const double (&x)[kTriNP / 64], double xd, double x1, int jj,
-                                                 int lane, double* vb, double* ptau, double* sd, double* se) {
+template <int NQ>
+__device__ __forceinline__ void householder_form(const double (&x)[NQ], double xd, double x1, int jj, int lane,
+                                                 double* vb, double* ptau, double* sd, double* se) {
   double part = 0.0;
 #pragma unroll
-  for (int r = 0; r < kTriNP / 64; r++)
+  for (int r = 0; r < NQ; r++)
     if (lane + 64 * r >= jj + 2) part = fma(x[r], x[r], part);
   const double sigma = wave_sum_dpp(part);
   double tau = 0.0, alpha = x1, v1 = 0.0;
@@ -127,7 +130,7 @@ __device__ __forceinline__ void householder_form(const double (&x)[kTriNP / 64],
     tau = 2.0 / (v1 * v1 + sigma);
   }
 #pragma unroll
-  for (int r = 0; r < kTriNP / 64; r++) {
+  for (int r = 0; r < NQ; r++) {
     const int c = lane + 64 * r;
     vb[c] = (c > jj + 1) ? x[r] : ((c == jj + 1) ? v1 : 0.0);
   }
@@ -451,12 +454,310 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
     }
     if (n >= 2) sym_publish(R, n - 1, r, c, hd, wave, lane, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
     __syncthreads();
-    double* out = p.spec + (size_t)it * kSpecStride;
+    const int nm = p.spec_nm;
+    double* out = p.spec + (size_t)it * 6 * nm;
     for (int i = tid; i < kTriNP; i += kTriThreads) {
       out[i] = sd[i];
-      out[kTriNP + i] = se[i];
+      out[nm + i] = se[i];
 #pragma unroll
-      for (int q = 0; q < 4; q++) out[(2 + q) * kTriNP + i] = su[q][i];
+      for (int q = 0; q < 4; q++) out[(2 + q) * nm + i] = su[q][i];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ k_tridiag_w
+// The same Householder tridiagonalisation for padded dimensions kTriNP < np <= kTriWNP: the
+// lower triangle of the 256 x 256 FP64 matrix (263 KB) lives in the registers of one
+// 1024-thread workgroup per CU. Thread (r, c) = (tid / 32, tid % 32) of the 32 x 32 grid holds
+// A(r + 32a, c + 32b) for b <= a (36 slots), h-weighted in the diagonal blocks as in
+// k_tridiag; wave w holds rows r = 2w (lanes 0-31) and 2w + 1 (lanes 32-63) of every block.
+// Row sums of A*v are reduced over each 16-lane DPP row (two partial sums per index), column
+// sums over the wave's two rows by a lane exchange (one partial sum per wave and index).
+constexpr int kTriWThreads = 1024;
+constexpr int kTriWWaves = kTriWThreads / 64;              // 16
+constexpr int kTriWBlk = kTriWNP / 32;                     // 8 blocks of 32 rows / columns
+constexpr int kTriWSlots = kTriWBlk * (kTriWBlk + 1) / 2;  // 36
+
+// sum over each 16-lane DPP row, returned in all 16 lanes; every lane of the wave active
+__device__ __forceinline__ double sum16(double v) {
+  v = sum8(v);
+  v += dpp_f64<0x140>(v);  // row_mirror: the other 8-group of the row
+  return v;
+}
+
+// The owners of column jj (c == jj % 32) copy its entries at rows >= jj to srow (rows
+// below jj in its block get 0 or a stale value, never read).
+__device__ __forceinline__ void symw_copy_col(const double (&R)[kTriWSlots], int jj, int r, int c, double hd,
+                                              double* srow) {
+  if (c != (jj & 31)) return;
+  const int bj = jj >> 5;
+#pragma unroll
+  for (int b = 0; b < kTriWBlk; b++) {
+    if (b == bj) {
+#pragma unroll
+      for (int a = b; a < kTriWBlk; a++) srow[r + 32 * a] = (a == b) ? hd * R[sym_slot(a, b)] : R[sym_slot(a, b)];
+    }
+  }
+}
+
+// Column jj as it stands in the registers, reflected by wave 0 (every thread calls it).
+__device__ __forceinline__ void symw_publish(const double (&R)[kTriWSlots], int jj, int r, int c, double hd,
+                                             int wave, int lane, double* srow, double* vb, double* ptau, double* sd,
+                                             double* se) {
+  symw_copy_col(R, jj, r, c, hd, srow);
+  __syncthreads();
+  if (wave == 0) {
+    double x[kTriWNP / 64];
+#pragma unroll
+    for (int q = 0; q < kTriWNP / 64; q++) x[q] = srow[lane + 64 * q];
+    householder_form(x, srow[jj], srow[jj + 1], jj, lane, vb, ptau, sd, se);  // srow[kTriWNP] stays zero
+  }
+}
+
+// A*v over this thread's slots of the live column blocks, in two passes over the registers so
+// that only one block-indexed array is live beside the matrix (1024 threads leave 128 VGPRs):
+// row sums (reduced over each 16-lane row, two partial sums per index into spr), then column
+// sums (the wave's two rows added, one partial sum per wave into spc). Returns this thread's
+// part of v'A v. Rows at or beyond np hold zeros.
+__device__ __forceinline__ double symw_matvec(const double (&R)[kTriWSlots], const double* v, double (*spr)[kTriWNP],
+                                              double (*spc)[kTriWNP], int wave, int lane, int r, int c, int bmin,
+                                              unsigned live) {
+  double s = 0.0;
+  {
+    double yr[kTriWBlk];
+#pragma unroll
+    for (int a = 0; a < kTriWBlk; a++) yr[a] = 0.0;
+    double vnext = v[c + 32 * bmin];
+#pragma unroll
+    for (int b = 0; b < kTriWBlk; b++) {
+      if ((live >> b) & 1u) {
+        const double vc = vnext;
+        if (b + 1 < kTriWBlk) vnext = v[c + 32 * (b + 1)];
+#pragma unroll
+        for (int a = b; a < kTriWBlk; a++) yr[a] = fma(R[sym_slot(a, b)], vc, yr[a]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < kTriWBlk; a++) {
+      if ((live >> a) & 1u) {
+        s = fma(v[r + 32 * a], yr[a], s);
+        const double t = sum16(yr[a]);
+        if ((lane & 15) == 0) spr[(lane >> 4) & 1][r + 32 * a] = t;
+      }
+    }
+  }
+  double vr[kTriWBlk];
+#pragma unroll
+  for (int a = 0; a < kTriWBlk; a++) vr[a] = v[r + 32 * a];
+#pragma unroll
+  for (int b = 0; b < kTriWBlk; b++) {
+    if ((live >> b) & 1u) {
+      double yc = 0.0;
+#pragma unroll
+      for (int a = b; a < kTriWBlk; a++) yc = fma(R[sym_slot(a, b)], vr[a], yc);
+      s = fma(v[c + 32 * b], yc, s);
+      const double yo = __shfl_xor(yc, 32);  // the wave's other row, same column
+      if (lane < 32) spc[wave][c + 32 * b] = yc + yo;
+    }
+  }
+  return s;
+}
+
+// A <- A - v w' - w v' over this thread's slots of the live column blocks
+__device__ __forceinline__ void symw_update(double (&R)[kTriWSlots], const double (&vr)[kTriWBlk],
+                                            const double (&wr)[kTriWBlk], const double* v, const double* sw, double h,
+                                            int c, int bmin, unsigned live) {
+  double vnext = v[c + 32 * bmin], wnext = sw[c + 32 * bmin];
+#pragma unroll
+  for (int b = 0; b < kTriWBlk; b++) {
+    if ((live >> b) & 1u) {
+      const double vc = vnext, wc = wnext;
+      if (b + 1 < kTriWBlk) {
+        vnext = v[c + 32 * (b + 1)];
+        wnext = sw[c + 32 * (b + 1)];
+      }
+#pragma unroll
+      for (int a = b; a < kTriWBlk; a++) {
+        const double va = (a == b) ? h * vr[a] : vr[a];
+        const double wa = (a == b) ? h * wr[a] : wr[a];
+        R[sym_slot(a, b)] = fma(-va, wc, fma(-wa, vc, R[sym_slot(a, b)]));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
+  __shared__ double sv[2][kTriWNP];  // Householder vectors, double-buffered
+  __shared__ double stau[2];
+  __shared__ double srow[kTriWNP + 1];  // the column being reflected; srow[kTriWNP] = 0
+  __shared__ double sw[kTriWNP];        // w = p - (tau/2)(v'p) v, p = tau*A*v
+  __shared__ double spr[2][kTriWNP];    // row sums of A*v over columns c < 16 and c >= 16
+  __shared__ double spc[kTriWWaves][kTriWNP];  // column sums of A*v, one row per wave
+  __shared__ double sq[kTriWWaves];
+  __shared__ double su[4][kTriWNP];
+  __shared__ double sd[kTriWNP], se[kTriWNP];
+  __shared__ double cre[34], cim[34];
+  __shared__ int sitem;
+  // the 32 rows of the generator under assembly (block row a in pass a); all zero between uses
+  constexpr int kAsmStride = kTriWNP + 4;
+  __shared__ double sA[32 * kAsmStride];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r = tid >> 5, c = tid & 31;
+  const double h = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
+  const double hd = (r == c) ? 2.0 : 1.0;
+  for (int k = tid; k < 32 * kAsmStride; k += kTriWThreads) sA[k] = 0.0;
+  for (int k = tid; k <= kTriWNP; k += kTriWThreads) srow[k] = 0.0;
+
+  for (;;) {
+    if (tid == 0) sitem = (int)atomicAdd(&p.queue[0], 1u);
+    __syncthreads();
+    const int64_t it = __builtin_amdgcn_readfirstlane(sitem);  // uniform (scalar branches below)
+    __syncthreads();
+    if (it >= p.n_items) break;
+    const Item item = p.items[it];
+    const SysDesc sdesc = p.sys[item.sys];
+    const int n = __builtin_amdgcn_readfirstlane(sdesc.n), np = __builtin_amdgcn_readfirstlane(sdesc.np);
+    const int bmax = (np + 31) >> 5;  // blocks below the padded dimension
+    const int64_t o = item.orient;
+    if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
+    if (tid < kTriWNP) {
+      const double* coilv = p.coil + 2 * sdesc.vec_off;
+      const double* rhov = p.rho_or ? p.rho_or + 2 * (size_t)o * p.rho_or_stride : p.rho0 + 2 * sdesc.vec_off;
+      const int i = tid;
+      double c0 = 0.0, c1 = 0.0, r0 = 0.0, r1 = 0.0;
+      if (i < n) {
+        c0 = coilv[2 * i];
+        c1 = -coilv[2 * i + 1];
+        r0 = rhov[2 * i];
+        r1 = rhov[2 * i + 1];
+      }
+      su[0][i] = c0;
+      su[1][i] = c1;
+      su[2][i] = r0;
+      su[3][i] = r1;
+      sd[i] = 0.0;
+      se[i] = 0.0;
+    }
+    __syncthreads();
+    // the lower triangle, one block row (32 rows) per pass: the pass's lower pattern entries
+    // (rows 32a .. 32a + 31: pattern entries rb[2a] .. rb[2a + 2]) are scattered into sA, then
+    // every thread takes its slots of row r + 32a and clears them
+    double R[kTriWSlots];
+    const int32_t* prow = p.prow + sdesc.pat_off;
+    const int32_t* pcol = p.pcol + sdesc.pat_off;
+#pragma unroll
+    for (int a = 0; a < kTriWBlk; a++) {
+      if (a < bmax) {
+        const int e1 = sdesc.rb[2 * a + 2];
+        for (int e = sdesc.rb[2 * a] + tid; e < e1; e += kTriWThreads) {
+          const int i = prow[e], jc = pcol[e];
+          if (jc <= i) sA[(i - 32 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b <= a; b++) {
+          double* s = &sA[r * kAsmStride + c + 32 * b];
+          R[sym_slot(a, b)] = (b == a) ? h * *s : *s;
+          *s = 0.0;
+        }
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int b = 0; b <= a; b++) R[sym_slot(a, b)] = 0.0;
+      }
+    }
+    symw_publish(R, 0, r, c, hd, wave, lane, srow, sv[0], &stau[0], sd, se);
+    __syncthreads();
+
+    for (int j = 0; j < n - 2; j++) {
+      const int cb = j & 1;
+      const double* v = sv[cb];
+      const double tau = uniform_f64(stau[cb]);  // reflection of column j
+      const int jn = j + 1;                      // the next column to reflect
+      const int bmin = jn >> 5;                  // blocks below bmin are eliminated
+      const unsigned live = ((1u << bmax) - 1u) & ~((1u << bmin) - 1u);
+      if (tau != 0.0) {  // uniform
+        const int la = jn & (kTriWWaves - 1);  // the wave that reflects column jn
+        symw_copy_col(R, jn, r, c, hd, srow);  // column jn as the previous step left it
+        const double s = symw_matvec(R, v, spr, spc, wave, lane, r, c, bmin, live);
+        __builtin_amdgcn_s_setprio(1);
+        const double qw = wave_sum_dpp(s) * tau;
+        if (lane == 0) sq[wave] = qw;
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(2);
+        // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v, assembled with p
+        if (tid < kTriWNP) {
+          double q4[4], t4[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            q4[q] = (sq[q] + sq[q + 4]) + (sq[q + 8] + sq[q + 12]);
+            t4[q] = (spc[q][tid] + spc[q + 4][tid]) + (spc[q + 8][tid] + spc[q + 12][tid]);
+          }
+          const double K = 0.5 * tau * ((q4[0] + q4[1]) + (q4[2] + q4[3]));
+          const double t = (spr[0][tid] + spr[1][tid]) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
+          const double pk = (tid > j && tid < np) ? t * tau : 0.0;
+          sw[tid] = pk - K * v[tid];
+        }
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(1);
+        if (wave == la) {
+          __builtin_amdgcn_s_setprio(3);
+          // column jn after this update, from its copy, and its reflection for the next step
+          const double vn = v[jn], wn = sw[jn];
+          double x[kTriWNP / 64];
+#pragma unroll
+          for (int q = 0; q < kTriWNP / 64; q++) {
+            const int i = lane + 64 * q;
+            x[q] = fma(-vn, sw[i], fma(-wn, v[i], srow[i]));
+          }
+          const double xd = fma(-vn, wn, fma(-wn, vn, srow[jn]));
+          const double x1 = fma(-vn, sw[jn + 1], fma(-wn, v[jn + 1], srow[jn + 1]));
+          householder_form(x, xd, x1, jn, lane, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+        } else {
+          // Q^T applied to conj(coil) and rho0 (real and imaginary parts): the four waves
+          // after la take one vector each
+          const int q = ((wave - la) & (kTriWWaves - 1)) - 1;
+          if (q < 4) {
+            double vv[kTriWNP / 64], uu[kTriWNP / 64];
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < kTriWNP / 64; k++) {
+              const int i = lane + 64 * k;
+              vv[k] = v[i];
+              uu[k] = su[q][i];
+              t = fma(vv[k], uu[k], t);
+            }
+            t = wave_sum_dpp(t) * tau;
+#pragma unroll
+            for (int k = 0; k < kTriWNP / 64; k++)
+              if (vv[k] != 0.0) su[q][lane + 64 * k] = uu[k] - t * vv[k];
+          }
+        }
+        double vr[kTriWBlk], wr[kTriWBlk];
+#pragma unroll
+        for (int a = 0; a < kTriWBlk; a++) {
+          vr[a] = v[r + 32 * a];
+          wr[a] = sw[r + 32 * a];
+        }
+        symw_update(R, vr, wr, v, sw, h, c, bmin, live);
+        __builtin_amdgcn_s_setprio(0);
+      } else {
+        // no reflection at this step: column jn stands as it is in the registers
+        symw_publish(R, jn, r, c, hd, wave, lane, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+      }
+      __syncthreads();
+    }
+    if (n >= 2) symw_publish(R, n - 1, r, c, hd, wave, lane, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
+    __syncthreads();
+    const int nm = p.spec_nm;  // kTriWNP
+    double* out = p.spec + (size_t)it * 6 * nm;
+    if (tid < kTriWNP) {
+      out[tid] = sd[tid];
+      out[nm + tid] = se[tid];
+#pragma unroll
+      for (int q = 0; q < 4; q++) out[(2 + q) * nm + tid] = su[q][tid];
     }
     __syncthreads();
   }
@@ -478,29 +779,36 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
 // parent merge). One 256-thread workgroup per orientation, everything in LDS.
 constexpr int kDcThreads = 256;
 constexpr int kDcLeaf = 6;
-constexpr int kDcMaxLevels = 5;  // up to 32 leaves
+constexpr int kDcMaxLevels = 6;  // up to 64 leaves (kTriWNP); 32 for kTriNP
 constexpr int kDcMaxIter = 100;  // secular-equation iterations per root (typically 3-8)
 constexpr int kDcPass1 = 3;      // evaluations per root before the open roots are packed (see M3a; 2 and 4-5 measured slower)
 
-struct DcShared {
-  double d[kTriNP], e[kTriNP];
-  double X[6][kTriNP];  // tracked vectors over the eigen-index of each segment
-  double Y[6][kTriNP];  // merge input, sorted
-  double D[kTriNP], z[kTriNP];
-  double Dc[kTriNP], Zc[kTriNP], Z2c[kTriNP], zh[kTriNP], tau[kTriNP];
-  double lorg[kTriNP];  // Dc[org] of each root: lambda = lorg + tau
-  int org[kTriNP], keep[kTriNP], defl[kTriNP];
-  int seg[kTriNP];  // merge index of each element at the current level (set by M1)
-  int bnd[kDcMaxLevels + 1][(1 << kDcMaxLevels) + 1];  // segment boundaries per level
-  double beta[(1 << kDcMaxLevels)];                     // rho = 2|coupling| of the merges of a level
-  double zsum[(1 << kDcMaxLevels)];                     // sum of the kept z^2 of each merge
-  int K[(1 << kDcMaxLevels)];
-  int its[kTriNP];
+// Sized for tridiagonal matrices of dimension at most NM (kTriNP or kTriWNP, with at most
+// kDcThreads = 256 roots per merge, one per thread).
+template <int NM>
+struct DcSharedT {
+  static constexpr int kNM = NM;
+  static constexpr int kLev = (NM <= kTriNP) ? 5 : kDcMaxLevels;  // leaves of <= kDcLeaf rows
+  double d[NM], e[NM];
+  double X[6][NM];  // tracked vectors over the eigen-index of each segment
+  double Y[6][NM];  // merge input, sorted
+  double D[NM], z[NM];
+  double Dc[NM], Zc[NM], Z2c[NM], zh[NM], tau[NM];
+  double lorg[NM];  // Dc[org] of each root: lambda = lorg + tau
+  int org[NM], keep[NM], defl[NM];
+  int seg[NM];  // merge index of each element at the current level (set by M1)
+  int bnd[kLev + 1][(1 << kLev) + 1];  // segment boundaries per level
+  double beta[(1 << kLev)];            // rho = 2|coupling| of the merges of a level
+  double zsum[(1 << kLev)];            // sum of the kept z^2 of each merge
+  int K[(1 << kLev)];
+  int its[NM];
   unsigned long long pend[kDcThreads / 64];  // roots the first secular pass left open, by thread
   int nlev;
   int fail;
   int max_iter;  // secular-equation evaluations allowed per root in the second pass
 };
+using DcShared = DcSharedT<kTriNP>;
+static_assert(kTriWNP <= kDcThreads && kDcLeaf << kDcMaxLevels >= kTriWNP, "eigensolver sizes");
 
 // 1/x: on the device v_rcp_f64 refined by two Newton steps (full double
 // precision from the hardware estimate), on the host a division
@@ -537,7 +845,8 @@ __host__ __device__ inline double dc_rcp_iter(double x) {
 
 // implicit QR on the segment [a, b) of (d, e) with the six tracked vectors
 // (Golub & Van Loan, algorithm 8.3.3); deflation at |e_i| <= eps*|T_seg|
-__host__ __device__ inline int dc_leaf_qr(DcShared& S, int a, int b) {
+template <class SH>
+__host__ __device__ inline int dc_leaf_qr(SH& S, int a, int b) {
   const int n = b - a;
   double anorm = 0.0;
   for (int i = a; i < b; i++) {
@@ -590,7 +899,8 @@ __host__ __device__ inline int dc_leaf_qr(DcShared& S, int a, int b) {
 }
 
 // insertion sort of the segment [a, b) by eigenvalue, carrying the vectors
-__host__ __device__ inline void dc_sort_segment(DcShared& S, int a, int b) {
+template <class SH>
+__host__ __device__ inline void dc_sort_segment(SH& S, int a, int b) {
   for (int i = a + 1; i < b; i++) {
     const double v = S.d[i];
     double x[6];
@@ -788,7 +1098,8 @@ __host__ __device__ __attribute__((always_inline)) inline bool dc_secular_run(co
 
 
 // merge index of element i at level l (bounded by the level's segment count)
-__host__ __device__ inline int dc_segment_of(const DcShared& S, int l, int i) {
+template <class SH>
+__host__ __device__ inline int dc_segment_of(const SH& S, int l, int i) {
   const int last = (1 << l) - 1;
   int s2 = 0;
   while (s2 < last && S.bnd[l][s2 + 1] <= i) s2++;
@@ -796,11 +1107,12 @@ __host__ __device__ inline int dc_segment_of(const DcShared& S, int l, int i) {
 }
 
 // phase 0: load T and the four vectors (every thread)
-__host__ __device__ inline void dc_phase_load(DcShared& S, const double* gsrc, int tid, int nthreads) {
-  for (int i = tid; i < kTriNP; i += nthreads) {
+template <class SH>
+__host__ __device__ inline void dc_phase_load(SH& S, const double* gsrc, int nm, int tid, int nthreads) {
+  for (int i = tid; i < SH::kNM; i += nthreads) {
     S.d[i] = gsrc[i];
-    S.e[i] = gsrc[kTriNP + i];
-    for (int q = 0; q < 4; q++) S.X[q][i] = gsrc[(2 + q) * kTriNP + i];
+    S.e[i] = gsrc[nm + i];
+    for (int q = 0; q < 4; q++) S.X[q][i] = gsrc[(2 + q) * nm + i];
     S.X[4][i] = 0.0;
     S.X[5][i] = 0.0;
     S.its[i] = 0;
@@ -808,9 +1120,10 @@ __host__ __device__ inline void dc_phase_load(DcShared& S, const double* gsrc, i
 }
 
 // phase 1 (one thread): the segment tree and the rank-one tears
-__host__ __device__ inline void dc_phase_plan(DcShared& S, int n, int max_iter = kDcMaxIter) {
+template <class SH>
+__host__ __device__ inline void dc_phase_plan(SH& S, int n, int max_iter = kDcMaxIter) {
   int L = 0;
-  while (L < kDcMaxLevels && ((n + (1 << L) - 1) >> L) > kDcLeaf) L++;
+  while (L < SH::kLev && ((n + (1 << L) - 1) >> L) > kDcLeaf) L++;
   S.nlev = L;
   S.fail = 0;
   S.max_iter = max_iter;
@@ -835,7 +1148,8 @@ __host__ __device__ inline void dc_phase_plan(DcShared& S, int n, int max_iter =
 }
 
 // phase 2: one thread per leaf
-__host__ __device__ inline void dc_phase_leaf(DcShared& S, int tid) {
+template <class SH>
+__host__ __device__ inline void dc_phase_leaf(SH& S, int tid) {
   const int L = S.nlev;
   if (tid >= (1 << L)) return;
   const int a = S.bnd[L][tid], b = S.bnd[L][tid + 1];
@@ -847,7 +1161,8 @@ __host__ __device__ inline void dc_phase_leaf(DcShared& S, int tid) {
 }
 
 // M1: merge-path of the two sorted children into Y, D, z (one thread per element)
-__host__ __device__ inline void dc_phase_m1(DcShared& S, int l, int n, int i) {
+template <class SH>
+__host__ __device__ inline void dc_phase_m1(SH& S, int l, int n, int i) {
   if (i >= n) return;
   const int s2 = dc_segment_of(S, l, i);
   S.seg[i] = s2;
@@ -878,7 +1193,8 @@ __host__ __device__ inline void dc_phase_m1(DcShared& S, int l, int n, int i) {
 // poles rotated so that one z component vanishes; the kept poles are compacted
 // into Dc, Zc, Z2c as they are found (a later rotation only removes the last kept
 // one, whose slot the rotated pole takes)
-__host__ __device__ inline void dc_phase_m2(DcShared& S, int l, int s2) {
+template <class SH>
+__host__ __device__ inline void dc_phase_m2(SH& S, int l, int s2) {
   if (s2 >= (1 << l)) return;
   if (s2 == 0)
     for (int w = 0; w < kDcThreads / 64; w++) S.pend[w] = 0;
@@ -959,7 +1275,8 @@ __host__ __device__ inline int dc_select_bit(unsigned long long m, int r) {
   return pos;
 }
 
-__host__ __device__ inline void dc_root_store(DcShared& S, int i, int a, int k, const SecState& st) {
+template <class SH>
+__host__ __device__ inline void dc_root_store(SH& S, int i, int a, int k, const SecState& st) {
   if (st.it > S.its[i]) S.its[i] = st.it;
   S.org[a + k] = st.org;
   S.tau[a + k] = st.tau;
@@ -971,7 +1288,8 @@ __host__ __device__ inline void dc_root_store(DcShared& S, int i, int a, int k, 
 // between M1 and M5) at its thread index and its bit in pend, so that M3b can
 // finish the open roots packed into as few waves as they fill: a wave's
 // iterations cost its slowest lane, and most roots converge in 2-4.
-__host__ __device__ inline void dc_phase_m3a(DcShared& S, int l, int n, int i) {
+template <class SH>
+__host__ __device__ inline void dc_phase_m3a(SH& S, int l, int n, int i) {
   if (i >= n) return;
   const int s2 = S.seg[i];
   const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
@@ -997,7 +1315,8 @@ __host__ __device__ inline void dc_phase_m3a(DcShared& S, int l, int n, int i) {
 }
 
 // M3b: thread u finishes the u-th open root (in thread order)
-__host__ __device__ inline void dc_phase_m3b(DcShared& S, int l, int u) {
+template <class SH>
+__host__ __device__ inline void dc_phase_m3b(SH& S, int l, int u) {
   int w = 0, base = 0;
   for (; w < kDcThreads / 64; w++) {
     const int c = __builtin_popcountll(S.pend[w]);
@@ -1024,7 +1343,8 @@ __host__ __device__ inline void dc_phase_m3b(DcShared& S, int l, int u) {
 // M4: z recomputed from the computed roots (Gu & Eisenstat), one thread per (merge, pole):
 // z_i^2 = (lambda_i - d_i)/rho prod_{k != i} (lambda_k - d_i)/(d_k - d_i), the products
 // taken four factors at a time with one reciprocal per group
-__host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
+template <class SH>
+__host__ __device__ inline void dc_phase_m4(SH& S, int l, int n, int t) {
   if (t >= n) return;
   const int s2 = S.seg[t];
   const int a = S.bnd[l][s2], i = t - a, K = S.K[s2];
@@ -1047,7 +1367,8 @@ __host__ __device__ inline void dc_phase_m4(DcShared& S, int l, int n, int t) {
 }
 
 // M5: eigenpairs of each merge into d, X: roots in slots 0..K-1, deflated pairs after
-__host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
+template <class SH>
+__host__ __device__ inline void dc_phase_m5(SH& S, int l, int n, int t) {
   if (t >= n) return;
   const int s2 = S.seg[t];
   const int a = S.bnd[l][s2], slot = t - a, K = S.K[s2];
@@ -1086,8 +1407,9 @@ __host__ __device__ inline void dc_phase_m5(DcShared& S, int l, int n, int t) {
 
 // M6a/M6b: rank sort of each merged segment (the rank into stage_*, then the scatter)
 // (the ranks and copies are staged in org, D and Y, which the merge no longer needs)
-__host__ __device__ inline void dc_phase_m6a(DcShared& S, int l, int n, int t) {
-  if (t >= kTriNP) return;
+template <class SH>
+__host__ __device__ inline void dc_phase_m6a(SH& S, int l, int n, int t) {
+  if (t >= SH::kNM) return;
   S.org[t] = -1;
   if (t >= n) return;
   const int s2 = S.seg[t];
@@ -1103,8 +1425,9 @@ __host__ __device__ inline void dc_phase_m6a(DcShared& S, int l, int n, int t) {
   for (int q = 0; q < 6; q++) S.Y[q][t] = S.X[q][t];
 }
 
-__host__ __device__ inline void dc_phase_m6b(DcShared& S, int t) {
-  if (t >= kTriNP) return;
+template <class SH>
+__host__ __device__ inline void dc_phase_m6b(SH& S, int t) {
+  if (t >= SH::kNM) return;
   const int pos = S.org[t];
   if (pos < 0) return;
   S.d[pos] = S.D[t];
@@ -1112,23 +1435,26 @@ __host__ __device__ inline void dc_phase_m6b(DcShared& S, int t) {
 }
 
 // output: eigenvalues and g_k = a_k * b_k (a = transformed conj(coil), b = transformed rho0)
-__host__ __device__ inline void dc_phase_store(const DcShared& S, double* gsrc, int n, int tid, int nthreads) {
+template <class SH>
+__host__ __device__ inline void dc_phase_store(const SH& S, double* gsrc, int nm, int n, int tid, int nthreads) {
   for (int k = tid; k < n; k += nthreads) {
     const double ar = S.X[0][k], ai = S.X[1][k], br = S.X[2][k], bi = S.X[3][k];
     gsrc[k] = S.d[k];
-    gsrc[2 * kTriNP + k] = ar * br - ai * bi;
-    gsrc[3 * kTriNP + k] = ar * bi + ai * br;
+    gsrc[2 * nm + k] = ar * br - ai * bi;
+    gsrc[3 * nm + k] = ar * bi + ai * br;
   }
 }
 
+template <int NM>
 __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
-  __shared__ DcShared S;
+  __shared__ DcSharedT<NM> S;
   const int tid = threadIdx.x;
   const int64_t it = blockIdx.x;
   if (it >= p.n_items) return;
   const int n = p.sys[p.items[it].sys].n;
-  double* gsrc = p.spec + (size_t)it * kSpecStride;
-  dc_phase_load(S, gsrc, tid, kDcThreads);
+  const int nm = p.spec_nm;
+  double* gsrc = p.spec + (size_t)it * 6 * nm;
+  dc_phase_load(S, gsrc, nm, tid, kDcThreads);
   __syncthreads();
   if (tid == 0) dc_phase_plan(S, n, p.dc_max_iter > 0 ? p.dc_max_iter : kDcMaxIter);
   __syncthreads();
@@ -1160,11 +1486,11 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
       __syncthreads();
     }
   }
-  dc_phase_store(S, gsrc, n, tid, kDcThreads);
+  dc_phase_store(S, gsrc, nm, n, tid, kDcThreads);
   __syncthreads();
   if (tid == 0) {
     int mx = 0;
-    for (int i = 0; i < kTriNP; i++) mx = (S.its[i] > mx) ? S.its[i] : mx;
+    for (int i = 0; i < NM; i++) mx = (S.its[i] > mx) ? S.its[i] : mx;
     p.status[it] = S.fail ? -1 : mx;
   }
 }
@@ -1178,10 +1504,11 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
 // one sincos each; an entry is its group's start (a binary power of the 8- or
 // 512-step, at most four rounded products) advanced by the 1- or 64-step (at
 // most seven more), so every phase factor carries at most ~12 roundings.
+template <int NM>
 struct SynthLds {
-  double s1r[kTriNP], s1i[kTriNP], s8r[kTriNP], s8i[kTriNP];
-  double s64r[kTriNP], s64i[kTriNP], s512r[kTriNP], s512i[kTriNP];
-  double gr[kTriNP], gi[kTriNP];  // g_k e^{-i lambda_k 64 i0} for the current row tile
+  double s1r[NM], s1i[NM], s8r[NM], s8i[NM];
+  double s64r[NM], s64i[NM], s512r[NM], s512i[NM];
+  double gr[NM], gi[NM];  // g_k e^{-i lambda_k 64 i0} for the current row tile
   GemmLds<double> g;
   double red[kThreads / 64][2];  // per-wave parts of a tail point
   int sitem;
@@ -1193,9 +1520,10 @@ __device__ __forceinline__ void cmul(double& ar, double& ai, double br, double b
   ar = t;
 }
 
+template <int NM>
 __global__ void __launch_bounds__(kThreads, 3) k_synth(SpecParams p) {
   typedef Mma<double>::v4 v4;
-  __shared__ SynthLds S;
+  __shared__ SynthLds<NM> S;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;  // the wave's 32 x 32 quarter of the 64 x 64 tile
@@ -1210,9 +1538,9 @@ __global__ void __launch_bounds__(kThreads, 3) k_synth(SpecParams p) {
     const Item item = p.items[it];
     const SysDesc sdesc = p.sys[item.sys];
     const int n = sdesc.n, np = sdesc.np;
-    const double* lam = p.spec + (size_t)it * kSpecStride;
-    const double* gre = lam + 2 * kTriNP;
-    const double* gim = lam + 3 * kTriNP;
+    const double* lam = p.spec + (size_t)it * 6 * p.spec_nm;
+    const double* gre = lam + 2 * p.spec_nm;
+    const double* gim = lam + 3 * p.spec_nm;
     double* out = p.per_or + 2 * (size_t)item.orient * npts;
     for (int k = tid; k < np; k += kThreads) {
       double c1 = 1.0, n1 = 0.0, c8 = 1.0, n8 = 0.0, c64 = 1.0, n64 = 0.0, c512 = 1.0, n512 = 0.0;

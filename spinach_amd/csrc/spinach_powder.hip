@@ -127,8 +127,10 @@ struct sp_plan {
   sp_stats last{};
   int64_t full_dim = 0, red_dim = 0, red_nnz = 0;
   int32_t real_systems = 0;
-  // spectral path (real generators, FP64, padded dimension <= spk::kTriNP)
-  int64_t n_prop = 0, n_spec = 0;
+  // spectral path (real generators, FP64, padded dimension <= spk::kTriWNP); the first
+  // n_spec_w spectral items have a padded dimension above spk::kTriNP (k_tridiag_w)
+  int64_t n_prop = 0, n_spec = 0, n_spec_w = 0;
+  int spec_nm = spk::kTriNP;  // per-item spectrum layout: 6 vectors of spec_nm doubles
   int64_t n_prop_complex = 0;  // the first n_prop_complex propagator items have a complex generator
   int32_t spectral_systems = 0;
   // stepping path (SP_MODE_KRYLOV, dimensions >= krylov_tol or above the dense limit)
@@ -333,7 +335,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
       throw SpError(SP_EINVAL, "a relaxation superoperator runs on the propagator path only (reduced dimension <= " +
                                    std::to_string(kDenseMaxDim) + ", below krylov_tol, no SP_MODE_KRYLOV)");
     spectral[s] =
-        (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriNP) ? 1 : 0;
+        (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriWNP) ? 1 : 0;
     if (spectral[s])
       np_spec = std::max(np_spec, sps[s].np);
     else if (!krylov[s])
@@ -395,7 +397,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
       rdiag.insert(rdiag.end(), sp.rdiag.begin(), sp.rdiag.end());
     }
     pl->relax_systems += sp.relax ? 1 : 0;
-    for (int k = 0; k < 13; k++)
+    for (int k = 0; k < spk::kRowBlocks; k++)
       d.rb[k] = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 16 * k) - sp.row.begin());
     pl->real_systems += d.real;
     pl->spectral_systems += spectral[s];
@@ -453,7 +455,13 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
                      if (ra != rb) return ra < rb;
                      return a.first > b.first;
                    });
-  std::stable_sort(spec_items.begin(), spec_items.end(), by_cost);
+  // spectral items: those above the 256-thread layout's kTriNP first (k_tridiag_w), then the rest
+  std::stable_sort(spec_items.begin(), spec_items.end(),
+                   [&](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
+                     const int wa = descs[a.second.sys].np > spk::kTriNP, wb = descs[b.second.sys].np > spk::kTriNP;
+                     if (wa != wb) return wa > wb;
+                     return a.first > b.first;
+                   });
   // stepping items: complex generators first, then real ones (two launches)
   std::stable_sort(kry_items.begin(), kry_items.end(),
                    [&](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
@@ -478,6 +486,10 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   if (const char* e = getenv("SPINACH_DC_MAX_ITER")) pl->dc_max_iter = std::max(0, atoi(e));
   pl->n_prop = (int64_t)prop_items.size();
   pl->n_spec = (int64_t)spec_items.size();
+  pl->n_spec_w = 0;
+  for (auto& c : spec_items)
+    if (descs[c.second.sys].np > spk::kTriNP) pl->n_spec_w++;
+  pl->spec_nm = pl->n_spec_w > 0 ? spk::kTriWNP : spk::kTriNP;
   pl->n_kry = (int64_t)kry_items.size();
   pl->n_items = pl->n_prop + pl->n_spec + pl->n_kry;
   pl->total_or = or_base;
@@ -531,9 +543,10 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   }
   if (pl->n_spec > 0) {
     // two 256-thread workgroups per CU, each holding the lower triangle of a 192 x 192
-    // matrix in registers; three synthesis workgroups per CU
+    // matrix in registers (one 1024-thread workgroup per CU for larger systems); three
+    // synthesis workgroups per CU
     configure_spec(ctx, pl.get(), 2, 3);
-    pl->d_spec.alloc((size_t)pl->n_spec * spk::kSpecStride * sizeof(double));
+    pl->d_spec.alloc((size_t)pl->n_spec * 6 * pl->spec_nm * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
   if (pl->n_kry > 0) setup_step(ctx, pl.get());
@@ -616,6 +629,7 @@ spk::SpecParams spec_params(sp_plan* pl) {
   sp.dt = pl->dt;
   sp.nsteps = pl->nsteps;
   sp.spec = pl->d_spec.as<double>();
+  sp.spec_nm = pl->spec_nm;
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
   sp.dc_max_iter = pl->dc_max_iter;
@@ -692,18 +706,52 @@ void launch_krylov(sp_plan* pl) {
   if (n_r > 0) launch_step_items(pl, true, items + n_c, n_r, 5, false, pl->dt, pl->nsteps, nullptr);
 }
 
-// The three spectral kernels; ev[0..2] are recorded after each of them.
+// The spectral items from `first` on, n of them (item, spectrum and status pointers offset).
+spk::SpecParams spec_params_range(sp_plan* pl, int64_t first, int64_t n) {
+  spk::SpecParams sp = spec_params(pl);
+  sp.items += first;
+  sp.n_items = n;
+  sp.spec += (size_t)first * 6 * pl->spec_nm;
+  sp.status += first;
+  return sp;
+}
+
+// The three spectral kernels; ev[0..2] are recorded after each of them. Items above kTriNP
+// (the first n_spec_w) take the 1024-thread tridiagonalisation (queue head 8) and the
+// eigensolver sized for kTriWNP; the synthesis runs over every spectral item.
 void launch_spectral(sp_plan* pl, hipEvent_t* ev) {  // ev == nullptr: no stage events (graph recording)
   hipStream_t st = pl->ctx->stream;
-  spk::SpecParams tri = spec_params(pl);
-  hipLaunchKernelGGL(spk::k_tridiag, dim3(pl->grid_tri), dim3(spk::kTriThreads), 0, st, tri);
-  HIPCHK(hipGetLastError());
+  const int64_t nw = pl->n_spec_w, nn = pl->n_spec - pl->n_spec_w;
+  if (nw > 0) {
+    spk::SpecParams tw = spec_params_range(pl, 0, nw);
+    tw.queue = pl->d_queue.as<unsigned int>() + 8;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)pl->ctx->n_cu, nw));
+    hipLaunchKernelGGL(spk::k_tridiag_w, dim3(g), dim3(spk::kTriWThreads), 0, st, tw);
+    HIPCHK(hipGetLastError());
+  }
+  if (nn > 0) {
+    spk::SpecParams tri = spec_params_range(pl, nw, nn);
+    hipLaunchKernelGGL(spk::k_tridiag, dim3(std::max<int64_t>(1, std::min<int64_t>(pl->grid_tri, nn))),
+                       dim3(spk::kTriThreads), 0, st, tri);
+    HIPCHK(hipGetLastError());
+  }
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)pl->n_spec), dim3(spk::kDcThreads), 0, st, tri);
-  HIPCHK(hipGetLastError());
+  if (nw > 0) {
+    spk::SpecParams tw = spec_params_range(pl, 0, nw);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriWNP>, dim3((unsigned)nw), dim3(spk::kDcThreads), 0, st, tw);
+    HIPCHK(hipGetLastError());
+  }
+  if (nn > 0) {
+    spk::SpecParams tri = spec_params_range(pl, nw, nn);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriNP>, dim3((unsigned)nn), dim3(spk::kDcThreads), 0, st, tri);
+    HIPCHK(hipGetLastError());
+  }
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
   spk::SpecParams syn = spec_params(pl);
-  hipLaunchKernelGGL(spk::k_synth, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
+  if (pl->spec_nm == spk::kTriWNP)
+    hipLaunchKernelGGL(spk::k_synth<spk::kTriWNP>, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
+  else
+    hipLaunchKernelGGL(spk::k_synth<spk::kTriNP>, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
 }
@@ -1197,7 +1245,7 @@ int noalloc_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al,
   d.nq = (int32_t)sp.slots.size();
   d.n_or = n_or;
   d.real = 1;
-  for (int k = 0; k < 13; k++)
+  for (int k = 0; k < spk::kRowBlocks; k++)
     d.rb[k] = (int32_t)(std::lower_bound(sp.row.begin(), sp.row.end(), 16 * k) - sp.row.begin());
   std::vector<spk::SysDesc> dv(1, d);
   auto* d_sys = a.take<spk::SysDesc>(1);
@@ -1281,15 +1329,16 @@ int noalloc_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al,
     sp2.dt = dt;
     sp2.nsteps = nsteps;
     sp2.spec = d_spec;
+    sp2.spec_nm = spk::kTriNP;
     sp2.status = d_status;
     sp2.per_or = d_per;
     const int g_tri = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)ctx->n_cu, nb));
     const int g_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, nb));
     hipLaunchKernelGGL(spk::k_tridiag, dim3(g_tri), dim3(spk::kTriThreads), 0, st, sp2);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(spk::k_dceig, dim3((unsigned)nb), dim3(spk::kDcThreads), 0, st, sp2);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriNP>, dim3((unsigned)nb), dim3(spk::kDcThreads), 0, st, sp2);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(spk::k_synth, dim3(g_syn), dim3(spk::kThreads), 0, st, sp2);
+    hipLaunchKernelGGL(spk::k_synth<spk::kTriNP>, dim3(g_syn), dim3(spk::kThreads), 0, st, sp2);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(spk::k_chunk_sum, dim3(nbt, (unsigned)chunks.size()), dim3(256), 0, st, d_per, d_w + b0,
                        d_chunks, npts, d_partial);

@@ -6,12 +6,12 @@
 
 using namespace spk;
 
-extern "C" int dc_host_solve(const double* in, int n, double* out, int* status, int* levels) {
-  static DcShared S;
-  if (n < 0 || n > kTriNP) return 1;
+template <int NM>
+static int dc_host_solve_t(const double* in, int n, double* out, int* status, int* levels) {
+  static DcSharedT<NM> S;
   const int T = kDcThreads;
-  for (int i = 0; i < kSpecStride; i++) out[i] = in[i];
-  for (int t = 0; t < T; t++) dc_phase_load(S, out, t, T);
+  for (int i = 0; i < 6 * NM; i++) out[i] = in[i];
+  for (int t = 0; t < T; t++) dc_phase_load(S, out, NM, t, T);
   dc_phase_plan(S, n);
   *levels = S.nlev;
   for (int t = 0; t < T; t++) dc_phase_leaf(S, t);
@@ -27,11 +27,19 @@ extern "C" int dc_host_solve(const double* in, int n, double* out, int* status, 
       for (int t = 0; t < T; t++) dc_phase_m6b(S, t);
     }
   }
-  for (int t = 0; t < T; t++) dc_phase_store(S, out, n, t, T);
+  for (int t = 0; t < T; t++) dc_phase_store(S, out, NM, n, t, T);
   int mx = 0;
-  for (int i = 0; i < kTriNP; i++) mx = S.its[i] > mx ? S.its[i] : mx;
+  for (int i = 0; i < NM; i++) mx = S.its[i] > mx ? S.its[i] : mx;
   *status = S.fail ? -1 : mx;
   return 0;
+}
+
+// in/out: 6 * nm doubles (d | e | four vectors), nm = kTriNP (n <= kTriNP, the solver sized for
+// the 256-thread tridiagonalisation) or kTriWNP (n <= kTriWNP, the one sized for k_tridiag_w)
+extern "C" int dc_host_solve(const double* in, int n, int nm, double* out, int* status, int* levels) {
+  if (nm == kTriNP && n >= 0 && n <= kTriNP) return dc_host_solve_t<kTriNP>(in, n, out, status, levels);
+  if (nm == kTriWNP && n >= 0 && n <= kTriWNP) return dc_host_solve_t<kTriWNP>(in, n, out, status, levels);
+  return 1;
 }
 
 // the per-slot Wigner coefficients (one thread per slot on the device) against

@@ -14,7 +14,8 @@ import pyoracle as O
 from spinach_amd import workloads
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-NP = 192
+NP = 192    # the solver sized for the 256-thread tridiagonalisation
+NPW = 256   # the one sized for the 1024-thread tridiagonalisation (dimensions 193-256)
 
 
 @pytest.fixture(scope='module')
@@ -25,8 +26,9 @@ def harness(tmp_path_factory):
                     '-I', os.path.join(ROOT, 'include'), src, '-o', str(out)], check=True,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     lib = ctypes.CDLL(str(out))
-    lib.dc_host_solve.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.dc_host_solve.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)]
     return lib
 
 
@@ -36,22 +38,23 @@ def tridiagonalise(A, vecs):
     return np.diag(H).copy(), np.diag(H, 1).copy(), [Q.T @ v for v in vecs]
 
 
-def solve(lib, A, coil, rho, nsteps=4096):
+def solve(lib, A, coil, rho, nsteps=4096, nm=None):
     n = len(A)
+    nm = nm or (NP if n <= NP else NPW)
     cc = np.conj(coil)
     d, e, (cr, ci, rr, ri) = tridiagonalise(A, [cc.real, cc.imag, rho.real, rho.imag])
-    buf = np.zeros(6 * NP)
+    buf = np.zeros(6 * nm)
     buf[:n] = d
-    buf[NP:NP + n - 1] = e
+    buf[nm:nm + n - 1] = e
     for q, v in enumerate((cr, ci, rr, ri)):
-        buf[(2 + q) * NP:(2 + q) * NP + n] = v
-    out = np.zeros(6 * NP)
+        buf[(2 + q) * nm:(2 + q) * nm + n] = v
+    out = np.zeros(6 * nm)
     st, lev = ctypes.c_int(), ctypes.c_int()
     dp = ctypes.POINTER(ctypes.c_double)
-    assert lib.dc_host_solve(buf.ctypes.data_as(dp), n, out.ctypes.data_as(dp), ctypes.byref(st),
+    assert lib.dc_host_solve(buf.ctypes.data_as(dp), n, nm, out.ctypes.data_as(dp), ctypes.byref(st),
                              ctypes.byref(lev)) == 0
     lam = out[:n]
-    g = out[2 * NP:2 * NP + n] + 1j * out[3 * NP:3 * NP + n]
+    g = out[2 * nm:2 * nm + n] + 1j * out[3 * nm:3 * nm + n]
     w, V = np.linalg.eigh(A)
     gref = (V.T @ cc) * (V.T @ rho)
     t = np.arange(nsteps + 1)
@@ -60,7 +63,7 @@ def solve(lib, A, coil, rho, nsteps=4096):
     return (np.abs(np.sort(lam) - w).max(), np.abs(f - fref).max() / np.abs(fref).max(), st.value, lev.value)
 
 
-@pytest.mark.parametrize('n', [1, 2, 3, 12, 13, 25, 97, 150, 192])
+@pytest.mark.parametrize('n', [1, 2, 3, 12, 13, 25, 97, 150, 192, 193, 231, 256])
 def test_random_symmetric(harness, n):
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
@@ -72,7 +75,7 @@ def test_random_symmetric(harness, n):
     assert le < 1e-13 and fe < 1e-10, (le, fe)
 
 
-@pytest.mark.parametrize('n', [30, 96, 192])
+@pytest.mark.parametrize('n', [30, 96, 192, 252])
 def test_degenerate_spectra(harness, n):
     # six-fold degenerate eigenvalues in a random basis: deflation by rotation
     rng = np.random.default_rng(100 + n)
