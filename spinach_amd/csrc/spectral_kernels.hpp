@@ -29,6 +29,7 @@ namespace spk {
 
 constexpr int kTriNP = 192;      // largest padded dimension of the 256-thread register layout
 constexpr int kTriWNP = 256;     // largest padded dimension of the 1024-thread layout (k_tridiag_w)
+constexpr int kTriGNP = 448;     // largest padded dimension with k_tridiag_w's global-memory prologue
 constexpr int kSpecStride = 6 * kTriNP;  // per item: d | e | four vectors (spec_nm = kTriNP)
 constexpr int kSynthB = 64;      // time block of the synthesis product
 
@@ -52,7 +53,9 @@ struct SpecParams {
   double dt;
   int64_t nsteps;
   double* spec;          // 6 * spec_nm doubles per spectral item: d | e | four vectors, each spec_nm long
-  int spec_nm;           // kTriNP, or kTriWNP in plans with systems above kTriNP
+  int spec_nm;           // kTriNP, kTriWNP or kTriGNP: the largest padded dimension class of the plan
+  double* tri_work;      // k_tridiag_w's prologue: kTriGWork doubles per workgroup (plans above kTriWNP)
+  int64_t tri_work_stride;
   int32_t* status;       // per item: most secular evaluations of a root, or -1 if a solve failed
   int32_t dc_max_iter;   // secular evaluations per root of the second pass (0: kDcMaxIter)
   double* per_or;        // per-orientation FIDs, interleaved complex, (nsteps+1) per orientation
@@ -587,28 +590,125 @@ __device__ __forceinline__ void symw_update(double (&R)[kTriWSlots], const doubl
   }
 }
 
+// Global-memory prologue of k_tridiag_w for padded dimensions kTriWNP < np <= kTriGNP: the
+// first k0 = np - kTriWNP (rounded up to a multiple of 32) columns are reduced with the matrix in
+// the workgroup's slot of a global workspace, after which the trailing block (at most kTriWNP
+// rows) is loaded into the register layout and the reduction continues there. The lower
+// triangle is stored as 32 x 32 blocks (block (a, b), b <= a, at sym_slot(a, b) * 1024, row-major
+// within the block), h-weighted in the diagonal blocks as the register layout is, so the
+// hand-off is a copy. Each step makes one pass over the live blocks, which applies the previous
+// step's rank-2 update, stores the block and forms the next symmetric product from the updated
+// values (the next reflection comes first, from its column updated in LDS).
+constexpr int kTriGBlk = kTriGNP / 32;                          // 14 block rows
+constexpr int kTriGBlocks = kTriGBlk * (kTriGBlk + 1) / 2;      // 105 stored blocks
+constexpr int64_t kTriGWork = (int64_t)kTriGBlocks * 1024;      // workspace doubles per workgroup
+
+// The pass over the live blocks (block rows and columns >= bmin, below nbk), split into equal
+// runs of consecutive blocks (row-major) over the 16 waves. Lane (c, h2) = (lane % 32, lane / 32)
+// takes column c and rows 16 h2 .. 16 h2 + 15 of each block. When upd: A <- A - v w' - w v'
+// (vo, wo) and the block is stored back; when prod: row sums of A*vn (reduced over the 32 lanes
+// of each half-wave, one per row index and wave) to rowp, column sums (the two halves added) to
+// colp, and the lane's part of vn'A vn is returned. rowp and colp are zero beforehand.
+__device__ __forceinline__ double trig_pass(double* W, int nbk, int bmin, bool upd, bool prod, const double* vo,
+                                            const double* wo, const double* vn, double (*rowp)[kTriGNP],
+                                            double (*colp)[kTriGNP], int wave, int lane) {
+  const int m = nbk - bmin;
+  const int L = m * (m + 1) / 2;
+  const int f0 = (L * wave) / kTriWWaves, f1 = (L * (wave + 1)) / kTriWWaves;
+  const int c = lane & 31, h2 = lane >> 5;
+  double s = 0.0;
+  if (f0 >= f1) return s;
+  int ap = 0;
+  while ((ap + 1) * (ap + 2) / 2 <= f0) ap++;
+  int bp = f0 - ap * (ap + 1) / 2;
+  double racc[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) racc[i] = 0.0;
+  for (int f = f0; f < f1; f++) {
+    const int a = bmin + ap, b = bmin + bp;
+    double* blk = W + (size_t)sym_slot(a, b) * 1024 + 512 * h2 + c;
+    const int C = 32 * b + c, R0 = 32 * a + 16 * h2;
+    const double vC = vo[C], wC = wo[C], nC = vn[C];
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = blk[32 * i];
+    if (upd) {
+      if (a == b) {  // h-weighted diagonal block
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int r = 16 * h2 + i;
+          const double hh = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
+          x[i] = fma(-hh * vo[R0 + i], wC, fma(-hh * wo[R0 + i], vC, x[i]));
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) x[i] = fma(-vo[R0 + i], wC, fma(-wo[R0 + i], vC, x[i]));
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) blk[32 * i] = x[i];
+    }
+    if (prod) {
+      double cacc = 0.0;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        racc[i] = fma(x[i], nC, racc[i]);
+        cacc = fma(x[i], vn[R0 + i], cacc);
+      }
+      s = fma(nC, cacc, s);
+      cacc += __shfl_xor(cacc, 32);  // rows of the other half, same column
+      if (h2 == 0) colp[wave][C] += cacc;
+    }
+    // next block; the block row's sums are complete when the run leaves it
+    const bool row_end = (bp == ap) || (f + 1 == f1);
+    if (prod && row_end) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        double t = sum16(racc[i]);
+        t += __shfl_xor(t, 16);  // the other 16 columns of the half
+        racc[i] = t;
+      }
+      if (c == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) rowp[wave][R0 + i] = racc[i];
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) racc[i] = 0.0;
+    }
+    if (++bp > ap) {
+      ap++;
+      bp = 0;
+    }
+  }
+  return 2.0 * s;  // the row sums' half of vn'A vn equals the column sums' half
+}
+
 __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
-  __shared__ double sv[2][kTriWNP];  // Householder vectors, double-buffered
+  __shared__ double sv[2][kTriGNP];  // Householder vectors, double-buffered
   __shared__ double stau[2];
-  __shared__ double srow[kTriWNP + 1];  // the column being reflected; srow[kTriWNP] = 0
-  __shared__ double sw[kTriWNP];        // w = p - (tau/2)(v'p) v, p = tau*A*v
+  __shared__ double srow[kTriGNP + 1];  // the column being reflected; zero beyond np
+  __shared__ double sw[kTriGNP];        // w = p - (tau/2)(v'p) v, p = tau*A*v
   __shared__ double spr[2][kTriWNP];    // row sums of A*v over columns c < 16 and c >= 16
-  __shared__ double spc[kTriWWaves][kTriWNP];  // column sums of A*v, one row per wave
   __shared__ double sq[kTriWWaves];
-  __shared__ double su[4][kTriWNP];
-  __shared__ double sd[kTriWNP], se[kTriWNP];
+  __shared__ double su[4][kTriGNP];
+  __shared__ double sd[kTriGNP], se[kTriGNP];
   __shared__ double cre[34], cim[34];
   __shared__ int sitem;
-  // the 32 rows of the generator under assembly (block row a in pass a); all zero between uses
+  // one region, used in turn by the assembly tile and the column sums of the register layout and
+  // by the prologue's partial sums
   constexpr int kAsmStride = kTriWNP + 4;
-  __shared__ double sA[32 * kAsmStride];
+  __shared__ double lds_u[2 * kTriWWaves * kTriGNP];
+  static_assert(32 * kAsmStride <= 2 * kTriWWaves * kTriGNP, "assembly tile");
+  double* sA = lds_u;
+  double (*spc)[kTriWNP] = reinterpret_cast<double (*)[kTriWNP]>(lds_u);  // column sums, one row per wave
+  double (*rowp)[kTriGNP] = reinterpret_cast<double (*)[kTriGNP]>(lds_u);
+  double (*colp)[kTriGNP] = reinterpret_cast<double (*)[kTriGNP]>(lds_u + kTriWWaves * kTriGNP);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r = tid >> 5, c = tid & 31;
   const double h = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
   const double hd = (r == c) ? 2.0 : 1.0;
-  for (int k = tid; k < 32 * kAsmStride; k += kTriWThreads) sA[k] = 0.0;
-  for (int k = tid; k <= kTriWNP; k += kTriWThreads) srow[k] = 0.0;
+  for (int k = tid; k <= kTriGNP; k += kTriWThreads) srow[k] = 0.0;
+  double* W = p.tri_work ? p.tri_work + (size_t)blockIdx.x * p.tri_work_stride : nullptr;
 
   for (;;) {
     if (tid == 0) sitem = (int)atomicAdd(&p.queue[0], 1u);
@@ -619,10 +719,13 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
     const Item item = p.items[it];
     const SysDesc sdesc = p.sys[item.sys];
     const int n = __builtin_amdgcn_readfirstlane(sdesc.n), np = __builtin_amdgcn_readfirstlane(sdesc.np);
-    const int bmax = (np + 31) >> 5;  // blocks below the padded dimension
+    // columns reduced by the prologue (a multiple of 32, so the trailing block is block-aligned)
+    const int k0 = (np > kTriWNP) ? ((np - kTriWNP + 31) >> 5) << 5 : 0;
+    const int n2 = n - k0, np2 = np - k0;
+    const int bmax = (np2 + 31) >> 5;  // blocks below the padded dimension of the register stage
     const int64_t o = item.orient;
     if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
-    if (tid < kTriWNP) {
+    if (tid < kTriGNP) {
       const double* coilv = p.coil + 2 * sdesc.vec_off;
       const double* rhov = p.rho_or ? p.rho_or + 2 * (size_t)o * p.rho_or_stride : p.rho0 + 2 * sdesc.vec_off;
       const int i = tid;
@@ -640,47 +743,170 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
       sd[i] = 0.0;
       se[i] = 0.0;
     }
-    __syncthreads();
-    // the lower triangle, one block row (32 rows) per pass: the pass's lower pattern entries
-    // (rows 32a .. 32a + 31: pattern entries rb[2a] .. rb[2a + 2]) are scattered into sA, then
-    // every thread takes its slots of row r + 32a and clears them
-    double R[kTriWSlots];
     const int32_t* prow = p.prow + sdesc.pat_off;
     const int32_t* pcol = p.pcol + sdesc.pat_off;
+    double R[kTriWSlots];
+    if (k0 > 0) {
+      // ---- prologue: the lower triangle in the global workspace, h-weighted diagonal blocks
+      const int nbk = (np + 31) >> 5;
+      const int nblk = nbk * (nbk + 1) / 2;
+      for (int e = tid; e < nblk * 1024; e += kTriWThreads) W[e] = 0.0;
+      __syncthreads();
+      for (int e = tid; e < sdesc.nnz; e += kTriWThreads) {
+        const int i = prow[e], jc = pcol[e];
+        if (jc <= i) {
+          const double x = real_generator_entry(p, sdesc, cre, cim, e);
+          W[(size_t)sym_slot(i >> 5, jc >> 5) * 1024 + (i & 31) * 32 + (jc & 31)] = (i == jc) ? 0.5 * x : x;
+        }
+      }
+      __syncthreads();
+      // column 0 and its reflection, then A*v0 (no update)
+      // (zero beyond np: the reflection sums the column to kTriGNP, and an earlier item may have
+      // left values there)
+      if (tid < kTriGNP)
+        srow[tid] = (tid < np) ? W[(size_t)sym_slot(tid >> 5, 0) * 1024 + (tid & 31) * 32] * (tid == 0 ? 2.0 : 1.0)
+                               : 0.0;
+      for (int k = tid; k < 2 * kTriWWaves * kTriGNP; k += kTriWThreads) lds_u[k] = 0.0;
+      __syncthreads();
+      if (wave == 0) {
+        double x[kTriGNP / 64];
 #pragma unroll
-    for (int a = 0; a < kTriWBlk; a++) {
-      if (a < bmax) {
-        const int e1 = sdesc.rb[2 * a + 2];
-        for (int e = sdesc.rb[2 * a] + tid; e < e1; e += kTriWThreads) {
-          const int i = prow[e], jc = pcol[e];
-          if (jc <= i) sA[(i - 32 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
+        for (int q = 0; q < kTriGNP / 64; q++) x[q] = srow[lane + 64 * q];
+        householder_form(x, srow[0], srow[1], 0, lane, sv[0], &stau[0], sd, se);
+      }
+      __syncthreads();
+      {
+        const double s = trig_pass(W, nbk, 0, false, true, sv[0], sw, sv[0], rowp, colp, wave, lane);
+        const double q = wave_sum_dpp(s);
+        if (lane == 0) sq[wave] = q;
+      }
+      __syncthreads();
+      for (int j = 0; j < k0; j++) {
+        const int cb = j & 1;
+        const double* v = sv[cb];
+        const double tau = uniform_f64(stau[cb]);
+        // w for reflection j from the partial sums of A*v (rows and columns > j)
+        if (tid < kTriGNP) {
+          double qs = 0.0, t = 0.0;
+#pragma unroll
+          for (int w = 0; w < kTriWWaves; w++) {
+            qs += sq[w];
+            t += rowp[w][tid] + colp[w][tid];
+          }
+          const double pk = (tid > j && tid < np) ? t * tau : 0.0;
+          sw[tid] = pk - 0.5 * tau * (qs * tau) * v[tid];
         }
         __syncthreads();
+        const bool nxt = (j + 1 < k0);
+        if (wave >= 1 && wave <= 4 && tau != 0.0) {
+          // Q^T applied to conj(coil) and rho0 (real and imaginary parts), one vector per wave
+          const int q = wave - 1;
+          double vv[kTriGNP / 64], uu[kTriGNP / 64];
+          double t = 0.0;
 #pragma unroll
-        for (int b = 0; b <= a; b++) {
-          double* s = &sA[r * kAsmStride + c + 32 * b];
-          R[sym_slot(a, b)] = (b == a) ? h * *s : *s;
-          *s = 0.0;
+          for (int k = 0; k < kTriGNP / 64; k++) {
+            const int i = lane + 64 * k;
+            vv[k] = v[i];
+            uu[k] = su[q][i];
+            t = fma(vv[k], uu[k], t);
+          }
+          t = wave_sum_dpp(t) * tau;
+#pragma unroll
+          for (int k = 0; k < kTriGNP / 64; k++)
+            if (vv[k] != 0.0) su[q][lane + 64 * k] = uu[k] - t * vv[k];
+        }
+        if (nxt) {
+          // column j + 1 as stored (before this step's update), and the partial sums cleared
+          const int jn = j + 1;
+          if (tid >= jn && tid < kTriGNP)
+            srow[tid] = (tid < np) ? W[(size_t)sym_slot(tid >> 5, jn >> 5) * 1024 + (tid & 31) * 32 + (jn & 31)] *
+                                         (tid == jn ? 2.0 : 1.0)
+                                   : 0.0;
+          for (int k = tid; k < 2 * kTriWWaves * kTriGNP; k += kTriWThreads) lds_u[k] = 0.0;
+          __syncthreads();
+          if (wave == 0) {
+            // the column after this step's update, and its reflection
+            const double vn = v[jn], wn = sw[jn];
+            double x[kTriGNP / 64];
+#pragma unroll
+            for (int q = 0; q < kTriGNP / 64; q++) {
+              const int i = lane + 64 * q;
+              x[q] = fma(-vn, sw[i], fma(-wn, v[i], srow[i]));
+            }
+            const double xd = fma(-vn, wn, fma(-wn, vn, srow[jn]));
+            const double x1 = fma(-vn, sw[jn + 1], fma(-wn, v[jn + 1], srow[jn + 1]));
+            householder_form(x, xd, x1, jn, lane, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+          }
+          __syncthreads();
+        }
+        if (tau != 0.0 || nxt) {
+          const double s = trig_pass(W, nbk, (j + 1) >> 5, tau != 0.0, nxt, v, sw, sv[cb ^ 1], rowp, colp, wave,
+                                     lane);
+          if (nxt) {
+            const double q = wave_sum_dpp(s);
+            if (lane == 0) sq[wave] = q;
+          }
         }
         __syncthreads();
-      } else {
+      }
+      // ---- hand-off: the trailing block (rows and columns >= k0) into the register layout
+      const int ob = k0 >> 5;
 #pragma unroll
-        for (int b = 0; b <= a; b++) R[sym_slot(a, b)] = 0.0;
+      for (int a = 0; a < kTriWBlk; a++) {
+#pragma unroll
+        for (int b = 0; b <= a; b++)
+          R[sym_slot(a, b)] =
+              (a < bmax) ? W[(size_t)sym_slot(a + ob, b + ob) * 1024 + r * 32 + c] : 0.0;
+      }
+      __syncthreads();
+    } else {
+      for (int k = tid; k < 32 * kAsmStride; k += kTriWThreads) sA[k] = 0.0;
+      __syncthreads();
+      // the lower triangle, one block row (32 rows) per pass: the pass's lower pattern entries
+      // (rows 32a .. 32a + 31: pattern entries rb[2a] .. rb[2a + 2]) are scattered into sA, then
+      // every thread takes its slots of row r + 32a and clears them
+#pragma unroll
+      for (int a = 0; a < kTriWBlk; a++) {
+        if (a < bmax) {
+          const int e1 = sdesc.rb[2 * a + 2];
+          for (int e = sdesc.rb[2 * a] + tid; e < e1; e += kTriWThreads) {
+            const int i = prow[e], jc = pcol[e];
+            if (jc <= i) sA[(i - 32 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
+          }
+          __syncthreads();
+#pragma unroll
+          for (int b = 0; b <= a; b++) {
+            double* s = &sA[r * kAsmStride + c + 32 * b];
+            R[sym_slot(a, b)] = (b == a) ? h * *s : *s;
+            *s = 0.0;
+          }
+          __syncthreads();
+        } else {
+#pragma unroll
+          for (int b = 0; b <= a; b++) R[sym_slot(a, b)] = 0.0;
+        }
       }
     }
-    symw_publish(R, 0, r, c, hd, wave, lane, srow, sv[0], &stau[0], sd, se);
+    // ---- register stage on rows and columns k0 .. np - 1 (local index i = global i - k0)
+    double* svo[2] = {sv[0] + k0, sv[1] + k0};
+    double* srowo = srow + k0;
+    double* swo = sw + k0;
+    double* sdo = sd + k0;
+    double* seo = se + k0;
+    double* suo[4] = {su[0] + k0, su[1] + k0, su[2] + k0, su[3] + k0};
+    symw_publish(R, 0, r, c, hd, wave, lane, srowo, svo[0], &stau[0], sdo, seo);
     __syncthreads();
 
-    for (int j = 0; j < n - 2; j++) {
+    for (int j = 0; j < n2 - 2; j++) {
       const int cb = j & 1;
-      const double* v = sv[cb];
+      const double* v = svo[cb];
       const double tau = uniform_f64(stau[cb]);  // reflection of column j
       const int jn = j + 1;                      // the next column to reflect
       const int bmin = jn >> 5;                  // blocks below bmin are eliminated
       const unsigned live = ((1u << bmax) - 1u) & ~((1u << bmin) - 1u);
       if (tau != 0.0) {  // uniform
         const int la = jn & (kTriWWaves - 1);  // the wave that reflects column jn
-        symw_copy_col(R, jn, r, c, hd, srow);  // column jn as the previous step left it
+        symw_copy_col(R, jn, r, c, hd, srowo);  // column jn as the previous step left it
         const double s = symw_matvec(R, v, spr, spc, wave, lane, r, c, bmin, live);
         __builtin_amdgcn_s_setprio(1);
         const double qw = wave_sum_dpp(s) * tau;
@@ -697,67 +923,69 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
           }
           const double K = 0.5 * tau * ((q4[0] + q4[1]) + (q4[2] + q4[3]));
           const double t = (spr[0][tid] + spr[1][tid]) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
-          const double pk = (tid > j && tid < np) ? t * tau : 0.0;
-          sw[tid] = pk - K * v[tid];
+          const double pk = (tid > j && tid < np2) ? t * tau : 0.0;
+          swo[tid] = pk - K * v[tid];
         }
         __syncthreads();
         __builtin_amdgcn_s_setprio(1);
         if (wave == la) {
           __builtin_amdgcn_s_setprio(3);
           // column jn after this update, from its copy, and its reflection for the next step
-          const double vn = v[jn], wn = sw[jn];
+          const double vn = v[jn], wn = swo[jn];
           double x[kTriWNP / 64];
 #pragma unroll
           for (int q = 0; q < kTriWNP / 64; q++) {
             const int i = lane + 64 * q;
-            x[q] = fma(-vn, sw[i], fma(-wn, v[i], srow[i]));
+            x[q] = fma(-vn, swo[i], fma(-wn, v[i], srowo[i]));
           }
-          const double xd = fma(-vn, wn, fma(-wn, vn, srow[jn]));
-          const double x1 = fma(-vn, sw[jn + 1], fma(-wn, v[jn + 1], srow[jn + 1]));
-          householder_form(x, xd, x1, jn, lane, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+          const double xd = fma(-vn, wn, fma(-wn, vn, srowo[jn]));
+          const double x1 = fma(-vn, swo[jn + 1], fma(-wn, v[jn + 1], srowo[jn + 1]));
+          householder_form(x, xd, x1, jn, lane, svo[cb ^ 1], &stau[cb ^ 1], sdo, seo);
         } else {
           // Q^T applied to conj(coil) and rho0 (real and imaginary parts): the four waves
           // after la take one vector each
           const int q = ((wave - la) & (kTriWWaves - 1)) - 1;
           if (q < 4) {
+            double* uq = suo[q];
             double vv[kTriWNP / 64], uu[kTriWNP / 64];
             double t = 0.0;
 #pragma unroll
             for (int k = 0; k < kTriWNP / 64; k++) {
               const int i = lane + 64 * k;
               vv[k] = v[i];
-              uu[k] = su[q][i];
+              uu[k] = uq[i];
               t = fma(vv[k], uu[k], t);
             }
             t = wave_sum_dpp(t) * tau;
 #pragma unroll
             for (int k = 0; k < kTriWNP / 64; k++)
-              if (vv[k] != 0.0) su[q][lane + 64 * k] = uu[k] - t * vv[k];
+              if (vv[k] != 0.0) uq[lane + 64 * k] = uu[k] - t * vv[k];
           }
         }
         double vr[kTriWBlk], wr[kTriWBlk];
 #pragma unroll
         for (int a = 0; a < kTriWBlk; a++) {
           vr[a] = v[r + 32 * a];
-          wr[a] = sw[r + 32 * a];
+          wr[a] = swo[r + 32 * a];
         }
-        symw_update(R, vr, wr, v, sw, h, c, bmin, live);
+        symw_update(R, vr, wr, v, swo, h, c, bmin, live);
         __builtin_amdgcn_s_setprio(0);
       } else {
         // no reflection at this step: column jn stands as it is in the registers
-        symw_publish(R, jn, r, c, hd, wave, lane, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+        symw_publish(R, jn, r, c, hd, wave, lane, srowo, svo[cb ^ 1], &stau[cb ^ 1], sdo, seo);
       }
       __syncthreads();
     }
-    if (n >= 2) symw_publish(R, n - 1, r, c, hd, wave, lane, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
+    if (n2 >= 2)
+      symw_publish(R, n2 - 1, r, c, hd, wave, lane, srowo, svo[(n2 - 1) & 1], &stau[(n2 - 1) & 1], sdo, seo);
     __syncthreads();
-    const int nm = p.spec_nm;  // kTriWNP
+    const int nm = p.spec_nm;  // kTriWNP or kTriGNP
     double* out = p.spec + (size_t)it * 6 * nm;
-    if (tid < kTriWNP) {
-      out[tid] = sd[tid];
-      out[nm + tid] = se[tid];
+    for (int i = tid; i < nm; i += kTriWThreads) {
+      out[i] = sd[i];
+      out[nm + i] = se[i];
 #pragma unroll
-      for (int q = 0; q < 4; q++) out[(2 + q) * nm + tid] = su[q][tid];
+      for (int q = 0; q < 4; q++) out[(2 + q) * nm + i] = su[q][i];
     }
     __syncthreads();
   }
@@ -779,16 +1007,17 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
 // parent merge). One 256-thread workgroup per orientation, everything in LDS.
 constexpr int kDcThreads = 256;
 constexpr int kDcLeaf = 6;
-constexpr int kDcMaxLevels = 6;  // up to 64 leaves (kTriWNP); 32 for kTriNP
+constexpr int kDcMaxLevels = 7;  // up to 128 leaves (kTriGNP)
 constexpr int kDcMaxIter = 100;  // secular-equation iterations per root (typically 3-8)
 constexpr int kDcPass1 = 3;      // evaluations per root before the open roots are packed (see M3a; 2 and 4-5 measured slower)
 
-// Sized for tridiagonal matrices of dimension at most NM (kTriNP or kTriWNP, with at most
-// kDcThreads = 256 roots per merge, one per thread).
+// Sized for tridiagonal matrices of dimension at most NM (kTriNP, kTriWNP or kTriGNP), solved
+// by kThr threads, one per root of a merge.
 template <int NM>
 struct DcSharedT {
   static constexpr int kNM = NM;
-  static constexpr int kLev = (NM <= kTriNP) ? 5 : kDcMaxLevels;  // leaves of <= kDcLeaf rows
+  static constexpr int kThr = (NM <= kDcThreads) ? kDcThreads : 2 * kDcThreads;
+  static constexpr int kLev = (NM <= kTriNP) ? 5 : ((NM <= kTriWNP) ? 6 : kDcMaxLevels);  // leaves of <= kDcLeaf rows
   double d[NM], e[NM];
   double X[6][NM];  // tracked vectors over the eigen-index of each segment
   double Y[6][NM];  // merge input, sorted
@@ -802,13 +1031,13 @@ struct DcSharedT {
   double zsum[(1 << kLev)];            // sum of the kept z^2 of each merge
   int K[(1 << kLev)];
   int its[NM];
-  unsigned long long pend[kDcThreads / 64];  // roots the first secular pass left open, by thread
+  unsigned long long pend[kThr / 64];  // roots the first secular pass left open, by thread
   int nlev;
   int fail;
   int max_iter;  // secular-equation evaluations allowed per root in the second pass
 };
 using DcShared = DcSharedT<kTriNP>;
-static_assert(kTriWNP <= kDcThreads && kDcLeaf << kDcMaxLevels >= kTriWNP, "eigensolver sizes");
+static_assert(kTriGNP <= 2 * kDcThreads && kDcLeaf << kDcMaxLevels >= kTriGNP, "eigensolver sizes");
 
 // 1/x: on the device v_rcp_f64 refined by two Newton steps (full double
 // precision from the hardware estimate), on the host a division
@@ -1197,7 +1426,7 @@ template <class SH>
 __host__ __device__ inline void dc_phase_m2(SH& S, int l, int s2) {
   if (s2 >= (1 << l)) return;
   if (s2 == 0)
-    for (int w = 0; w < kDcThreads / 64; w++) S.pend[w] = 0;
+    for (int w = 0; w < SH::kThr / 64; w++) S.pend[w] = 0;
   const int a = S.bnd[l][s2], b = S.bnd[l][s2 + 1], m = S.bnd[l + 1][2 * s2 + 1];
   const double bcoup = (m > a && m < b) ? S.e[m - 1] : 0.0;
   const double rho = 2.0 * fabs(bcoup);
@@ -1318,12 +1547,12 @@ __host__ __device__ inline void dc_phase_m3a(SH& S, int l, int n, int i) {
 template <class SH>
 __host__ __device__ inline void dc_phase_m3b(SH& S, int l, int u) {
   int w = 0, base = 0;
-  for (; w < kDcThreads / 64; w++) {
+  for (; w < SH::kThr / 64; w++) {
     const int c = __builtin_popcountll(S.pend[w]);
     if (u < base + c) break;
     base += c;
   }
-  if (w == kDcThreads / 64) return;
+  if (w == SH::kThr / 64) return;
   const int i = 64 * w + dc_select_bit(S.pend[w], u - base);
   const int s2 = S.seg[i];
   const int a = S.bnd[l][s2], k = i - a, K = S.K[s2];
@@ -1446,15 +1675,16 @@ __host__ __device__ inline void dc_phase_store(const SH& S, double* gsrc, int nm
 }
 
 template <int NM>
-__global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
+__global__ void __launch_bounds__(DcSharedT<NM>::kThr) k_dceig(SpecParams p) {
   __shared__ DcSharedT<NM> S;
+  constexpr int kT = DcSharedT<NM>::kThr;
   const int tid = threadIdx.x;
   const int64_t it = blockIdx.x;
   if (it >= p.n_items) return;
   const int n = p.sys[p.items[it].sys].n;
   const int nm = p.spec_nm;
   double* gsrc = p.spec + (size_t)it * 6 * nm;
-  dc_phase_load(S, gsrc, nm, tid, kDcThreads);
+  dc_phase_load(S, gsrc, nm, tid, kT);
   __syncthreads();
   if (tid == 0) dc_phase_plan(S, n, p.dc_max_iter > 0 ? p.dc_max_iter : kDcMaxIter);
   __syncthreads();
@@ -1486,7 +1716,7 @@ __global__ void __launch_bounds__(kDcThreads) k_dceig(SpecParams p) {
       __syncthreads();
     }
   }
-  dc_phase_store(S, gsrc, nm, n, tid, kDcThreads);
+  dc_phase_store(S, gsrc, nm, n, tid, kT);
   __syncthreads();
   if (tid == 0) {
     int mx = 0;

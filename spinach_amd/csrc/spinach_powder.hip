@@ -127,10 +127,12 @@ struct sp_plan {
   sp_stats last{};
   int64_t full_dim = 0, red_dim = 0, red_nnz = 0;
   int32_t real_systems = 0;
-  // spectral path (real generators, FP64, padded dimension <= spk::kTriWNP); the first
-  // n_spec_w spectral items have a padded dimension above spk::kTriNP (k_tridiag_w)
-  int64_t n_prop = 0, n_spec = 0, n_spec_w = 0;
+  // spectral path (real generators, FP64, padded dimension <= spk::kTriGNP); the first
+  // n_spec_w spectral items have a padded dimension above spk::kTriNP (k_tridiag_w), the first
+  // n_spec_g of them above spk::kTriWNP (with its global-memory prologue)
+  int64_t n_prop = 0, n_spec = 0, n_spec_w = 0, n_spec_g = 0;
   int spec_nm = spk::kTriNP;  // per-item spectrum layout: 6 vectors of spec_nm doubles
+  DevBuf d_tri_work;          // the prologue's workspace, spk::kTriGWork doubles per workgroup
   int64_t n_prop_complex = 0;  // the first n_prop_complex propagator items have a complex generator
   int32_t spectral_systems = 0;
   // stepping path (SP_MODE_KRYLOV, dimensions >= krylov_tol or above the dense limit)
@@ -335,7 +337,7 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
       throw SpError(SP_EINVAL, "a relaxation superoperator runs on the propagator path only (reduced dimension <= " +
                                    std::to_string(kDenseMaxDim) + ", below krylov_tol, no SP_MODE_KRYLOV)");
     spectral[s] =
-        (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriWNP) ? 1 : 0;
+        (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriGNP) ? 1 : 0;
     if (spectral[s])
       np_spec = std::max(np_spec, sps[s].np);
     else if (!krylov[s])
@@ -455,10 +457,15 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
                      if (ra != rb) return ra < rb;
                      return a.first > b.first;
                    });
-  // spectral items: those above the 256-thread layout's kTriNP first (k_tridiag_w), then the rest
+  // spectral items by size class (above kTriWNP, above kTriNP, the rest: the eigensolver and the
+  // tridiagonalisation are sized per class), then by cost
+  auto spec_class = [&](const spk::Item& it) {
+    const int np = descs[it.sys].np;
+    return np > spk::kTriWNP ? 2 : (np > spk::kTriNP ? 1 : 0);
+  };
   std::stable_sort(spec_items.begin(), spec_items.end(),
                    [&](const std::pair<double, spk::Item>& a, const std::pair<double, spk::Item>& b) {
-                     const int wa = descs[a.second.sys].np > spk::kTriNP, wb = descs[b.second.sys].np > spk::kTriNP;
+                     const int wa = spec_class(a.second), wb = spec_class(b.second);
                      if (wa != wb) return wa > wb;
                      return a.first > b.first;
                    });
@@ -486,10 +493,13 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   if (const char* e = getenv("SPINACH_DC_MAX_ITER")) pl->dc_max_iter = std::max(0, atoi(e));
   pl->n_prop = (int64_t)prop_items.size();
   pl->n_spec = (int64_t)spec_items.size();
-  pl->n_spec_w = 0;
-  for (auto& c : spec_items)
-    if (descs[c.second.sys].np > spk::kTriNP) pl->n_spec_w++;
-  pl->spec_nm = pl->n_spec_w > 0 ? spk::kTriWNP : spk::kTriNP;
+  pl->n_spec_w = pl->n_spec_g = 0;
+  for (auto& c : spec_items) {
+    const int k = spec_class(c.second);
+    pl->n_spec_w += (k > 0);
+    pl->n_spec_g += (k > 1);
+  }
+  pl->spec_nm = pl->n_spec_g > 0 ? spk::kTriGNP : (pl->n_spec_w > 0 ? spk::kTriWNP : spk::kTriNP);
   pl->n_kry = (int64_t)kry_items.size();
   pl->n_items = pl->n_prop + pl->n_spec + pl->n_kry;
   pl->total_or = or_base;
@@ -547,6 +557,8 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
     // synthesis workgroups per CU
     configure_spec(ctx, pl.get(), 2, 3);
     pl->d_spec.alloc((size_t)pl->n_spec * 6 * pl->spec_nm * sizeof(double));
+    if (pl->n_spec_g > 0)
+      pl->d_tri_work.alloc((size_t)std::min<int64_t>(ctx->n_cu, pl->n_spec_w) * spk::kTriGWork * sizeof(double));
     pl->d_status.alloc((size_t)pl->n_spec * sizeof(int32_t));
   }
   if (pl->n_kry > 0) setup_step(ctx, pl.get());
@@ -630,6 +642,8 @@ spk::SpecParams spec_params(sp_plan* pl) {
   sp.nsteps = pl->nsteps;
   sp.spec = pl->d_spec.as<double>();
   sp.spec_nm = pl->spec_nm;
+  sp.tri_work = pl->n_spec_g > 0 ? pl->d_tri_work.as<double>() : nullptr;
+  sp.tri_work_stride = spk::kTriGWork;
   sp.status = pl->d_status.as<int32_t>();
   sp.per_or = pl->d_per.as<double>();
   sp.dc_max_iter = pl->dc_max_iter;
@@ -717,8 +731,9 @@ spk::SpecParams spec_params_range(sp_plan* pl, int64_t first, int64_t n) {
 }
 
 // The three spectral kernels; ev[0..2] are recorded after each of them. Items above kTriNP
-// (the first n_spec_w) take the 1024-thread tridiagonalisation (queue head 8) and the
-// eigensolver sized for kTriWNP; the synthesis runs over every spectral item.
+// (the first n_spec_w) take the 1024-thread tridiagonalisation (queue head 8), with its
+// global-memory prologue for the first n_spec_g (above kTriWNP); the eigensolver is sized per
+// class; the synthesis runs over every spectral item.
 void launch_spectral(sp_plan* pl, hipEvent_t* ev) {  // ev == nullptr: no stage events (graph recording)
   hipStream_t st = pl->ctx->stream;
   const int64_t nw = pl->n_spec_w, nn = pl->n_spec - pl->n_spec_w;
@@ -736,19 +751,30 @@ void launch_spectral(sp_plan* pl, hipEvent_t* ev) {  // ev == nullptr: no stage 
     HIPCHK(hipGetLastError());
   }
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  if (nw > 0) {
-    spk::SpecParams tw = spec_params_range(pl, 0, nw);
-    hipLaunchKernelGGL(spk::k_dceig<spk::kTriWNP>, dim3((unsigned)nw), dim3(spk::kDcThreads), 0, st, tw);
+  const int64_t ng = pl->n_spec_g;
+  if (ng > 0) {
+    spk::SpecParams tg = spec_params_range(pl, 0, ng);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriGNP>, dim3((unsigned)ng), dim3(spk::DcSharedT<spk::kTriGNP>::kThr), 0,
+                       st, tg);
+    HIPCHK(hipGetLastError());
+  }
+  if (nw > ng) {
+    spk::SpecParams tw = spec_params_range(pl, ng, nw - ng);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriWNP>, dim3((unsigned)(nw - ng)), dim3(spk::DcSharedT<spk::kTriWNP>::kThr),
+                       0, st, tw);
     HIPCHK(hipGetLastError());
   }
   if (nn > 0) {
     spk::SpecParams tri = spec_params_range(pl, nw, nn);
-    hipLaunchKernelGGL(spk::k_dceig<spk::kTriNP>, dim3((unsigned)nn), dim3(spk::kDcThreads), 0, st, tri);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriNP>, dim3((unsigned)nn), dim3(spk::DcSharedT<spk::kTriNP>::kThr), 0, st,
+                       tri);
     HIPCHK(hipGetLastError());
   }
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
   spk::SpecParams syn = spec_params(pl);
-  if (pl->spec_nm == spk::kTriWNP)
+  if (pl->spec_nm == spk::kTriGNP)
+    hipLaunchKernelGGL(spk::k_synth<spk::kTriGNP>, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
+  else if (pl->spec_nm == spk::kTriWNP)
     hipLaunchKernelGGL(spk::k_synth<spk::kTriWNP>, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
   else
     hipLaunchKernelGGL(spk::k_synth<spk::kTriNP>, dim3(pl->grid_syn), dim3(spk::kThreads), 0, st, syn);
@@ -1336,7 +1362,8 @@ int noalloc_fid(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* al,
     const int g_syn = (int)std::max<int64_t>(1, std::min<int64_t>(3 * (int64_t)ctx->n_cu, nb));
     hipLaunchKernelGGL(spk::k_tridiag, dim3(g_tri), dim3(spk::kTriThreads), 0, st, sp2);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(spk::k_dceig<spk::kTriNP>, dim3((unsigned)nb), dim3(spk::kDcThreads), 0, st, sp2);
+    hipLaunchKernelGGL(spk::k_dceig<spk::kTriNP>, dim3((unsigned)nb), dim3(spk::DcSharedT<spk::kTriNP>::kThr), 0, st,
+                       sp2);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(spk::k_synth<spk::kTriNP>, dim3(g_syn), dim3(spk::kThreads), 0, st, sp2);
     HIPCHK(hipGetLastError());

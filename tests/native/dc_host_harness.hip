@@ -9,7 +9,7 @@ using namespace spk;
 template <int NM>
 static int dc_host_solve_t(const double* in, int n, double* out, int* status, int* levels) {
   static DcSharedT<NM> S;
-  const int T = kDcThreads;
+  const int T = DcSharedT<NM>::kThr;
   for (int i = 0; i < 6 * NM; i++) out[i] = in[i];
   for (int t = 0; t < T; t++) dc_phase_load(S, out, NM, t, T);
   dc_phase_plan(S, n);
@@ -35,10 +35,11 @@ static int dc_host_solve_t(const double* in, int n, double* out, int* status, in
 }
 
 // in/out: 6 * nm doubles (d | e | four vectors), nm = kTriNP (n <= kTriNP, the solver sized for
-// the 256-thread tridiagonalisation) or kTriWNP (n <= kTriWNP, the one sized for k_tridiag_w)
+// the 256-thread tridiagonalisation), kTriWNP or kTriGNP (the ones sized for k_tridiag_w)
 extern "C" int dc_host_solve(const double* in, int n, int nm, double* out, int* status, int* levels) {
   if (nm == kTriNP && n >= 0 && n <= kTriNP) return dc_host_solve_t<kTriNP>(in, n, out, status, levels);
   if (nm == kTriWNP && n >= 0 && n <= kTriWNP) return dc_host_solve_t<kTriWNP>(in, n, out, status, levels);
+  if (nm == kTriGNP && n >= 0 && n <= kTriGNP) return dc_host_solve_t<kTriGNP>(in, n, out, status, levels);
   return 1;
 }
 

@@ -16,6 +16,7 @@ from spinach_amd import workloads
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 NP = 192    # the solver sized for the 256-thread tridiagonalisation
 NPW = 256   # the one sized for the 1024-thread tridiagonalisation (dimensions 193-256)
+NPG = 448   # and for its global-memory prologue (dimensions 257-448, 512 threads)
 
 
 @pytest.fixture(scope='module')
@@ -40,7 +41,7 @@ def tridiagonalise(A, vecs):
 
 def solve(lib, A, coil, rho, nsteps=4096, nm=None):
     n = len(A)
-    nm = nm or (NP if n <= NP else NPW)
+    nm = nm or (NP if n <= NP else (NPW if n <= NPW else NPG))
     cc = np.conj(coil)
     d, e, (cr, ci, rr, ri) = tridiagonalise(A, [cc.real, cc.imag, rho.real, rho.imag])
     buf = np.zeros(6 * nm)
@@ -63,7 +64,7 @@ def solve(lib, A, coil, rho, nsteps=4096, nm=None):
     return (np.abs(np.sort(lam) - w).max(), np.abs(f - fref).max() / np.abs(fref).max(), st.value, lev.value)
 
 
-@pytest.mark.parametrize('n', [1, 2, 3, 12, 13, 25, 97, 150, 192, 193, 231, 256])
+@pytest.mark.parametrize('n', [1, 2, 3, 12, 13, 25, 97, 150, 192, 193, 231, 256, 257, 300, 385, 441, 448])
 def test_random_symmetric(harness, n):
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
@@ -75,7 +76,7 @@ def test_random_symmetric(harness, n):
     assert le < 1e-13 and fe < 1e-10, (le, fe)
 
 
-@pytest.mark.parametrize('n', [30, 96, 192, 252])
+@pytest.mark.parametrize('n', [30, 96, 192, 252, 444])
 def test_degenerate_spectra(harness, n):
     # six-fold degenerate eigenvalues in a random basis: deflation by rotation
     rng = np.random.default_rng(100 + n)

@@ -41,6 +41,9 @@ def test_large_members_vs_exact_oracle(mode):
     fids = plan.result_batch()
     print(f'[{mode}] stats', st, flush=True)
     assert st['reduced_dim'] == 441 and st['padded_dim'] == 448
+    # real generators: the spectral path takes every member in FP64 (k_tridiag_w, with its
+    # global-memory prologue above 256 states); '+complex' keeps them on the propagator path
+    assert st['spectral_systems'] == (len(LARGE) if mode == 'fp64' else 0)
     worst = 0.0
     for (s, n), y, f in zip(LARGE.items(), ys, fids):
         assert len(O.union_states(y['I'], y['Q'], y['coil'])) == n
@@ -122,6 +125,43 @@ def test_wide_spectral_members_vs_exact_oracle_and_propagator(graph):
         n = len(O.union_states(y['I'], y['Q'], y['coil']))
         if s in WIDE:
             assert n == WIDE[s]
+        exact = O.refcpu_powder(y['I'], y['Q'], y['grid'], y['rho0'], y['coil'], DT, NSTEPS, variant=2)
+        e, ep = O.rel_err(f, exact), O.rel_err(f, g)
+        print(f'[{mode}] member {s} (reduced dim {n}): spectral vs exact-expm oracle {e:.2e}, '
+              f'vs propagator path {ep:.2e}', flush=True)
+        worst, worst_p = max(worst, e), max(worst_p, ep)
+    assert worst < 1e-10 and worst_p < 1e-10
+
+
+# one plan with every spectral size class: above 256 states (k_tridiag_w's global-memory
+# prologue, eigensolver for 448), 193-256 (register layout only) and up to 192 (256-thread kernel)
+GLOBAL = {363: 441, 304: 360, 482: 320, 74: 308, 3: 276}
+
+
+@pytest.mark.parametrize('graph', [False, True])
+def test_global_prologue_members_vs_exact_oracle_and_propagator(graph):
+    members = list(GLOBAL) + [264, 5]
+    ys = []
+    for s in members:
+        y = workloads.batch_member(2, s)
+        y['grid'] = grids.subset(y['grid'], np.arange(2))
+        ys.append(y)
+    mode = 'fp64+graph' if graph else 'fp64'
+    plan = _native.plan_powder_batch(ys, DT, NSTEPS, mode=mode)
+    st = plan.execute()
+    if graph:
+        st = plan.execute()  # a replay
+    fids = plan.result_batch()
+    print(f'[{mode}] stats', st, flush=True)
+    assert st['spectral_systems'] == len(ys) and st['padded_dim'] == 448
+    prop = _native.plan_powder_batch(ys, DT, NSTEPS, mode='fp64+propagator')
+    prop.execute()
+    pf = prop.result_batch()
+    worst = worst_p = 0.0
+    for s, y, f, g in zip(members, ys, fids, pf):
+        n = len(O.union_states(y['I'], y['Q'], y['coil']))
+        if s in GLOBAL:
+            assert n == GLOBAL[s]
         exact = O.refcpu_powder(y['I'], y['Q'], y['grid'], y['rho0'], y['coil'], DT, NSTEPS, variant=2)
         e, ep = O.rel_err(f, exact), O.rel_err(f, g)
         print(f'[{mode}] member {s} (reduced dim {n}): spectral vs exact-expm oracle {e:.2e}, '
