@@ -471,16 +471,24 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
 
 // ------------------------------------------------------------ k_tridiag_w
 // The same Householder tridiagonalisation for padded dimensions kTriNP < np <= kTriWNP: the
-// lower triangle of the 256 x 256 FP64 matrix (263 KB) lives in the registers of one
-// 1024-thread workgroup per CU. Thread (r, c) = (tid / 32, tid % 32) of the 32 x 32 grid holds
-// A(r + 32a, c + 32b) for b <= a (36 slots), h-weighted in the diagonal blocks as in
-// k_tridiag; wave w holds rows r = 2w (lanes 0-31) and 2w + 1 (lanes 32-63) of every block.
-// Row sums of A*v are reduced over each 16-lane DPP row (two partial sums per index), column
-// sums over the wave's two rows by a lane exchange (one partial sum per wave and index).
-constexpr int kTriWThreads = 1024;
-constexpr int kTriWWaves = kTriWThreads / 64;              // 16
-constexpr int kTriWBlk = kTriWNP / 32;                     // 8 blocks of 32 rows / columns
-constexpr int kTriWSlots = kTriWBlk * (kTriWBlk + 1) / 2;  // 36
+// lower triangle of the 256 x 256 FP64 matrix lives in the registers of one 512-thread
+// workgroup per CU (two waves per SIMD, so that each thread may hold 144 VGPRs of it without
+// spilling). Thread (r, c) = (tid / 32, tid % 32) holds A(r + 16a, c + 32b) for the blocks of
+// 16 rows and 32 columns that reach the lower triangle, a >= 2b (72 slots). The two blocks of
+// column block b that make up its 32 x 32 diagonal square (a = 2b, 2b + 1) hold the whole
+// square, above the diagonal too: the product takes their row sums only (each entry of the
+// square once), the blocks below the square their row and column sums (each entry for itself
+// and for its transpose), and the update and the column copies need no weights. Wave w holds
+// rows r = 2w (lanes 0-31) and 2w + 1 (lanes 32-63). Row sums of A*v are reduced over each
+// 16-lane DPP row (two partial sums per index), column sums over the wave's two rows by a lane
+// exchange (one per wave and index).
+constexpr int kTriWThreads = 512;
+constexpr int kTriWWaves = kTriWThreads / 64;  // 8
+constexpr int kTriWRB = kTriWNP / 16;          // 16 row blocks of 16
+constexpr int kTriWCB = kTriWNP / 32;          // 8 column blocks of 32
+constexpr int kTriWSlots = kTriWCB * (kTriWCB + 1);  // 72: column block b has rows blocks 2b .. 15
+// slot of row block a (>= 2b) of column block b
+__host__ __device__ constexpr int symw_slot(int a, int b) { return b * (2 * kTriWCB + 1 - b) + (a - 2 * b); }
 
 // sum over each 16-lane DPP row, returned in all 16 lanes; every lane of the wave active
 __device__ __forceinline__ double sum16(double v) {
@@ -489,76 +497,77 @@ __device__ __forceinline__ double sum16(double v) {
   return v;
 }
 
-// The owners of column jj (c == jj % 32) copy its entries at rows >= jj to srow (rows
-// below jj in its block get 0 or a stale value, never read).
-__device__ __forceinline__ void symw_copy_col(const double (&R)[kTriWSlots], int jj, int r, int c, double hd,
-                                              double* srow) {
+// The owners of column jj (c == jj % 32) copy its entries at rows >= 32 (jj / 32) to srow (rows
+// below jj are never read).
+__device__ __forceinline__ void symw_copy_col(const double (&R)[kTriWSlots], int jj, int r, int c, double* srow) {
   if (c != (jj & 31)) return;
   const int bj = jj >> 5;
 #pragma unroll
-  for (int b = 0; b < kTriWBlk; b++) {
+  for (int b = 0; b < kTriWCB; b++) {
     if (b == bj) {
+      // (the empty volatile statement keeps the eight branches apart: merged, they would read
+      // the matrix at a run-time index, which puts the whole array in scratch memory)
+      asm volatile("" ::: "memory");
 #pragma unroll
-      for (int a = b; a < kTriWBlk; a++) srow[r + 32 * a] = (a == b) ? hd * R[sym_slot(a, b)] : R[sym_slot(a, b)];
+      for (int a = 2 * b; a < kTriWRB; a++) srow[r + 16 * a] = R[symw_slot(a, b)];
     }
   }
 }
 
 // Column jj as it stands in the registers, reflected by wave 0 (every thread calls it).
-__device__ __forceinline__ void symw_publish(const double (&R)[kTriWSlots], int jj, int r, int c, double hd,
-                                             int wave, int lane, double* srow, double* vb, double* ptau, double* sd,
-                                             double* se) {
-  symw_copy_col(R, jj, r, c, hd, srow);
+__device__ __forceinline__ void symw_publish(const double (&R)[kTriWSlots], int jj, int r, int c, int wave, int lane,
+                                             double* srow, double* vb, double* ptau, double* sd, double* se) {
+  symw_copy_col(R, jj, r, c, srow);
   __syncthreads();
   if (wave == 0) {
     double x[kTriWNP / 64];
 #pragma unroll
     for (int q = 0; q < kTriWNP / 64; q++) x[q] = srow[lane + 64 * q];
-    householder_form(x, srow[jj], srow[jj + 1], jj, lane, vb, ptau, sd, se);  // srow[kTriWNP] stays zero
+    householder_form(x, srow[jj], srow[jj + 1], jj, lane, vb, ptau, sd, se);
   }
 }
 
-// A*v over this thread's slots of the live column blocks, in two passes over the registers so
-// that only one block-indexed array is live beside the matrix (1024 threads leave 128 VGPRs):
-// row sums (reduced over each 16-lane row, two partial sums per index into spr), then column
-// sums (the wave's two rows added, one partial sum per wave into spc). Returns this thread's
-// part of v'A v. Rows at or beyond np hold zeros.
+// A*v over this thread's slots of the live column blocks (b >= bmin), in two passes over the
+// registers (one row-block-indexed array live at a time beside the matrix): row sums, reduced
+// over each 16-lane row (two partial sums per index, to spr), then column sums of the blocks
+// below the diagonal squares, the wave's two rows added (one partial sum per wave and index, to
+// spc). Returns this thread's part of v'A v. Rows and columns at or beyond np hold zeros.
 __device__ __forceinline__ double symw_matvec(const double (&R)[kTriWSlots], const double* v, double (*spr)[kTriWNP],
                                               double (*spc)[kTriWNP], int wave, int lane, int r, int c, int bmin,
                                               unsigned live) {
   double s = 0.0;
   {
-    double yr[kTriWBlk];
+    double yr[kTriWRB];
 #pragma unroll
-    for (int a = 0; a < kTriWBlk; a++) yr[a] = 0.0;
+    for (int a = 0; a < kTriWRB; a++) yr[a] = 0.0;
     double vnext = v[c + 32 * bmin];
 #pragma unroll
-    for (int b = 0; b < kTriWBlk; b++) {
+    for (int b = 0; b < kTriWCB; b++) {
       if ((live >> b) & 1u) {
         const double vc = vnext;
-        if (b + 1 < kTriWBlk) vnext = v[c + 32 * (b + 1)];
+        if (b + 1 < kTriWCB) vnext = v[c + 32 * (b + 1)];
 #pragma unroll
-        for (int a = b; a < kTriWBlk; a++) yr[a] = fma(R[sym_slot(a, b)], vc, yr[a]);
+        for (int a = 2 * b; a < kTriWRB; a++) yr[a] = fma(R[symw_slot(a, b)], vc, yr[a]);
       }
     }
 #pragma unroll
-    for (int a = 0; a < kTriWBlk; a++) {
-      if ((live >> a) & 1u) {
-        s = fma(v[r + 32 * a], yr[a], s);
+    for (int a = 0; a < kTriWRB; a++) {
+      if ((live >> (a >> 1)) & 1u) {  // row block a meets the live column blocks bmin .. a / 2
+        s = fma(v[r + 16 * a], yr[a], s);
         const double t = sum16(yr[a]);
-        if ((lane & 15) == 0) spr[(lane >> 4) & 1][r + 32 * a] = t;
+        if ((lane & 15) == 0) spr[(lane >> 4) & 1][r + 16 * a] = t;
       }
     }
   }
-  double vr[kTriWBlk];
+  double vr[kTriWRB];
 #pragma unroll
-  for (int a = 0; a < kTriWBlk; a++) vr[a] = v[r + 32 * a];
+  for (int a = 0; a < kTriWRB; a++) vr[a] = v[r + 16 * a];
 #pragma unroll
-  for (int b = 0; b < kTriWBlk; b++) {
+  for (int b = 0; b < kTriWCB; b++) {
     if ((live >> b) & 1u) {
-      double yc = 0.0;
+      double yc = 0.0;  // below the diagonal square only
 #pragma unroll
-      for (int a = b; a < kTriWBlk; a++) yc = fma(R[sym_slot(a, b)], vr[a], yc);
+      for (int a = 2 * b + 2; a < kTriWRB; a++) yc = fma(R[symw_slot(a, b)], vr[a], yc);
       s = fma(v[c + 32 * b], yc, s);
       const double yo = __shfl_xor(yc, 32);  // the wave's other row, same column
       if (lane < 32) spc[wave][c + 32 * b] = yc + yo;
@@ -567,25 +576,29 @@ __device__ __forceinline__ double symw_matvec(const double (&R)[kTriWSlots], con
   return s;
 }
 
-// A <- A - v w' - w v' over this thread's slots of the live column blocks
-__device__ __forceinline__ void symw_update(double (&R)[kTriWSlots], const double (&vr)[kTriWBlk],
-                                            const double (&wr)[kTriWBlk], const double* v, const double* sw, double h,
-                                            int c, int bmin, unsigned live) {
+// A <- A - v w' - w v' over this thread's slots of the live column blocks, for the row blocks
+// A0 .. A0 + 7 (two calls cover the sixteen: half the row values are live at a time)
+template <int A0>
+__device__ __forceinline__ void symw_update(double (&R)[kTriWSlots], const double* v, const double* sw, int r, int c,
+                                            int bmin, unsigned live) {
+  double vr[8], wr[8];
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    vr[a] = v[r + 16 * (A0 + a)];
+    wr[a] = sw[r + 16 * (A0 + a)];
+  }
   double vnext = v[c + 32 * bmin], wnext = sw[c + 32 * bmin];
 #pragma unroll
-  for (int b = 0; b < kTriWBlk; b++) {
+  for (int b = 0; 2 * b < A0 + 8; b++) {
     if ((live >> b) & 1u) {
       const double vc = vnext, wc = wnext;
-      if (b + 1 < kTriWBlk) {
+      if (2 * (b + 1) < A0 + 8) {
         vnext = v[c + 32 * (b + 1)];
         wnext = sw[c + 32 * (b + 1)];
       }
 #pragma unroll
-      for (int a = b; a < kTriWBlk; a++) {
-        const double va = (a == b) ? h * vr[a] : vr[a];
-        const double wa = (a == b) ? h * wr[a] : wr[a];
-        R[sym_slot(a, b)] = fma(-va, wc, fma(-wa, vc, R[sym_slot(a, b)]));
-      }
+      for (int a = (2 * b > A0 ? 2 * b : A0); a < A0 + 8; a++)
+        R[symw_slot(a, b)] = fma(-vr[a - A0], wc, fma(-wr[a - A0], vc, R[symw_slot(a, b)]));
     }
   }
 }
@@ -598,17 +611,19 @@ __device__ __forceinline__ void symw_update(double (&R)[kTriWSlots], const doubl
 // within the block), h-weighted in the diagonal blocks as the register layout is, so the
 // hand-off is a copy. Each step makes one pass over the live blocks, which applies the previous
 // step's rank-2 update, stores the block and forms the next symmetric product from the updated
-// values (the next reflection comes first, from its column updated in LDS).
+// values (the next reflection comes first, from its column updated in LDS). The pass streams
+// the trailing lower triangle once from and to HBM per step: it is bound by that traffic.
 constexpr int kTriGBlk = kTriGNP / 32;                          // 14 block rows
 constexpr int kTriGBlocks = kTriGBlk * (kTriGBlk + 1) / 2;      // 105 stored blocks
 constexpr int64_t kTriGWork = (int64_t)kTriGBlocks * 1024;      // workspace doubles per workgroup
 
 // The pass over the live blocks (block rows and columns >= bmin, below nbk), split into equal
-// runs of consecutive blocks (row-major) over the 16 waves. Lane (c, h2) = (lane % 32, lane / 32)
-// takes column c and rows 16 h2 .. 16 h2 + 15 of each block. When upd: A <- A - v w' - w v'
-// (vo, wo) and the block is stored back; when prod: row sums of A*vn (reduced over the 32 lanes
-// of each half-wave, one per row index and wave) to rowp, column sums (the two halves added) to
-// colp, and the lane's part of vn'A vn is returned. rowp and colp are zero beforehand.
+// runs of consecutive blocks (row-major) over the waves. Lane (c, h2) = (lane % 32, lane / 32)
+// takes column c and rows 16 h2 .. 16 h2 + 15 of each block; the next block's values are loaded
+// while the current one is processed. When upd: A <- A - v w' - w v' (vo, wo) and the block is
+// stored back; when prod: row sums of A*vn (reduced over the 32 lanes of each half-wave, one
+// per row index and wave) to rowp, column sums (the two halves added) to colp, and the lane's
+// part of vn'A vn is returned. rowp and colp are zero beforehand.
 __device__ __forceinline__ double trig_pass(double* W, int nbk, int bmin, bool upd, bool prod, const double* vo,
                                             const double* wo, const double* vn, double (*rowp)[kTriGNP],
                                             double (*colp)[kTriGNP], int wave, int lane) {
@@ -621,17 +636,32 @@ __device__ __forceinline__ double trig_pass(double* W, int nbk, int bmin, bool u
   int ap = 0;
   while ((ap + 1) * (ap + 2) / 2 <= f0) ap++;
   int bp = f0 - ap * (ap + 1) / 2;
-  double racc[16];
+  double racc[16], xn[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) racc[i] = 0.0;
+  {
+    const double* nb = W + (size_t)sym_slot(bmin + ap, bmin + bp) * 1024 + 512 * h2 + c;
+#pragma unroll
+    for (int i = 0; i < 16; i++) xn[i] = nb[32 * i];
+  }
   for (int f = f0; f < f1; f++) {
     const int a = bmin + ap, b = bmin + bp;
     double* blk = W + (size_t)sym_slot(a, b) * 1024 + 512 * h2 + c;
-    const int C = 32 * b + c, R0 = 32 * a + 16 * h2;
-    const double vC = vo[C], wC = wo[C], nC = vn[C];
     double x[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) x[i] = blk[32 * i];
+    for (int i = 0; i < 16; i++) x[i] = xn[i];
+    const bool row_end = (bp == ap) || (f + 1 == f1);
+    if (++bp > ap) {
+      ap++;
+      bp = 0;
+    }
+    if (f + 1 < f1) {
+      const double* nb = W + (size_t)sym_slot(bmin + ap, bmin + bp) * 1024 + 512 * h2 + c;
+#pragma unroll
+      for (int i = 0; i < 16; i++) xn[i] = nb[32 * i];
+    }
+    const int C = 32 * b + c, R0 = 32 * a + 16 * h2;
+    const double vC = vo[C], wC = wo[C], nC = vn[C];
     if (upd) {
       if (a == b) {  // h-weighted diagonal block
 #pragma unroll
@@ -657,26 +687,21 @@ __device__ __forceinline__ double trig_pass(double* W, int nbk, int bmin, bool u
       s = fma(nC, cacc, s);
       cacc += __shfl_xor(cacc, 32);  // rows of the other half, same column
       if (h2 == 0) colp[wave][C] += cacc;
-    }
-    // next block; the block row's sums are complete when the run leaves it
-    const bool row_end = (bp == ap) || (f + 1 == f1);
-    if (prod && row_end) {
+      // the block row's sums are complete when the run leaves it
+      if (row_end) {
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        double t = sum16(racc[i]);
-        t += __shfl_xor(t, 16);  // the other 16 columns of the half
-        racc[i] = t;
+        for (int i = 0; i < 16; i++) {
+          double t = sum16(racc[i]);
+          t += __shfl_xor(t, 16);  // the other 16 columns of the half
+          racc[i] = t;
+        }
+        if (c == 0) {
+#pragma unroll
+          for (int i = 0; i < 16; i++) rowp[wave][R0 + i] = racc[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) racc[i] = 0.0;
       }
-      if (c == 0) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) rowp[wave][R0 + i] = racc[i];
-      }
-#pragma unroll
-      for (int i = 0; i < 16; i++) racc[i] = 0.0;
-    }
-    if (++bp > ap) {
-      ap++;
-      bp = 0;
     }
   }
   return 2.0 * s;  // the row sums' half of vn'A vn equals the column sums' half
@@ -697,16 +722,16 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
   // by the prologue's partial sums
   constexpr int kAsmStride = kTriWNP + 4;
   __shared__ double lds_u[2 * kTriWWaves * kTriGNP];
-  static_assert(32 * kAsmStride <= 2 * kTriWWaves * kTriGNP, "assembly tile");
-  double* sA = lds_u;
+  static_assert(16 * kAsmStride <= 2 * kTriWWaves * kTriGNP, "assembly tile");
+  double* sA = lds_u;  // 16 rows of the generator under assembly
   double (*spc)[kTriWNP] = reinterpret_cast<double (*)[kTriWNP]>(lds_u);  // column sums, one row per wave
   double (*rowp)[kTriGNP] = reinterpret_cast<double (*)[kTriGNP]>(lds_u);
   double (*colp)[kTriGNP] = reinterpret_cast<double (*)[kTriGNP]>(lds_u + kTriWWaves * kTriGNP);
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int r = tid >> 5, c = tid & 31;
-  const double h = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
-  const double hd = (r == c) ? 2.0 : 1.0;
+  // the wave index as a wave-uniform value: the prologue's block runs and the reflecting wave's
+  // branches are then scalar
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = tid >> 5, c = tid & 31;  // r < 16
   for (int k = tid; k <= kTriGNP; k += kTriWThreads) srow[k] = 0.0;
   double* W = p.tri_work ? p.tri_work + (size_t)blockIdx.x * p.tri_work_stride : nullptr;
 
@@ -722,7 +747,7 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
     // columns reduced by the prologue (a multiple of 32, so the trailing block is block-aligned)
     const int k0 = (np > kTriWNP) ? ((np - kTriWNP + 31) >> 5) << 5 : 0;
     const int n2 = n - k0, np2 = np - k0;
-    const int bmax = (np2 + 31) >> 5;  // blocks below the padded dimension of the register stage
+    const int cmax = (np2 + 31) >> 5;  // column blocks below the padded dimension of the register stage
     const int64_t o = item.orient;
     if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, cre, cim);
     if (tid < kTriGNP) {
@@ -760,9 +785,8 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
         }
       }
       __syncthreads();
-      // column 0 and its reflection, then A*v0 (no update)
-      // (zero beyond np: the reflection sums the column to kTriGNP, and an earlier item may have
-      // left values there)
+      // column 0 and its reflection, then A*v0 (no update); the column is zero beyond np (the
+      // reflection sums it to kTriGNP, and an earlier item may have left values there)
       if (tid < kTriGNP)
         srow[tid] = (tid < np) ? W[(size_t)sym_slot(tid >> 5, 0) * 1024 + (tid & 31) * 32] * (tid == 0 ? 2.0 : 1.0)
                                : 0.0;
@@ -849,64 +873,78 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
         }
         __syncthreads();
       }
-      // ---- hand-off: the trailing block (rows and columns >= k0) into the register layout
+      // ---- hand-off: the trailing block (rows and columns >= k0) into the register layout;
+      // local row block a (16 rows) lies in stored block row (k0 + 16a) / 32, half a % 2; the
+      // diagonal squares are filled above the diagonal from the transposed entries and their
+      // diagonal restored from the stored halves
       const int ob = k0 >> 5;
 #pragma unroll
-      for (int a = 0; a < kTriWBlk; a++) {
+      for (int b = 0; b < kTriWCB; b++) {
 #pragma unroll
-        for (int b = 0; b <= a; b++)
-          R[sym_slot(a, b)] =
-              (a < bmax) ? W[(size_t)sym_slot(a + ob, b + ob) * 1024 + r * 32 + c] : 0.0;
+        for (int a = 2 * b; a < kTriWRB; a++) {
+          const int rr = 16 * (a & 1) + r;  // row within the stored 32 x 32 block
+          const double* blk = W + (size_t)sym_slot((a >> 1) + ob, b + ob) * 1024;
+          double x = 0.0;
+          if (b < cmax) {
+            if (a >= 2 * b + 2 || rr > c)
+              x = blk[rr * 32 + c];
+            else if (rr == c)
+              x = 2.0 * blk[rr * 32 + c];
+            else
+              x = blk[c * 32 + rr];
+          }
+          R[symw_slot(a, b)] = x;
+        }
       }
       __syncthreads();
     } else {
-      for (int k = tid; k < 32 * kAsmStride; k += kTriWThreads) sA[k] = 0.0;
+      for (int k = tid; k < 16 * kAsmStride; k += kTriWThreads) sA[k] = 0.0;
       __syncthreads();
-      // the lower triangle, one block row (32 rows) per pass: the pass's lower pattern entries
-      // (rows 32a .. 32a + 31: pattern entries rb[2a] .. rb[2a + 2]) are scattered into sA, then
-      // every thread takes its slots of row r + 32a and clears them
+      // the lower triangle and the diagonal squares, one row block (16 rows) per pass: the pass's
+      // pattern entries (rb[a] .. rb[a + 1]) at or below the diagonal square are scattered into
+      // sA, then every thread takes its slots of row r + 16a and clears them
 #pragma unroll
-      for (int a = 0; a < kTriWBlk; a++) {
-        if (a < bmax) {
-          const int e1 = sdesc.rb[2 * a + 2];
-          for (int e = sdesc.rb[2 * a] + tid; e < e1; e += kTriWThreads) {
+      for (int a = 0; a < kTriWRB; a++) {
+        if (a < 2 * cmax) {
+          const int e1 = sdesc.rb[a + 1];
+          for (int e = sdesc.rb[a] + tid; e < e1; e += kTriWThreads) {
             const int i = prow[e], jc = pcol[e];
-            if (jc <= i) sA[(i - 32 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
+            if ((jc >> 5) <= (i >> 5)) sA[(i - 16 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
           }
           __syncthreads();
 #pragma unroll
-          for (int b = 0; b <= a; b++) {
+          for (int b = 0; 2 * b <= a; b++) {
             double* s = &sA[r * kAsmStride + c + 32 * b];
-            R[sym_slot(a, b)] = (b == a) ? h * *s : *s;
+            R[symw_slot(a, b)] = *s;
             *s = 0.0;
           }
           __syncthreads();
         } else {
 #pragma unroll
-          for (int b = 0; b <= a; b++) R[sym_slot(a, b)] = 0.0;
+          for (int b = 0; 2 * b <= a; b++) R[symw_slot(a, b)] = 0.0;
         }
       }
     }
-    // ---- register stage on rows and columns k0 .. np - 1 (local index i = global i - k0)
-    double* svo[2] = {sv[0] + k0, sv[1] + k0};
+    // ---- register stage on rows and columns k0 .. np - 1 (local index i = global i - k0); the
+    // offset vectors are formed by address arithmetic (an array of pointers indexed at run time
+    // would live in scratch memory)
     double* srowo = srow + k0;
     double* swo = sw + k0;
     double* sdo = sd + k0;
     double* seo = se + k0;
-    double* suo[4] = {su[0] + k0, su[1] + k0, su[2] + k0, su[3] + k0};
-    symw_publish(R, 0, r, c, hd, wave, lane, srowo, svo[0], &stau[0], sdo, seo);
+    symw_publish(R, 0, r, c, wave, lane, srowo, &sv[0][k0], &stau[0], sdo, seo);
     __syncthreads();
 
     for (int j = 0; j < n2 - 2; j++) {
       const int cb = j & 1;
-      const double* v = svo[cb];
+      const double* v = &sv[cb][k0];
       const double tau = uniform_f64(stau[cb]);  // reflection of column j
       const int jn = j + 1;                      // the next column to reflect
-      const int bmin = jn >> 5;                  // blocks below bmin are eliminated
-      const unsigned live = ((1u << bmax) - 1u) & ~((1u << bmin) - 1u);
+      const int bmin = jn >> 5;                  // column blocks below bmin are eliminated
+      const unsigned live = ((1u << cmax) - 1u) & ~((1u << bmin) - 1u);
       if (tau != 0.0) {  // uniform
         const int la = jn & (kTriWWaves - 1);  // the wave that reflects column jn
-        symw_copy_col(R, jn, r, c, hd, srowo);  // column jn as the previous step left it
+        symw_copy_col(R, jn, r, c, srowo);  // column jn as the previous step left it
         const double s = symw_matvec(R, v, spr, spc, wave, lane, r, c, bmin, live);
         __builtin_amdgcn_s_setprio(1);
         const double qw = wave_sum_dpp(s) * tau;
@@ -915,13 +953,10 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
         __builtin_amdgcn_s_setprio(2);
         // A <- A - v w' - w v' with w = p - (tau/2)(v'p) v, assembled with p
         if (tid < kTriWNP) {
-          double q4[4], t4[4];
+          double t4[4];
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            q4[q] = (sq[q] + sq[q + 4]) + (sq[q + 8] + sq[q + 12]);
-            t4[q] = (spc[q][tid] + spc[q + 4][tid]) + (spc[q + 8][tid] + spc[q + 12][tid]);
-          }
-          const double K = 0.5 * tau * ((q4[0] + q4[1]) + (q4[2] + q4[3]));
+          for (int q = 0; q < 4; q++) t4[q] = spc[q][tid] + spc[q + 4][tid];
+          const double K = 0.5 * tau * (((sq[0] + sq[4]) + (sq[1] + sq[5])) + ((sq[2] + sq[6]) + (sq[3] + sq[7])));
           const double t = (spr[0][tid] + spr[1][tid]) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
           const double pk = (tid > j && tid < np2) ? t * tau : 0.0;
           swo[tid] = pk - K * v[tid];
@@ -940,13 +975,13 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
           }
           const double xd = fma(-vn, wn, fma(-wn, vn, srowo[jn]));
           const double x1 = fma(-vn, swo[jn + 1], fma(-wn, v[jn + 1], srowo[jn + 1]));
-          householder_form(x, xd, x1, jn, lane, svo[cb ^ 1], &stau[cb ^ 1], sdo, seo);
+          householder_form(x, xd, x1, jn, lane, &sv[cb ^ 1][k0], &stau[cb ^ 1], sdo, seo);
         } else {
           // Q^T applied to conj(coil) and rho0 (real and imaginary parts): the four waves
           // after la take one vector each
           const int q = ((wave - la) & (kTriWWaves - 1)) - 1;
           if (q < 4) {
-            double* uq = suo[q];
+            double* uq = &su[q][k0];
             double vv[kTriWNP / 64], uu[kTriWNP / 64];
             double t = 0.0;
 #pragma unroll
@@ -962,22 +997,17 @@ __global__ void __launch_bounds__(kTriWThreads, 1) k_tridiag_w(SpecParams p) {
               if (vv[k] != 0.0) uq[lane + 64 * k] = uu[k] - t * vv[k];
           }
         }
-        double vr[kTriWBlk], wr[kTriWBlk];
-#pragma unroll
-        for (int a = 0; a < kTriWBlk; a++) {
-          vr[a] = v[r + 32 * a];
-          wr[a] = swo[r + 32 * a];
-        }
-        symw_update(R, vr, wr, v, swo, h, c, bmin, live);
+        symw_update<0>(R, v, swo, r, c, bmin, live);
+        symw_update<8>(R, v, swo, r, c, bmin, live);
         __builtin_amdgcn_s_setprio(0);
       } else {
         // no reflection at this step: column jn stands as it is in the registers
-        symw_publish(R, jn, r, c, hd, wave, lane, srowo, svo[cb ^ 1], &stau[cb ^ 1], sdo, seo);
+        symw_publish(R, jn, r, c, wave, lane, srowo, &sv[cb ^ 1][k0], &stau[cb ^ 1], sdo, seo);
       }
       __syncthreads();
     }
     if (n2 >= 2)
-      symw_publish(R, n2 - 1, r, c, hd, wave, lane, srowo, svo[(n2 - 1) & 1], &stau[(n2 - 1) & 1], sdo, seo);
+      symw_publish(R, n2 - 1, r, c, wave, lane, srowo, &sv[(n2 - 1) & 1][k0], &stau[(n2 - 1) & 1], sdo, seo);
     __syncthreads();
     const int nm = p.spec_nm;  // kTriWNP or kTriGNP
     double* out = p.spec + (size_t)it * 6 * nm;
