@@ -44,9 +44,10 @@ BOUND = {'spk::k_tridiag': 'valu', 'spk::k_dceig': 'valu', 'spk::k_synth': 'mfma
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs of this script (counters scaled by a device copy of known size:
 # FETCH x2.000, WRITE x1.000); see profiles/k_spectral_static8spin_fp64.md and
-# profiles/k_powder_static8spin_fp64.md and profiles/k_powder_batch512_fp64.md
+# profiles/k_powder_static8spin_fp64.md and profiles/k_tridiag_w_batch512_fp64.md (the batch's
+# tridiagonalisation stage: k_tridiag_w and k_tridiag together)
 TRAFFIC = {
-    ('batch', 'fp64', 'spk::k_powder<double,true>'): 2534736904 * 1024 * 2.0 + 528656886 * 1024 * 1.0,
+    ('batch', 'fp64', 'spk::k_tridiag'): (178670707 + 159255) * 1024 * 2.0 + (344737580 + 2384462) * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_tridiag'): 13775 * 1024 * 2.0 + 67885 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,false>'): 165530392 * 1024 * 2.0 + 53150413 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,true>'): 139595089 * 1024 * 2.0 + 49487957 * 1024 * 1.0,
@@ -97,6 +98,23 @@ def spectral_stage_flops(n, npts):
     time point (complex multiply-add)."""
     return {'spk::k_tridiag': 4.0 / 3.0 * n ** 3 + 16.0 * n ** 2, 'spk::k_dceig': 0.0,
             'spk::k_synth': 8.0 * n * npts}
+
+
+def prologue_bytes(np_):
+    """HBM bytes per orientation of k_tridiag_w's global-memory prologue (padded dimension
+    np_ > 256, 32 x 32 FP64 blocks of the lower triangle): the workspace cleared and read once
+    for the first product, one read and one write of the live blocks (block rows and columns
+    >= (j + 1) / 32) for each of the first k0 reflections, and the trailing block read into
+    the register layout."""
+    k0 = -(-(np_ - 256) // 32) * 32
+    nbk = -(-np_ // 32)
+    blk = 32 * 32 * 8
+    total = nbk * (nbk + 1) // 2 * blk * 2
+    for j in range(k0):
+        m = nbk - ((j + 1) >> 5)
+        total += m * (m + 1) // 2 * blk * 2
+    m2 = nbk - k0 // 32
+    return total + m2 * (m2 + 1) // 2 * blk
 
 
 def step_product_work(n, nnz, real=True):
@@ -240,18 +258,25 @@ def bench_batch(args):
     spec_flops = {k: 0.0 for k in ('spk::k_tridiag', 'spk::k_dceig', 'spk::k_synth')}
     dims = []
     spectral_ok = st['spectral_systems'] > 0
+    tri_bytes = 0.0  # the prologue's HBM traffic (systems above 256 states)
     for y in b['systems']:
         n = len(O.union_states(y['I'], y['Q'], y['coil']))
         dims.append(n)
         n_or = len(y['grid']['weights'])
-        if spectral_ok and n <= 192:
+        np_ = -(-n // 16) * 16
+        # every member's generator is real: in FP64 the spectral path takes padded dimensions <= 448
+        if spectral_ok and np_ <= 448:
             for k, v in spectral_stage_flops(n, b['nsteps'] + 1).items():
                 spec_flops[k] += n_or * v
+            if np_ > 256:
+                tri_bytes += n_or * prologue_bytes(np_)
         else:
             prop_flops += n_or * powder_flops_per_orientation(
                 n, b['nsteps'], st['taylor_terms'], st['ps_block'], st['sum_squarings_lo'] / n_or_tot, st['block'],
                 real=st['real_systems'] > 0)
     stages = stage_table(st, prop_flops, spec_flops if spectral_ok else None, mode=args.mode)
+    if spectral_ok and 'spk::k_tridiag' in stages:
+        stages['spk::k_tridiag'] = stages['spk::k_tridiag'] + (tri_bytes,)
     cpu = err = None
     if not args.no_cpu:
         # sample: 2 orientations of every 8th system, summed in the same way
@@ -395,7 +420,7 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=1000, help='orientations in the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--path', default='auto', choices=['auto', 'propagator', 'complex', 'krylov'],
-                    help='auto: spectral path for real generators (FP64, dim <= 192); propagator: the power '
+                    help='auto: spectral path for real generators (FP64, padded dim <= 448); propagator: the power '
                          'scheme with real products; complex: the general complex power scheme; krylov: '
                          'step() per time step (sys.enable krylov)')
     ap.add_argument('--autotune', type=int, default=2, metavar='PASSES',
@@ -437,6 +462,15 @@ def main():
                 'frac': round(lds / LDS_PEAK, 4), 'traffic': TRAFFIC.get((args.workload, args.mode, kernel)),
                 'kernel': kernel, 'kernel_ms_avg': round(k_ms, 3), 'lds_bytes_per_launch': stages[kernel][2],
                 'flops_per_launch': k_flops, 'fp64_frac': round(achieved / peak, 5)}
+    elif args.workload == 'batch' and kernel == 'spk::k_tridiag' and len(stages[kernel]) > 2:
+        # the batch's tridiagonalisation stage (k_tridiag_w and k_tridiag): bound by the HBM
+        # traffic of k_tridiag_w's global-memory prologue (systems above 256 states)
+        hbm = stages[kernel][2] / (k_ms * 1e-3) if k_ms > 0 else 0.0
+        roof = {'bound': 'hbm', 'achieved': round(hbm / 1e9, 1), 'peak': 8000.0, 'unit': 'GB/s',
+                'frac': round(hbm / 8e12, 4), 'traffic': TRAFFIC.get((args.workload, args.mode, kernel)),
+                'kernel': 'spk::k_tridiag_w + spk::k_tridiag', 'kernel_ms_avg': round(k_ms, 3),
+                'algorithmic_bytes_per_launch': stages[kernel][2], 'flops_per_launch': k_flops,
+                'fp64_frac': round(achieved / peak, 5)}
     else:
         bound = BOUND.get(kernel, 'mfma' if kernel.startswith('spk::k_powder') else 'valu')
         roof = {'bound': bound, 'achieved': round(achieved / 1e12, 3), 'peak': round(peak / 1e12, 1),

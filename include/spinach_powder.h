@@ -50,8 +50,8 @@ extern "C" {
  * result agrees with the complex path to rounding. */
 #define SP_MODE_COMPLEX 16
 /* Flag OR-ed into the mode: use the propagator power scheme for every system.
- * Without it, FP64 systems with a real generator and a coil-connected
- * dimension of at most 192 take the spectral path: one Householder
+ * Without it, FP64 systems with a real generator and a padded coil-connected
+ * dimension of at most 448 take the spectral path: one Householder
  * tridiagonalisation and one divide-and-conquer tridiagonal eigensolution per
  * orientation give f(t) = sum_k g_k exp(-i lambda_k t) directly (agreement with the
  * propagator path: ~1e-12 relative over thousands of steps). */

@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
   constexpr int kAsmStride = kTriNP + 4;
   __shared__ double sA[16 * kAsmStride];
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
   const int r = tid >> 4, c = tid & 15;
   const double h = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
   const double hd = (r == c) ? 2.0 : 1.0;
