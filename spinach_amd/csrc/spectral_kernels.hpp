@@ -149,10 +149,11 @@ __device__ __forceinline__ void householder_form(const double (&x)[NQ], double x
 // per orientation, so that two orientations (workgroups) share a CU and hide each
 // other's barrier and reduction latency (the whole matrix would fill the CU's
 // register file). 256 threads on a 16 x 16 grid, thread (r, c) = (tid / 16, tid % 16);
-// the thread holds A(r + 16a, c + 16b) for b <= a (78 slots). In the diagonal
-// blocks (a == b) the slot holds h * A with h = 1 (r > c), 1/2 (r == c, the
-// diagonal, so that the symmetric product counts it once) or 0 (r < c, an upper
-// entry, held by the thread (c, r) as its transpose).
+// the thread holds A(r + 16a, c + 16b) for b <= a (78 slots). The diagonal blocks
+// (a == b) are held whole, above the diagonal too: the symmetric product takes their
+// row sums only (each entry once), the blocks below them their row and column sums
+// (each entry for itself and for its transpose), and the update and the column copy
+// need no weights.
 constexpr int kTriThreads = 256;
 constexpr int kTriBlk = kTriNP / 16;                      // blocks of 16 rows / columns
 constexpr int kTriSlots = kTriBlk * (kTriBlk + 1) / 2;  // 78
@@ -160,24 +161,25 @@ __host__ __device__ constexpr int sym_slot(int a, int b) { return a * (a + 1) / 
 
 // The owners of column jj (the threads with c == jj % 16) copy its entries at rows
 // >= jj (rows below jj in its block get 0 or a stale value, never read) to srow.
-__device__ __forceinline__ void sym_copy_col(const double (&R)[kTriSlots], int jj, int r, int c, double hd,
-                                             double* srow) {
+__device__ __forceinline__ void sym_copy_col(const double (&R)[kTriSlots], int jj, int r, int c, double* srow) {
   if (c != (jj & 15)) return;
   const int bj = jj >> 4;
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
     if (b == bj) {
+      // (the empty volatile statement keeps the branches apart: merged, they would read the
+      // matrix at a run-time index, which puts the whole array in scratch memory)
+      asm volatile("" ::: "memory");
 #pragma unroll
-      for (int a = b; a < kTriBlk; a++) srow[r + 16 * a] = (a == b) ? hd * R[sym_slot(a, b)] : R[sym_slot(a, b)];
+      for (int a = b; a < kTriBlk; a++) srow[r + 16 * a] = R[sym_slot(a, b)];
     }
   }
 }
 
 // Column jj as it stands in the registers, reflected by wave 0 (every thread calls it).
-__device__ __forceinline__ void sym_publish(const double (&R)[kTriSlots], int jj, int r, int c, double hd,
-                                            int wave, int lane, double* srow, double* vb, double* ptau, double* sd,
-                                            double* se) {
-  sym_copy_col(R, jj, r, c, hd, srow);
+__device__ __forceinline__ void sym_publish(const double (&R)[kTriSlots], int jj, int r, int c, int wave, int lane,
+                                            double* srow, double* vb, double* ptau, double* sd, double* se) {
+  sym_copy_col(R, jj, r, c, srow);
   __syncthreads();
   if (wave == 0) {
     double x[kTriNP / 64];
@@ -216,7 +218,7 @@ __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const
       for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros
         const double x = R[sym_slot(a, b)];
         yr[a] = fma(x, vc, yr[a]);
-        yc = fma(x, vr[a], yc);
+        if (a > b) yc = fma(x, vr[a], yc);  // the diagonal block gives row sums only
       }
       spc[r][c + 16 * b] = yc;
       s = fma(vc, yc, s);
@@ -228,8 +230,8 @@ __device__ __forceinline__ double sym_matvec(const double (&R)[kTriSlots], const
 // A <- A - v w' - w v' over this thread's slots of the live column blocks (h weights the
 // diagonal blocks)
 __device__ __forceinline__ void sym_update(double (&R)[kTriSlots], const double (&vr)[kTriBlk],
-                                           const double (&wr)[kTriBlk], const double* v, const double* sw, double h,
-                                           int c, int bmin, unsigned live) {
+                                           const double (&wr)[kTriBlk], const double* v, const double* sw, int c,
+                                           int bmin, unsigned live) {
   double vnext = v[c + 16 * bmin], wnext = sw[c + 16 * bmin];  // (next block's loads ahead, as above)
 #pragma unroll
   for (int b = 0; b < kTriBlk; b++) {
@@ -240,11 +242,8 @@ __device__ __forceinline__ void sym_update(double (&R)[kTriSlots], const double 
         wnext = sw[c + 16 * (b + 1)];
       }
 #pragma unroll
-      for (int a = b; a < kTriBlk; a++) {  // rows at or beyond np hold zeros and stay zero
-        const double va = (a == b) ? h * vr[a] : vr[a];
-        const double wa = (a == b) ? h * wr[a] : wr[a];
-        R[sym_slot(a, b)] = fma(-va, wc, fma(-wa, vc, R[sym_slot(a, b)]));
-      }
+      for (int a = b; a < kTriBlk; a++)  // rows at or beyond np hold zeros and stay zero
+        R[sym_slot(a, b)] = fma(-vr[a], wc, fma(-wr[a], vc, R[sym_slot(a, b)]));
     }
   }
 }
@@ -267,8 +266,6 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
   const int r = tid >> 4, c = tid & 15;
-  const double h = (r > c) ? 1.0 : ((r == c) ? 0.5 : 0.0);
-  const double hd = (r == c) ? 2.0 : 1.0;
   for (int k = tid; k < 16 * kAsmStride; k += kTriThreads) sA[k] = 0.0;
   for (int k = tid; k <= kTriNP; k += kTriThreads) srow[k] = 0.0;
 
@@ -322,7 +319,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
       const int ee = (e < nnz) ? e : 0;
       const int i = prow[ee], jc = pcol[ee];
       cval[k] = (e < nnz) ? real_generator_entry(p, sdesc, cre, cim, ee) : 0.0;
-      cpos[k] = (e < nnz && jc <= i) ? ((i << 8) | jc) : -1;
+      cpos[k] = (e < nnz && (jc >> 4) <= (i >> 4)) ? ((i << 8) | jc) : -1;
     }
 #pragma unroll
     for (int a = 0; a < kTriBlk; a++) {
@@ -333,13 +330,13 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
             sA[((cpos[k] >> 8) - 16 * a) * kAsmStride + (cpos[k] & 255)] = cval[k];
         for (int e = max(sdesc.rb[a], kAsmCache * kTriThreads) + tid; e < sdesc.rb[a + 1]; e += kTriThreads) {
           const int i = prow[e], jc = pcol[e];
-          if (jc <= i) sA[(i - 16 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
+          if ((jc >> 4) <= (i >> 4)) sA[(i - 16 * a) * kAsmStride + jc] = real_generator_entry(p, sdesc, cre, cim, e);
         }
         __syncthreads();
 #pragma unroll
         for (int b = 0; b <= a; b++) {
           double* s = &sA[r * kAsmStride + c + 16 * b];
-          R[sym_slot(a, b)] = (b == a) ? h * *s : *s;
+          R[sym_slot(a, b)] = *s;
           *s = 0.0;
         }
         __syncthreads();
@@ -348,7 +345,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         for (int b = 0; b <= a; b++) R[sym_slot(a, b)] = 0.0;
       }
     }
-    sym_publish(R, 0, r, c, hd, wave, lane, srow, sv[0], &stau[0], sd, se);
+    sym_publish(R, 0, r, c, wave, lane, srow, sv[0], &stau[0], sd, se);
     __syncthreads();
 
     for (int j = 0; j < n - 2; j++) {
@@ -363,7 +360,7 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         // the wave that reflects column jn rotates with the step, so that the two workgroups
         // sharing a CU rarely have their look-ahead waves on the same SIMD
         const int la = jn & 3;
-        sym_copy_col(R, jn, r, c, hd, srow);  // column jn as the previous step left it
+        sym_copy_col(R, jn, r, c, srow);  // column jn as the previous step left it
         // A*v: row sums yr over this thread's columns, column sums yc over its rows
         double vr[kTriBlk], yr[kTriBlk];
 #pragma unroll
@@ -447,15 +444,15 @@ __global__ void __launch_bounds__(kTriThreads, 2) k_tridiag(SpecParams p) {
         double wr[kTriBlk];
 #pragma unroll
         for (int a = 0; a < kTriBlk; a++) wr[a] = sw[r + 16 * a];
-        sym_update(R, vr, wr, v, sw, h, c, bmin, live);
+        sym_update(R, vr, wr, v, sw, c, bmin, live);
         __builtin_amdgcn_s_setprio(0);
       } else {
         // no reflection at this step: column jn stands as it is in the registers
-        sym_publish(R, jn, r, c, hd, wave, lane, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
+        sym_publish(R, jn, r, c, wave, lane, srow, sv[cb ^ 1], &stau[cb ^ 1], sd, se);
       }
       __syncthreads();
     }
-    if (n >= 2) sym_publish(R, n - 1, r, c, hd, wave, lane, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
+    if (n >= 2) sym_publish(R, n - 1, r, c, wave, lane, srow, sv[(n - 1) & 1], &stau[(n - 1) & 1], sd, se);
     __syncthreads();
     const int nm = p.spec_nm;
     double* out = p.spec + (size_t)it * 6 * nm;
