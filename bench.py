@@ -39,7 +39,7 @@ PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12}  # MI355X dense MFMA FP64 / FP32, FLO
 # LDS bandwidth of MI355X: 128 bytes per clock per CU, 256 CUs, 2.4 GHz
 LDS_PEAK = 128 * 256 * 2.4e9
 # what bounds each kernel: FP64/FP32 vector ALU, matrix cores, LDS
-BOUND = {'spk::k_tridiag': 'valu', 'spk::k_dceig': 'valu', 'spk::k_synth': 'mfma', 'spk::k_mas': 'mfma',
+BOUND = {'spk::k_tridiag': 'valu', 'spk::k_dceig': 'valu', 'spk::k_synth': 'mfma', 'spk::k_mas': 'mfma', 'spk::k_mas_tiled': 'mfma',
          'sgk::k_gemm_bf16': 'mfma'}
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs of this script (counters scaled by a device copy of known size:
@@ -85,6 +85,19 @@ def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
     n_rho = math.ceil(npts / ticks)
     per_tick = (ps_q - 1) + taylor_k // ps_q + sexp + 3
     return ticks * per_tick * 8.0 * n ** 3 + 2 * (n_rho - 1) * 8.0 * n ** 3 + ticks * n_rho * 8.0 * n * n
+
+
+def mas_tiled_flops_per_orientation(ntile, npair, nsteps, ticks, taylor_k, ps_q, sexp):
+    """The same algorithm on 16 x 16 tiles: every tick, per tile an expm (q-1 + K//q products
+    + s squarings) and the U update, per tile pair C_j = U_s' coil_st U_t (2 products); per
+    rotor period two products per pair for the rho chain; detection over the pairs' 256
+    elements each."""
+    npts = nsteps + 1
+    n_rho = math.ceil(npts / ticks)
+    prod = 8.0 * 16 ** 3
+    per_tick = ntile * ((ps_q - 1) + taylor_k // ps_q + sexp + 1) + 2 * npair
+    return (ticks * per_tick * prod + 2 * (n_rho - 1) * npair * prod
+            + ticks * n_rho * 8.0 * npair * 256)
 
 
 def spectral_stage_flops(n, npts):
@@ -321,8 +334,15 @@ def bench_mas(args):
     sec, kms, st = time_plan(plan, args.steps, args.warmup)
     fid, per = plan.result(per_orientation=True)
     log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms')
-    flops = n_or * mas_flops_per_orientation(64, w['nsteps'], w['ticks'], st['taylor_terms'], st['ps_block'],
-                                             st['sum_squarings_lo'] / n_or)
+    if st['mas_tiles']:
+        kname = 'spk::k_mas_tiled'
+        flops = n_or * mas_tiled_flops_per_orientation(st['mas_tiles'], st['mas_pairs'], w['nsteps'], w['ticks'],
+                                                       st['taylor_terms'], st['ps_block'],
+                                                       st['sum_squarings_lo'] / n_or)
+    else:
+        kname = 'spk::k_mas'
+        flops = n_or * mas_flops_per_orientation(64, w['nsteps'], w['ticks'], st['taylor_terms'], st['ps_block'],
+                                                 st['sum_squarings_lo'] / n_or)
     cpu = err = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
@@ -342,7 +362,7 @@ def bench_mas(args):
     cfg = {'workload': f'MAS 12.5 kHz, 6 spins (4x13C, 2x1H) in Hilbert space (dim 64), 128 ticks per period '
                        f'x 64 periods, {n_or} orientations', 'orientations': n_or, 'nsteps': w['nsteps'],
            'hilbert_dim': 64, 'mode': args.mode}
-    return n_or * w['nsteps'] / sec, sec, kms, stage_table(st, flops, None, 'spk::k_mas'), cfg, cpu, err
+    return n_or * w['nsteps'] / sec, sec, kms, stage_table(st, flops, None, kname), cfg, cpu, err
 
 
 def bench_surrogate(args):

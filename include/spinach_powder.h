@@ -111,7 +111,8 @@ typedef struct sp_stats {
   int64_t full_dim;          /* Liouville dimension of I and Q */
   int64_t reduced_dim;       /* dimension of the coil-connected subspace */
   int64_t reduced_nnz;       /* structural nonzeros of the reduced union pattern */
-  int32_t padded_dim;        /* reduced dimension rounded up to the tile size */
+  int32_t padded_dim;        /* reduced dimension rounded up to the tile size (sp_mas_fid: 16 when
+                                the invariant subspaces are propagated as 16 x 16 tiles) */
   int32_t block;             /* time-block length B of the power scheme */
   int32_t taylor_terms;      /* Taylor degree of the scaled exponential */
   int32_t max_squarings;     /* largest scaling exponent over orientations */
@@ -128,6 +129,9 @@ typedef struct sp_stats {
   double plan_ms;            /* host planning (reduction, layouts) and upload of the plan */
   int32_t eig_fallbacks;     /* spectral orientations whose tridiagonal eigensolution did not
                                 converge, re-run on the propagator path (real products) */
+  int32_t mas_tiles;         /* sp_mas_fid: 16 x 16 tiles the invariant subspaces are packed into
+                                (0: the dense kernel ran) */
+  int32_t mas_pairs;         /* sp_mas_fid: tile pairs where both coil and rho0 have entries */
   int32_t pad;
 } sp_stats;
 

@@ -291,3 +291,419 @@ __global__ void __launch_bounds__(kThreads) k_mas(MasParams<R> p) {
 }
 
 }  // namespace spk
+
+namespace spk {
+
+// ------------------------------------------------------------------ tiled MAS (FP64)
+// The secular Hamiltonian of a system with several species conserves each species' total Fz,
+// so H(phi) = I + sum D(phi)(k,m) Q{r}{k,m} is block diagonal at every rotor phase: the
+// connected components of the union pattern of I and the Q slots are invariant subspaces of
+// every tick propagator and of every U_j. When no component exceeds 16 states, the planner packs
+// the components into tiles of 16 (first-fit decreasing) and numbers the basis tile by tile;
+// then H(phi), each P_j and each U_j are block diagonal in 16 x 16 tiles, every product of the
+// dense algorithm above splits into independent 16 x 16 products, and
+//     C_j = U_j' coil U_j,  rho_k = U^k rho0 U'^k
+// need only the tile pairs (s, t) where both coil and rho0 have entries (the detection
+// sum(conj(C_j) .* rho_k) sees nothing else). Same mathematics, same evaluation order of the
+// Taylor polynomial; one wave owns a tile (tiles wave, wave + 4, ...) and a 16 x 16 complex
+// product is 16 FP64 MFMAs on registers.
+//
+// Register layouts of a 16 x 16 matrix in a wave (the MFMA f64 16x16x4 operand layouts):
+//   C layout (accumulator):  lane l holds M[4e + l/16][l%16], e = 0..3;
+//   A operand, k-chunk s:    lane l holds M[l%16][4s + l/16]  (the C layout of M');
+//   B operand, k-chunk s:    lane l holds M[4s + l/16][l%16]  (the C layout itself, e = s).
+// A matrix in the C layout therefore serves as a B operand as it is; as an A operand it is
+// transposed through the wave's LDS scratch tile, or read transposed from LDS where it is stored.
+
+constexpr int kTileN = 16;
+constexpr int kTileLd = 17;                 // LDS row stride (one pad double: transposed reads spread over banks)
+constexpr int kTilePlane = kTileN * kTileLd;  // doubles per real plane of an LDS tile
+constexpr int kMasMaxTiles = 8;
+
+struct MasTParams {
+  const int32_t* epos;  // per pattern entry (sorted by tile): tile-local position r * 16 + c
+  const int32_t* toff;  // per tile: first entry; ntile + 1 values
+  int32_t ntile;
+  int32_t nnz;
+  int32_t nq;
+  const double* vals;    // (1+nq)*nnz interleaved complex, entry order as epos: I, then Q slots
+  const int32_t* slots;  // Q slot ids
+  int32_t npair;
+  const int32_t* pair_s;  // tile pairs where both coil and rho0 have entries
+  const int32_t* pair_t;
+  const double* coil_t;  // npair x 256 interleaved complex, the tile row-major
+  const double* rho0_t;  // npair x 256 interleaved complex
+  const double* alphas;
+  const double* betas;
+  const double* gammas;
+  int64_t n_or;
+  const double* rot2lab;
+  const double* lab2rot;
+  double dt;
+  int32_t ticks;
+  int32_t n_rho;
+  int64_t nsteps;
+  int32_t taylor_k;
+  int32_t ps_q;
+  double theta;
+  double* work;  // per workgroup: C (ticks x npair*256) and rho (n_rho x npair*256), planar complex
+  int64_t work_stride;
+  unsigned int* queue;
+  double* per_or;
+  int32_t* sq_out;
+};
+
+struct MasTShared {
+  union {
+    GemmLds<double> g;  // the detection product, after the propagation
+    struct {
+      double U[kMasMaxTiles][2 * kTilePlane];     // the tiles of U_j, row-major planar
+      double scr[kThreads / 64][2 * kTilePlane];  // per-wave scratch tile
+    } t;
+  } u;
+  double B1r[9], B1i[9], B2r[25], B2i[25];
+  double cre[34], cim[34];
+  double tc[32];
+  int item;
+  int smax;
+};
+
+struct CTile {  // a 16 x 16 complex matrix in the C layout
+  Mma<double>::v4 r, i;
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// D = A * B: A as its A-operand chunks (ar, ai), B in the C layout
+__device__ __forceinline__ CTile cmm16(const double (&ar)[4], const double (&ai)[4], const CTile& b) {
+  typedef Mma<double>::v4 v4;
+  v4 dr = v4{0, 0, 0, 0}, di = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    dr = Mma<double>::fma(ar[s], b.r[s], dr);
+    di = Mma<double>::fma(ar[s], b.i[s], di);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    dr = Mma<double>::fma(-ai[s], b.i[s], dr);
+    di = Mma<double>::fma(ai[s], b.r[s], di);
+  }
+  return CTile{dr, di};
+}
+
+// the A-operand chunks of M (in the C layout), through the wave's scratch tile
+__device__ __forceinline__ void tile_aop(const CTile& m, double* scr, int lane, double (&ar)[4], double (&ai)[4]) {
+  const int rq = lane >> 4, cq = lane & 15;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    scr[(4 * e + rq) * kTileLd + cq] = m.r[e];
+    scr[kTilePlane + (4 * e + rq) * kTileLd + cq] = m.i[e];
+  }
+  wave_sync_lds();
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    ar[s] = scr[cq * kTileLd + 4 * s + rq];
+    ai[s] = scr[kTilePlane + cq * kTileLd + 4 * s + rq];
+  }
+  wave_sync_lds();
+}
+
+// loads of a row-major planar LDS tile: the C layout, and the A-operand chunks (optionally of
+// the conjugate)
+__device__ __forceinline__ CTile tile_load_c(const double* t, int lane) {
+  const int rq = lane >> 4, cq = lane & 15;
+  CTile m;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    m.r[e] = t[(4 * e + rq) * kTileLd + cq];
+    m.i[e] = t[kTilePlane + (4 * e + rq) * kTileLd + cq];
+  }
+  return m;
+}
+
+__device__ __forceinline__ void tile_load_aop(const double* t, int lane, bool conj, double (&ar)[4],
+                                              double (&ai)[4]) {
+  const int rq = lane >> 4, cq = lane & 15;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    ar[s] = t[cq * kTileLd + 4 * s + rq];
+    const double x = t[kTilePlane + cq * kTileLd + 4 * s + rq];
+    ai[s] = conj ? -x : x;
+  }
+}
+
+__device__ __forceinline__ void tile_store(double* t, const CTile& m, int lane) {
+  const int rq = lane >> 4, cq = lane & 15;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    t[(4 * e + rq) * kTileLd + cq] = m.r[e];
+    t[kTilePlane + (4 * e + rq) * kTileLd + cq] = m.i[e];
+  }
+}
+
+// a global tile (256 interleaved complex, row-major) in the C layout
+__device__ __forceinline__ CTile tile_global_c(const double* g, int lane) {
+  const int rq = lane >> 4, cq = lane & 15;
+  CTile m;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    m.r[e] = g[2 * ((4 * e + rq) * kTileN + cq)];
+    m.i[e] = g[2 * ((4 * e + rq) * kTileN + cq) + 1];
+  }
+  return m;
+}
+
+// acc += sum_{i2 < q, j0*q + i2 <= K} tc[j0*q + i2] X^i2 (X^0 = identity; x1, x2, x3 = X, X^2, X^3)
+__device__ __forceinline__ void tile_taylor_block(CTile& acc, int j0, int q, int K, const double* tc, const CTile& x1,
+                                                  const CTile& x2, const CTile& x3, int rq, int cq) {
+#pragma unroll
+  for (int i2 = 0; i2 < 4; i2++) {
+    const int k = j0 * q + i2;
+    if (i2 < q && k <= K) {
+      const double c = tc[k];
+      if (i2 == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (4 * e + rq == cq) acc.r[e] += c;
+      } else {
+        const CTile& x = (i2 == 1) ? x1 : ((i2 == 2) ? x2 : x3);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          acc.r[e] = fma(c, x.r[e], acc.r[e]);
+          acc.i[e] = fma(c, x.i[e], acc.i[e]);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
+  typedef Mma<double>::v4 v4;
+  __shared__ MasTShared sh;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rq = lane >> 4, cq = lane & 15;
+  constexpr int kWaves = kThreads / 64;
+  const int64_t L = (int64_t)p.npair * 256;
+  double* Cre = p.work + (size_t)blockIdx.x * p.work_stride;  // ticks x L
+  double* Cim = Cre + (size_t)p.ticks * L;
+  double* Rre = Cim + (size_t)p.ticks * L;  // n_rho x L
+  double* Rim = Rre + (size_t)p.n_rho * L;
+  double* scr = sh.u.t.scr[wave];
+  if (tid < 32) {
+    double f = 1.0;
+    for (int k = 2; k <= tid; k++) f *= k;
+    sh.tc[tid] = 1.0 / f;
+  }
+  __syncthreads();
+  const int K = p.taylor_k, q = p.ps_q, rr = K / q;
+
+  for (;;) {
+    if (tid == 0) sh.item = (int)atomicAdd(p.queue, 1u);
+    __syncthreads();
+    const int64_t o = sh.item;
+    __syncthreads();
+    if (o >= p.n_or) break;
+    if (tid == 0) {
+      // B = D_lab2rot * D_initial for ranks 1 and 2
+      double dr[34], di[34];
+      wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], dr, di);
+      for (int k = 0; k < 3; k++)
+        for (int m = 0; m < 3; m++) {
+          double sr = 0, si = 0;
+          for (int j = 0; j < 3; j++) {
+            double ar = p.lab2rot[2 * (k * 3 + j)], ai = p.lab2rot[2 * (k * 3 + j) + 1];
+            double br = dr[j * 3 + m], bi = di[j * 3 + m];
+            sr += ar * br - ai * bi;
+            si += ar * bi + ai * br;
+          }
+          sh.B1r[k * 3 + m] = sr;
+          sh.B1i[k * 3 + m] = si;
+        }
+      for (int k = 0; k < 5; k++)
+        for (int m = 0; m < 5; m++) {
+          double sr = 0, si = 0;
+          for (int j = 0; j < 5; j++) {
+            double ar = p.lab2rot[2 * (9 + k * 5 + j)], ai = p.lab2rot[2 * (9 + k * 5 + j) + 1];
+            double br = dr[9 + j * 5 + m], bi = di[9 + j * 5 + m];
+            sr += ar * br - ai * bi;
+            si += ar * bi + ai * br;
+          }
+          sh.B2r[k * 5 + m] = sr;
+          sh.B2i[k * 5 + m] = si;
+        }
+      sh.smax = 0;
+    }
+    // U_0 = identity on every tile
+    for (int t = wave; t < p.ntile; t += kWaves) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        sh.u.t.U[t][(4 * e + rq) * kTileLd + cq] = (4 * e + rq == cq) ? 1.0 : 0.0;
+        sh.u.t.U[t][kTilePlane + (4 * e + rq) * kTileLd + cq] = 0.0;
+      }
+    }
+    int smax = 0;
+    for (int j = 0; j < p.ticks; j++) {
+      __syncthreads();  // every tile of U_j stored; the rotor coefficients free
+      // C_j = U_j' coil U_j on the tile pairs: W = coil_st U_t, C = U_s' W
+      for (int pp = wave; pp < p.npair; pp += kWaves) {
+        const int s = p.pair_s[pp], t = p.pair_t[pp];
+        double car[4], cai[4];
+        {
+          const double* g = p.coil_t + (size_t)pp * 512;  // A operand of coil_st: element (cq, 4s' + rq)
+#pragma unroll
+          for (int s2 = 0; s2 < 4; s2++) {
+            car[s2] = g[2 * (cq * kTileN + 4 * s2 + rq)];
+            cai[s2] = g[2 * (cq * kTileN + 4 * s2 + rq) + 1];
+          }
+        }
+        const CTile W = cmm16(car, cai, tile_load_c(sh.u.t.U[t], lane));
+        // A operand of U_s' (conjugate transpose): chunk s2, lane l: conj(U_s[4 s2 + l/16][l%16])
+        double uar[4], uai[4];
+        {
+          const CTile us = tile_load_c(sh.u.t.U[s], lane);
+#pragma unroll
+          for (int s2 = 0; s2 < 4; s2++) {
+            uar[s2] = us.r[s2];
+            uai[s2] = -us.i[s2];
+          }
+        }
+        const CTile C = cmm16(uar, uai, W);
+        double* cr = Cre + (size_t)j * L + (size_t)pp * 256;
+        double* ci = Cim + (size_t)j * L + (size_t)pp * 256;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          cr[(4 * e + rq) * kTileN + cq] = C.r[e];
+          ci[(4 * e + rq) * kTileN + cq] = C.i[e];
+        }
+      }
+      if (tid == 0) {
+        const double phi = 6.283185307179586476925 * (double)j / (double)p.ticks;
+        rotor_coeffs(3, p.rot2lab, sh.B1r, sh.B1i, phi, sh.cre, sh.cim);
+        rotor_coeffs(5, p.rot2lab + 18, sh.B2r, sh.B2i, phi, sh.cre + 9, sh.cim + 9);
+      }
+      __syncthreads();  // the coefficients written; every wave has read the U tiles it needs for C_j
+      for (int t = wave; t < p.ntile; t += kWaves) {
+        // X = -i dt H on tile t, assembled in the scratch tile
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          scr[(4 * e + rq) * kTileLd + cq] = 0.0;
+          scr[kTilePlane + (4 * e + rq) * kTileLd + cq] = 0.0;
+        }
+        wave_sync_lds();
+        for (int e = p.toff[t] + lane; e < p.toff[t + 1]; e += 64) {
+          double xr = p.vals[2 * e], xi = p.vals[2 * e + 1];
+          for (int qq = 0; qq < p.nq; qq++) {
+            const double* qv = p.vals + 2 * (size_t)(qq + 1) * p.nnz;
+            const double cr = sh.cre[p.slots[qq]], ci = sh.cim[p.slots[qq]];
+            xr += cr * qv[2 * e] - ci * qv[2 * e + 1];
+            xi += cr * qv[2 * e + 1] + ci * qv[2 * e];
+          }
+          const int pos = p.epos[e];
+          const int r0 = pos >> 4, c0 = pos & 15;
+          scr[r0 * kTileLd + c0] = p.dt * xi;  // -i*dt*H
+          scr[kTilePlane + r0 * kTileLd + c0] = -p.dt * xr;
+        }
+        wave_sync_lds();
+        CTile X = tile_load_c(scr, lane);
+        // 1-norm (largest column sum of moduli) -> scaling exponent
+        double cs = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) cs += sqrt(X.r[e] * X.r[e] + X.i[e] * X.i[e]);
+        cs += __shfl_xor(cs, 16);
+        cs += __shfl_xor(cs, 32);
+        double nrm = cs;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) nrm = fmax(nrm, __shfl_xor(nrm, off));
+        nrm = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(nrm)),
+                               __builtin_amdgcn_readfirstlane(__double2loint(nrm)));  // the same in every lane
+        int sexp = 0;
+        if (nrm > p.theta) sexp = (int)ceil(log2(nrm / p.theta));
+        smax = sexp > smax ? sexp : smax;
+        const double scl = ldexp(1.0, -sexp);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          X.r[e] *= scl;
+          X.i[e] *= scl;
+        }
+        // Paterson-Stockmeyer Taylor polynomial of degree K in blocks of q (q <= 4): powers
+        // X^2 .. X^q, the top block, then K/q Horner steps with Y = X^q (as wg_expm)
+        // powers X^2 .. X^q (q <= 4), Y = X^q
+        double xar[4], xai[4];
+        tile_aop(X, scr, lane, xar, xai);
+        const CTile X2 = cmm16(xar, xai, X);
+        const CTile X3 = (q >= 3) ? cmm16(xar, xai, X2) : X2;
+        const CTile X4 = (q >= 4) ? cmm16(xar, xai, X3) : X3;
+        const CTile Y = (q >= 4) ? X4 : ((q == 3) ? X3 : ((q == 2) ? X2 : X));
+        CTile acc{v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+        tile_taylor_block(acc, rr, q, K, sh.tc, X, X2, X3, rq, cq);
+        double yar[4], yai[4];
+        tile_aop(Y, scr, lane, yar, yai);
+        for (int jj = rr - 1; jj >= 0; jj--) {
+          acc = cmm16(yar, yai, acc);
+          tile_taylor_block(acc, jj, q, K, sh.tc, X, X2, X3, rq, cq);
+        }
+        for (int s = 0; s < sexp; s++) {
+          double par[4], pai[4];
+          tile_aop(acc, scr, lane, par, pai);
+          acc = cmm16(par, pai, acc);
+        }
+        // U_t <- P U_t
+        double par[4], pai[4];
+        tile_aop(acc, scr, lane, par, pai);
+        const CTile Un = cmm16(par, pai, tile_load_c(sh.u.t.U[t], lane));
+        tile_store(sh.u.t.U[t], Un, lane);
+      }
+    }
+    __syncthreads();  // U = U_ticks, the period propagator, on every tile
+    // rho_k = U rho_{k-1} U' on the pairs: T = U_s rho, rho <- T U_t'
+    for (int pp = wave; pp < p.npair; pp += kWaves) {
+      const int s = p.pair_s[pp], t = p.pair_t[pp];
+      double usr[4], usi[4], utr[4], uti[4];
+      tile_load_aop(sh.u.t.U[s], lane, false, usr, usi);  // A operand of U_s
+      // B operand of U_t' (C layout of U_t'): element (4e + l/16, l%16) = conj(U_t[l%16][4e + l/16])
+      tile_load_aop(sh.u.t.U[t], lane, true, utr, uti);
+      const CTile Ut{v4{utr[0], utr[1], utr[2], utr[3]}, v4{uti[0], uti[1], uti[2], uti[3]}};
+      CTile rho = tile_global_c(p.rho0_t + (size_t)pp * 512, lane);
+      for (int k = 0; k < p.n_rho; k++) {
+        if (k > 0) {
+          const CTile T = cmm16(usr, usi, rho);
+          double tar[4], tai[4];
+          tile_aop(T, scr, lane, tar, tai);
+          rho = cmm16(tar, tai, Ut);
+        }
+        double* rr_ = Rre + (size_t)k * L + (size_t)pp * 256;
+        double* ri_ = Rim + (size_t)k * L + (size_t)pp * 256;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          rr_[(4 * e + rq) * kTileN + cq] = rho.r[e];
+          ri_[(4 * e + rq) * kTileN + cq] = rho.i[e];
+        }
+      }
+    }
+    if (lane == 0) atomicMax(&sh.smax, smax);
+    __syncthreads();
+    if (tid == 0 && p.sq_out) p.sq_out[o] = sh.smax;
+    // f(k*ticks + j) = sum conj(C_j) .* rho_k over the pairs' elements
+    {
+      const int64_t npts = p.nsteps + 1;
+      double* out = p.per_or + 2 * (size_t)o * npts;
+      const int ticks = p.ticks;
+      wg_gemm_t<double, false, true, true, false>(ticks, p.n_rho, (int)L, Cre, Cim, (int)L, Rre, Rim, (int)L,
+                                                  [=](int jj, int kk, double vr, double vi) {
+                                                    int64_t tt = (int64_t)kk * ticks + jj;
+                                                    if (tt < npts) {
+                                                      out[2 * tt] = vr;
+                                                      out[2 * tt + 1] = vi;
+                                                    }
+                                                  },
+                                                  sh.u.g);
+    }
+  }
+}
+
+}  // namespace spk

@@ -178,6 +178,11 @@ struct sp_plan {
   bool mas = false;
   int32_t ticks = 0, n_rho = 0, mas_np = 0, mas_nnz = 0, mas_nq = 0;
   DevBuf d_rot2lab, d_lab2rot;
+  // tiled MAS (FP64, every invariant component of the union pattern <= 16 states): tiles, the
+  // pattern's tile-local positions and per-tile ranges, the tile pairs of coil and rho0
+  bool mas_tiled = false;
+  int32_t mas_ntile = 0, mas_npair = 0;
+  DevBuf d_epos, d_toff, d_pair_s, d_pair_t, d_coil_t, d_rho0_t;
 };
 
 namespace {
@@ -901,6 +906,127 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
     for (int k = 0; k < 5; k++) lab2rot[2 * (9 + k * 5 + k)] = 1.0;
   }
   hipStream_t st = ctx->stream;
+  // invariant subspaces of every H(phi): connected components of the union pattern. When none
+  // exceeds 16 states (and in FP64), pack them into tiles of 16, first-fit decreasing, and
+  // propagate tile by tile (spk::k_mas_tiled); SPINACH_MAS_DENSE=1 keeps the dense kernel.
+  {
+    std::vector<int64_t> par(N);
+    for (int64_t i = 0; i < N; i++) par[i] = i;
+    auto root = [&](int64_t x) {
+      while (par[x] != x) x = par[x] = par[par[x]];
+      return x;
+    };
+    for (int32_t e = 0; e < nnz; e++) {
+      const int64_t a = root(row[e]), b = root(col[e]);
+      if (a != b) par[std::max(a, b)] = std::min(a, b);
+    }
+    std::vector<std::vector<int64_t>> comps;
+    std::vector<int64_t> cid(N, -1);
+    for (int64_t i = 0; i < N; i++) {
+      const int64_t rt = root(i);
+      if (cid[rt] < 0) {
+        cid[rt] = (int64_t)comps.size();
+        comps.emplace_back();
+      }
+      comps[cid[rt]].push_back(i);
+    }
+    size_t biggest = 0;
+    for (auto& c : comps) biggest = std::max(biggest, c.size());
+    const char* ed = getenv("SPINACH_MAS_DENSE");
+    if (mode == SP_MODE_FP64 && biggest <= (size_t)spk::kTileN && !(ed && atoi(ed))) {
+      std::vector<size_t> order(comps.size());
+      for (size_t k = 0; k < order.size(); k++) order[k] = k;
+      std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return comps[a].size() > comps[b].size(); });
+      std::vector<int> used;                       // states placed in each tile
+      std::vector<int64_t> tile_of(N), local(N);   // tile and position of each state
+      for (size_t k : order) {
+        const int sz = (int)comps[k].size();
+        size_t t = 0;
+        while (t < used.size() && used[t] + sz > spk::kTileN) t++;
+        if (t == used.size()) used.push_back(0);
+        for (int64_t i : comps[k]) {
+          tile_of[i] = (int64_t)t;
+          local[i] = used[t]++;
+        }
+      }
+      const int ntile = (int)used.size();
+      if (ntile <= spk::kMasMaxTiles) {
+        // pattern entries by tile, with tile-local positions; values in the same order
+        std::vector<int32_t> eorder(nnz);
+        for (int32_t e = 0; e < nnz; e++) eorder[e] = e;
+        std::stable_sort(eorder.begin(), eorder.end(),
+                         [&](int32_t a, int32_t b) { return tile_of[row[a]] < tile_of[row[b]]; });
+        std::vector<int32_t> epos(nnz), toff(ntile + 1, 0);
+        const int nplanes = 1 + (int)slots.size();
+        std::vector<double> tvals(vals.size());
+        for (int32_t k = 0; k < nnz; k++) {
+          const int32_t e = eorder[k];
+          epos[k] = (int32_t)(local[row[e]] * spk::kTileN + local[col[e]]);
+          toff[tile_of[row[e]] + 1]++;
+          for (int pln = 0; pln < nplanes; pln++) {
+            tvals[2 * ((size_t)pln * nnz + k)] = vals[2 * ((size_t)pln * nnz + e)];
+            tvals[2 * ((size_t)pln * nnz + k) + 1] = vals[2 * ((size_t)pln * nnz + e) + 1];
+          }
+        }
+        for (int t = 0; t < ntile; t++) toff[t + 1] += toff[t];
+        std::vector<int32_t> trow(nnz), tcol(nnz);  // row and col follow the values' order
+        for (int32_t k = 0; k < nnz; k++) {
+          trow[k] = row[eorder[k]];
+          tcol[k] = col[eorder[k]];
+        }
+        // tile pairs where both coil and rho0 have entries (MATLAB column-major N x N input)
+        std::vector<double> cblk((size_t)ntile * ntile * 512, 0.0), rblk((size_t)ntile * ntile * 512, 0.0);
+        std::vector<char> cnz((size_t)ntile * ntile, 0), rnz((size_t)ntile * ntile, 0);
+        for (int64_t c = 0; c < N; c++)
+          for (int64_t r = 0; r < N; r++) {
+            const size_t src = 2 * (size_t)(c * N + r);
+            const size_t pr = (size_t)tile_of[r] * ntile + tile_of[c];
+            const size_t dst = pr * 512 + 2 * (size_t)(local[r] * spk::kTileN + local[c]);
+            if (coil[src] != 0.0 || coil[src + 1] != 0.0) {
+              cnz[pr] = 1;
+              cblk[dst] = coil[src];
+              cblk[dst + 1] = coil[src + 1];
+            }
+            if (rho0[src] != 0.0 || rho0[src + 1] != 0.0) {
+              rnz[pr] = 1;
+              rblk[dst] = rho0[src];
+              rblk[dst + 1] = rho0[src + 1];
+            }
+          }
+        std::vector<int32_t> ps, pt;
+        std::vector<double> ct, rt;
+        for (int a = 0; a < ntile; a++)
+          for (int b = 0; b < ntile; b++) {
+            const size_t pr = (size_t)a * ntile + b;
+            if (cnz[pr] && rnz[pr]) {
+              ps.push_back(a);
+              pt.push_back(b);
+              ct.insert(ct.end(), cblk.begin() + pr * 512, cblk.begin() + (pr + 1) * 512);
+              rt.insert(rt.end(), rblk.begin() + pr * 512, rblk.begin() + (pr + 1) * 512);
+            }
+          }
+        if (ps.empty()) {  // nothing detected: keep one pair so the kernel's shapes stay valid
+          ps.push_back(0);
+          pt.push_back(0);
+          ct.assign(512, 0.0);
+          rt.assign(512, 0.0);
+        }
+        pl->mas_tiled = true;
+        pl->np_max = spk::kTileN;  // reported as the padded dimension: every product is 16 x 16
+        pl->mas_ntile = ntile;
+        pl->mas_npair = (int32_t)ps.size();
+        upload(pl->d_epos, epos, st);
+        upload(pl->d_toff, toff, st);
+        upload(pl->d_pair_s, ps, st);
+        upload(pl->d_pair_t, pt, st);
+        upload(pl->d_coil_t, ct, st);
+        upload(pl->d_rho0_t, rt, st);
+        vals.swap(tvals);
+        row.swap(trow);
+        col.swap(tcol);
+      }
+    }
+  }
   upload(pl->d_row, row, st);
   upload(pl->d_col, col, st);
   upload(pl->d_vals, vals, st);
@@ -927,7 +1053,13 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * per_cu, n_or));
   const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
   const int64_t mat = 2 * (int64_t)np * np;
-  pl->work_stride = 10 * mat + mat * (int64_t)pl->ticks + mat * (int64_t)pl->n_rho;
+  if (pl->mas_tiled) {
+    // per workgroup: C (ticks x npair*256) and rho (n_rho x npair*256), planar complex FP64
+    const int64_t Lt = (int64_t)pl->mas_npair * 256;
+    pl->work_stride = 2 * Lt * ((int64_t)pl->ticks + pl->n_rho);
+  } else {
+    pl->work_stride = 10 * mat + mat * (int64_t)pl->ticks + mat * (int64_t)pl->n_rho;
+  }
   pl->work_stride = (pl->work_stride + 63) / 64 * 64;
   pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
   pl->d_per.alloc((size_t)n_or * npts * 2 * sizeof(double));
@@ -972,6 +1104,42 @@ void launch_mas(sp_plan* pl) {
   HIPCHK(hipGetLastError());
 }
 
+void launch_mas_tiled(sp_plan* pl) {
+  spk::MasTParams kp{};
+  kp.epos = pl->d_epos.as<int32_t>();
+  kp.toff = pl->d_toff.as<int32_t>();
+  kp.ntile = pl->mas_ntile;
+  kp.nnz = pl->mas_nnz;
+  kp.nq = pl->mas_nq;
+  kp.vals = pl->d_vals.as<double>();
+  kp.slots = pl->d_slots.as<int32_t>();
+  kp.npair = pl->mas_npair;
+  kp.pair_s = pl->d_pair_s.as<int32_t>();
+  kp.pair_t = pl->d_pair_t.as<int32_t>();
+  kp.coil_t = pl->d_coil_t.as<double>();
+  kp.rho0_t = pl->d_rho0_t.as<double>();
+  kp.alphas = pl->d_al.as<double>();
+  kp.betas = pl->d_be.as<double>();
+  kp.gammas = pl->d_ga.as<double>();
+  kp.n_or = pl->total_or;
+  kp.rot2lab = pl->d_rot2lab.as<double>();
+  kp.lab2rot = pl->d_lab2rot.as<double>();
+  kp.dt = pl->dt;
+  kp.ticks = pl->ticks;
+  kp.n_rho = pl->n_rho;
+  kp.nsteps = pl->nsteps;
+  kp.taylor_k = pl->taylor_k;
+  kp.ps_q = pl->ps_q;
+  kp.theta = pl->theta;
+  kp.work = pl->d_work.as<double>();
+  kp.work_stride = pl->work_stride;
+  kp.queue = pl->d_queue.as<unsigned int>();
+  kp.per_or = pl->d_per.as<double>();
+  kp.sq_out = pl->d_sq.as<int32_t>();
+  hipLaunchKernelGGL(spk::k_mas_tiled, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
+  HIPCHK(hipGetLastError());
+}
+
 // One pass on the context's stream: queue reset, propagation, orientation sum.
 // With stage events (eager execution) the boundaries ev1, evs[0..3], ev2 are
 // recorded; without (graph recording) nothing but the kernels is enqueued.
@@ -983,7 +1151,9 @@ void enqueue_pass(sp_plan* pl, bool events) {
   HIPCHK(hipGetLastError());
   if (events) HIPCHK(hipEventRecord(ctx->ev1, st));
   if (pl->mas) {
-    if (pl->mode == SP_MODE_FP64)
+    if (pl->mas_tiled)
+      launch_mas_tiled(pl);
+    else if (pl->mode == SP_MODE_FP64)
       launch_mas<double>(pl);
     else
       launch_mas<float>(pl);
@@ -1195,6 +1365,10 @@ int run(sp_plan* pl, sp_stats* stats) {
   for (int k = 0; k < 4; k++) s.stage_ms[k] = ms_stage[k];
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   s.eig_fallbacks = eig_failures;
+  if (pl->mas_tiled) {
+    s.mas_tiles = pl->mas_ntile;
+    s.mas_pairs = pl->mas_npair;
+  }
   pl->last = s;
   if (stats) *stats = s;
   pl->executed = true;  // sp_plan_result refuses the results of a failed pass
