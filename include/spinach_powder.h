@@ -132,7 +132,8 @@ typedef struct sp_stats {
   int32_t mas_tiles;         /* sp_mas_fid: 16 x 16 tiles the invariant subspaces are packed into
                                 (0: the dense kernel ran) */
   int32_t mas_pairs;         /* sp_mas_fid: tile pairs where both coil and rho0 have entries */
-  int32_t pad;
+  int32_t mas_detected;      /* sp_mas_fid: elements of those pairs the detection sums over (the
+                                component blocks where both have entries), padded to 16 */
 } sp_stats;
 
 /* ---- context --------------------------------------------------------- */

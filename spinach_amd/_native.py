@@ -54,7 +54,7 @@ class SpStats(C.Structure):
                 ('real_systems', C.c_int32), ('spectral_systems', C.c_int32), ('eig_max_sweeps', C.c_int32),
                 ('krylov_systems', C.c_int32), ('stage_ms', C.c_double * 4), ('step_products', C.c_int64),
                 ('plan_ms', C.c_double), ('eig_fallbacks', C.c_int32), ('mas_tiles', C.c_int32),
-                ('mas_pairs', C.c_int32), ('pad', C.c_int32)]
+                ('mas_pairs', C.c_int32), ('mas_detected', C.c_int32)]
 
     def as_dict(self):
         out = {k: getattr(self, k) for k, _ in self._fields_}

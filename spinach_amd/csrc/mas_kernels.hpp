@@ -304,9 +304,12 @@ namespace spk {
 // dense algorithm above splits into independent 16 x 16 products, and
 //     C_j = U_j' coil U_j,  rho_k = U^k rho0 U'^k
 // need only the tile pairs (s, t) where both coil and rho0 have entries (the detection
-// sum(conj(C_j) .* rho_k) sees nothing else). Same mathematics, same evaluation order of the
-// Taylor polynomial; one wave owns a tile (tiles wave, wave + 4, ...) and a 16 x 16 complex
-// product is 16 FP64 MFMAs on registers.
+// sum(conj(C_j) .* rho_k) sees nothing else); inside a pair, only the elements of the component
+// blocks where both have entries are kept (emap), so the detection product runs over those.
+// Same mathematics, same evaluation order of the Taylor polynomial; one wave owns a tile (tiles
+// wave, wave + 4, ...) and a 16 x 16 complex product is 16 FP64 MFMAs on registers. The
+// generator is assembled from its Fourier components in the rotor phase (formed once per
+// orientation), so a tick reads five dense tiles instead of the sparse pattern and slot values.
 //
 // Register layouts of a 16 x 16 matrix in a wave (the MFMA f64 16x16x4 operand layouts):
 //   C layout (accumulator):  lane l holds M[4e + l/16][l%16], e = 0..3;
@@ -333,6 +336,8 @@ struct MasTParams {
   const int32_t* pair_t;
   const double* coil_t;  // npair x 256 interleaved complex, the tile row-major
   const double* rho0_t;  // npair x 256 interleaved complex
+  const int32_t* emap;   // npair x 256: compact index of a detected element, -1 elsewhere
+  int32_t ldet;          // detected elements, padded to a multiple of 16
   const double* alphas;
   const double* betas;
   const double* gammas;
@@ -346,7 +351,8 @@ struct MasTParams {
   int32_t taylor_k;
   int32_t ps_q;
   double theta;
-  double* work;  // per workgroup: C (ticks x npair*256) and rho (n_rho x npair*256), planar complex
+  double* work;  // per workgroup: C (ticks x ldet) and rho (n_rho x ldet), planar complex, then the
+                 // generator's Fourier components G_n (ntile x 5 x 256 complex, lane order)
   int64_t work_stride;
   unsigned int* queue;
   double* per_or;
@@ -361,8 +367,6 @@ struct MasTShared {
       double scr[kThreads / 64][2 * kTilePlane];  // per-wave scratch tile
     } t;
   } u;
-  double B1r[9], B1i[9], B2r[25], B2i[25];
-  double cre[34], cim[34];
   double tc[32];
   int item;
   int smax;
@@ -481,14 +485,14 @@ __device__ __forceinline__ void tile_taylor_block(CTile& acc, int j0, int q, int
   }
 }
 
-__global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
+__global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
   typedef Mma<double>::v4 v4;
   __shared__ MasTShared sh;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int rq = lane >> 4, cq = lane & 15;
   constexpr int kWaves = kThreads / 64;
-  const int64_t L = (int64_t)p.npair * 256;
+  const int64_t L = p.ldet;
   double* Cre = p.work + (size_t)blockIdx.x * p.work_stride;  // ticks x L
   double* Cim = Cre + (size_t)p.ticks * L;
   double* Rre = Cim + (size_t)p.ticks * L;  // n_rho x L
@@ -508,35 +512,73 @@ __global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
     const int64_t o = sh.item;
     __syncthreads();
     if (o >= p.n_or) break;
-    if (tid == 0) {
-      // B = D_lab2rot * D_initial for ranks 1 and 2
-      double dr[34], di[34];
-      wigner_coeffs(p.alphas[o], p.betas[o], p.gammas[o], dr, di);
-      for (int k = 0; k < 3; k++)
-        for (int m = 0; m < 3; m++) {
-          double sr = 0, si = 0;
-          for (int j = 0; j < 3; j++) {
-            double ar = p.lab2rot[2 * (k * 3 + j)], ai = p.lab2rot[2 * (k * 3 + j) + 1];
-            double br = dr[j * 3 + m], bi = di[j * 3 + m];
-            sr += ar * br - ai * bi;
-            si += ar * bi + ai * br;
-          }
-          sh.B1r[k * 3 + m] = sr;
-          sh.B1i[k * 3 + m] = si;
+    // Setup of the orientation, one slot per thread, staged in the wave scratch tiles:
+    // D = D_initial (34 Wigner coefficients), B = D_lab2rot * D per rank.
+    // The slot coefficients are trigonometric polynomials of the rotor phase:
+    //   c_(k,m)(phi) = sum_j A(k,j) exp(-i (l - j) phi) B(j,m) = sum_n a_(k,m),n exp(-i n phi),
+    //   a_(k,m),n = A(k, l - n) B(l - n, m),  |n| <= l,
+    // so X(phi) = -i dt H(phi) = sum_{n=-2..2} exp(-i n phi) G_n with G_n = -i dt (delta_n0 I +
+    // sum_slots a_slot,n Q_slot), formed once per orientation.
+    double* acr = &sh.u.t.scr[0][0];  // 34 slots x 5 (n + 2)
+    double* aci = acr + 34 * 5;
+    double* dre = &sh.u.t.scr[1][0];  // D by slot
+    double* dim = dre + 34;
+    double* bre = &sh.u.t.scr[2][0];  // B by slot
+    double* bim = bre + 34;
+    double* Gw = Rim + (size_t)p.n_rho * L;  // ntile x 5 x (64 lanes x 4 complex)
+    if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, dre, dim);
+    if (tid == 0) sh.smax = 0;
+    for (size_t x = tid; x < (size_t)p.ntile * 5 * 512; x += kThreads) Gw[x] = 0.0;
+    __syncthreads();  // D written
+    if (tid < 34) {
+      const bool r1 = tid < 9;
+      const int sz = r1 ? 3 : 5, base = r1 ? 0 : 9, km = tid - base, k = km / sz, m = km % sz;
+      double sr = 0, si = 0;
+      for (int j = 0; j < sz; j++) {
+        const double ar = p.lab2rot[2 * (base + k * sz + j)], ai = p.lab2rot[2 * (base + k * sz + j) + 1];
+        const double br = dre[base + j * sz + m], bi = dim[base + j * sz + m];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+      bre[tid] = sr;
+      bim[tid] = si;
+    }
+    __syncthreads();  // B written
+    if (tid < 34 * 5) {
+      const int slot = tid / 5, n = tid % 5 - 2;
+      const bool r1 = slot < 9;
+      const int l = r1 ? 1 : 2, sz = r1 ? 3 : 5, km = r1 ? slot : slot - 9;
+      const int k = km / sz, m = km % sz, jn = l - n;
+      double xr = 0.0, xi = 0.0;
+      if (jn >= 0 && jn < sz) {
+        const double* A = p.rot2lab + (r1 ? 0 : 18) + 2 * (k * sz + jn);
+        const double br = bre[(r1 ? 0 : 9) + jn * sz + m], bi = bim[(r1 ? 0 : 9) + jn * sz + m];
+        xr = A[0] * br - A[1] * bi;
+        xi = A[0] * bi + A[1] * br;
+      }
+      acr[tid] = xr;
+      aci[tid] = xi;
+    }
+    __syncthreads();  // a staged; G cleared
+    {
+      const int ne = p.toff[p.ntile];
+      for (int x = tid; x < ne * 5; x += kThreads) {
+        const int e = x / 5, nn = x % 5;
+        int t = 0;
+        while (p.toff[t + 1] <= e) t++;
+        double hr = (nn == 2) ? p.vals[2 * e] : 0.0, hi = (nn == 2) ? p.vals[2 * e + 1] : 0.0;
+        for (int qq = 0; qq < p.nq; qq++) {
+          const double* qv = p.vals + 2 * (size_t)(qq + 1) * p.nnz;
+          const double cr = acr[p.slots[qq] * 5 + nn], ci = aci[p.slots[qq] * 5 + nn];
+          hr += cr * qv[2 * e] - ci * qv[2 * e + 1];
+          hi += cr * qv[2 * e + 1] + ci * qv[2 * e];
         }
-      for (int k = 0; k < 5; k++)
-        for (int m = 0; m < 5; m++) {
-          double sr = 0, si = 0;
-          for (int j = 0; j < 5; j++) {
-            double ar = p.lab2rot[2 * (9 + k * 5 + j)], ai = p.lab2rot[2 * (9 + k * 5 + j) + 1];
-            double br = dr[9 + j * 5 + m], bi = di[9 + j * 5 + m];
-            sr += ar * br - ai * bi;
-            si += ar * bi + ai * br;
-          }
-          sh.B2r[k * 5 + m] = sr;
-          sh.B2i[k * 5 + m] = si;
-        }
-      sh.smax = 0;
+        const int pos = p.epos[e];
+        const int r0 = pos >> 4, c0 = pos & 15;
+        double* g = Gw + ((size_t)t * 5 + nn) * 512 + ((r0 & 3) * 16 + c0) * 8 + 2 * (r0 >> 2);
+        g[0] = p.dt * hi;  // -i dt H
+        g[1] = -p.dt * hr;
+      }
     }
     // U_0 = identity on every tile
     for (int t = wave; t < p.ntile; t += kWaves) {
@@ -548,7 +590,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
     }
     int smax = 0;
     for (int j = 0; j < p.ticks; j++) {
-      __syncthreads();  // every tile of U_j stored; the rotor coefficients free
+      __syncthreads();  // every tile of U_j stored (and, before the first tick, G written)
       // C_j = U_j' coil U_j on the tile pairs: W = coil_st U_t, C = U_s' W
       for (int pp = wave; pp < p.npair; pp += kWaves) {
         const int s = p.pair_s[pp], t = p.pair_t[pp];
@@ -573,43 +615,41 @@ __global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
           }
         }
         const CTile C = cmm16(uar, uai, W);
-        double* cr = Cre + (size_t)j * L + (size_t)pp * 256;
-        double* ci = Cim + (size_t)j * L + (size_t)pp * 256;
+        double* cr = Cre + (size_t)j * L;
+        double* ci = Cim + (size_t)j * L;
+        const int32_t* em = p.emap + (size_t)pp * 256;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          cr[(4 * e + rq) * kTileN + cq] = C.r[e];
-          ci[(4 * e + rq) * kTileN + cq] = C.i[e];
-        }
-      }
-      if (tid == 0) {
-        const double phi = 6.283185307179586476925 * (double)j / (double)p.ticks;
-        rotor_coeffs(3, p.rot2lab, sh.B1r, sh.B1i, phi, sh.cre, sh.cim);
-        rotor_coeffs(5, p.rot2lab + 18, sh.B2r, sh.B2i, phi, sh.cre + 9, sh.cim + 9);
-      }
-      __syncthreads();  // the coefficients written; every wave has read the U tiles it needs for C_j
-      for (int t = wave; t < p.ntile; t += kWaves) {
-        // X = -i dt H on tile t, assembled in the scratch tile
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          scr[(4 * e + rq) * kTileLd + cq] = 0.0;
-          scr[kTilePlane + (4 * e + rq) * kTileLd + cq] = 0.0;
-        }
-        wave_sync_lds();
-        for (int e = p.toff[t] + lane; e < p.toff[t + 1]; e += 64) {
-          double xr = p.vals[2 * e], xi = p.vals[2 * e + 1];
-          for (int qq = 0; qq < p.nq; qq++) {
-            const double* qv = p.vals + 2 * (size_t)(qq + 1) * p.nnz;
-            const double cr = sh.cre[p.slots[qq]], ci = sh.cim[p.slots[qq]];
-            xr += cr * qv[2 * e] - ci * qv[2 * e + 1];
-            xi += cr * qv[2 * e + 1] + ci * qv[2 * e];
+          const int x = em[(4 * e + rq) * kTileN + cq];
+          if (x >= 0) {
+            cr[x] = C.r[e];
+            ci[x] = C.i[e];
           }
-          const int pos = p.epos[e];
-          const int r0 = pos >> 4, c0 = pos & 15;
-          scr[r0 * kTileLd + c0] = p.dt * xi;  // -i*dt*H
-          scr[kTilePlane + r0 * kTileLd + c0] = -p.dt * xr;
         }
-        wave_sync_lds();
-        CTile X = tile_load_c(scr, lane);
+      }
+      __syncthreads();  // every wave has read the U tiles it needs for C_j
+      const double phi = 6.283185307179586476925 * (double)j / (double)p.ticks;
+      double s1, c1, s2, c2;
+      sincos(phi, &s1, &c1);
+      sincos(2.0 * phi, &s2, &c2);
+      // exp(-i n phi), n = -2..2
+      const double er[5] = {c2, c1, 1.0, c1, c2}, ei[5] = {s2, s1, 0.0, -s1, -s2};
+      for (int t = wave; t < p.ntile; t += kWaves) {
+        // X = sum_n exp(-i n phi) G_n on tile t, in the C layout
+        CTile X{v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+        {
+          const double* g = Gw + (size_t)t * 5 * 512 + lane * 8;
+#pragma unroll
+          for (int nn = 0; nn < 5; nn++) {
+            const double2* g2 = reinterpret_cast<const double2*>(g + nn * 512);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              const double2 z = g2[e];
+              X.r[e] = fma(er[nn], z.x, fma(-ei[nn], z.y, X.r[e]));
+              X.i[e] = fma(er[nn], z.y, fma(ei[nn], z.x, X.i[e]));
+            }
+          }
+        }
         // 1-norm (largest column sum of moduli) -> scaling exponent
         double cs = 0.0;
 #pragma unroll
@@ -669,6 +709,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
       tile_load_aop(sh.u.t.U[t], lane, true, utr, uti);
       const CTile Ut{v4{utr[0], utr[1], utr[2], utr[3]}, v4{uti[0], uti[1], uti[2], uti[3]}};
       CTile rho = tile_global_c(p.rho0_t + (size_t)pp * 512, lane);
+      const int32_t* em = p.emap + (size_t)pp * 256;
       for (int k = 0; k < p.n_rho; k++) {
         if (k > 0) {
           const CTile T = cmm16(usr, usi, rho);
@@ -676,19 +717,22 @@ __global__ void __launch_bounds__(kThreads, 2) k_mas_tiled(MasTParams p) {
           tile_aop(T, scr, lane, tar, tai);
           rho = cmm16(tar, tai, Ut);
         }
-        double* rr_ = Rre + (size_t)k * L + (size_t)pp * 256;
-        double* ri_ = Rim + (size_t)k * L + (size_t)pp * 256;
+        double* rr_ = Rre + (size_t)k * L;
+        double* ri_ = Rim + (size_t)k * L;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          rr_[(4 * e + rq) * kTileN + cq] = rho.r[e];
-          ri_[(4 * e + rq) * kTileN + cq] = rho.i[e];
+          const int x = em[(4 * e + rq) * kTileN + cq];
+          if (x >= 0) {
+            rr_[x] = rho.r[e];
+            ri_[x] = rho.i[e];
+          }
         }
       }
     }
     if (lane == 0) atomicMax(&sh.smax, smax);
     __syncthreads();
     if (tid == 0 && p.sq_out) p.sq_out[o] = sh.smax;
-    // f(k*ticks + j) = sum conj(C_j) .* rho_k over the pairs' elements
+    // f(k*ticks + j) = sum conj(C_j) .* rho_k over the detected elements
     {
       const int64_t npts = p.nsteps + 1;
       double* out = p.per_or + 2 * (size_t)o * npts;

@@ -181,8 +181,8 @@ struct sp_plan {
   // tiled MAS (FP64, every invariant component of the union pattern <= 16 states): tiles, the
   // pattern's tile-local positions and per-tile ranges, the tile pairs of coil and rho0
   bool mas_tiled = false;
-  int32_t mas_ntile = 0, mas_npair = 0;
-  DevBuf d_epos, d_toff, d_pair_s, d_pair_t, d_coil_t, d_rho0_t;
+  int32_t mas_ntile = 0, mas_npair = 0, mas_ldet = 0;  // ldet: detected elements, padded to 16
+  DevBuf d_epos, d_toff, d_pair_s, d_pair_t, d_coil_t, d_rho0_t, d_emap;
 };
 
 namespace {
@@ -1011,6 +1011,34 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
           ct.assign(512, 0.0);
           rt.assign(512, 0.0);
         }
+        // Within a pair's tiles, C_j = U_j' coil U_j and rho_k = U^k rho0 U'^k are nonzero only on
+        // the component blocks (a, b) where coil (resp. rho0) has entries, since U_j is block
+        // diagonal over the components. The detection needs the elements where both can be
+        // nonzero: emap numbers them (pair, tile-local position -> compact index, -1 elsewhere).
+        const size_t ncomp = comps.size();
+        std::vector<char> ccomp(ncomp * ncomp, 0), rcomp(ncomp * ncomp, 0);
+        for (int64_t c = 0; c < N; c++)
+          for (int64_t r = 0; r < N; r++) {
+            const size_t src = 2 * (size_t)(c * N + r);
+            const size_t cp = (size_t)cid[root(r)] * ncomp + cid[root(c)];
+            if (coil[src] != 0.0 || coil[src + 1] != 0.0) ccomp[cp] = 1;
+            if (rho0[src] != 0.0 || rho0[src + 1] != 0.0) rcomp[cp] = 1;
+          }
+        std::vector<int64_t> state_at((size_t)ntile * spk::kTileN, -1);
+        for (int64_t i = 0; i < N; i++) state_at[(size_t)tile_of[i] * spk::kTileN + local[i]] = i;
+        std::vector<int32_t> emap(ps.size() * 256, -1);
+        int32_t ndet = 0;
+        for (size_t pp = 0; pp < ps.size(); pp++)
+          for (int r = 0; r < spk::kTileN; r++)
+            for (int c = 0; c < spk::kTileN; c++) {
+              const int64_t i = state_at[(size_t)ps[pp] * spk::kTileN + r];
+              const int64_t k = state_at[(size_t)pt[pp] * spk::kTileN + c];
+              if (i < 0 || k < 0) continue;
+              const size_t cp = (size_t)cid[root(i)] * ncomp + cid[root(k)];
+              if (ccomp[cp] && rcomp[cp]) emap[pp * 256 + r * spk::kTileN + c] = ndet++;
+            }
+        pl->mas_ldet = std::max<int32_t>(16, (ndet + 15) / 16 * 16);
+        upload(pl->d_emap, emap, st);
         pl->mas_tiled = true;
         pl->np_max = spk::kTileN;  // reported as the padded dimension: every product is 16 x 16
         pl->mas_ntile = ntile;
@@ -1047,21 +1075,24 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   pl->full_dim = N;
   pl->red_dim = N;
   pl->red_nnz = nnz;
-  int per_cu = 2;
+  int per_cu = pl->mas_tiled ? 3 : 2;  // k_mas_tiled: 168 VGPRs and 52.5 KB of LDS, three per CU
   if (const char* e = getenv("SPINACH_WG_PER_CU")) per_cu = std::max(1, atoi(e));
   pl->per_cu = per_cu;
   pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * per_cu, n_or));
   const size_t esz = (mode == SP_MODE_FP64) ? sizeof(double) : sizeof(float);
   const int64_t mat = 2 * (int64_t)np * np;
   if (pl->mas_tiled) {
-    // per workgroup: C (ticks x npair*256) and rho (n_rho x npair*256), planar complex FP64
-    const int64_t Lt = (int64_t)pl->mas_npair * 256;
-    pl->work_stride = 2 * Lt * ((int64_t)pl->ticks + pl->n_rho);
+    // per workgroup: C (ticks x ldet) and rho (n_rho x ldet) on the detected elements, planar
+    // complex FP64, and the generator's five Fourier components on every tile
+    const int64_t Lt = pl->mas_ldet;
+    pl->work_stride = 2 * Lt * ((int64_t)pl->ticks + pl->n_rho) + (int64_t)pl->mas_ntile * 5 * 512;
   } else {
     pl->work_stride = 10 * mat + mat * (int64_t)pl->ticks + mat * (int64_t)pl->n_rho;
   }
   pl->work_stride = (pl->work_stride + 63) / 64 * 64;
   pl->d_work.alloc((size_t)pl->grid * pl->work_stride * esz);
+  // the padding columns of the detected elements (ldet - elements) are never written: zero once
+  if (pl->mas_tiled) HIPCHK(hipMemsetAsync(pl->d_work.p, 0, (size_t)pl->grid * pl->work_stride * esz, st));
   pl->d_per.alloc((size_t)n_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)npts * 2 * sizeof(double));
   pl->d_sq.alloc((size_t)n_or * sizeof(int32_t));
@@ -1118,6 +1149,8 @@ void launch_mas_tiled(sp_plan* pl) {
   kp.pair_t = pl->d_pair_t.as<int32_t>();
   kp.coil_t = pl->d_coil_t.as<double>();
   kp.rho0_t = pl->d_rho0_t.as<double>();
+  kp.emap = pl->d_emap.as<int32_t>();
+  kp.ldet = pl->mas_ldet;
   kp.alphas = pl->d_al.as<double>();
   kp.betas = pl->d_be.as<double>();
   kp.gammas = pl->d_ga.as<double>();
@@ -1368,6 +1401,7 @@ int run(sp_plan* pl, sp_stats* stats) {
   if (pl->mas_tiled) {
     s.mas_tiles = pl->mas_ntile;
     s.mas_pairs = pl->mas_npair;
+    s.mas_detected = pl->mas_ldet;
   }
   pl->last = s;
   if (stats) *stats = s;

@@ -68,10 +68,13 @@ def test_mas_tiled_matches_dense_and_oracle(monkeypatch):
     fd, pd, sd = _native.mas_fid(*args, mode='fp64', per_orientation=True)
     ref, rper = O.mas_powder(*args, per_orientation=True)
     print(f"MAS 6-spin tiled {O.rel_err(ft, ref):.2e}, dense {O.rel_err(fd, ref):.2e}, "
-          f"tiled vs dense {O.rel_err(ft, fd):.2e}; {st['mas_tiles']} tiles, {st['mas_pairs']} pairs", flush=True)
+          f"tiled vs dense {O.rel_err(ft, fd):.2e}; {st['mas_tiles']} tiles, {st['mas_pairs']} pairs, {st['mas_detected']} detected", flush=True)
     assert st['padded_dim'] == 16 and sd['padded_dim'] == 64
     # 15 subspaces of 12, 8, 8, 6, 6, 4 x 4, 2, 2, 1 x 4 states fill four tiles exactly
     assert st['mas_tiles'] == 4 and 1 <= st['mas_pairs'] <= 16 and sd['mas_tiles'] == 0
+    # L+(13C) and Lx(13C) meet on the blocks (m_C, m_H) -> (m_C + 1, m_H): (1*4 + 4*6 + 6*4 + 4*1) *
+    # (1 + 4 + 1) = 336 elements at most (fewer where the union pattern splits a block)
+    assert 16 <= st['mas_detected'] <= 336
     assert O.rel_err(ft, ref) < 1e-10 and O.rel_err(ft, fd) < 1e-10
     for k in range(len(pt)):
         assert O.rel_err(pt[k], rper[k]) < 1e-10
