@@ -87,17 +87,14 @@ def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
     return ticks * per_tick * 8.0 * n ** 3 + 2 * (n_rho - 1) * 8.0 * n ** 3 + ticks * n_rho * 8.0 * n * n
 
 
-def mas_tiled_flops_per_orientation(ntile, npair, ndet, nsteps, ticks, taylor_k, ps_q, sexp):
-    """The same algorithm on 16 x 16 tiles: every tick, per tile an expm (q-1 + K//q products
-    + s squarings) and the U update, per tile pair C_j = U_s' coil_st U_t (2 products); per
-    rotor period two products per pair for the rho chain; detection over the ndet elements
-    where both C_j and rho_k can be nonzero."""
-    npts = nsteps + 1
-    n_rho = math.ceil(npts / ticks)
-    prod = 8.0 * 16 ** 3
-    per_tick = ntile * ((ps_q - 1) + taylor_k // ps_q + sexp + 1) + 2 * npair
-    return (ticks * per_tick * prod + 2 * (n_rho - 1) * npair * prod
-            + ticks * n_rho * 8.0 * ndet)
+def mas_tiled_flops(products, ndet, n_or, nsteps, ticks):
+    """The same algorithm on 16 x 16 tiles: the complex tile products the kernel counted (per
+    tick the expm of every tile at its own Taylor degree, the U update and C_j = U_s' coil_st
+    U_t per tile pair; per rotor period two products per pair for the rho chain; 8 * 16^3
+    FLOP each), plus the detection over the ndet elements where both C_j and rho_k can be
+    nonzero."""
+    n_rho = math.ceil((nsteps + 1) / ticks)
+    return products * 8.0 * 16 ** 3 + n_or * ticks * n_rho * 8.0 * ndet
 
 
 def spectral_stage_flops(n, npts):
@@ -336,10 +333,7 @@ def bench_mas(args):
     log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms')
     if st['mas_tiles']:
         kname = 'spk::k_mas_tiled'
-        flops = n_or * mas_tiled_flops_per_orientation(st['mas_tiles'], st['mas_pairs'], st['mas_detected'],
-                                                       w['nsteps'], w['ticks'],
-                                                       st['taylor_terms'], st['ps_block'],
-                                                       st['sum_squarings_lo'] / n_or)
+        flops = mas_tiled_flops(st['step_products'], st['mas_detected'], n_or, w['nsteps'], w['ticks'])
     else:
         kname = 'spk::k_mas'
         flops = n_or * mas_flops_per_orientation(64, w['nsteps'], w['ticks'], st['taylor_terms'], st['ps_block'],

@@ -352,11 +352,12 @@ struct MasTParams {
   int32_t ps_q;
   double theta;
   double* work;  // per workgroup: C (ticks x ldet) and rho (n_rho x ldet), planar complex, then the
-                 // generator's Fourier components G_n (ntile x 5 x 256 complex, lane order)
+                 // generator's Fourier components G_n (nnz x 5 complex, entry order as epos)
   int64_t work_stride;
   unsigned int* queue;
   double* per_or;
   int32_t* sq_out;
+  int32_t* prod_out;  // per orientation: 16 x 16 complex tile products made
 };
 
 struct MasTShared {
@@ -370,6 +371,7 @@ struct MasTShared {
   double tc[32];
   int item;
   int smax;
+  int nprod;
 };
 
 struct CTile {  // a 16 x 16 complex matrix in the C layout
@@ -504,7 +506,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
     sh.tc[tid] = 1.0 / f;
   }
   __syncthreads();
-  const int K = p.taylor_k, q = p.ps_q, rr = K / q;
+  const int K = p.taylor_k, q = p.ps_q;
 
   for (;;) {
     if (tid == 0) sh.item = (int)atomicAdd(p.queue, 1u);
@@ -525,10 +527,12 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
     double* dim = dre + 34;
     double* bre = &sh.u.t.scr[2][0];  // B by slot
     double* bim = bre + 34;
-    double* Gw = Rim + (size_t)p.n_rho * L;  // ntile x 5 x (64 lanes x 4 complex)
+    double* Gw = Rim + (size_t)p.n_rho * L;  // per pattern entry (tile order): G_n, n = -2..2
     if (tid < 34) wigner_coeff_slot(p.alphas[o], p.betas[o], p.gammas[o], tid, dre, dim);
-    if (tid == 0) sh.smax = 0;
-    for (size_t x = tid; x < (size_t)p.ntile * 5 * 512; x += kThreads) Gw[x] = 0.0;
+    if (tid == 0) {
+      sh.smax = 0;
+      sh.nprod = 0;
+    }
     __syncthreads();  // D written
     if (tid < 34) {
       const bool r1 = tid < 9;
@@ -559,13 +563,11 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
       acr[tid] = xr;
       aci[tid] = xi;
     }
-    __syncthreads();  // a staged; G cleared
+    __syncthreads();  // a staged
     {
       const int ne = p.toff[p.ntile];
       for (int x = tid; x < ne * 5; x += kThreads) {
         const int e = x / 5, nn = x % 5;
-        int t = 0;
-        while (p.toff[t + 1] <= e) t++;
         double hr = (nn == 2) ? p.vals[2 * e] : 0.0, hi = (nn == 2) ? p.vals[2 * e + 1] : 0.0;
         for (int qq = 0; qq < p.nq; qq++) {
           const double* qv = p.vals + 2 * (size_t)(qq + 1) * p.nnz;
@@ -573,11 +575,8 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
           hr += cr * qv[2 * e] - ci * qv[2 * e + 1];
           hi += cr * qv[2 * e + 1] + ci * qv[2 * e];
         }
-        const int pos = p.epos[e];
-        const int r0 = pos >> 4, c0 = pos & 15;
-        double* g = Gw + ((size_t)t * 5 + nn) * 512 + ((r0 & 3) * 16 + c0) * 8 + 2 * (r0 >> 2);
-        g[0] = p.dt * hi;  // -i dt H
-        g[1] = -p.dt * hr;
+        Gw[10 * (size_t)e + 2 * nn] = p.dt * hi;  // -i dt H
+        Gw[10 * (size_t)e + 2 * nn + 1] = -p.dt * hr;
       }
     }
     // U_0 = identity on every tile
@@ -588,7 +587,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
         sh.u.t.U[t][kTilePlane + (4 * e + rq) * kTileLd + cq] = 0.0;
       }
     }
-    int smax = 0;
+    int smax = 0, nprod = 0;  // per wave
     for (int j = 0; j < p.ticks; j++) {
       __syncthreads();  // every tile of U_j stored (and, before the first tick, G written)
       // C_j = U_j' coil U_j on the tile pairs: W = coil_st U_t, C = U_s' W
@@ -615,6 +614,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
           }
         }
         const CTile C = cmm16(uar, uai, W);
+        nprod += 2;
         double* cr = Cre + (size_t)j * L;
         double* ci = Cim + (size_t)j * L;
         const int32_t* em = p.emap + (size_t)pp * 256;
@@ -635,21 +635,29 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
       // exp(-i n phi), n = -2..2
       const double er[5] = {c2, c1, 1.0, c1, c2}, ei[5] = {s2, s1, 0.0, -s1, -s2};
       for (int t = wave; t < p.ntile; t += kWaves) {
-        // X = sum_n exp(-i n phi) G_n on tile t, in the C layout
-        CTile X{v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
-        {
-          const double* g = Gw + (size_t)t * 5 * 512 + lane * 8;
+        // X = sum_n exp(-i n phi) G_n on tile t, assembled in the scratch tile from the
+        // tile's pattern entries
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          scr[(4 * e + rq) * kTileLd + cq] = 0.0;
+          scr[kTilePlane + (4 * e + rq) * kTileLd + cq] = 0.0;
+        }
+        wave_sync_lds();
+        for (int e = p.toff[t] + lane; e < p.toff[t + 1]; e += 64) {
+          const double2* g2 = reinterpret_cast<const double2*>(Gw + 10 * (size_t)e);
+          double xr = 0.0, xi = 0.0;
 #pragma unroll
           for (int nn = 0; nn < 5; nn++) {
-            const double2* g2 = reinterpret_cast<const double2*>(g + nn * 512);
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-              const double2 z = g2[e];
-              X.r[e] = fma(er[nn], z.x, fma(-ei[nn], z.y, X.r[e]));
-              X.i[e] = fma(er[nn], z.y, fma(ei[nn], z.x, X.i[e]));
-            }
+            const double2 z = g2[nn];
+            xr = fma(er[nn], z.x, fma(-ei[nn], z.y, xr));
+            xi = fma(er[nn], z.y, fma(ei[nn], z.x, xi));
           }
+          const int pos = p.epos[e];
+          scr[(pos >> 4) * kTileLd + (pos & 15)] = xr;
+          scr[kTilePlane + (pos >> 4) * kTileLd + (pos & 15)] = xi;
         }
+        wave_sync_lds();
+        CTile X = tile_load_c(scr, lane);
         // 1-norm (largest column sum of moduli) -> scaling exponent
         double cs = 0.0;
 #pragma unroll
@@ -670,22 +678,49 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
           X.r[e] *= scl;
           X.i[e] *= scl;
         }
-        // Paterson-Stockmeyer Taylor polynomial of degree K in blocks of q (q <= 4): powers
-        // X^2 .. X^q, the top block, then K/q Horner steps with Y = X^q (as wg_expm)
-        // powers X^2 .. X^q (q <= 4), Y = X^q
+        // Taylor degree of this tile: the least kt <= K whose remainder bound
+        //   sum_{k > kt} x^k / k! <= x^(kt+1) / (kt+1)! / (1 - x / (kt+2))
+        // is at most 2^-53 for x = the scaled 1-norm (the plan's K meets it at x = theta = 1)
+        const double xs = nrm * scl;
+        int kt = K;
+        {
+          double term = xs;  // x^k / k! at k = 1
+          for (int k = 1; k < K; k++) {
+            const double next = term * xs / (k + 1);
+            if (next <= 1.1102230246251565e-16 * (1.0 - xs / (k + 2))) {
+              kt = k;
+              break;
+            }
+            term = next;
+          }
+        }
+        // Paterson-Stockmeyer in blocks of qt <= q: powers X^2 .. X^qt, the top block, then
+        // kt/qt Horner steps with Y = X^qt; qt minimises the products (qt - 1) + kt/qt
+        int qt = 1;
+        for (int c = 2; c <= q; c++)
+          if ((c - 1) + kt / c < (qt - 1) + kt / qt) qt = c;
+        const int rt = kt / qt;
         double xar[4], xai[4];
         tile_aop(X, scr, lane, xar, xai);
-        const CTile X2 = cmm16(xar, xai, X);
-        const CTile X3 = (q >= 3) ? cmm16(xar, xai, X2) : X2;
-        const CTile X4 = (q >= 4) ? cmm16(xar, xai, X3) : X3;
-        const CTile Y = (q >= 4) ? X4 : ((q == 3) ? X3 : ((q == 2) ? X2 : X));
+        const CTile X2 = (qt >= 2) ? cmm16(xar, xai, X) : X;
+        const CTile X3 = (qt >= 3) ? cmm16(xar, xai, X2) : X2;
+        const CTile X4 = (qt >= 4) ? cmm16(xar, xai, X3) : X3;
+        const CTile Y = (qt >= 4) ? X4 : ((qt == 3) ? X3 : ((qt == 2) ? X2 : X));
         CTile acc{v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
-        tile_taylor_block(acc, rr, q, K, sh.tc, X, X2, X3, rq, cq);
+        tile_taylor_block(acc, rt, qt, kt, sh.tc, X, X2, X3, rq, cq);
         double yar[4], yai[4];
-        tile_aop(Y, scr, lane, yar, yai);
-        for (int jj = rr - 1; jj >= 0; jj--) {
+        if (qt == 1) {
+#pragma unroll
+          for (int s2 = 0; s2 < 4; s2++) {
+            yar[s2] = xar[s2];
+            yai[s2] = xai[s2];
+          }
+        } else {
+          tile_aop(Y, scr, lane, yar, yai);
+        }
+        for (int jj = rt - 1; jj >= 0; jj--) {
           acc = cmm16(yar, yai, acc);
-          tile_taylor_block(acc, jj, q, K, sh.tc, X, X2, X3, rq, cq);
+          tile_taylor_block(acc, jj, qt, kt, sh.tc, X, X2, X3, rq, cq);
         }
         for (int s = 0; s < sexp; s++) {
           double par[4], pai[4];
@@ -697,6 +732,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
         tile_aop(acc, scr, lane, par, pai);
         const CTile Un = cmm16(par, pai, tile_load_c(sh.u.t.U[t], lane));
         tile_store(sh.u.t.U[t], Un, lane);
+        nprod += (qt - 1) + rt + sexp + 1;
       }
     }
     __syncthreads();  // U = U_ticks, the period propagator, on every tile
@@ -716,6 +752,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
           double tar[4], tai[4];
           tile_aop(T, scr, lane, tar, tai);
           rho = cmm16(tar, tai, Ut);
+          nprod += 2;
         }
         double* rr_ = Rre + (size_t)k * L;
         double* ri_ = Rim + (size_t)k * L;
@@ -729,9 +766,15 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
         }
       }
     }
-    if (lane == 0) atomicMax(&sh.smax, smax);
+    if (lane == 0) {
+      atomicMax(&sh.smax, smax);
+      atomicAdd(&sh.nprod, nprod);
+    }
     __syncthreads();
-    if (tid == 0 && p.sq_out) p.sq_out[o] = sh.smax;
+    if (tid == 0) {
+      if (p.sq_out) p.sq_out[o] = sh.smax;
+      if (p.prod_out) p.prod_out[o] = sh.nprod;
+    }
     // f(k*ticks + j) = sum conj(C_j) .* rho_k over the detected elements
     {
       const int64_t npts = p.nsteps + 1;

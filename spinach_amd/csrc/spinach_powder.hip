@@ -182,7 +182,7 @@ struct sp_plan {
   // pattern's tile-local positions and per-tile ranges, the tile pairs of coil and rho0
   bool mas_tiled = false;
   int32_t mas_ntile = 0, mas_npair = 0, mas_ldet = 0;  // ldet: detected elements, padded to 16
-  DevBuf d_epos, d_toff, d_pair_s, d_pair_t, d_coil_t, d_rho0_t, d_emap;
+  DevBuf d_epos, d_toff, d_pair_s, d_pair_t, d_coil_t, d_rho0_t, d_emap, d_nprod;
 };
 
 namespace {
@@ -1083,9 +1083,9 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   const int64_t mat = 2 * (int64_t)np * np;
   if (pl->mas_tiled) {
     // per workgroup: C (ticks x ldet) and rho (n_rho x ldet) on the detected elements, planar
-    // complex FP64, and the generator's five Fourier components on every tile
+    // complex FP64, and the five Fourier components of the generator per pattern entry
     const int64_t Lt = pl->mas_ldet;
-    pl->work_stride = 2 * Lt * ((int64_t)pl->ticks + pl->n_rho) + (int64_t)pl->mas_ntile * 5 * 512;
+    pl->work_stride = 2 * Lt * ((int64_t)pl->ticks + pl->n_rho) + (int64_t)nnz * 10;
   } else {
     pl->work_stride = 10 * mat + mat * (int64_t)pl->ticks + mat * (int64_t)pl->n_rho;
   }
@@ -1096,6 +1096,7 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
   pl->d_per.alloc((size_t)n_or * npts * 2 * sizeof(double));
   pl->d_fid.alloc((size_t)npts * 2 * sizeof(double));
   pl->d_sq.alloc((size_t)n_or * sizeof(int32_t));
+  if (pl->mas_tiled) pl->d_nprod.alloc((size_t)n_or * sizeof(int32_t));
   setup_sum(pl.get(), st);
   HIPCHK(hipStreamSynchronize(st));
   return pl.release();
@@ -1169,6 +1170,7 @@ void launch_mas_tiled(sp_plan* pl) {
   kp.queue = pl->d_queue.as<unsigned int>();
   kp.per_or = pl->d_per.as<double>();
   kp.sq_out = pl->d_sq.as<int32_t>();
+  kp.prod_out = pl->d_nprod.as<int32_t>();
   hipLaunchKernelGGL(spk::k_mas_tiled, dim3(pl->grid), dim3(spk::kThreads), 0, pl->ctx->stream, kp);
   HIPCHK(hipGetLastError());
 }
@@ -1399,6 +1401,11 @@ int run(sp_plan* pl, sp_stats* stats) {
   s.sum_squarings_lo = (int32_t)(sum & 0x7fffffff);
   s.eig_fallbacks = eig_failures;
   if (pl->mas_tiled) {
+    std::vector<int32_t> np_or((size_t)pl->total_or);
+    HIPCHK(hipMemcpy(np_or.data(), pl->d_nprod.p, np_or.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (int32_t v : np_or) tot += v;
+    s.step_products = tot;
     s.mas_tiles = pl->mas_ntile;
     s.mas_pairs = pl->mas_npair;
     s.mas_detected = pl->mas_ldet;
