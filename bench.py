@@ -87,14 +87,13 @@ def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
     return ticks * per_tick * 8.0 * n ** 3 + 2 * (n_rho - 1) * 8.0 * n ** 3 + ticks * n_rho * 8.0 * n * n
 
 
-def mas_tiled_flops(products, ndet, n_or, nsteps, ticks):
+def mas_tiled_flops(products, ndet, n_or, nsteps):
     """The same algorithm on 16 x 16 tiles: the complex tile products the kernel counted (per
     tick the expm of every tile at its own Taylor degree, the U update and C_j = U_s' coil_st
     U_t per tile pair; per rotor period two products per pair for the rho chain; 8 * 16^3
-    FLOP each), plus the detection over the ndet elements where both C_j and rho_k can be
-    nonzero."""
-    n_rho = math.ceil((nsteps + 1) / ticks)
-    return products * 8.0 * 16 ** 3 + n_or * ticks * n_rho * 8.0 * ndet
+    FLOP each), plus the detection of every time point over the ndet elements where both C_j
+    and rho_k can be nonzero."""
+    return products * 8.0 * 16 ** 3 + n_or * (nsteps + 1) * 8.0 * ndet
 
 
 def spectral_stage_flops(n, npts):
@@ -333,7 +332,7 @@ def bench_mas(args):
     log(f'{args.steps} steps, {1e3 * sec:.2f} ms/step, kernel {kms:.2f} ms')
     if st['mas_tiles']:
         kname = 'spk::k_mas_tiled'
-        flops = mas_tiled_flops(st['step_products'], st['mas_detected'], n_or, w['nsteps'], w['ticks'])
+        flops = mas_tiled_flops(st['step_products'], st['mas_detected'], n_or, w['nsteps'])
     else:
         kname = 'spk::k_mas'
         flops = n_or * mas_flops_per_orientation(64, w['nsteps'], w['ticks'], st['taylor_terms'], st['ps_block'],

@@ -775,20 +775,45 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
       if (p.sq_out) p.sq_out[o] = sh.smax;
       if (p.prod_out) p.prod_out[o] = sh.nprod;
     }
-    // f(k*ticks + j) = sum conj(C_j) .* rho_k over the detected elements
+    // f(k*ticks + j) = sum conj(C_j) .* rho_k over the detected elements: one product for the
+    // rotor periods detected whole, dot products for the ticks of a last, partial period
     {
       const int64_t npts = p.nsteps + 1;
       double* out = p.per_or + 2 * (size_t)o * npts;
       const int ticks = p.ticks;
-      wg_gemm_t<double, false, true, true, false>(ticks, p.n_rho, (int)L, Cre, Cim, (int)L, Rre, Rim, (int)L,
-                                                  [=](int jj, int kk, double vr, double vi) {
-                                                    int64_t tt = (int64_t)kk * ticks + jj;
-                                                    if (tt < npts) {
+      const int kfull = (int)(npts / ticks);
+      const int rem = (int)(npts - (int64_t)kfull * ticks);
+      if (kfull > 0)
+        wg_gemm_t<double, false, true, true, false>(ticks, kfull, (int)L, Cre, Cim, (int)L, Rre, Rim, (int)L,
+                                                    [=](int jj, int kk, double vr, double vi) {
+                                                      const int64_t tt = (int64_t)kk * ticks + jj;
                                                       out[2 * tt] = vr;
                                                       out[2 * tt + 1] = vi;
-                                                    }
-                                                  },
-                                                  sh.u.g);
+                                                    },
+                                                    sh.u.g);
+      if (rem > 0) {
+        const double* rr_ = Rre + (size_t)kfull * L;
+        const double* ri_ = Rim + (size_t)kfull * L;
+        for (int jj = wave; jj < rem; jj += kWaves) {
+          const double* cr = Cre + (size_t)jj * L;
+          const double* ci = Cim + (size_t)jj * L;
+          double sr = 0.0, si = 0.0;
+          for (int x = lane; x < L; x += 64) {
+            sr = fma(cr[x], rr_[x], fma(ci[x], ri_[x], sr));
+            si = fma(cr[x], ri_[x], fma(-ci[x], rr_[x], si));
+          }
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off);
+            si += __shfl_xor(si, off);
+          }
+          if (lane == 0) {
+            const int64_t tt = (int64_t)kfull * ticks + jj;
+            out[2 * tt] = sr;
+            out[2 * tt + 1] = si;
+          }
+        }
+      }
     }
   }
 }

@@ -79,3 +79,17 @@ def test_mas_tiled_matches_dense_and_oracle(monkeypatch):
     for k in range(len(pt)):
         assert O.rel_err(pt[k], rper[k]) < 1e-10
         assert O.rel_err(pt[k], pd[k]) < 1e-10
+
+
+@pytest.mark.parametrize('nsteps', [5, 15, 16, 33])
+def test_mas_tiled_record_lengths(nsteps):
+    """Records shorter than a rotor period, ending on a period boundary and one or two steps past
+    it: the whole periods go through the detection product, the rest through dot products."""
+    ss, I, q, rho0, coil, g = _small(11)
+    args = (I, q, g, rho0, coil, (1.0, 1.0, 1.0), 10000.0, 16, nsteps)
+    fid, per, st = _native.mas_fid(*args, mode='fp64', per_orientation=True)
+    ref, rper = O.mas_powder(*args, per_orientation=True)
+    assert st['mas_tiles'] == 1
+    assert O.rel_err(fid, ref) < 1e-10
+    for k in range(len(per)):
+        assert O.rel_err(per[k], rper[k]) < 1e-10
