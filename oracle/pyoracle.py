@@ -419,6 +419,31 @@ def np_cpmg(I, qlist, grid, rho0, coil, pulse_op, dt, npoints, nloops, per_orien
     return (fid, per) if per_orientation else fid
 
 
+def np_hp_echo(I, qlist, grid, rho0, coil, pulse_op, pulse_angle, echo_oper, echo_angle, echo_time, dt, nsteps,
+               per_orientation=False):
+    """experiments/hp_acquire.m:47-64 with parameters.echo_time, per orientation: rho =
+    step(pulse_op, rho0, pulse_angle) (:47); rho <- expm(-i*echo_time*H) rho (evolution 'final',
+    one step, :52); rho <- step(echo_oper, rho, echo_angle) (:55); rho <- expm(-i*echo_time*H) rho
+    (:57); then acquire(): nsteps+1 points of f = coil'*rho, rho <- expm(-i*dt*H) rho."""
+    dense = lambda m: np.asarray(m.toarray() if sps.issparse(m) else m, complex)
+    U0 = sla.expm(-1j * pulse_angle * dense(pulse_op))
+    Ue = sla.expm(-1j * echo_angle * dense(echo_oper))
+    n_or = len(grid['weights'])
+    c = np.asarray(coil, complex)
+    per = np.zeros((n_or, int(nsteps) + 1), complex)
+    r0 = U0 @ np.asarray(rho0, complex)
+    for o in range(n_or):
+        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        E = sla.expm(-1j * echo_time * h)
+        P = sla.expm(-1j * dt * h)
+        r = E @ (Ue @ (E @ r0))
+        for t in range(int(nsteps) + 1):
+            per[o, t] = np.vdot(c, r)
+            r = P @ r
+    fid = _weighted(grid, per)
+    return (fid, per) if per_orientation else fid
+
+
 def np_sp_acquire(I, qlist, grid, rho0, coil, ex, ey, frq, pwr, dur, phi, rnk, dt, nsteps, per_orientation=False,
                   plant=0):
     """experiments/sp_acquire.m:38-54 per orientation with one constant slice of

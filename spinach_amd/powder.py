@@ -28,8 +28,11 @@ def hp_acquire(spin_system, parameters, H, R, K):
     """Marker for experiments/hp_acquire.m: an ideal pulse
     rho0 <- step(pulse_op, rho0, pulse_angle), then acquire(). powder()
     applies the pulse on the host (it does not depend on the orientation) and
-    acquires on the GPU; the echo variant (parameters.echo_time) is not
-    supported."""
+    acquires on the GPU. With parameters.echo_time (and echo_oper, echo_angle)
+    every orientation's state evolves for echo_time, takes the echo pulse and
+    evolves for echo_time again before the acquisition (hp_acquire.m:51-58):
+    the evolutions run on the GPU (sp_plan_propagate), the echo pulse on the
+    host."""
     raise NotImplementedError('hp_acquire() runs inside powder() on the GPU')
 
 
@@ -63,8 +66,7 @@ def _hp_pulse(parameters):
     for f in ('pulse_op', 'pulse_angle'):
         if f not in parameters:
             raise ValueError(f'parameters.{f} subfield must be present.')
-    if 'echo_time' in parameters:
-        raise NotImplementedError('the echo variant of hp_acquire (hp_acquire.m:51-58) is not supported on this path')
+    _hp_echo_grumble(parameters)
     ang = parameters['pulse_angle']
     if not (np.isscalar(ang) and np.isreal(ang) and np.isfinite(ang)):
         raise ValueError('parameters.pulse_angle should be a real number.')
@@ -73,6 +75,23 @@ def _hp_pulse(parameters):
     if op.shape != (rho.size, rho.size):
         raise ValueError('parameters.pulse_op must be a square operator of the state dimension.')
     return spl.expm_multiply(-1j * float(ang) * op, rho)
+
+
+def _hp_echo_grumble(parameters):
+    """hp_acquire.m:112-130: the echo fields come together and are real scalars."""
+    if 'echo_time' in parameters:
+        et = parameters['echo_time']
+        if not (np.isscalar(et) and np.isreal(et) and np.isfinite(et) and et > 0):
+            raise ValueError('parameters.echo_time must be a positive real scalar.')
+        if 'echo_oper' not in parameters:
+            raise ValueError('echo pulse operator must be specified in parameters.echo_oper variable.')
+        if 'echo_angle' not in parameters:
+            raise ValueError('echo pulse angle must be specified in parameters.echo_angle variable.')
+        ea = parameters['echo_angle']
+        if not (np.isscalar(ea) and np.isreal(ea) and np.isfinite(ea)):
+            raise ValueError('parameters.echo_angle must be a real scalar.')
+    elif 'echo_oper' in parameters or 'echo_angle' in parameters:
+        raise ValueError('echo time must be specified in parameters.echo_time variable.')
 
 
 def _grumble(parameters):
@@ -159,6 +178,52 @@ def _closure(I, qlist, extra, vec):
     _, lab = cg.connected_components((u + u.T) > 0, directed=False)
     hit = np.unique(lab[np.abs(np.asarray(vec).ravel()) > 0])
     return np.nonzero(np.isin(lab, hit))[0]
+
+
+def _hp_echo_on_grid(ss, I, qlist, parameters):
+    """hp_acquire.m:51-64 at every orientation: the pulsed rho0 (already in parameters) evolves
+    for echo_time under the orientation's generator, takes step(echo_oper, ., echo_angle),
+    evolves for echo_time again, and acquire() runs from those per-orientation states. The
+    evolutions run on a plan whose reduction is seeded by every state the echo pulse and L can
+    reach (sp_plan_propagate); the acquisition on a plan with per-orientation initial states."""
+    import scipy.linalg as sla
+    import scipy.sparse as sps
+    if ss.get('rlx') is not None:
+        raise NotImplementedError('hp_acquire with an echo and relaxation is not supported on this path')
+    if parameters.get('dead_time'):
+        raise NotImplementedError('hp_acquire takes no dead time')
+    grid = parameters['grid']
+    n_or = len(grid['weights'])
+    dim = I.shape[0]
+    eop = sps.csr_matrix(parameters['echo_oper'], dtype=complex)
+    rho = np.asarray(parameters['rho0'], complex).ravel()
+    coil = np.asarray(parameters['coil'], complex).ravel()
+    if eop.shape != (dim, dim):
+        raise ValueError('parameters.echo_oper must be a square operator of the state dimension.')
+    st = _closure(I, qlist, [eop], np.abs(rho) + np.abs(coil))
+    seed = np.zeros(dim, complex)
+    seed[st] = 1.0
+    U = sla.expm(-1j * float(parameters['echo_angle']) * eop[st][:, st].toarray())
+    mode = parameters.get('mode', 'fp64')
+    tau = float(parameters['echo_time'])
+    prop = _native.plan_powder(I, qlist, grid, rho, seed, tau, 1, tols=ss['tols'], mode=mode)
+    acq = _native.plan_powder(I, qlist, grid, rho, coil, 1.0 / parameters['sweep'], int(parameters['npoints']) - 1,
+                              tols=ss['tols'], mode=mode)
+    try:
+        prop.set_initial_states(np.tile(rho, (n_or, 1)))
+        states = prop.propagate(tau, 1, dim)
+        states[:, st] = states[:, st] @ U.T
+        prop.set_initial_states(states)
+        states = prop.propagate(tau, 1, dim)
+        acq.set_initial_states(states)
+        acq.execute()
+        fid, per = acq.result(per_orientation=True)
+    finally:
+        prop.close()
+        acq.close()
+    if parameters.get('sum_up', True):
+        return fid, grid
+    return [per[k] for k in range(n_or)], grid
 
 
 def _cpmg_on_grid(ss, I, qlist, parameters):
@@ -313,6 +378,8 @@ def powder(spin_system, pulse_sequence, parameters, assumptions):
     params = dict(parameters, grid=_grid(parameters['grid']))  # powder.m:173-179
     if pulse_sequence is cpmg:
         return _cpmg_on_grid(ss, I, S.q_list(Q), params)
+    if pulse_sequence is hp_acquire and 'echo_time' in params:
+        return _hp_echo_on_grid(ss, I, S.q_list(Q), params)
     if pulse_sequence is sp_acquire:
         return _sp_acquire_on_grid(ss, I, S.q_list(Q), params)
     return _acquire_on_grid(ss, I, S.q_list(Q), params)

@@ -76,3 +76,40 @@ def test_sp_acquire_reference_known_answer():
     import scipy.linalg as sla
     ref = np.vdot(coil, sla.expm(-1j * dur * amp * (math.cos(phi) * ex + math.sin(phi) * ey).toarray()) @ rho0)
     assert np.allclose(fid, ref, rtol=1e-10, atol=1e-13)
+
+
+def test_hp_acquire_echo_refocuses_one_spin():
+    """hp_acquire with an echo on the GPU: pi/2 and pi pulses about y around two echo_time
+    evolutions refocus a one-spin chemical shift at every orientation, so the FID is that of
+    -Lx (oracle without the echo)."""
+    from test_sequences import one_spin_csa
+    ss, I, q = one_spin_csa()
+    g = grids.haar(10, np.random.default_rng(5))
+    ly = S.operator(ss, 'Ly', '1H')
+    params = {'rho0': S.state(ss, 'Lz', '1H'), 'coil': S.state(ss, 'L+', '1H'), 'sweep': 5e4, 'npoints': 41,
+              'grid': g, 'pulse_op': ly, 'pulse_angle': math.pi / 2, 'echo_time': 3.7e-5, 'echo_oper': ly,
+              'echo_angle': math.pi}
+    fid, _ = P.powder(ss, P.hp_acquire, params, 'nmr')
+    ref = O.np_powder(I, q, g, -S.state(ss, 'Lx', '1H'), params['coil'], 2e-5, 40, reduce=False)
+    e = O.rel_err(fid, ref)
+    print(f'hp_acquire echo, one spin: vs refocused oracle {e:.2e}', flush=True)
+    assert e < 1e-10
+
+
+def test_hp_acquire_echo_two_spin_vs_oracle():
+    w = workloads.static_2spin(npoints=128, n_orient=12)
+    ss = w['ss']
+    g = grids.haar(12, np.random.default_rng(8))
+    ly, lx = S.operator(ss, 'Ly', '1H'), S.operator(ss, 'Lx', '1H')
+    rho0 = S.state(ss, 'Lz', '1H')
+    params = {'rho0': rho0, 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 128, 'grid': g, 'pulse_op': ly,
+              'pulse_angle': math.pi / 2, 'echo_time': 2.9e-5, 'echo_oper': lx, 'echo_angle': 2 * math.pi / 3,
+              'sum_up': False}
+    per, _ = P.powder(ss, P.hp_acquire, params, 'nmr')
+    fid = (np.asarray(g['weights'])[:, None] * np.array(per)).sum(axis=0)
+    ref, rper = O.np_hp_echo(w['I'], w['Q'], g, rho0, w['coil'], ly, math.pi / 2, lx, 2 * math.pi / 3, 2.9e-5,
+                             w['dt'], 127, per_orientation=True)
+    e = O.rel_err(fid, ref)
+    ep = max(O.rel_err(per[k], rper[k]) for k in range(12))
+    print(f'hp_acquire echo, 2-spin: vs oracle {e:.2e} (per orientation {ep:.2e})', flush=True)
+    assert e < 1e-10 and ep < 1e-10

@@ -76,6 +76,37 @@ def test_oracle_cpmg_echo_train_closed_form():
     assert O.rel_err(O.np_cpmg(I, q, g, rho0, coil, pop, dt, npts, nl, angle=math.pi / 4), cf) > 0.1
 
 
+def test_oracle_hp_echo_refocuses_a_one_spin_csa():
+    """hp_acquire with an echo (hp_acquire.m:51-58): pi/2 pulse about y on Lz, echo_time,
+    a pi pulse about y, echo_time. For one spin the chemical shift is refocused at every
+    orientation, so the echo equals the pulses alone: acquire() from exp(-i pi Ly) Lx = -Lx.
+    A pi/2 echo pulse (planted) does not refocus."""
+    ss, I, q = one_spin_csa()
+    g = grids.haar(10, np.random.default_rng(5))
+    rho0, coil, ly = S.state(ss, 'Lz', '1H'), S.state(ss, 'L+', '1H'), S.operator(ss, 'Ly', '1H')
+    dt, nsteps, tau = 2e-5, 40, 3.7e-5
+    f = O.np_hp_echo(I, q, g, rho0, coil, ly, math.pi / 2, ly, math.pi, tau, dt, nsteps)
+    ref = O.np_powder(I, q, g, -S.state(ss, 'Lx', '1H'), coil, dt, nsteps, reduce=False)
+    assert O.rel_err(f, ref) < 1e-10
+    bad = O.np_hp_echo(I, q, g, rho0, coil, ly, math.pi / 2, ly, math.pi / 2, tau, dt, nsteps)
+    assert O.rel_err(bad, ref) > 0.1
+
+
+def test_oracle_hp_echo_without_a_pulse_is_a_time_shift():
+    """With echo_angle = 0 the echo is free evolution for 2 echo_time: the two-spin FID of Lx
+    from t0 = 2 echo_time (closed form). An evolution of the wrong sign (planted through a
+    negative echo time) gives the FID from -2 echo_time instead."""
+    w = workloads.static_2spin(npoints=64, n_orient=12)
+    ss, g = w['ss'], w['grid']
+    rho0, ly = S.state(ss, 'Lz', '1H'), S.operator(ss, 'Ly', '1H')
+    tau = 1.3e-5
+    f = O.np_hp_echo(w['I'], w['Q'], g, rho0, w['coil'], ly, math.pi / 2, ly, 0.0, tau, w['dt'], 63)
+    cf = O.closed_form_2spin(ss, g, w['dt'], 63, t0=2 * tau)
+    assert O.rel_err(f, cf) < 1e-9
+    bad = O.np_hp_echo(w['I'], w['Q'], g, rho0, w['coil'], ly, math.pi / 2, ly, 0.0, -tau, w['dt'], 63)
+    assert O.rel_err(bad, cf) > 1e-2
+
+
 def test_oracle_soft_pulse_reference_known_answer():
     """Zero drift and zero offset: the Fokker-Planck pulse folds back to
     step(amp*(cos(phi) Lx + sin(phi) Ly), rho, dur) (the reference's test)."""
@@ -132,3 +163,22 @@ def test_sequence_inputs_are_validated(seq, bad, match):
     base.update(bad)
     with pytest.raises(ValueError, match=match):
         P.powder(ss, seq, base, 'nmr')
+
+
+@pytest.mark.parametrize('bad, match', [
+    ({'echo_time': 0.0, 'echo_oper': None, 'echo_angle': 1.0}, 'echo_time'),
+    ({'echo_time': 1e-5, 'echo_angle': 1.0}, 'echo_oper'),
+    ({'echo_time': 1e-5, 'echo_oper': None}, 'echo_angle'),
+    ({'echo_oper': None, 'echo_angle': 1.0}, 'echo time'),
+    ({'echo_time': 1e-5, 'echo_oper': None, 'echo_angle': 1j}, 'echo_angle'),
+])
+def test_hp_acquire_echo_inputs_are_validated(bad, match):
+    """hp_acquire.m:112-130; checked on the host before any device work."""
+    w = workloads.static_2spin(npoints=16, n_orient=2)
+    ss = w['ss']
+    ly = S.operator(ss, 'Ly', '1H')
+    params = {'rho0': S.state(ss, 'Lz', '1H'), 'coil': w['coil'], 'grid': w['grid'], 'sweep': 1e5, 'npoints': 16,
+              'pulse_op': ly, 'pulse_angle': math.pi / 2}
+    params.update({k: (ly if v is None else v) for k, v in bad.items()})
+    with pytest.raises(ValueError, match=match):
+        P.powder(ss, P.hp_acquire, params, 'nmr')
