@@ -216,3 +216,124 @@ def test_crystal_validates_the_orientation(bad):
         params['orientation'] = bad
     with pytest.raises(ValueError, match='orientation'):
         Cx.crystal(w['ss'], P.acquire, params, 'nmr')
+
+
+# ------------------------------------------------------------------ gridfree (SLE)
+
+def sle_dense_fid(H, R, rho0, coil, dt, nsteps):
+    """FID of the stochastic Liouville equation by dense expm stepping on the states reachable
+    from rho0 (connected component of the generator's pattern): f(n) = coil' exp(-i L dt)^n
+    rho0, L = H + iR (evolution.m's definition)."""
+    import scipy.sparse as sps
+    import scipy.sparse.csgraph as cg
+    L = sps.csr_matrix(H, dtype=complex)
+    if R is not None:
+        L = L + 1j * sps.csr_matrix(R)
+    _, lab = cg.connected_components((abs(L) + abs(L).T) > 0, directed=False)
+    keep = np.isin(lab, np.unique(lab[np.abs(rho0) > 0]))
+    Ls = L[keep][:, keep].toarray()
+    P_ = sla.expm(-1j * dt * Ls)
+    x, c = np.asarray(rho0, complex)[keep], np.asarray(coil, complex)[keep]
+    out = []
+    for _ in range(int(nsteps) + 1):
+        out.append(np.vdot(c, x))
+        x = P_ @ x
+    return np.array(out)
+
+
+def sle_one_spin_case(max_rank, **kw):
+    """One 1H with a chemical shift tensor; rho0 = coil = L+ (the SLE state kron(e_000, .))."""
+    ss, I, q = _one_spin_csa_nested()
+    H, R, spc = Cx.sle_generator(I, q, max_rank, **kw)
+    e0 = np.zeros(spc)
+    e0[0] = 1.0
+    lp = S.state(ss, 'L+', '1H')
+    return ss, I, q, H, R, np.kron(e0, lp), np.kron(e0, lp)
+
+
+def _one_spin_csa_nested():
+    sys = {'magnet': 14.1, 'isotopes': ['1H']}
+    inter = {'zeeman': {'eigs': [[-20.0, -5.0, 25.0]], 'euler': [[0.3, 0.9, -0.4]]}}
+    ss = S.basis(S.create(sys, inter), {'formalism': 'sphten-liouv', 'approximation': 'none'})
+    I, Q = S.hamiltonian(S.assume(ss, 'nmr'))
+    return ss, I, Q
+
+
+def test_clebsch_gordan_matches_the_library_and_is_orthogonal():
+    from sympy.physics.wigner import clebsch_gordan as scg
+    for L1 in range(4):
+        for L2 in range(5):
+            for L in range(abs(L1 - L2), L1 + L2 + 1):
+                for M1 in range(-L1, L1 + 1):
+                    for M2 in range(-L2, L2 + 1):
+                        if abs(M1 + M2) <= L:
+                            assert abs(Cx.clebsch_gordan(L, M1 + M2, L1, M1, L2, M2)
+                                       - float(scg(L1, L2, L, M1, M2, M1 + M2))) < 1e-15
+    # sum over M1 + M2 = M of <2 M1 3 M2|L M><2 M1 3 M2|L' M> = delta(L, L')
+    for M in range(-5, 6):
+        for L in range(max(1, abs(M)), 6):
+            for Lp in range(max(1, abs(M)), 6):
+                s = sum(Cx.clebsch_gordan(L, M, 2, m1, 3, M - m1) * Cx.clebsch_gordan(Lp, M, 2, m1, 3, M - m1)
+                        for m1 in range(-2, 3))
+                assert abs(s - (L == Lp)) < 1e-14
+
+
+def test_sle_operators_represent_rotations_and_multiplication():
+    """sle_operators.m: Lx, Ly, Lz satisfy [Lx, Ly] = i Lz and L^2 = L(L+1) on rank L; the
+    multiplication operators give <e_000|D^2_ab D^2_cd|e_000> = int D^2_ab D^2_cd dOmega/8pi^2
+    = (-1)^(c'-d') delta(a', -c') delta(b', -d') / 5 (a' = 2 - a, ...), and e_000 times D^r_ab
+    is e_r,a',b' / sqrt(2r+1)."""
+    Lx, Ly, Lz, D, sb = Cx.sle_operators(4, [1, 2])
+    assert Lx.shape[0] == 5 * 9 * 11 // 3
+    assert abs(Lx @ Ly - Ly @ Lx - 1j * Lz).max() < 1e-13
+    L2 = (Lx @ Lx + Ly @ Ly + Lz @ Lz).toarray()
+    assert np.abs(L2 - np.diag(sb[:, 0] * (sb[:, 0] + 1.0))).max() < 1e-12
+    e0 = np.zeros(Lx.shape[0])
+    e0[0] = 1.0
+    for a in range(5):
+        for b in range(5):
+            v = D[2][(a, b)] @ e0
+            assert abs(v[Cx.sle_index(2, 2 - a, 2 - b)] - 1 / math.sqrt(5)) < 1e-15
+            assert abs(np.abs(v).sum() - 1 / math.sqrt(5)) < 1e-15
+            for c in range(5):
+                for d in range(5):
+                    val = e0 @ (D[2][(a, b)] @ (D[2][(c, d)] @ e0))
+                    ref = (-1.0) ** ((2 - c) - (2 - d)) / 5 if (a + c == 4 and b + d == 4) else 0.0
+                    assert abs(val - ref) < 1e-15
+    # multiplication operators commute where the truncation does not reach (ranks <= 4 - 2 - 2)
+    C = (D[2][(0, 1)] @ D[2][(3, 2)] - D[2][(3, 2)] @ D[2][(0, 1)]).toarray()
+    low = sb[:, 0] == 0
+    assert np.abs(C[np.ix_(low, low)]).max() < 1e-15
+
+
+def test_gridfree_static_converges_to_the_powder_average():
+    """Without spinning or diffusion the SLE FID is the orientation average of the static FID:
+    the error against a 20,000-point powder average falls spectrally with max_rank (short
+    times: |omega t| <= 3)."""
+    ss, I, Q = _one_spin_csa_nested()
+    lp = S.state(ss, 'L+', '1H')
+    dt, nst = 1e-6, 30
+    ref = O.np_powder(I, S.q_list(Q), grids.zcwn(20000), lp, lp, dt, nst, reduce=False)
+    errs = []
+    for mr in (2, 4, 6):
+        _, _, _, H, R, r0, c0 = sle_one_spin_case(mr)
+        f = sle_dense_fid(H, R, r0, c0, dt, nst)
+        assert abs(f[0] - np.vdot(lp, lp)) < 1e-14
+        errs.append(O.rel_err(f, ref))
+    assert errs[0] > 10 * errs[1] > 100 * errs[2] and errs[2] < 1e-4, errs
+
+
+@pytest.mark.parametrize('bad, match', [
+    ({'grid': 'zcwn_10'}, 'spherical integration'),
+    ({'max_rank': -1}, 'max_rank'),
+    ({'rate': 1000.0}, 'requires parameters.axis'),
+    ({'tau_c': -1e-9}, 'tau_c must be positive'),
+    ({'tau_c': np.ones((3, 3))}, 'symmetric positive definite'),
+])
+def test_gridfree_validates_its_inputs(bad, match):
+    ss, _, _ = _one_spin_csa_nested()
+    lp = S.state(ss, 'L+', '1H')
+    params = {'rho0': lp, 'coil': lp, 'sweep': 1e5, 'npoints': 4, 'max_rank': 2}
+    params.update(bad)
+    with pytest.raises(ValueError, match=match):
+        Cx.gridfree(ss, P.acquire, params, 'nmr')

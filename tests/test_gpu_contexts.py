@@ -128,3 +128,61 @@ def test_powder_on_a_spinach_grid_file(tmp_path, monkeypatch):
     fid, grid = P.powder(w['ss'], P.acquire, params, 'nmr')
     ref = O.refcpu_powder(w['I'], w['Q'], grid, w['rho0'], w['coil'], w['dt'], 127, variant=0)
     assert len(grid['weights']) == 14 and O.rel_err(fid, ref) < 1e-10
+
+
+def test_gridfree_reference_case():
+    """tests/kernel/test_ctx_gridfree_acquire.m: one 1H, CSA [-2 -2 4] ppm, projections +1,
+    rho0 = coil = L+, sweep 2000 Hz, 3 points, spinning at 1 kHz about [1 1 1], max_rank 2:
+    3 points, f(0) = coil' rho0 to 1e-12, finite; and the dense oracle of the same SLE
+    generator."""
+    from test_contexts import sle_dense_fid
+    ss, rho0, coil = _ref_one_spin()
+    params = {'rho0': rho0, 'coil': coil, 'sweep': 2000.0, 'npoints': 3, 'rate': 1000.0, 'axis': [1, 1, 1],
+              'max_rank': 2, 'offset': {'1H': 0.0}}
+    fid = Cx.gridfree(ss, P.acquire, params, 'nmr')
+    assert len(fid) == 3 and np.all(np.isfinite(fid))
+    assert abs(fid[0] - np.vdot(coil, rho0)) < 1e-12
+    I, Q = S.hamiltonian(S.assume(ss, 'nmr'))
+    H, R, spc = Cx.sle_generator(I, Q, 2, 1000.0, [1, 1, 1])
+    e0 = np.zeros(spc)
+    e0[0] = 1.0
+    ref = sle_dense_fid(H, R, np.kron(e0, rho0), np.kron(e0, coil), 1 / 2000.0, 2)
+    e = O.rel_err(fid, ref)
+    print(f'gridfree reference case: f(0) {fid[0]:.6f}, vs dense oracle {e:.2e}', flush=True)
+    assert e < 1e-10
+
+
+def test_gridfree_static_vs_dense_and_powder_average():
+    from test_contexts import _one_spin_csa_nested, sle_dense_fid, sle_one_spin_case
+    ss, I, Q = _one_spin_csa_nested()
+    lp = S.state(ss, 'L+', '1H')
+    dt, nst = 1e-6, 30
+    params = {'rho0': lp, 'coil': lp, 'sweep': 1 / dt, 'npoints': nst + 1, 'max_rank': 6}
+    fid = Cx.gridfree(ss, P.acquire, params, 'nmr')
+    _, _, _, H, R, r0, c0 = sle_one_spin_case(6)
+    ref = sle_dense_fid(H, R, r0, c0, dt, nst)
+    avg = O.np_powder(I, S.q_list(Q), grids.zcwn(20000), lp, lp, dt, nst, reduce=False)
+    e, ea = O.rel_err(fid, ref), O.rel_err(fid, avg)
+    print(f'gridfree static, max_rank 6: vs dense oracle {e:.2e}, vs powder average {ea:.2e}', flush=True)
+    assert e < 1e-10 and ea < 1e-4
+
+
+def test_gridfree_diffusion_and_spinning_two_spin_vs_dense():
+    """Two spins, rotational diffusion (tau_c, scalar and an anisotropic 3x3 tensor) and
+    spinning: the library's relaxation path on the SLE generator against dense expm."""
+    from test_contexts import sle_dense_fid
+    w = workloads.static_2spin(npoints=64, n_orient=4)
+    ss = w['ss']
+    I, Q = S.hamiltonian(S.assume(ss, 'nmr'))
+    for extra in ({'tau_c': 2e-8}, {'tau_c': np.diag([1e-8, 2e-8, 4e-8]), 'rate': 5000.0, 'axis': [0, 1, 1]}):
+        params = dict({'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 64, 'max_rank': 3},
+                      **extra)
+        fid = Cx.gridfree(ss, P.acquire, params, 'nmr')
+        H, R, spc = Cx.sle_generator(I, Q, 3, extra.get('rate', 0.0), extra.get('axis'), extra['tau_c'])
+        e0 = np.zeros(spc)
+        e0[0] = 1.0
+        ref = sle_dense_fid(H, R, np.kron(e0, w['rho0']), np.kron(e0, w['coil']), w['dt'], 63)
+        e = O.rel_err(fid, ref)
+        print(f'gridfree 2-spin {sorted(extra)}: vs dense oracle {e:.2e}, |f(end)|/|f(0)| '
+              f'{abs(fid[-1]) / abs(fid[0]):.3f}', flush=True)
+        assert e < 1e-10
