@@ -202,7 +202,8 @@ def test_hp_acquire_pulse_turns_lz_into_lx():
     rho = P._hp_pulse({'rho0': S.state(ss, 'Lz', '1H'), 'pulse_op': S.operator(ss, 'Ly', '1H'),
                        'pulse_angle': math.pi / 2})
     assert np.allclose(rho, S.state(ss, 'Lx', '1H'), atol=1e-12)
-    with pytest.raises(NotImplementedError):
+    # an echo needs its operator and angle (hp_acquire.m:118-124)
+    with pytest.raises(ValueError, match='echo_oper'):
         P._hp_pulse({'rho0': rho, 'pulse_op': S.operator(ss, 'Ly', '1H'), 'pulse_angle': 1.0, 'echo_time': 1e-3})
 
 
