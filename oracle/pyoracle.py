@@ -338,11 +338,15 @@ def relax_diag_at(spec, states, euler, plant=0):
 
 
 def np_powder_relax(I, qlist, grid, rho0, coil, dt, nsteps, spec, states, reduce=True, per_orientation=False,
-                    plant=0):
+                    plant=0, R0=None):
     """np_powder with L = H + iR(Omega) at every orientation (kernel/contexts/
-    powder.m:263, kernel/evolution.m: L = H + 1i*R), R from relax_diag_at:
+    powder.m:263, kernel/evolution.m: L = H + 1i*R), R from relax_diag_at
+    plus an optional orientation-independent R0 (dense; the Redfield term):
     P = expm(-i dt L), f(t) = coil' P^t rho0, weighted sum in grid order.
-    R is diagonal, so the union reduction of np_powder stays exact."""
+    The diagonal R keeps the union reduction of np_powder exact; with R0 pass
+    reduce=False."""
+    if R0 is not None and reduce:
+        raise ValueError('np_powder_relax: R0 needs reduce=False')
     st = union_states(I, qlist, coil) if reduce else np.arange(I.shape[0])
     n_or = len(grid['weights'])
     npts = int(nsteps) + 1
@@ -353,7 +357,8 @@ def np_powder_relax(I, qlist, grid, rho0, coil, dt, nsteps, spec, states, reduce
         eul = (grid['alphas'][o], grid['betas'][o], grid['gammas'][o])
         h = assemble_dense(Ir, qr, eul)
         r_diag = relax_diag_at(spec, np.asarray(states)[st], eul, plant)
-        P = sla.expm(-1j * dt * (h + 1j * np.diag(r_diag)))
+        Rm = np.diag(r_diag) if R0 is None else np.diag(r_diag) + np.asarray(R0)[np.ix_(st, st)]
+        P = sla.expm(-1j * dt * (h + 1j * Rm))
         r = np.asarray(rho0)[st].astype(complex)
         c = np.asarray(coil)[st].astype(complex)
         for t in range(npts):

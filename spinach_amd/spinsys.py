@@ -57,6 +57,8 @@ DEFAULT_TOLS = dict(
     dense_matrix=0.15,    # tolerances.m:279
     small_matrix=200,     # tolerances.m:290
     prox_cutoff=100.0,    # tolerances.m:235
+    rlx_integration=1e-4,  # tolerances.m:296-311
+    rlx_zero=1e-10,       # tolerances.m:74-90
 )
 
 
@@ -256,7 +258,7 @@ def create(sys, inter):
 
 
 # -------------------------------------------------------------- relaxation
-_RLX_THEORIES = ('t1_t2', 'damp')
+_RLX_THEORIES = ('t1_t2', 'damp', 'redfield')
 _RLX_KEEP = ('diagonal', 'kite', 'secular', 'labframe')
 
 
@@ -287,7 +289,10 @@ def _relaxation_spec(inter, nspins):
     eq = inter.get('equilibrium', 'zero')
     if eq != 'zero':
         raise NotImplementedError("only inter.equilibrium = 'zero' (relaxation to the unit state) is supported")
-    spec = dict(theories=theories, keep=keep, equilibrium=eq, r1=None, r2=None, damp=None)
+    dfs = inter.get('rlx_dfs', 'ignore')                      # create.m:1181-1185
+    if dfs not in ('keep', 'ignore'):
+        raise ValueError('invalid value of the inter.rlx_dfs parameter.')
+    spec = dict(theories=theories, keep=keep, equilibrium=eq, r1=None, r2=None, damp=None, tau_c=None, dfs=dfs)
     if 't1_t2' in theories:
         for f in ('r1_rates', 'r2_rates'):
             if f not in inter or len(inter[f]) != nspins:
@@ -298,6 +303,18 @@ def _relaxation_spec(inter, nspins):
         if 'damp_rate' not in inter:
             raise ValueError('inter.damp_rate must be specified for the damp theory.')
         spec['damp'] = _rate_spec(inter['damp_rate'], 'inter.damp_rate')
+    if 'redfield' in theories:
+        if 'tau_c' not in inter:
+            raise ValueError('Redfield theory requires rotational correlation time(s) in inter.tau_c.')
+        tcs = list(inter['tau_c'])
+        if len(tcs) != 1:
+            raise NotImplementedError('one chemical species (one entry of inter.tau_c) is supported')
+        tc = np.atleast_1d(np.asarray(tcs[0], float))
+        if tc.ndim != 1 or tc.size not in (1, 2, 3) or not np.all(np.isfinite(tc)):
+            raise ValueError('inter.tau_c entries must hold one, two or three correlation times.')
+        if not np.any(tc):
+            raise ValueError('zero rotational correlation times are not supported.')   # relaxation.m:102-107
+        spec['tau_c'] = [tc]
     return spec
 
 
@@ -340,6 +357,141 @@ def relaxation(ss, euler=None):
     return sps.diags(-rates, 0, shape=(dim, dim), format='csc')
 
 
+def _clean_up_dense(A, tol):
+    """clean_up.m:48-51 on a dense array: A <- tol*round(A/tol), real and imaginary parts
+    rounded half away from zero (MATLAB's round)."""
+    def rnd(x):
+        return np.sign(x) * np.floor(np.abs(x) + 0.5)
+    A = np.asarray(A) / tol
+    if np.iscomplexobj(A):
+        return tol * (rnd(A.real) + 1j * rnd(A.imag))
+    return tol * rnd(A)
+
+
+def corrfun(tau_c, n, k, m, p, q):
+    """kernel/utilities/corrfun.m:34-118 for one chemical species: (weights, rates) of the
+    rotational correlation function of the rank-n Wigner functions (k, m) and (p, q) (0-based
+    rows, projections n - index) as a sum of exponentials. One correlation time: isotropic
+    diffusion, D = 1/(6 tau_c), weight delta_kp delta_mq / (2n+1), rate -n(n+1)D; two (axial,
+    equatorial): rate -(n(n+1) D_eq + m'^2 (D_ax - D_eq)) with m' = n - m; three (rhombic, n = 2
+    only): five exponentials with the weights (1/5) delta_kp h(j,m) h(j,q)."""
+    tc = np.atleast_1d(np.asarray(tau_c, float))
+    if tc.size == 1:
+        D = 1.0 / (6.0 * tc[0])
+        return [float(k == p and m == q) / (2 * n + 1)], [-n * (n + 1) * D]
+    if tc.size == 2:
+        Dax, Deq = 1.0 / (6.0 * tc[0]), 1.0 / (6.0 * tc[1])
+        return ([float(k == p and m == q) / (2 * n + 1)],
+                [-(n * (n + 1) * Deq + (n - m) ** 2 * (Dax - Deq))])
+    Dxx, Dyy, Dzz = 1.0 / (6.0 * tc)
+    mean = (Dxx + Dyy + Dzz) / 3
+    if min(abs(Dxx - Dyy), abs(Dyy - Dzz), abs(Dzz - Dxx)) < 1e-6 * mean:
+        raise ValueError('the three rotational correlation times must be different.')
+    if n != 2:
+        raise ValueError('rhombic rotational diffusion only implemented for n=2.')
+    delta = math.sqrt(Dxx ** 2 + Dyy ** 2 + Dzz ** 2 - Dxx * Dyy - Dxx * Dzz - Dyy * Dzz)
+    rates = [-(4 * Dxx + Dyy + Dzz), -(Dxx + 4 * Dyy + Dzz), -(Dxx + Dyy + 4 * Dzz),
+             -(2 * Dxx + 2 * Dyy + 2 * Dzz - 2 * delta), -(2 * Dxx + 2 * Dyy + 2 * Dzz + 2 * delta)]
+    lp = math.sqrt(2 / 3) * (Dxx + Dyy - 2 * Dzz + 2 * delta) / (Dxx - Dyy)
+    lm = math.sqrt(2 / 3) * (Dxx + Dyy - 2 * Dzz - 2 * delta) / (Dxx - Dyy)
+    r2 = 1 / math.sqrt(2)
+    h = np.zeros((5, 5))
+    h[0, 1], h[0, 3], h[1, 1], h[1, 3], h[2, 4], h[2, 0] = r2, r2, -r2, r2, r2, -r2
+    h[3, 0], h[3, 2], h[3, 4] = np.array([1, lm, 1]) / math.sqrt(2 + lm ** 2)
+    h[4, 0], h[4, 2], h[4, 4] = np.array([1, lp, 1]) / math.sqrt(2 + lp ** 2)
+    return [float(k == p) * h[j, m] * h[j, q] / 5 for j in range(5)], rates
+
+
+def _expmint(A, B, C, T):
+    """kernel/utilities/expmint.m:41-63: int_0^T exp(-iAt) B exp(iCt) dt by the block
+    exponential of aux = [[-A, iB], [0, -C]]: with E = exp(-i aux T), the integral is
+    E11' * E12 (Van Loan), cleaned at prop_chop."""
+    d = A.shape[0]
+    aux = np.zeros((2 * d, 2 * d), complex)
+    aux[:d, :d] = -A
+    aux[:d, d:] = 1j * B
+    aux[d:, d:] = -C
+    import scipy.linalg as sla
+    E = sla.expm(-1j * T * aux)
+    return E[:d, :d].conj().T @ E[:d, d:]
+
+
+def redfield_superoperator(ss):
+    """The Redfield term of kernel/relaxation.m:96-137 (kernel/includes/redfield_integral_serial.m)
+    for one chemical species, before the dynamic-shift and retention policies: over the
+    lab-frame Hamiltonian's rotational components Q{n}{k,m} and each exponential (w, rate) of
+    corrfun, R -= w * Q{n}{k,m} * int_0^T exp(-iBt) Q{n}{p,q}' exp(i(B - i rate)t) dt with
+    B = clean_up(L0, rlx_integration/T), the on-shell kernel at zero shift, T =
+    -1.5 log(1/rlx_integration)/rate, the block of states outside the species zeroed; every term
+    cleaned at 1e-2*rlx_zero. Dense FP64 (the integrals by scipy's expm)."""
+    spec = ss['rlx']
+    tols = ss['tols']
+    L0, Q = hamiltonian(assume(ss, 'labframe'))
+    L0 = L0.toarray()
+    dim = L0.shape[0]
+    outside = ~(ss['basis'].sum(axis=1) > 0)           # corrfun.m:52-54, one species
+    blk = np.ix_(outside, outside)
+    R = np.zeros((dim, dim), complex)
+    tau = spec['tau_c'][0]
+    for n in (1, 2):
+        sz = 2 * n + 1
+        comps = Q[n - 1]
+        for k in range(sz):
+            for m in range(sz):
+                A = comps[k][m]
+                if A is None or abs(A).sum() == 0:
+                    continue
+                A = A.toarray()
+                for p in range(sz):
+                    for q in range(sz):
+                        Cq = comps[p][q]
+                        if Cq is None or abs(Cq).sum() == 0:
+                            continue
+                        weights, rates = corrfun(tau, n, k, m, p, q)
+                        for w, rate in zip(weights, rates):
+                            if w == 0:
+                                continue
+                            T = -1.5 * (1.0 / rate) * math.log(1.0 / tols['rlx_integration'])
+                            B = _clean_up_dense(L0, tols['rlx_integration'] / abs(T))
+                            C = Cq.toarray().conj().T
+                            Dm = B - 1j * rate * np.eye(dim)
+                            Ab = A.copy()
+                            for X in (Ab, B, C, Dm):
+                                X[blk] = 0
+                            Ri = -w * Ab @ _expmint(B, C, Dm, T)
+                            R += _clean_up_dense(Ri, 1e-2 * tols['rlx_zero'])
+    return R
+
+
+def _redfield_R0(ss):
+    """The orientation-independent part the library takes as R0: the Redfield term with the
+    dynamic-shift policy (relaxation.m:533-552), the retention policy (relaxation.m:556-634) and
+    the validity check against the correlation times (relaxation.m:126-134)."""
+    spec = ss['rlx']
+    R = redfield_superoperator(ss)
+    rmax = np.abs(np.diag(R)).max()
+    for tc in spec['tau_c']:
+        if rmax > 0 and (1.0 / rmax) < np.max(tc):
+            raise ValueError('T1,2>>tau_c validity condition violation in Redfield theory')
+    if spec['dfs'] == 'ignore':
+        R = R.real.astype(complex)
+    states = ss['basis']
+    M = np.where(states == 1, 1, np.where(states == 3, -1, 0))   # projection of each spin's T(1,m)
+    keep = spec['keep']
+    if keep == 'diagonal':
+        R = np.diag(np.diag(R))
+        unit = np.all(states == 0, axis=1)
+        R[unit, unit] = 0.0
+    elif keep == 'kite':
+        lon = np.all(M == 0, axis=1)
+        mask = np.outer(lon, lon) | np.eye(len(R), dtype=bool)
+        R = np.where(mask, R, 0.0)
+    elif keep == 'secular':
+        f = M @ np.asarray(ss['basefrqs'], float)
+        R = np.where(f[:, None] == f[None, :], R, 0.0)
+    return sps.csr_matrix(R)
+
+
 def relaxation_terms(ss):
     """The library's form of relaxation(ss, euler) for every orientation
     (sp_relax): R(Omega) = diag(g0 + sum_p q_p(Omega) g_p) with the monomials
@@ -371,14 +523,22 @@ def relaxation_terms(ss):
             add((states[:, n] == 1) | (states[:, n] == 3), spec['r2'][n])
     if spec['damp'] is not None:
         add(~np.all(states == 0, axis=1), spec['damp'])
-    return {'R0': None, 'diag': g}
+    R0 = _redfield_R0(ss) if spec['tau_c'] is not None else None
+    return {'R0': R0, 'diag': g}
 
 
 def assume(ss, assumptions):
-    """Interaction strengths under the 'nmr' assumption set
-    (kernel/assume.m:89-137). Returns a modified copy."""
+    """Interaction strengths under the 'nmr' assumption set (kernel/assume.m:89-137), or
+    'labframe' (assume.m:55-66: full Zeeman interactions including the carrier, strong
+    couplings; the Hamiltonian Redfield theory integrates over). Returns a modified copy."""
+    if assumptions == 'labframe':
+        ss = dict(ss)
+        n = ss['nspins']
+        ss['zeeman_strength'] = ['full'] * n
+        ss['coupling_strength'] = [['strong'] * n for _ in range(n)]
+        return ss
     if assumptions != 'nmr':
-        raise NotImplementedError("only the 'nmr' assumption set is supported on this path")
+        raise NotImplementedError("only the 'nmr' and 'labframe' assumption sets are supported on this path")
     ss = dict(ss)
     n = ss['nspins']
     ss['zeeman_strength'] = ['secular'] * n

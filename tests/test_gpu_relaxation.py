@@ -80,6 +80,29 @@ def test_powder_host_mirror_applies_relaxation():
     assert e < 5e-9
 
 
+@pytest.mark.parametrize('keep', ['secular', 'kite'])
+def test_redfield_powder_vs_oracle(keep):
+    """Redfield theory (isotropic tumbling, tau_c = 0.5 ns) on the two-spin system through
+    powder(): the host builds the orientation-independent superoperator, the library takes it
+    as R0 (sp_plan_powder_relax); against the oracle with the same R0 per orientation. The
+    rates themselves are pinned on the CPU (CSA and Solomon dipolar rates)."""
+    relax = {'relaxation': ['redfield'], 'tau_c': [5e-10], 'rlx_keep': keep, 'equilibrium': 'zero'}
+    w = workloads.static_2spin(npoints=256, n_orient=16, relax=relax)
+    params = {'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 256, 'grid': w['grid'],
+              'sum_up': False}
+    per, grid = P.powder(w['ss'], P.acquire, params, 'nmr')
+    fid = (np.asarray(grid['weights'])[:, None] * np.array(per)).sum(axis=0)
+    R0 = S.relaxation_terms(S.assume(w['ss'], 'nmr'))['R0'].toarray()
+    ref, rper = O.np_powder_relax(w['I'], w['Q'], grid, w['rho0'], w['coil'], w['dt'], 255, {}, w['ss']['basis'],
+                                  reduce=False, per_orientation=True, R0=R0)
+    free = O.np_powder(w['I'], w['Q'], grid, w['rho0'], w['coil'], w['dt'], 255)
+    e = O.rel_err(fid, ref)
+    ep = max(O.rel_err(per[k], rper[k]) for k in range(len(per)))
+    print(f'Redfield ({keep}) 2-spin powder: vs oracle {e:.2e} (per orientation {ep:.2e}); relaxation changes '
+          f'the FID by {O.rel_err(ref, free):.2f}', flush=True)
+    assert e < 1e-10 and ep < 1e-10 and O.rel_err(ref, free) > 1e-3
+
+
 def test_relaxation_refuses_the_paths_that_do_not_take_it():
     w = workloads.static_2spin(npoints=64, n_orient=4, relax=RELAX2)
     rl = S.relaxation_terms(w['ss'])
