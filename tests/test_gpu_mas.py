@@ -93,3 +93,42 @@ def test_mas_tiled_record_lengths(nsteps):
     assert O.rel_err(fid, ref) < 1e-10
     for k in range(len(per)):
         assert O.rel_err(per[k], rper[k]) < 1e-10
+
+
+def _mas_system(iso, seed):
+    rng = np.random.default_rng(seed)
+    inter = {'zeeman': {'eigs': [list(rng.uniform(-60, 60, 3)) for _ in iso],
+                        'euler': [list(rng.uniform(0, 6, 3)) for _ in iso]},
+             'coordinates': [np.array(x) for x in workloads._coords(rng, len(iso), box=3.5, min_sep=1.0)]}
+    ss = S.basis(S.create({'magnet': 14.1, 'isotopes': iso}, inter), {'formalism': 'zeeman-hilb'})
+    I, Q = S.hamiltonian(S.assume(ss, 'nmr'))
+    return ss, I, S.q_list(Q)
+
+
+def test_mas_tiled_eight_tiles_two_per_wave():
+    """4 13C, 2 1H and one 15N (Hilbert dimension 128): 30 invariant subspaces of at most 12
+    states fill eight tiles, so every wave propagates two tiles and several pairs."""
+    ss, I, q = _mas_system(['13C'] * 4 + ['1H'] * 2 + ['15N'], 21)
+    g = grids.haar(2, np.random.default_rng(22))
+    args = (I, q, g, S.state(ss, 'Lx', '13C'), S.state(ss, 'L+', '13C'), (1.0, 1.0, 1.0), 12500.0, 32, 70)
+    fid, per, st = _native.mas_fid(*args, mode='fp64', per_orientation=True)
+    ref, rper = O.mas_powder(*args, per_orientation=True)
+    e = O.rel_err(fid, ref)
+    print(f"MAS 7-spin (dim 128): {st['mas_tiles']} tiles, {st['mas_pairs']} pairs, {st['mas_detected']} detected; "
+          f"vs oracle {e:.2e}", flush=True)
+    assert st['mas_tiles'] == 8 and e < 1e-10
+    for k in range(len(per)):
+        assert O.rel_err(per[k], rper[k]) < 1e-10
+
+
+def test_mas_component_above_a_tile_takes_the_dense_kernel():
+    """Six protons: the m = 0 subspace holds 20 states, more than a tile, so the dense kernel
+    runs (mas_tiles = 0), still against the oracle."""
+    ss, I, q = _mas_system(['1H'] * 6, 23)
+    g = grids.haar(2, np.random.default_rng(24))
+    args = (I, q, g, S.state(ss, 'Lx', '1H'), S.state(ss, 'L+', '1H'), (1.0, 1.0, 1.0), 12500.0, 32, 40)
+    fid, _, st = _native.mas_fid(*args, mode='fp64')
+    ref = O.mas_powder(*args)
+    e = O.rel_err(fid, ref)
+    print(f"MAS 6 protons: mas_tiles {st['mas_tiles']}, padded {st['padded_dim']}, vs oracle {e:.2e}", flush=True)
+    assert st['mas_tiles'] == 0 and st['padded_dim'] == 64 and e < 1e-10
