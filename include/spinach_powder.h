@@ -239,10 +239,12 @@ typedef struct sp_relax {
 
 /* sp_plan_powder with a relaxation superoperator (rlx may be NULL: then the
  * same as sp_plan_powder). The generator is not Hermitian, so such systems take
- * the propagator path in complex arithmetic (reduced dimension <= 1024,
- * SP_EINVAL with SP_MODE_KRYLOV); R0's pattern joins the union pattern of the
- * coil-connected reduction. Replaces: the relaxation part of the parfor body
- * of powder.m:262-269. */
+ * the propagator path in complex arithmetic (reduced dimension <= 1024), or the
+ * time-stepping path (SP_MODE_KRYLOV, krylov_tol, or larger reduced dimensions);
+ * a dead time (sp_plan_set_dead_time) steps under the same L = H + iR. R0's
+ * pattern, and the diagonal when rlx->diag is given, join the union pattern of
+ * the coil-connected reduction. Replaces: the relaxation part of the parfor
+ * body of powder.m:262-269. */
 sp_plan* sp_plan_powder_relax(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const double* alphas,
                               const double* betas, const double* gammas, const double* weights,
                               int64_t n_or, const double* rho0, const double* coil, double dt,
@@ -287,7 +289,8 @@ int sp_plan_set_initial_states(sp_plan* plan, const double* rho_or);
  * as its coil a vector whose support seeds every state the sequence can
  * reach. Replaces: the per-orientation evolution(...,'final') and step() of
  * experiments/cpmg.m:41-55 and kernel/pulses/shaped_pulse_af.m (its 'expv'
- * branch). Not with a relaxation superoperator or a dead time. */
+ * branch); with a relaxation superoperator under L = H + iR. Not with a dead
+ * time. */
 int sp_plan_propagate(sp_plan* plan, double dt, int64_t nsteps, double* rho_out);
 
 /* ---- launch configuration and autotuning ------------------------------ */

@@ -178,9 +178,19 @@ inline SysPlan plan_system(const sp_csc* I, const sp_csc* Q, const double* al, c
   Triplets r0;
   const bool have_diag = rlx && rlx->diag;
   if (rlx && rlx->R0) r0 = read_csc(rlx->R0, dim, "R0");
-  if (have_diag)
+  if (have_diag) {
     for (int64_t i = 0; i < 7 * dim; i++)
       if (!std::isfinite(rlx->diag[i])) throw SpError(SP_EINVAL, "non-finite relaxation diagonal");
+    // the diagonal joins the pattern (value 0 in R0's plane), so that the stepping path's sparse
+    // layout has a place for the per-orientation rates; the reduction is unchanged (it follows
+    // off-diagonal entries only)
+    for (int64_t i = 0; i < dim; i++) {
+      r0.row.push_back(i);
+      r0.col.push_back(i);
+      r0.re.push_back(0.0);
+      r0.im.push_back(0.0);
+    }
+  }
   mats.push_back(r0);
 
   // union adjacency (symmetrised) and the coil-connected component

@@ -338,9 +338,6 @@ sp_plan* build_plan(sp_ctx* ctx, std::vector<SysPlan>& sps, double dt, int64_t n
   int np_prop = 16, np_spec = 16;
   for (size_t s = 0; s < sps.size(); s++) {
     krylov[s] = (force_krylov || (double)sps[s].n >= tl.krylov_tol || sps[s].np > kDenseMaxDim) ? 1 : 0;
-    if (sps[s].relax && krylov[s])
-      throw SpError(SP_EINVAL, "a relaxation superoperator runs on the propagator path only (reduced dimension <= " +
-                                   std::to_string(kDenseMaxDim) + ", below krylov_tol, no SP_MODE_KRYLOV)");
     spectral[s] =
         (!krylov[s] && !force_prop && mode == SP_MODE_FP64 && sps[s].real && sps[s].np <= spk::kTriGNP) ? 1 : 0;
     if (spectral[s])
@@ -709,6 +706,9 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.rho_out = rho_out;
   sp.nsub_out = final_state ? pl->d_nsub_fs.as<int32_t>() : pl->d_nsub.as<int32_t>();
   sp.terms_out = final_state ? nullptr : pl->d_terms.as<int32_t>();
+  sp.prow = pl->d_row.as<int32_t>();
+  sp.rvals = pl->d_rvals.as<double>();
+  sp.rdiag = pl->d_rdiag.as<double>();
   const int g = (int)std::min<int64_t>(pl->grid_step, n);
   const int r = real ? 1 : 0;
   if (real)
@@ -1956,8 +1956,6 @@ int sp_plan_set_dead_time(sp_plan* plan, double dead_time) {
   if (!plan) return SP_EINVAL;
   try {
     if (plan->mas) throw SpError(SP_EINVAL, "a dead time is not available on MAS plans");
-    if (plan->relax_systems > 0 && dead_time > 0)
-      throw SpError(SP_EINVAL, "a dead time is not available with a relaxation superoperator");
     if (!std::isfinite(dead_time) || dead_time < 0) throw SpError(SP_EINVAL, "dead_time must be finite and >= 0");
     HIPCHK(hipSetDevice(plan->ctx->device));
     if (dead_time > 0 && plan->init_states)
@@ -2030,8 +2028,6 @@ int sp_plan_propagate(sp_plan* plan, double dt, int64_t nsteps, double* rho_out)
     if (plan->mas) throw SpError(SP_EINVAL, "propagation of states is not available on MAS plans");
     if (plan->n_systems != 1 || plan->red_states.empty())
       throw SpError(SP_EINVAL, "propagation of states needs a single-system plan");
-    if (plan->relax_systems > 0)
-      throw SpError(SP_EINVAL, "propagation of states is not available with a relaxation superoperator");
     if (plan->dead_time > 0) throw SpError(SP_EINVAL, "propagation of states does not apply the dead time");
     if (!std::isfinite(dt) || dt <= 0) throw SpError(SP_EINVAL, "dt must be positive and finite");
     if (nsteps < 1 || nsteps > ((int64_t)1 << 30)) throw SpError(SP_EINVAL, "nsteps must be between 1 and 2^30");

@@ -17,8 +17,9 @@
 //      when the generator is real at every orientation, complex otherwise;
 //      staged in LDS when the plan's largest slice layout fits, else in the
 //      workgroup's slot of a global workspace,
-//   3. the 1-norm of L in FP64 (cheap_norm.m:50; L is Hermitian, so the
-//      column sums equal the row sums) and the substep count
+//   3. the largest row sum of |L| in FP64 (cheap_norm.m:45-50; equal to the
+//      1-norm for a Hermitian L, and with relaxation a bound of the same use:
+//      each substep's product has norm <= 2 in it) and the substep count
 //      nsub = ceil(|L|_1 |dt| / 2) (step.m:222-227),
 //   4. per time step: scaling = max|rho| (a workgroup max reduction),
 //      rho /= scaling; nsub substeps of next = -(i/k)(dt/nsub) L next,
@@ -90,6 +91,9 @@ struct StepParams {
   double* rho_out;        // final states (final_state == 1)
   int32_t* nsub_out;      // per orientation: substeps per time step (diagnostics)
   int32_t* terms_out;     // per orientation: Taylor terms (matrix-vector products) over all steps
+  const int32_t* prow;    // row of every union-pattern entry (reduced state)
+  const double* rvals;    // relaxation R0 on the union pattern (SysDesc::rlx_off), interleaved complex
+  const double* rdiag;    // relaxation diagonal planes g0..g6 (SysDesc::rdiag_off)
 };
 
 // Sum and max over the workgroup: DPP within each wave, then four partials in LDS.
@@ -170,6 +174,8 @@ __global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // tw
       const int32_t* slots = p.slots + sd.slot_off;
       const int32_t* ptp = p.ptpos + sd.pat_off;
       const int nnz = sd.nnz;
+      double rq[6] = {0, 0, 0, 0, 0, 0};  // q_k(Omega) of the relaxation rate tensors
+      if (!REAL && sd.rdiag_off >= 0) relax_monomials(p.betas[o], p.gammas[o], rq);
       for (int e = tid; e < nnz; e += kStepThreads) {
         const int et = ptp[e];
         double xr = v0[2 * e], xi = v0[2 * e + 1];
@@ -186,8 +192,23 @@ __global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // tw
         if (REAL) {
           val[pos] = 0.5 * (xr + tr);
         } else {
-          val[2 * pos] = 0.5 * (xr + tr);
-          val[2 * pos + 1] = 0.5 * (xi - ti);
+          // L = H + iR (relaxation.m, powder.m:263): R0 as given (not symmetrised) and, on the
+          // diagonal, the orientation's rates g0 + sum_k q_k g_k
+          double lr = 0.5 * (xr + tr), li = 0.5 * (xi - ti);
+          if (sd.rlx_off >= 0) {
+            lr -= p.rvals[2 * (sd.rlx_off + e) + 1];
+            li += p.rvals[2 * (sd.rlx_off + e)];
+          }
+          if (sd.rdiag_off >= 0 && et == e) {
+            const int i = p.prow[sd.pat_off + e];
+            const double* g = p.rdiag + sd.rdiag_off;
+            double r = g[i];
+#pragma unroll
+            for (int k = 0; k < 6; k++) r = fma(rq[k], g[(size_t)(k + 1) * sd.n + i], r);
+            li += r;
+          }
+          val[2 * pos] = lr;
+          val[2 * pos + 1] = li;
         }
       }
     }

@@ -27,7 +27,7 @@
 //                       ort = [0 0 1]*euler2dcm(alpha,beta,gamma)
 //                       (rlx_t1_t2.m rate tensors, relaxation.m 'damp');
 //                       with R0 or rlx_diag the generator is L = H + 1i*R
-//                       (evolution.m), and dead_time and krylov are refused
+//                       (evolution.m) on every path, the dead time included
 //   fid    npoints x 1 complex, the weighted sum (sum_up true)
 //   per    npoints x n_or complex, unweighted per-orientation FIDs (requested
 //          as a second output, or the first output when sum_up is false;

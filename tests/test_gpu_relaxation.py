@@ -103,15 +103,49 @@ def test_redfield_powder_vs_oracle(keep):
     assert e < 1e-10 and ep < 1e-10 and O.rel_err(ref, free) > 1e-3
 
 
-def test_relaxation_refuses_the_paths_that_do_not_take_it():
-    w = workloads.static_2spin(npoints=64, n_orient=4, relax=RELAX2)
-    rl = S.relaxation_terms(w['ss'])
-    with pytest.raises(_native.SpinachPowderError, match='propagator path'):
-        _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'],
-                            mode='fp64+krylov', relax=rl)
-    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], w['nsteps'], relax=rl)
+def test_stepping_path_with_relaxation_vs_oracle():
+    """The time-stepping path (k_step, SP_MODE_KRYLOV) with L = H + iR(Omega): the rate tensors
+    and damping of the two-spin system on the diagonal, against the oracle."""
+    w = workloads.static_2spin(npoints=256, n_orient=16, relax=RELAX2)
+    w['nsteps'] = 255
+    (fid, per), st = _run(w, w['grid'], mode='fp64+krylov', per=True)
+    ref, rper = O.np_powder_relax(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], 255, w['ss']['rlx'],
+                                  w['ss']['basis'], per_orientation=True)
+    e = O.rel_err(fid, ref)
+    ep = max(O.rel_err(per[k], rper[k]) for k in range(16))
+    print(f"stepping path with relaxation: {st['krylov_systems']} stepping system(s), vs oracle {e:.2e} "
+          f"(per orientation {ep:.2e})", flush=True)
+    assert st['krylov_systems'] == 1 and e < 1e-10 and ep < 1e-10
+
+
+def test_redfield_on_the_stepping_path_vs_oracle():
+    relax = {'relaxation': ['redfield'], 'tau_c': [5e-10], 'rlx_keep': 'kite', 'equilibrium': 'zero'}
+    w = workloads.static_2spin(npoints=128, n_orient=8, relax=relax)
+    w['nsteps'] = 127
+    (fid, _), st = _run(w, w['grid'], mode='fp64+krylov', per=True)
+    R0 = S.relaxation_terms(S.assume(w['ss'], 'nmr'))['R0'].toarray()
+    ref = O.np_powder_relax(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], 127, {}, w['ss']['basis'],
+                            reduce=False, R0=R0)
+    e = O.rel_err(fid, ref)
+    print(f'Redfield on the stepping path: vs oracle {e:.2e}', flush=True)
+    assert st['krylov_systems'] == 1 and e < 1e-10
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'fp64+krylov'])
+def test_dead_time_with_relaxation_closed_form(mode):
+    """acquire.m:65-67 with relaxation: the dead-time step (k_step) under L = H + iR, then the
+    acquisition on the propagator or the stepping path; the two-spin closed form from t0."""
+    w = workloads.static_2spin(npoints=128, n_orient=16, relax=RELAX2)
+    t0 = 3.3e-5
+    plan = _native.plan_powder(w['I'], w['Q'], w['grid'], w['rho0'], w['coil'], w['dt'], 127, mode=mode,
+                               relax=S.relaxation_terms(w['ss']))
     try:
-        with pytest.raises(_native.SpinachPowderError, match='dead time'):
-            plan.set_dead_time(1e-5)
+        plan.set_dead_time(t0)
+        plan.execute()
+        fid = plan.result()
     finally:
         plan.close()
+    cf = O.closed_form_2spin(w['ss'], w['grid'], w['dt'], 127, t0=t0, relax=w['ss']['rlx'])
+    e = O.rel_err(fid, cf)
+    print(f'dead time {t0:g} s with relaxation ({mode}): vs closed form {e:.2e}', flush=True)
+    assert e < 5e-9
