@@ -393,12 +393,24 @@ def _weighted(grid, per):
     return fid
 
 
-def np_cpmg(I, qlist, grid, rho0, coil, pulse_op, dt, npoints, nloops, per_orientation=False, angle=math.pi / 2):
+def _gen(I, qlist, eul, rlx=None, states=None, R0=None):
+    """H(Omega), plus i*R(Omega) when relaxation is given (relax_diag_at's spec and the
+    basis states for the diagonal, R0 dense for the orientation-independent part)."""
+    h = assemble_dense(I, qlist, eul)
+    if rlx:
+        h = h + 1j * np.diag(relax_diag_at(rlx, states, eul))
+    if R0 is not None:
+        h = h + 1j * np.asarray(R0)
+    return h
+
+
+def np_cpmg(I, qlist, grid, rho0, coil, pulse_op, dt, npoints, nloops, per_orientation=False, angle=math.pi / 2,
+            rlx=None, states=None, R0=None):
     """experiments/cpmg.m:36-57 per orientation: rho = step(pulse_op, rho0, pi/2)
     (`angle`: a planted mistake changes it); npoints detected points of
-    rho <- P rho, P = expm(-i dt H(Omega)); then nloops times the last state
-    takes step(pulse_op, ., pi) and 2*npoints points follow (the first being the
-    pulsed state)."""
+    rho <- P rho, P = expm(-i dt L(Omega)), L = H (+ iR with relaxation); then nloops times
+    the last state takes step(pulse_op, ., pi) and 2*npoints points follow (the first being
+    the pulsed state)."""
     pop = np.asarray(pulse_op.toarray() if sps.issparse(pulse_op) else pulse_op, complex)
     U2 = sla.expm(-1j * angle * pop)
     U1 = sla.expm(-1j * math.pi * pop)
@@ -407,7 +419,7 @@ def np_cpmg(I, qlist, grid, rho0, coil, pulse_op, dt, npoints, nloops, per_orien
     total = npoints + nloops * 2 * npoints
     per = np.zeros((n_or, total), complex)
     for o in range(n_or):
-        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        h = _gen(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]), rlx, states, R0)
         P = sla.expm(-1j * dt * h)
         r = U2 @ np.asarray(rho0, complex)
         col = 0
@@ -425,7 +437,7 @@ def np_cpmg(I, qlist, grid, rho0, coil, pulse_op, dt, npoints, nloops, per_orien
 
 
 def np_hp_echo(I, qlist, grid, rho0, coil, pulse_op, pulse_angle, echo_oper, echo_angle, echo_time, dt, nsteps,
-               per_orientation=False):
+               per_orientation=False, rlx=None, states=None, R0=None):
     """experiments/hp_acquire.m:47-64 with parameters.echo_time, per orientation: rho =
     step(pulse_op, rho0, pulse_angle) (:47); rho <- expm(-i*echo_time*H) rho (evolution 'final',
     one step, :52); rho <- step(echo_oper, rho, echo_angle) (:55); rho <- expm(-i*echo_time*H) rho
@@ -438,7 +450,7 @@ def np_hp_echo(I, qlist, grid, rho0, coil, pulse_op, pulse_angle, echo_oper, ech
     per = np.zeros((n_or, int(nsteps) + 1), complex)
     r0 = U0 @ np.asarray(rho0, complex)
     for o in range(n_or):
-        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        h = _gen(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]), rlx, states, R0)
         E = sla.expm(-1j * echo_time * h)
         P = sla.expm(-1j * dt * h)
         r = E @ (Ue @ (E @ r0))

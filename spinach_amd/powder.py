@@ -188,8 +188,6 @@ def _hp_echo_on_grid(ss, I, qlist, parameters):
     reach (sp_plan_propagate); the acquisition on a plan with per-orientation initial states."""
     import scipy.linalg as sla
     import scipy.sparse as sps
-    if ss.get('rlx') is not None:
-        raise NotImplementedError('hp_acquire with an echo and relaxation is not supported on this path')
     if parameters.get('dead_time'):
         raise NotImplementedError('hp_acquire takes no dead time')
     grid = parameters['grid']
@@ -206,9 +204,10 @@ def _hp_echo_on_grid(ss, I, qlist, parameters):
     U = sla.expm(-1j * float(parameters['echo_angle']) * eop[st][:, st].toarray())
     mode = parameters.get('mode', 'fp64')
     tau = float(parameters['echo_time'])
-    prop = _native.plan_powder(I, qlist, grid, rho, seed, tau, 1, tols=ss['tols'], mode=mode)
+    relax = S.relaxation_terms(ss)              # L = H + iR(Omega) in both evolutions and the acquisition
+    prop = _native.plan_powder(I, qlist, grid, rho, seed, tau, 1, tols=ss['tols'], mode=mode, relax=relax)
     acq = _native.plan_powder(I, qlist, grid, rho, coil, 1.0 / parameters['sweep'], int(parameters['npoints']) - 1,
-                              tols=ss['tols'], mode=mode)
+                              tols=ss['tols'], mode=mode, relax=relax)
     try:
         prop.set_initial_states(np.tile(rho, (n_or, 1)))
         states = prop.propagate(tau, 1, dim)
@@ -246,8 +245,6 @@ def _cpmg_on_grid(ss, I, qlist, parameters):
         raise ValueError('parameters.npoints should be a positive integer.')
     if not (np.isscalar(dt) and np.isreal(dt) and dt > 0):
         raise ValueError('parameters.timestep should be a positive real number.')
-    if ss.get('rlx') is not None:
-        raise NotImplementedError('cpmg with relaxation is not supported on this path')
     grid = parameters['grid']
     n_or = len(grid['weights'])
     dim = I.shape[0]
@@ -259,8 +256,9 @@ def _cpmg_on_grid(ss, I, qlist, parameters):
     seed[st] = 1.0
     U = sla.expm(-1j * np.pi * pop[st][:, st].toarray())    # step(pulse_op, ., pi) on the reachable states
     mode = parameters.get('mode', 'fp64')
-    prop = _native.plan_powder(I, qlist, grid, rho, seed, dt, 1, tols=ss['tols'], mode=mode)
-    plans = [_native.plan_powder(I, qlist, grid, rho, coil, dt, n - 1, tols=ss['tols'], mode=mode)
+    relax = S.relaxation_terms(ss)              # L = H + iR(Omega) between and during the acquisitions
+    prop = _native.plan_powder(I, qlist, grid, rho, seed, dt, 1, tols=ss['tols'], mode=mode, relax=relax)
+    plans = [_native.plan_powder(I, qlist, grid, rho, coil, dt, n - 1, tols=ss['tols'], mode=mode, relax=relax)
              for n in (npts, 2 * npts)]
     try:
         states = np.tile(rho, (n_or, 1))

@@ -113,3 +113,37 @@ def test_hp_acquire_echo_two_spin_vs_oracle():
     ep = max(O.rel_err(per[k], rper[k]) for k in range(12))
     print(f'hp_acquire echo, 2-spin: vs oracle {e:.2e} (per orientation {ep:.2e})', flush=True)
     assert e < 1e-10 and ep < 1e-10
+
+
+
+def test_cpmg_and_echo_with_relaxation_vs_oracle():
+    """cpmg (rate tensors and damping on the diagonal) and hp_acquire's echo (Redfield R0) with
+    L = H + iR between the pulses and during the acquisitions, against the oracle."""
+    T1 = np.array([[1.0, 0.3, -0.2], [0.3, 3.0, 0.5], [-0.2, 0.5, 5.0]]) * 40.0
+    relax = {'relaxation': ['t1_t2', 'damp'], 'r1_rates': [4.0, T1], 'r2_rates': [T1 * 2, 9.0],
+             'damp_rate': 25.0, 'rlx_keep': 'secular'}
+    w = workloads.static_2spin(npoints=32, n_orient=8, relax=relax)
+    ss = w['ss']
+    g = grids.haar(8, np.random.default_rng(9))
+    pop, rho0 = S.operator(ss, 'Ly', '1H'), S.state(ss, 'Lz', '1H')
+    params = {'rho0': rho0, 'coil': w['coil'], 'pulse_op': pop, 'timestep': w['dt'], 'npoints': 32, 'nloops': 2,
+              'grid': g}
+    fid, _ = P.powder(ss, P.cpmg, params, 'nmr')
+    ref = O.np_cpmg(w['I'], w['Q'], g, rho0, w['coil'], pop, w['dt'], 32, 2, rlx=ss['rlx'], states=ss['basis'])
+    free = O.np_cpmg(w['I'], w['Q'], g, rho0, w['coil'], pop, w['dt'], 32, 2)
+    e = O.rel_err(fid, ref)
+    print(f'CPMG with relaxation: vs oracle {e:.2e} (relaxation changes it by {O.rel_err(ref, free):.2f})', flush=True)
+    assert e < 1e-10 and O.rel_err(ref, free) > 1e-3
+    rf = {'relaxation': ['redfield'], 'tau_c': [5e-10], 'rlx_keep': 'kite', 'equilibrium': 'zero'}
+    w = workloads.static_2spin(npoints=64, n_orient=8, relax=rf)
+    ss = w['ss']
+    lx = S.operator(ss, 'Lx', '1H')
+    params = {'rho0': rho0, 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 64, 'grid': g, 'pulse_op': pop,
+              'pulse_angle': math.pi / 2, 'echo_time': 4.1e-5, 'echo_oper': lx, 'echo_angle': math.pi}
+    fid, _ = P.powder(ss, P.hp_acquire, params, 'nmr')
+    R0 = S.relaxation_terms(S.assume(ss, 'nmr'))['R0'].toarray()
+    ref = O.np_hp_echo(w['I'], w['Q'], g, rho0, w['coil'], pop, math.pi / 2, lx, math.pi, 4.1e-5, w['dt'], 63,
+                       R0=R0)
+    e = O.rel_err(fid, ref)
+    print(f'hp_acquire echo with Redfield relaxation: vs oracle {e:.2e}', flush=True)
+    assert e < 1e-10
