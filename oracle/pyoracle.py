@@ -356,7 +356,7 @@ def np_powder_relax(I, qlist, grid, rho0, coil, dt, nsteps, spec, states, reduce
     for o in range(n_or):
         eul = (grid['alphas'][o], grid['betas'][o], grid['gammas'][o])
         h = assemble_dense(Ir, qr, eul)
-        r_diag = relax_diag_at(spec, np.asarray(states)[st], eul, plant)
+        r_diag = relax_diag_at(spec, np.asarray(states)[st], eul, plant) if spec else np.zeros(len(st))
         Rm = np.diag(r_diag) if R0 is None else np.diag(r_diag) + np.asarray(R0)[np.ix_(st, st)]
         P = sla.expm(-1j * dt * (h + 1j * Rm))
         r = np.asarray(rho0)[st].astype(complex)

@@ -139,8 +139,6 @@ def singlerot(spin_system, pulse_sequence, parameters, assumptions):
     if not np.isscalar(parameters['rate']) or not np.isreal(parameters['rate']):
         raise ValueError('parameters.rate must be a real number.')
     P._grumble(parameters)
-    if spin_system.get('rlx') is not None:
-        raise NotImplementedError('relaxation in the Fokker-Planck MAS context is not supported on this path')
     ss = S.assume(spin_system, assumptions)
     I, Q = S.hamiltonian(ss)
     I = S.frqoffset(ss, I, parameters.get('offset'))
@@ -156,7 +154,11 @@ def singlerot(spin_system, pulse_sequence, parameters, assumptions):
     rho0 = np.kron(space, np.asarray(parameters['rho0'], complex))
     coil = np.kron(np.ones(N), np.asarray(parameters['coil'], complex))
     params = dict(parameters, rho0=rho0, coil=coil, grid=grid)
-    return P._acquire_on_grid(ss, I_fp, q_fp, params)
+    # relaxation (singlerot.m:137, 176-177): R = relaxation(spin_system) without angles,
+    # kron(1_N, R) in the rotor space
+    R = S.relaxation_isotropic(ss)
+    relax = None if R is None else {'R0': sps.kron(sps.identity(N, format='csr'), R, format='csr'), 'diag': None}
+    return P._acquire_on_grid(ss, I_fp, q_fp, params, relax=relax)
 
 
 def crystal(spin_system, pulse_sequence, parameters, assumptions):

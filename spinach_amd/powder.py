@@ -133,7 +133,7 @@ def _grid(g):
     raise ValueError(f'unknown grid specification {g!r}')
 
 
-def _acquire_on_grid(ss, I, qlist, parameters):
+def _acquire_on_grid(ss, I, qlist, parameters, relax='orientation'):
     """acquire() at every orientation of parameters['grid'] and the weighted
     sum (powder.m:226-294) on the GPU: dt = 1/sweep, nsteps = npoints-1
     (acquire.m:70-71), the dead time (acquire.m:65-67), spin_system.tols and
@@ -145,8 +145,10 @@ def _acquire_on_grid(ss, I, qlist, parameters):
     mode = parameters.get('mode', 'fp64')
     if 'krylov' in ss.get('enable', []):
         mode = mode + '+krylov'
-    # R(Omega) at each orientation (powder.m:263) in the library's form
-    relax = S.relaxation_terms(ss)
+    # R(Omega) at each orientation (powder.m:263) in the library's form, unless the caller
+    # passes its own (a context whose generator lives in a larger space)
+    if isinstance(relax, str):
+        relax = S.relaxation_terms(ss)
     plan = _native.plan_powder(I, qlist, grid, parameters['rho0'], parameters['coil'], dt, nsteps,
                                tols=ss['tols'], mode=mode, relax=relax)
     try:

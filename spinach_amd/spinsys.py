@@ -353,8 +353,25 @@ def relaxation(ss, euler=None):
             rates += lon.astype(float) @ r1 + tra.astype(float) @ r2
         if spec['damp'] is not None:
             unit = np.all(states == 0, axis=1)
-            rates += np.where(unit, 0.0, _rate_at(spec['damp'], euler))
+            d = spec['damp']
+            if euler is None and not np.isscalar(d):
+                d = float(np.trace(d)) / 3          # relaxation.m:652-655: the isotropic damping rate
+            rates += np.where(unit, 0.0, _rate_at(d, euler))
     return sps.diags(-rates, 0, shape=(dim, dim), format='csc')
+
+
+def relaxation_isotropic(ss):
+    """relaxation(spin_system) without Euler angles (kernel/relaxation.m as singlerot.m:137
+    calls it): scalar T1/T2 rates (rate tensors are an error, as in rlx_t1_t2.m), the mean of a
+    damping tensor's diagonal, and the Redfield term. Returns a sparse matrix, or None without
+    relaxation theories."""
+    spec = ss.get('rlx')
+    if spec is None:
+        return None
+    R = sps.csr_matrix(relaxation(ss, None), dtype=complex)
+    if spec['tau_c'] is not None:
+        R = R + _redfield_R0(ss)
+    return R.tocsr()
 
 
 def _clean_up_dense(A, tol):

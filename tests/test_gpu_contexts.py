@@ -100,6 +100,31 @@ def test_singlerot_two_spin_powder_vs_oracle():
     assert e < 1e-10
 
 
+def test_singlerot_with_relaxation_vs_oracle():
+    """singlerot.m:137, 176-177: R = relaxation(spin_system) without angles (scalar T1/T2 rates,
+    the mean damping rate, Redfield), kron(1_N, R) in the rotor space; the GPU against the
+    oracle on the same Fokker-Planck matrices with that R."""
+    relax = {'relaxation': ['t1_t2', 'damp', 'redfield'], 'r1_rates': [4.0, 2.0], 'r2_rates': [30.0, 9.0],
+             'damp_rate': np.diag([10.0, 20.0, 45.0]), 'tau_c': [3e-10], 'rlx_keep': 'kite'}
+    w = workloads.static_2spin(npoints=8, n_orient=2, relax=relax)
+    g = grids.haar(6, np.random.default_rng(10))
+    params = {'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1e5, 'npoints': 257, 'rate': 12500.0,
+              'axis': [1, 1, 1], 'max_rank': 3, 'grid': g}
+    fid, _ = Cx.singlerot(w['ss'], P.acquire, params, 'nmr')
+    ss = S.assume(w['ss'], 'nmr')
+    I, Q = S.hamiltonian(ss)
+    Ifp, qfp = Cx.fokker_planck(I, Q, params['axis'], params['rate'], 3)
+    r0 = np.kron(np.ones(7) / 7, w['rho0'])
+    c0 = np.kron(np.ones(7), w['coil'])
+    R = np.kron(np.eye(7), S.relaxation_isotropic(ss).toarray())
+    ref = O.np_powder_relax(Ifp, qfp, g, r0, c0, 1e-5, 256, {}, None, reduce=False, R0=R)
+    free = O.refcpu_powder(Ifp, qfp, g, r0, c0, 1e-5, 256, variant=0)
+    e = O.rel_err(fid, ref)
+    print(f'singlerot 2-spin with relaxation: vs oracle {e:.2e} (relaxation changes it by {O.rel_err(ref, free):.2f})',
+          flush=True)
+    assert e < 1e-10 and O.rel_err(ref, free) > 1e-3
+
+
 def test_hp_acquire_pulse_then_acquire_vs_closed_form():
     """hp_acquire: rho0 = Lz, pi/2 pulse about y, then the two-spin FID of
     rho0 = Lx (closed form)."""
