@@ -48,7 +48,7 @@ BOUND = {'spk::k_tridiag': 'valu', 'spk::k_dceig': 'valu', 'spk::k_synth': 'mfma
 # tridiagonalisation stage: k_tridiag_w and k_tridiag together) and profiles/k_mas_tiled_6spin_fp64.md
 TRAFFIC = {
     ('batch', 'fp64', 'spk::k_tridiag'): (178670707 + 159255) * 1024 * 2.0 + (344737580 + 2384462) * 1024 * 1.0,
-    ('static8', 'fp64', 'spk::k_tridiag'): 13775 * 1024 * 2.0 + 67885 * 1024 * 1.0,
+    ('static8', 'fp64', 'spk::k_tridiag'): 6203 * 1024 * 2.0 + 58669 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,false>'): 165530392 * 1024 * 2.0 + 53150413 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,true>'): 139595089 * 1024 * 2.0 + 49487957 * 1024 * 1.0,
     ('mas', 'fp64', 'spk::k_mas_tiled'): 3708907 * 1024 * 2.0 + 3615744 * 1024 * 1.0,
