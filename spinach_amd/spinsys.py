@@ -488,7 +488,8 @@ def _redfield_R0(ss):
     R = redfield_superoperator(ss)
     rmax = np.abs(np.diag(R)).max()
     for tc in spec['tau_c']:
-        if rmax > 0 and (1.0 / rmax) < np.max(tc):
+        # MATLAB's `if` on an array holds when every element does: the shortest time decides
+        if rmax > 0 and (1.0 / rmax) < np.min(tc):
             raise ValueError('T1,2>>tau_c validity condition violation in Redfield theory')
     if spec['dfs'] == 'ignore':
         R = R.real.astype(complex)

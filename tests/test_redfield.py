@@ -129,3 +129,12 @@ def test_redfield_dipolar_rates_match_solomon(tau_c):
     print(f'tau_c {tau_c:g}: rho {rho:.6g} vs {rho_ref:.6g}, sigma {sig:.6g} vs {sig_ref:.6g}, '
           f'R2 {r2:.6g} vs {r2_ref:.6g}')
     assert abs(rho / rho_ref - 1) < 1e-4 and abs(sig / sig_ref - 1) < 1e-4 and abs(r2 / r2_ref - 1) < 1e-4
+
+
+def test_redfield_validity_condition():
+    """relaxation.m:126-134: a correlation time longer than the fastest relaxation time is an
+    error (a slow-tumbling CSA spin at 10 ms); an axial pair of times errs only when both are."""
+    with pytest.raises(ValueError, match='validity'):
+        S.relaxation_terms(_one_spin([-2000.0, -500.0, 2500.0], 1e-2))
+    R = S.relaxation_terms(_one_spin([-20.0, -5.0, 25.0], [1e-9, 2e-9]))['R0']
+    assert R.nnz > 0
