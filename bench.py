@@ -89,12 +89,13 @@ def mas_flops_per_orientation(n, nsteps, ticks, taylor_k, ps_q, sexp):
 
 
 def mas_tiled_flops(products, ndet, n_or, nsteps):
-    """The same algorithm on 16 x 16 tiles: the complex tile products the kernel counted (per
+    """The same algorithm on 16 x 16 tiles: the real tile products the kernel counted (per
     tick the expm of every tile at its own Taylor degree, the U update and C_j = U_s' coil_st
-    U_t per tile pair; per rotor period two products per pair for the rho chain; 8 * 16^3
-    FLOP each), plus the detection of every time point over the ndet elements where both C_j
-    and rho_k can be nonzero."""
-    return products * 8.0 * 16 ** 3 + n_or * (nsteps + 1) * 8.0 * ndet
+    U_t per tile pair, W = coil_st U_t in two real products when the coil is real; per rotor
+    period two complex products per pair for the rho chain; a complex product is four real
+    ones, 2 * 16^3 FLOP each), plus the detection of every time point over the ndet elements
+    where both C_j and rho_k can be nonzero."""
+    return products * 2.0 * 16 ** 3 + n_or * (nsteps + 1) * 8.0 * ndet
 
 
 def spectral_stage_flops(n, npts):

@@ -126,7 +126,8 @@ typedef struct sp_stats {
                                 eigensolver, synthesis */
   int64_t step_products;     /* sparse matrix-vector products the stepping path made in the pass
                                 (Taylor terms over all substeps, steps and orientations);
-                                sp_mas_fid on tiles: the 16 x 16 complex tile products */
+                                sp_mas_fid on tiles: the 16 x 16 real tile products (a
+                                complex product counts 4) */
   double plan_ms;            /* host planning (reduction, layouts) and upload of the plan */
   int32_t eig_fallbacks;     /* spectral orientations whose tridiagonal eigensolution did not
                                 converge, re-run on the propagator path (real products) */

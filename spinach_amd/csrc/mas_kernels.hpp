@@ -332,6 +332,7 @@ struct MasTParams {
   const double* vals;    // (1+nq)*nnz interleaved complex, entry order as epos: I, then Q slots
   const int32_t* slots;  // Q slot ids
   int32_t npair;
+  int32_t coil_real;      // every coil tile is real: W = coil_st U_t takes two real products, not four
   const int32_t* pair_s;  // tile pairs where both coil and rho0 have entries
   const int32_t* pair_t;
   const double* coil_t;  // npair x 256 interleaved complex, the tile row-major
@@ -357,7 +358,7 @@ struct MasTParams {
   unsigned int* queue;
   double* per_or;
   int32_t* sq_out;
-  int32_t* prod_out;  // per orientation: 16 x 16 complex tile products made
+  int32_t* prod_out;  // per orientation: 16 x 16 real tile products made (a complex product counts 4)
 };
 
 struct MasTShared {
@@ -397,6 +398,18 @@ __device__ __forceinline__ CTile cmm16(const double (&ar)[4], const double (&ai)
   for (int s = 0; s < 4; s++) {
     dr = Mma<double>::fma(-ai[s], b.i[s], dr);
     di = Mma<double>::fma(ai[s], b.r[s], di);
+  }
+  return CTile{dr, di};
+}
+
+// D = A * B with A real (its A-operand chunks ar), B complex in the C layout: 8 MFMAs
+__device__ __forceinline__ CTile rmm16(const double (&ar)[4], const CTile& b) {
+  typedef Mma<double>::v4 v4;
+  v4 dr = v4{0, 0, 0, 0}, di = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    dr = Mma<double>::fma(ar[s], b.r[s], dr);
+    di = Mma<double>::fma(ar[s], b.i[s], di);
   }
   return CTile{dr, di};
 }
@@ -602,7 +615,8 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
             cai[s2] = g[2 * (cq * kTileN + 4 * s2 + rq) + 1];
           }
         }
-        const CTile W = cmm16(car, cai, tile_load_c(sh.u.t.U[t], lane));
+        const CTile W = p.coil_real ? rmm16(car, tile_load_c(sh.u.t.U[t], lane))
+                                    : cmm16(car, cai, tile_load_c(sh.u.t.U[t], lane));
         // A operand of U_s' (conjugate transpose): chunk s2, lane l: conj(U_s[4 s2 + l/16][l%16])
         double uar[4], uai[4];
         {
@@ -614,7 +628,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
           }
         }
         const CTile C = cmm16(uar, uai, W);
-        nprod += 2;
+        nprod += p.coil_real ? 6 : 8;
         double* cr = Cre + (size_t)j * L;
         double* ci = Cim + (size_t)j * L;
         const int32_t* em = p.emap + (size_t)pp * 256;
@@ -732,7 +746,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
         tile_aop(acc, scr, lane, par, pai);
         const CTile Un = cmm16(par, pai, tile_load_c(sh.u.t.U[t], lane));
         tile_store(sh.u.t.U[t], Un, lane);
-        nprod += (qt - 1) + rt + sexp + 1;
+        nprod += 4 * ((qt - 1) + rt + sexp + 1);
       }
     }
     __syncthreads();  // U = U_ticks, the period propagator, on every tile
@@ -752,7 +766,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_mas_tiled(MasTParams p) {
           double tar[4], tai[4];
           tile_aop(T, scr, lane, tar, tai);
           rho = cmm16(tar, tai, Ut);
-          nprod += 2;
+          nprod += 8;
         }
         double* rr_ = Rre + (size_t)k * L;
         double* ri_ = Rim + (size_t)k * L;

@@ -182,6 +182,7 @@ struct sp_plan {
   // pattern's tile-local positions and per-tile ranges, the tile pairs of coil and rho0
   bool mas_tiled = false;
   int32_t mas_ntile = 0, mas_npair = 0, mas_ldet = 0;  // ldet: detected elements, padded to 16
+  bool mas_coil_real = false;
   DevBuf d_epos, d_toff, d_pair_s, d_pair_t, d_coil_t, d_rho0_t, d_emap, d_nprod;
 };
 
@@ -1038,6 +1039,9 @@ sp_plan* build_mas_plan(sp_ctx* ctx, const sp_csc* I, const sp_csc* Q, const dou
               if (ccomp[cp] && rcomp[cp]) emap[pp * 256 + r * spk::kTileN + c] = ndet++;
             }
         pl->mas_ldet = std::max<int32_t>(16, (ndet + 15) / 16 * 16);
+        pl->mas_coil_real = true;
+        for (size_t k = 1; k < ct.size(); k += 2)
+          if (ct[k] != 0.0) pl->mas_coil_real = false;
         upload(pl->d_emap, emap, st);
         pl->mas_tiled = true;
         pl->np_max = spk::kTileN;  // reported as the padded dimension: every product is 16 x 16
@@ -1146,6 +1150,7 @@ void launch_mas_tiled(sp_plan* pl) {
   kp.vals = pl->d_vals.as<double>();
   kp.slots = pl->d_slots.as<int32_t>();
   kp.npair = pl->mas_npair;
+  kp.coil_real = pl->mas_coil_real ? 1 : 0;
   kp.pair_s = pl->d_pair_s.as<int32_t>();
   kp.pair_t = pl->d_pair_t.as<int32_t>();
   kp.coil_t = pl->d_coil_t.as<double>();
