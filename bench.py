@@ -51,7 +51,7 @@ TRAFFIC = {
     ('static8', 'fp64', 'spk::k_tridiag'): 6203 * 1024 * 2.0 + 58669 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,false>'): 165530392 * 1024 * 2.0 + 53150413 * 1024 * 1.0,
     ('static8', 'fp64', 'spk::k_powder<double,true>'): 139595089 * 1024 * 2.0 + 49487957 * 1024 * 1.0,
-    ('mas', 'fp64', 'spk::k_mas_tiled'): 3708907 * 1024 * 2.0 + 3615744 * 1024 * 1.0,
+    ('mas', 'fp64', 'spk::k_mas_tiled'): 4100499 * 1024 * 2.0 + 3624603 * 1024 * 1.0,
 }
 
 
