@@ -337,13 +337,16 @@ def gridfree(spin_system, pulse_sequence, parameters, assumptions):
     mr = int(parameters['max_rank'])
     if mr < 1:
         raise ValueError('max_rank must be a real positive integer.')
-    if spin_system.get('rlx') is not None:
-        raise NotImplementedError('spin relaxation theories in the SLE context are not supported on this path')
     ss = S.assume(spin_system, assumptions)
     I, Q = S.hamiltonian(ss)
     I = S.frqoffset(ss, I, parameters.get('offset'))
     H, R, spc = sle_generator(I, Q, mr, parameters.get('rate', 0.0), parameters.get('axis'),
                               parameters.get('tau_c'))
+    # spin relaxation (gridfree.m:95-96, 105): relaxation(spin_system) without angles, kron(1, R)
+    Rs = S.relaxation_isotropic(ss)
+    if Rs is not None:
+        Rk = sps.kron(sps.identity(spc, format='csr'), Rs, format='csr')
+        R = Rk if R is None else (R + Rk).tocsr()
     e0 = np.zeros(spc)
     e0[0] = 1.0
     grid = {'alphas': np.zeros(1), 'betas': np.zeros(1), 'gammas': np.zeros(1), 'weights': np.ones(1)}

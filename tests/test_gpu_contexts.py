@@ -199,15 +199,23 @@ def test_gridfree_diffusion_and_spinning_two_spin_vs_dense():
     w = workloads.static_2spin(npoints=64, n_orient=4)
     ss = w['ss']
     I, Q = S.hamiltonian(S.assume(ss, 'nmr'))
-    for extra in ({'tau_c': 2e-8}, {'tau_c': np.diag([1e-8, 2e-8, 4e-8]), 'rate': 5000.0, 'axis': [0, 1, 1]}):
+    wr = workloads.static_2spin(npoints=64, n_orient=4, relax={'relaxation': ['t1_t2', 'damp'], 'r1_rates': [4.0, 2.0],
+                                                               'r2_rates': [30.0, 9.0], 'damp_rate': 15.0,
+                                                               'rlx_keep': 'secular'})
+    for extra in ({'tau_c': 2e-8}, {'tau_c': np.diag([1e-8, 2e-8, 4e-8]), 'rate': 5000.0, 'axis': [0, 1, 1]},
+                  {'tau_c': 2e-8, 'spin_relaxation': True}):
+        sr = extra.pop('spin_relaxation', False)
+        ss = wr['ss'] if sr else w['ss']
         params = dict({'rho0': w['rho0'], 'coil': w['coil'], 'sweep': 1 / w['dt'], 'npoints': 64, 'max_rank': 3},
                       **extra)
         fid = Cx.gridfree(ss, P.acquire, params, 'nmr')
         H, R, spc = Cx.sle_generator(I, Q, 3, extra.get('rate', 0.0), extra.get('axis'), extra['tau_c'])
+        if sr:  # gridfree.m:95-96, 105: kron(1, relaxation(spin_system)) joins the diffusion term
+            R = R.toarray() + np.kron(np.eye(spc), S.relaxation_isotropic(S.assume(ss, 'nmr')).toarray())
         e0 = np.zeros(spc)
         e0[0] = 1.0
         ref = sle_dense_fid(H, R, np.kron(e0, w['rho0']), np.kron(e0, w['coil']), w['dt'], 63)
         e = O.rel_err(fid, ref)
-        print(f'gridfree 2-spin {sorted(extra)}: vs dense oracle {e:.2e}, |f(end)|/|f(0)| '
+        print(f'gridfree 2-spin {sorted(extra)}{" + spin relaxation" if sr else ""}: vs dense oracle {e:.2e}, |f(end)|/|f(0)| '
               f'{abs(fid[-1]) / abs(fid[0]):.3f}', flush=True)
         assert e < 1e-10
