@@ -462,12 +462,13 @@ def np_hp_echo(I, qlist, grid, rho0, coil, pulse_op, pulse_angle, echo_oper, ech
 
 
 def np_sp_acquire(I, qlist, grid, rho0, coil, ex, ey, frq, pwr, dur, phi, rnk, dt, nsteps, per_orientation=False,
-                  plant=0):
+                  plant=0, rlx=None, states=None, R0=None):
     """experiments/sp_acquire.m:38-54 per orientation with one constant slice of
     kernel/pulses/shaped_pulse_af.m:46-110: N = 2*rnk+1 collocation phases
     phi_n = 2*pi*n/N + phi; F = kron(1_N, H) + pwr*(kron(diag(cos phi_n), Ex) +
     kron(diag(sin phi_n), Ey)) + 2i*pi*frq*kron(D, 1); rho_F = expm(-i*dur*F)
-    kron(e_1, rho0); rho = the sum of its N phase blocks; then acquire()."""
+    kron(e_1, rho0); rho = the sum of its N phase blocks; then acquire(). With relaxation H is
+    H + iR(Omega) in both (sp_acquire.m: L = H + 1i*R, kron(1_N, L) in the phase space)."""
     N = 2 * int(rnk) + 1
     ph, D = fourdif1(N)
     ph = ph + phi
@@ -483,7 +484,7 @@ def np_sp_acquire(I, qlist, grid, rho0, coil, ex, ey, frq, pwr, dur, phi, rnk, d
     per = np.zeros((n_or, nsteps + 1), complex)
     c = np.asarray(coil, complex)
     for o in range(n_or):
-        h = assemble_dense(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]))
+        h = _gen(I, qlist, (grid['alphas'][o], grid['betas'][o], grid['gammas'][o]), rlx, states, R0)
         F = np.kron(np.eye(N), h) + pwr * F1 + 2j * math.pi * frq * M
         rf = (sla.expm(-1j * dur * F) @ r0).reshape(N, dim)
         r = rf[0].copy() if plant == PLANT_FOLD else rf.sum(axis=0)

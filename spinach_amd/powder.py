@@ -316,8 +316,6 @@ def _sp_acquire_on_grid(ss, I, qlist, parameters):
     rnk = parameters['pulse_rnk']
     if not (isinstance(rnk, (int, np.integer)) and rnk >= 0):
         raise ValueError('parameters.pulse_rnk must be a non-negative integer.')
-    if ss.get('rlx') is not None:
-        raise NotImplementedError('sp_acquire with relaxation is not supported on this path')
     spin = parameters['spins'][0]
     off = (parameters.get('offset') or {}).get(spin, 0.0)
     frq = float(parameters['pulse_frq']) - float(off)
@@ -343,7 +341,15 @@ def _sp_acquire_on_grid(ss, I, qlist, parameters):
     k = max(1, int(np.ceil(parameters['pulse_dur'] / dt_acq)))
     dur = float(parameters['pulse_dur'])
     mode = parameters.get('mode', 'fp64')
-    plan = _native.plan_powder(I_fp.tocsr(), q_fp, grid, rho_fp, seed, dur / k, k, tols=ss['tols'], mode=mode)
+    # relaxation (sp_acquire.m: L = H + 1i*R, kron(1_N, L) in the phase space): R0 and the
+    # per-orientation diagonal planes replicated over the phase blocks
+    rl = S.relaxation_terms(ss)
+    relax_fp = None
+    if rl is not None:
+        relax_fp = {'R0': None if rl['R0'] is None else sps.kron(one, rl['R0'], format='csr'),
+                    'diag': None if rl['diag'] is None else np.tile(rl['diag'], (1, N))}
+    plan = _native.plan_powder(I_fp.tocsr(), q_fp, grid, rho_fp, seed, dur / k, k, tols=ss['tols'], mode=mode,
+                               relax=relax_fp)
     try:
         st_fp = plan.propagate(dur / k, k, N * dim)
     finally:

@@ -147,3 +147,30 @@ def test_cpmg_and_echo_with_relaxation_vs_oracle():
     e = O.rel_err(fid, ref)
     print(f'hp_acquire echo with Redfield relaxation: vs oracle {e:.2e}', flush=True)
     assert e < 1e-10
+
+
+
+def test_sp_acquire_with_relaxation_vs_oracle():
+    """sp_acquire under L = H + iR(Omega) (rate tensors and damping): the relaxation planes
+    replicated over the Fokker-Planck phase blocks during the pulse, and the acquisition with R."""
+    T1 = np.array([[1.0, 0.3, -0.2], [0.3, 3.0, 0.5], [-0.2, 0.5, 5.0]]) * 400.0
+    relax = {'relaxation': ['t1_t2', 'damp'], 'r1_rates': [400.0, T1], 'r2_rates': [T1 * 2, 900.0],
+             'damp_rate': 2500.0, 'rlx_keep': 'secular'}
+    w = workloads.static_2spin(npoints=128, n_orient=6, relax=relax)
+    ss = w['ss']
+    g = grids.haar(6, np.random.default_rng(12))
+    ep = S.operator(ss, 'L+', '1H')
+    ex, ey = (ep + ep.conj().T) / 2, (ep - ep.conj().T) / 2j
+    rho0 = S.state(ss, 'Lz', '1H')
+    pulse = dict(spins=['1H'], pulse_frq=3.0e3, pulse_pwr=2 * math.pi * 2.0e4, pulse_dur=1.25e-5, pulse_phi=0.4,
+                 pulse_rnk=2)
+    params = dict(pulse, rho0=rho0, coil=w['coil'], sweep=1 / w['dt'], npoints=128, grid=g)
+    fid, _ = P.powder(ss, P.sp_acquire, params, 'nmr')
+    ref = O.np_sp_acquire(w['I'], w['Q'], g, rho0, w['coil'], ex, ey, 3.0e3, pulse['pulse_pwr'], 1.25e-5, 0.4, 2,
+                          w['dt'], 127, rlx=ss['rlx'], states=ss['basis'])
+    free = O.np_sp_acquire(w['I'], w['Q'], g, rho0, w['coil'], ex, ey, 3.0e3, pulse['pulse_pwr'], 1.25e-5, 0.4, 2,
+                           w['dt'], 127)
+    e = O.rel_err(fid, ref)
+    print(f'sp_acquire with relaxation: vs oracle {e:.2e} (relaxation changes it by {O.rel_err(ref, free):.2f})',
+          flush=True)
+    assert e < 1e-10 and O.rel_err(ref, free) > 1e-3
