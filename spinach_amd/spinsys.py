@@ -246,13 +246,22 @@ def create(sys, inter):
                 _add(n, k, d)
 
     rlx = _relaxation_spec(inter, nspins)
+    # chemical species (create.m: inter.chem.parts; one species of every spin by default)
+    parts = [list(range(nspins))]
+    if inter.get('chem', {}).get('parts') is not None:
+        parts = [sorted(int(x) for x in pt) for pt in inter['chem']['parts']]
+        flat = sorted(x for pt in parts for x in pt)
+        if flat != list(range(nspins)):
+            raise ValueError('inter.chem.parts must assign every spin to exactly one species.')
+    if rlx is not None and rlx['tau_c'] is not None and len(rlx['tau_c']) != len(parts):
+        raise ValueError('inter.tau_c must have one entry per chemical species.')
 
     enable = list(sys.get('enable', []))  # sys.enable switches (create.m:1556); 'krylov' is read by powder()
     for e in enable:
         if e not in ('krylov', 'gpu', 'ham_cache'):
             raise ValueError(f'sys.enable option {e!r} is not supported on this path')
     return dict(isotopes=isotopes, nspins=nspins, magnet=magnet, gammas=gammas, enable=enable,
-                basefrqs=basefrqs, zeeman=zmat, coupling=coup, tols=tols, rlx=rlx,
+                basefrqs=basefrqs, zeeman=zmat, coupling=coup, tols=tols, rlx=rlx, parts=parts,
                 zeeman_strength=['full'] * nspins,
                 coupling_strength=[['strong'] * nspins for _ in range(nspins)])
 
@@ -306,15 +315,14 @@ def _relaxation_spec(inter, nspins):
     if 'redfield' in theories:
         if 'tau_c' not in inter:
             raise ValueError('Redfield theory requires rotational correlation time(s) in inter.tau_c.')
-        tcs = list(inter['tau_c'])
-        if len(tcs) != 1:
-            raise NotImplementedError('one chemical species (one entry of inter.tau_c) is supported')
-        tc = np.atleast_1d(np.asarray(tcs[0], float))
-        if tc.ndim != 1 or tc.size not in (1, 2, 3) or not np.all(np.isfinite(tc)):
-            raise ValueError('inter.tau_c entries must hold one, two or three correlation times.')
-        if not np.any(tc):
-            raise ValueError('zero rotational correlation times are not supported.')   # relaxation.m:102-107
-        spec['tau_c'] = [tc]
+        spec['tau_c'] = []
+        for t in list(inter['tau_c']):
+            tc = np.atleast_1d(np.asarray(t, float))
+            if tc.ndim != 1 or tc.size not in (1, 2, 3) or not np.all(np.isfinite(tc)):
+                raise ValueError('inter.tau_c entries must hold one, two or three correlation times.')
+            if not np.any(tc):
+                raise ValueError('zero rotational correlation times are not supported.')   # relaxation.m:102-107
+            spec['tau_c'].append(tc)
     return spec
 
 
@@ -435,7 +443,7 @@ def _expmint(A, B, C, T):
 
 def redfield_superoperator(ss):
     """The Redfield term of kernel/relaxation.m:96-137 (kernel/includes/redfield_integral_serial.m)
-    for one chemical species, before the dynamic-shift and retention policies: over the
+    over the chemical species, before the dynamic-shift and retention policies: over the
     lab-frame Hamiltonian's rotational components Q{n}{k,m} and each exponential (w, rate) of
     corrfun, R -= w * Q{n}{k,m} * int_0^T exp(-iBt) Q{n}{p,q}' exp(i(B - i rate)t) dt with
     B = clean_up(L0, rlx_integration/T), the on-shell kernel at zero shift, T =
@@ -446,10 +454,20 @@ def redfield_superoperator(ss):
     L0, Q = hamiltonian(assume(ss, 'labframe'))
     L0 = L0.toarray()
     dim = L0.shape[0]
-    outside = ~(ss['basis'].sum(axis=1) > 0)           # corrfun.m:52-54, one species
-    blk = np.ix_(outside, outside)
     R = np.zeros((dim, dim), complex)
-    tau = spec['tau_c'][0]
+    parts = ss.get('parts') or [list(range(ss['nspins']))]
+    for s_, part in enumerate(parts):
+        # the species' states: a non-unit state on one of its spins (corrfun.m:52-54)
+        outside = ~(ss['basis'][:, part].sum(axis=1) > 0)
+        R += _redfield_species(L0, Q, spec['tau_c'][s_], np.ix_(outside, outside), tols)
+    return R
+
+
+def _redfield_species(L0, Q, tau, blk, tols):
+    """One species' share of the Redfield sum (corrfun's weights and rates for its correlation
+    time, the block of states outside the species zeroed in every operand)."""
+    dim = L0.shape[0]
+    R = np.zeros((dim, dim), complex)
     for n in (1, 2):
         sz = 2 * n + 1
         comps = Q[n - 1]

@@ -138,3 +138,28 @@ def test_redfield_validity_condition():
         S.relaxation_terms(_one_spin([-2000.0, -500.0, 2500.0], 1e-2))
     R = S.relaxation_terms(_one_spin([-20.0, -5.0, 25.0], [1e-9, 2e-9]))['R0']
     assert R.nnz > 0
+
+
+def test_redfield_two_species_take_their_own_correlation_times():
+    """inter.chem.parts with one tau_c per species (corrfun.m:49-54): an uncoupled 1H and 13C,
+    each with a shielding tensor, in two species tumbling at 1 ns and 50 ps: each spin's rates are
+    the one-spin CSA rates at its own correlation time; one species for both would not be."""
+    eig = [[-20.0, -5.0, 25.0], [40.0, -10.0, -30.0]]
+    sys = {'magnet': 14.1, 'isotopes': ['1H', '13C']}
+    base = {'zeeman': {'eigs': eig, 'euler': [[0.4, 1.1, -0.7], [1.3, 0.2, 2.0]]}, 'relaxation': ['redfield'],
+            'rlx_keep': 'secular', 'equilibrium': 'zero'}
+    ss = S.basis(S.create(sys, dict(base, tau_c=[1e-9, 5e-11], chem={'parts': [[0], [1]]})),
+                 {'formalism': 'sphten-liouv', 'approximation': 'none'})
+    R = S.relaxation_terms(ss)['R0'].toarray()
+    for k, (iso, tc) in enumerate((('1H', 1e-9), ('13C', 5e-11))):
+        lz = S.state(ss, 'Lz', iso)
+        r1 = -np.vdot(lz, R @ lz).real / np.vdot(lz, lz).real
+        w0 = abs(ss['basefrqs'][k])
+        ref = w0 ** 2 * sum((e * 1e-6) ** 2 for e in eig[k]) * tc / (1 + (w0 * tc) ** 2) / 5
+        assert abs(r1 / ref - 1) < 1e-4, (iso, r1, ref)
+    one = S.basis(S.create(sys, dict(base, tau_c=[1e-9])), {'formalism': 'sphten-liouv', 'approximation': 'none'})
+    R1 = S.relaxation_terms(one)['R0'].toarray()
+    lz = S.state(one, 'Lz', '13C')
+    assert abs(np.vdot(lz, R1 @ lz).real / np.vdot(lz, R @ lz).real - 1) > 0.1
+    with pytest.raises(ValueError, match='one entry per chemical species'):
+        S.create(sys, dict(base, tau_c=[1e-9], chem={'parts': [[0], [1]]}))
