@@ -52,7 +52,7 @@ def test_reference_known_answers_scalar_rates_and_damping():
 @pytest.mark.parametrize('bad, exc', [
     ({'relaxation': ['lindblad'], 'rlx_keep': 'secular'}, NotImplementedError),
     ({'relaxation': ['redfield'], 'rlx_keep': 'secular'}, ValueError),                      # no tau_c
-    ({'relaxation': ['redfield'], 'rlx_keep': 'secular', 'tau_c': [1e-9, 2e-9]}, NotImplementedError),  # 2 species
+    ({'relaxation': ['redfield'], 'rlx_keep': 'secular', 'tau_c': [1e-9, 2e-9]}, ValueError),  # 2 times, 1 species
     ({'relaxation': ['t1_t2'], 'r1_rates': [1.0], 'r2_rates': [1.0], 'rlx_keep': 'secular',
       'equilibrium': 'IME'}, NotImplementedError),
     ({'relaxation': ['t1_t2'], 'r1_rates': [1.0], 'r2_rates': [1.0]}, ValueError),           # no rlx_keep
