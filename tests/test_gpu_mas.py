@@ -132,3 +132,18 @@ def test_mas_component_above_a_tile_takes_the_dense_kernel():
     e = O.rel_err(fid, ref)
     print(f"MAS 6 protons: mas_tiles {st['mas_tiles']}, padded {st['padded_dim']}, vs oracle {e:.2e}", flush=True)
     assert st['mas_tiles'] == 0 and st['padded_dim'] == 64 and e < 1e-10
+
+
+def test_mas_tiled_nothing_detected():
+    """coil Lz against rho0 Lx: the secular Hamiltonian keeps the transverse state transverse, so
+    no component block holds both and the detection sums over no element: the FID is zero (the
+    oracle's too), with one padded group of 16 columns in the detection product."""
+    ss, I, q, rho0, coil, g = _small(5)
+    lz = S.state(ss, 'Lz', '13C')
+    args = (I, q, g, rho0, lz, (1.0, 1.0, 1.0), 10000.0, 16, 40)
+    fid, _, st = _native.mas_fid(*args, mode='fp64')
+    ref = O.mas_powder(*args)
+    print(f"MAS nothing detected: max |f| {np.abs(fid).max():.1e}, oracle {np.abs(ref).max():.1e}, "
+          f"detected {st['mas_detected']}", flush=True)
+    assert st['mas_tiles'] >= 1 and st['mas_detected'] == 16
+    assert np.abs(fid).max() == 0.0 and np.abs(ref).max() < 1e-12
