@@ -163,3 +163,12 @@ def test_redfield_two_species_take_their_own_correlation_times():
     assert abs(np.vdot(lz, R1 @ lz).real / np.vdot(lz, R @ lz).real - 1) > 0.1
     with pytest.raises(ValueError, match='one entry per chemical species'):
         S.create(sys, dict(base, tau_c=[1e-9], chem={'parts': [[0], [1]]}))
+
+
+def test_redfield_dynamic_frequency_shifts():
+    """relaxation.m:533-552: rlx_dfs = 'keep' leaves the imaginary part (the dynamic frequency
+    shift) of the integral; 'ignore' (the default) takes the real part only."""
+    Rk = S.relaxation_terms(_one_spin([-20.0, -5.0, 25.0], 1e-9, keep='labframe', dfs='keep'))['R0'].toarray()
+    Ri = S.relaxation_terms(_one_spin([-20.0, -5.0, 25.0], 1e-9, keep='labframe'))['R0'].toarray()
+    assert np.abs(Rk.imag).max() > 1e-3 * np.abs(Rk.real).max()
+    assert np.abs(Ri.imag).max() == 0.0 and np.allclose(Rk.real, Ri.real, rtol=0, atol=1e-12)
