@@ -163,6 +163,42 @@ const char* sp_last_error(const sp_ctx* ctx);
  * the path choice the planner makes. */
 int sp_generator_is_real(const sp_csc* I, const sp_csc* Q, int32_t* is_real);
 
+/* ---- native host setup (no device needed) ------------------------------ */
+
+/* One term of the spin Hamiltonian (kernel/hamiltonian.m:673-690's descriptor): the product of
+ * one or two single-spin ISTs (ist: 1 = T(1,+1), 2 = T(1,0), 3 = T(1,-1); coef: the product of
+ * the operator names' coefficients, 'L+' = -sqrt(2) T(1,+1), 'L-' = sqrt(2) T(1,-1),
+ * 'Lz' = T(1,0)), its isotropic coefficient and its rank-1 (0..2) and rank-2 (3..7) IST
+ * coefficients and irreducible components, complex interleaved. */
+typedef struct sp_ham_term {
+  int32_t nspin;         /* 1 or 2 */
+  int32_t spin[2];       /* 0-based spin numbers */
+  int32_t ist[2];
+  int32_t pad;
+  double coef;
+  double iso[2];
+  double ist_coef[16];
+  double irr[16];
+} sp_ham_term;
+
+typedef struct sp_ham sp_ham; /* opaque: the built I and Q */
+
+/* I and the 34 Q slots of the Liouvillian commutation superoperator in the spin-1/2
+ * sphten-liouv basis (basis: dim rows of nspins IST indices 0..3, rows ascending as
+ * kernel/basis.m sorts them), each the sum over the terms of the coefficient times the term's
+ * commutation superoperator, quantised to tol (clean_up.m). NULL on failure (message from
+ * sp_last_error(NULL)). Replaces: kernel/hamiltonian.m:673-792 with the product-table
+ * construction of kernel/operators/superop.m:62-170. */
+sp_ham* sp_ham_build(int32_t nspins, const int8_t* basis, int64_t dim, const sp_ham_term* terms, int32_t nterms,
+                     double tol);
+
+/* Matrix `which` (0 = I, 1..34 = the Q slots in the order of sp_powder_fid's Q array) as a
+ * complex CSC view into h (valid until sp_ham_free). */
+int sp_ham_matrix(const sp_ham* h, int32_t which, sp_csc* out);
+
+/* Releases h (NULL is accepted). */
+void sp_ham_free(sp_ham* h);
+
 /* ---- one-shot call ----------------------------------------------------- */
 
 /* Replaces: kernel/contexts/powder.m:226-286 (parfor body with

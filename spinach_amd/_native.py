@@ -88,6 +88,75 @@ _ctx = {}
 _dp = C.POINTER(C.c_double)
 
 
+class SpHamTerm(C.Structure):
+    _fields_ = [('nspin', C.c_int32), ('spin', C.c_int32 * 2), ('ist', C.c_int32 * 2), ('pad', C.c_int32),
+                ('coef', C.c_double), ('iso', C.c_double * 2), ('ist_coef', C.c_double * 16),
+                ('irr', C.c_double * 16)]
+
+
+_OPNAME = {'L+': (1, -2 ** 0.5), 'Lz': (2, 1.0), 'L-': (3, 2 ** 0.5)}
+
+
+def build_hamiltonian(nspins, basis, descr, tol):
+    """sp_ham_build: I and Q (nested [rank1 3x3, rank2 5x5], None where empty) of the
+    Liouvillian commutation superoperator from the Hamiltonian's descriptors
+    (ops, spins, iso, ist[8], irr[8]), natively. basis: (dim, nspins) IST indices, rows
+    ascending."""
+    import numpy as np
+    import scipy.sparse as sps
+    lib = load()
+    if not getattr(lib, '_ham_ready', False):
+        lib.sp_ham_build.restype = C.c_void_p
+        lib.sp_ham_build.argtypes = [C.c_int32, C.POINTER(C.c_int8), C.c_int64, C.POINTER(SpHamTerm), C.c_int32,
+                                     C.c_double]
+        lib.sp_ham_matrix.restype = C.c_int
+        lib.sp_ham_matrix.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SpCsc)]
+        lib.sp_ham_free.restype = None
+        lib.sp_ham_free.argtypes = [C.c_void_p]
+        lib._ham_ready = True
+    b = np.ascontiguousarray(basis, dtype=np.int8)
+    dim = b.shape[0]
+    terms = (SpHamTerm * max(1, len(descr)))()
+    for i, (ops, spins, iso, ist, irr) in enumerate(descr):
+        t = terms[i]
+        t.nspin = len(ops)
+        coef = 1.0
+        for a, (name, sp) in enumerate(zip(ops, spins)):
+            idx, c = _OPNAME[name]
+            t.spin[a] = int(sp)
+            t.ist[a] = idx
+            coef *= c
+        t.coef = coef
+        t.iso[0], t.iso[1] = complex(iso).real, complex(iso).imag
+        for k in range(8):
+            t.ist_coef[2 * k], t.ist_coef[2 * k + 1] = complex(ist[k]).real, complex(ist[k]).imag
+            t.irr[2 * k], t.irr[2 * k + 1] = complex(irr[k]).real, complex(irr[k]).imag
+    h = lib.sp_ham_build(int(nspins), b.ctypes.data_as(C.POINTER(C.c_int8)), int(dim), terms, len(descr),
+                         float(tol))
+    if not h:
+        raise SpinachPowderError(-1, 'sp_ham_build: ' + lib.sp_last_error(None).decode())
+    try:
+        mats = []
+        for w in range(35):
+            m = SpCsc()
+            if lib.sp_ham_matrix(h, w, C.byref(m)):
+                raise SpinachPowderError(-1, 'sp_ham_matrix failed')
+            nnz = int(m.jc[dim]) if dim > 0 else 0
+            jc = np.ctypeslib.as_array(m.jc, shape=(dim + 1,)).astype(np.int64)
+            if nnz:
+                ir = np.ctypeslib.as_array(m.ir, shape=(nnz,)).astype(np.int64)
+                pr = np.ctypeslib.as_array(m.pr, shape=(2 * nnz,)).copy().view(np.complex128)
+                mats.append(sps.csc_matrix((pr, ir, jc), shape=(dim, dim)).tocsr())
+            else:
+                mats.append(None)
+    finally:
+        lib.sp_ham_free(h)
+    I = mats[0] if mats[0] is not None else sps.csr_matrix((dim, dim), dtype=complex)
+    Q = [[mats[1 + 3 * k + m] for m in range(3)] for k in range(3)], \
+        [[mats[10 + 5 * k + m] for m in range(5)] for k in range(5)]
+    return I, [Q[0], Q[1]]
+
+
 def load():
     """Load the library and declare its signatures (no GPU call)."""
     global _lib

@@ -33,6 +33,7 @@
 #include "spectral_kernels.hpp"
 #include "step_kernels.hpp"
 #include "planner.hpp"
+#include "builder.hpp"
 
 // Device memory of sp_powder_fid_noalloc, which calls no allocator: the staged
 // inputs, per-item spectra, per-orientation FIDs, orientation-sum partials and
@@ -1828,6 +1829,60 @@ int sp_generator_is_real(const sp_csc* I, const sp_csc* Q, int32_t* is_real) {
     return SP_EINVAL;
   }
 }
+
+struct sp_ham {
+  std::vector<spb::Csc> m;
+};
+
+sp_ham* sp_ham_build(int32_t nspins, const int8_t* basis, int64_t dim, const sp_ham_term* terms, int32_t nterms,
+                     double tol) {
+  try {
+    if (!basis || dim < 1 || nterms < 0 || (nterms > 0 && !terms)) throw SpError(SP_EINVAL, "empty basis or NULL terms");
+    if (!(tol > 0) || !std::isfinite(tol)) throw SpError(SP_EINVAL, "tol must be positive and finite");
+    std::vector<int8_t> digits(basis, basis + (size_t)dim * nspins);
+    std::vector<spb::Term> tv((size_t)nterms);
+    for (int32_t i = 0; i < nterms; i++) {
+      const sp_ham_term& a = terms[i];
+      spb::Term& t = tv[(size_t)i];
+      t.nspin = a.nspin;
+      for (int k = 0; k < 2; k++) {
+        t.spin[k] = a.spin[k];
+        t.ist[k] = a.ist[k];
+      }
+      t.coef = a.coef;
+      t.iso = spb::cd(a.iso[0], a.iso[1]);
+      for (int k = 0; k < 8; k++) {
+        t.istc[k] = spb::cd(a.ist_coef[2 * k], a.ist_coef[2 * k + 1]);
+        t.irr[k] = spb::cd(a.irr[2 * k], a.irr[2 * k + 1]);
+      }
+    }
+    std::unique_ptr<sp_ham> h(new sp_ham);
+    h->m = spb::build_hamiltonian(nspins, digits, dim, tv, tol);
+    return h.release();
+  } catch (SpError& e) {
+    g_create_err = e.what();
+  } catch (std::invalid_argument& e) {
+    g_create_err = e.what();
+  } catch (std::bad_alloc&) {
+    g_create_err = "host allocation failed";
+  } catch (std::exception& e) {
+    g_create_err = e.what();
+  }
+  return nullptr;
+}
+
+int sp_ham_matrix(const sp_ham* h, int32_t which, sp_csc* out) {
+  if (!h || !out || which < 0 || which >= (int32_t)h->m.size()) return SP_EINVAL;
+  const spb::Csc& c = h->m[(size_t)which];
+  out->n = c.n;
+  out->jc = c.jc.data();
+  out->ir = c.ir.data();
+  out->pr = c.pr.data();
+  out->is_complex = 1;
+  return SP_OK;
+}
+
+void sp_ham_free(sp_ham* h) { delete h; }
 
 const char* sp_last_error(const sp_ctx* ctx) {
   if (!ctx) return g_create_err.c_str();

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import itertools
 import math
+import os
 
 import numpy as np
 import scipy.sparse as sps
@@ -894,7 +895,12 @@ def _hamiltonian(ss):
         for ops in set(iso_terms) | set(ist):
             descr.append((ops, (l, s), iso_terms.get(ops, 0.0),
                           _vec8(ist.get(ops, {})), irr))
-    # operators (hamiltonian.m:673-690), assembly of I (:718-738) and Q (:741-792)
+    # operators (hamiltonian.m:673-690), assembly of I (:718-738) and Q (:741-792): natively
+    # (sp_ham_build) in the sphten-liouv basis; in Python for zeeman-hilb, or with
+    # SPINACH_PY_BUILDER=1 (the restatement the native builder is tested against)
+    if ss.get('formalism') != 'zeeman-hilb' and os.environ.get('SPINACH_PY_BUILDER', '0') != '1':
+        from . import _native
+        return _native.build_hamiltonian(n, ss['basis'], descr, tol)
     dim = 2 ** n if ss.get('formalism') == 'zeeman-hilb' else len(ss['basis_keys'])
     opers = []
     for ops, sp_, iso, ist, irr in descr:

@@ -28,7 +28,7 @@ def test_fp32_storage_floor_on_the_8spin_workload():
     nst = w['nsteps']
     for o in (0, 1):
         H = _orientation(w, st, o)
-        assert np.abs(H.imag).max() == 0.0
+        assert np.abs(H.imag).max() <= 1e-15 * np.abs(H).max()   # real to rounding
         lam, V = np.linalg.eigh(w['dt'] * H.real)
         gk = (V.T @ c.conj()) * (V.T @ r)
         t = np.arange(nst + 1)

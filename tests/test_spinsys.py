@@ -13,6 +13,7 @@ import scipy.linalg as sla
 
 import pyoracle as O
 from spinach_amd import spinsys as S
+from spinach_amd import workloads
 
 
 def _system(isotopes, seed=5, coords=None, scalar=None, level=None):
@@ -232,3 +233,35 @@ def test_dipolar_constant_and_direction():
     assert abs(b - 1.43e5) / 1.43e5 < 0.02  # ~22.7 kHz * 2pi
     D = ss['coupling'][0][1] + ss['coupling'][1][0]
     assert np.allclose(D, b * np.diag([1, 1, -2]), rtol=1e-12)
+
+
+@pytest.mark.parametrize('approx, level, iso', [('none', None, ['1H', '13C', '15N']),
+                                                ('IK-0', 3, ['1H'] * 5 + ['13C'] * 2 + ['15N'])])
+def test_native_builder_equals_the_restatement(approx, level, iso, monkeypatch):
+    """sp_ham_build (the native host setup of I and Q) against the Python restatement of
+    hamiltonian.m and superop.m it replaces, on a complete and an IK-0 basis with shielding
+    tensors, dipolar and J couplings: the same patterns, values to 1e-15 relative plus one
+    quantum of the clean-up grid (liouv_zero = 1e-10: a value whose quotient by the grid step lies
+    within rounding of a half-step may round to either neighbour)."""
+    rng = np.random.default_rng(7)
+    inter = {'zeeman': {'eigs': [list(rng.uniform(-100, 100, 3)) for _ in iso],
+                        'euler': [list(rng.uniform(0, 6, 3)) for _ in iso]},
+             'coupling': {'scalar': {(0, 1): 140.0}},
+             'coordinates': [np.array(x) for x in workloads._coords(rng, len(iso))]}
+    bas = {'formalism': 'sphten-liouv', 'approximation': approx}
+    if level:
+        bas['level'] = level
+    ss = S.assume(S.basis(S.create({'magnet': 14.1, 'isotopes': iso}, inter), bas), 'nmr')
+    I1, Q1 = S.hamiltonian(ss)
+    monkeypatch.setenv('SPINACH_PY_BUILDER', '1')
+    I2, Q2 = S.hamiltonian(ss)
+    scale = abs(I2).max()
+    q = 1.01e-10
+    assert (I1 != 0).sum() == (I2 != 0).sum() and abs(I1 - I2).max() <= 1e-15 * scale + q
+    for r in range(2):
+        for k in range(2 * r + 3):
+            for m in range(2 * r + 3):
+                a, b = Q1[r][k][m], Q2[r][k][m]
+                assert (a is None) == (b is None), (r, k, m)
+                if a is not None:
+                    assert (a != 0).sum() == (b != 0).sum() and abs(a - b).max() <= 1e-15 * abs(b).max() + q
