@@ -10,7 +10,11 @@ UndefinedBehaviorSanitizer:
   convention); a leak or invalid access fails the sanitizers;
 * the host planner (spinach_amd/csrc/planner.hpp) on workload systems and on
   corrupted inputs it must reject;
-* the oracle (oracle/refcpu.cpp) against its normal build.
+* the oracle (oracle/refcpu.cpp) against its normal build;
+* the native Hamiltonian builder (spinach_amd/csrc/builder.hpp) on
+  superoperators whose structure the algebra fixes (Zeeman diagonal,
+  flip-flop symmetry and projection conservation, rank-2 slot occupancy)
+  and on inputs it must reject.
 The host mirror of powder() (spinach_amd/powder.py) is checked for the
 arguments it must refuse before it reaches the library.
 """
@@ -262,6 +266,16 @@ def test_oracle_under_sanitizers(native, tmp_path):
             O.refcpu_powder(*args, variant=1, tols=O.make_tols(merge_dim=2))]
     for k, ref in enumerate(refs):
         assert O.rel_err(f[k], ref) < 1e-12, k
+
+
+def test_builder_under_sanitizers(native):
+    r = __import__('subprocess').run([os.path.join(native, 'builder_san')], capture_output=True, text=True,
+                                     timeout=600)
+    _sanitizer_clean(r)
+    lines = r.stdout.strip().splitlines()
+    print('\n'.join(lines))
+    assert [l.split()[0] for l in lines] == ['ZEEMAN', 'FLIPFLOP', 'PRUNED', 'ANISO'] + ['REJECT'] * 5 + ['DONE']
+    assert all(l.endswith('ok=1') for l in lines[:4]) and lines[-1] == 'DONE failures=0'
 
 
 # ------------------------------------------------------ host mirror checks
