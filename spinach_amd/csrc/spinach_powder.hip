@@ -141,6 +141,8 @@ struct sp_plan {
   int32_t krylov_systems = 0;
   int32_t step_vec_words = 0;            // LDS doubles of the four state vectors
   int32_t step_max_sell = 0;             // largest slice layout over the plan's systems
+  int32_t step_vec_global = 0;           // rho and the coil copy in the workgroup's global slot
+  int64_t step_gvec_off = 0;             // their offset in the slot (doubles)
   bool step_vl[2] = {false, false};      // [REAL] values staged in LDS
   int step_rcs = 0;                      // register-cached product: slices per wave (1, 2; 0: off)
   size_t step_lds[2] = {0, 0};           // [REAL] dynamic LDS bytes
@@ -289,6 +291,18 @@ void setup_step(sp_ctx* ctx, sp_plan* pl) {
     pl->step_vl[real] = with_vals <= kStepLdsMax;
     pl->step_lds[real] = pl->step_vl[real] ? with_vals : vec_bytes;
     if (!pl->step_vl[real]) work_words = std::max(work_words, (vw * (size_t)pl->step_max_sell + 1) / 2 * 2);
+  }
+  // values in global memory for both kinds and one workgroup per CU with four vectors in LDS:
+  // rho and the coil copy move to the workgroup's slot when that fits a second workgroup
+  pl->step_vec_global = 0;
+  pl->step_gvec_off = 0;
+  const size_t cu_lds = 160 * 1024, wg_static = 1024;
+  if (!pl->step_vl[0] && !pl->step_vl[1] && 2 * (vec_bytes + wg_static) > cu_lds &&
+      2 * (vec_bytes / 2 + wg_static) <= cu_lds && std::getenv("SPINACH_STEP_VEC_LDS") == nullptr) {
+    pl->step_vec_global = 1;
+    pl->step_gvec_off = (int64_t)work_words;
+    work_words += (size_t)pl->step_vec_words / 2;  // two of the four vectors
+    pl->step_lds[0] = pl->step_lds[1] = vec_bytes / 2;
   }
   // both variants of each kind may run in one pass
   if (pl->step_vl[0]) step_attr<false, true>(pl->step_lds[0]); else step_attr<false, false>(pl->step_lds[0]);
@@ -702,6 +716,8 @@ void launch_step_items(sp_plan* pl, bool real, const spk::Item* items, int64_t n
   sp.nsteps = nsteps;
   sp.final_state = final_state ? 1 : 0;
   sp.vec_words = pl->step_vec_words;
+  sp.vec_global = pl->step_vec_global;
+  sp.gvec_off = pl->step_gvec_off;
   sp.work = pl->d_step_work.as<double>();
   sp.work_stride = (int64_t)(pl->d_step_work.bytes / sizeof(double) / std::max(1, pl->grid_step));
   sp.per_or = pl->d_per.as<double>();

@@ -84,8 +84,10 @@ struct StepParams {
   double dt;              // time step
   int64_t nsteps;         // steps; the observable has nsteps+1 points
   int32_t final_state;    // 0: write f(t) to per_or; 1: write rho after nsteps steps to rho_out
-  int32_t vec_words;      // LDS doubles reserved for the four state vectors (8 * largest n)
-  double* work;           // per-workgroup value slots (values not staged in LDS)
+  int32_t vec_words;      // doubles of the four state vectors (8 * largest n, rounded up to 64)
+  int32_t vec_global;     // 1: rho and the coil copy live in the workgroup's slot at gvec_off
+  int64_t gvec_off;       //    (only the two term buffers in LDS: twice the workgroups per CU)
+  double* work;           // per-workgroup slots: values not staged in LDS, global state vectors
   int64_t work_stride;    // doubles per workgroup slot
   double* per_or;         // per-orientation FIDs, interleaved complex, nsteps+1 per orientation
   double* rho_out;        // final states (final_state == 1)
@@ -128,12 +130,15 @@ __global__ void __launch_bounds__(kStepThreads, 2) k_step(StepParams p) {  // tw
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int kVW = REAL ? 1 : 2;  // doubles per generator value
-  double* va = lds;                  // next term, read buffer (interleaved complex)
-  double* vb = lds + p.vec_words / 4;  // next term, write buffer
-  double* vr = lds + p.vec_words / 2;  // rho (each thread touches only its own rows)
-  double* vc = lds + 3 * (p.vec_words / 4);  // conj-free coil copy
-  double* lval = lds + p.vec_words;  // staged values (VL)
+  const int q4 = p.vec_words / 4;    // doubles of one state vector
   double* wslot = p.work + (size_t)blockIdx.x * p.work_stride;
+  double* va = lds;                  // next term, read buffer (interleaved complex)
+  double* vb = lds + q4;             // next term, write buffer
+  // rho: between barriers each thread touches only its own rows, so it may live in the
+  // workgroup's global slot as well as in LDS; the coil copy after it
+  double* vr = p.vec_global ? wslot + p.gvec_off : lds + 2 * q4;
+  double* vc = vr + q4;              // conj-free coil copy
+  double* lval = lds + (p.vec_global ? 2 : 4) * q4;  // staged values (VL)
   const double eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
   for (int k = tid; k <= kStepMaxTerms; k += kStepThreads) rk[k] = k > 0 ? 1.0 / (double)k : 0.0;
 

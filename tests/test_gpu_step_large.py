@@ -6,7 +6,8 @@ vectors, so every workgroup reads them from its own slot of a global workspace
 (the k_step<REAL, false> variant). Random 11- and 12-spin systems (IK-0 level 4)
 against the oracle's stepping restatement (refcpu step_liouv), real and complex
 generator arithmetic; the norm bound |coil'*rho(t)| <= |coil| |rho0| over a
-longer run; a dimension above 2304 is refused before anything is launched.
+longer run; rho and the coil copy in the workgroup's global slot (the default at
+these sizes: two workgroups per CU) and in LDS; a dimension above 2304 is refused before anything is launched.
 """
 import time
 
@@ -28,11 +29,14 @@ def _system(nspins, seed):
 
 @pytest.mark.parametrize('nspins,seed,dim', [(11, 1, 1419), (11, 2, 1690), (12, 2, 1806)])
 @pytest.mark.parametrize('mode', ['fp64', 'fp64+krylov+complex'])
-def test_stepping_above_the_dense_limit_vs_oracle(nspins, seed, dim, mode):
+@pytest.mark.parametrize('vectors', ['global', 'lds'])
+def test_stepping_above_the_dense_limit_vs_oracle(nspins, seed, dim, mode, vectors, monkeypatch):
+    if vectors == 'lds':
+        monkeypatch.setenv('SPINACH_STEP_VEC_LDS', '1')
     w = _system(nspins, seed)
     assert len(O.union_states(w['I'], w['Q'], w['coil'])) == dim
     g = grids.haar(3, np.random.default_rng(5))
-    print(f'[{nspins} spins, seed {seed}, {mode}] {dim} coil-connected states, 3 orientations x 64 steps',
+    print(f'[{nspins} spins, seed {seed}, {mode}, vectors in {vectors}] {dim} coil-connected states, 3 orientations x 64 steps',
           flush=True)
     plan = _native.plan_powder(w['I'], w['Q'], g, w['rho0'], w['coil'], DT, 64, mode=mode)
     st = plan.execute()
@@ -50,8 +54,9 @@ def test_stepping_above_the_dense_limit_vs_oracle(nspins, seed, dim, mode):
 
 def test_stepping_norm_bound_and_rate_at_1806_states():
     w = _system(12, 2)
-    g = grids.haar(256, np.random.default_rng(6))
-    nsteps = 256
+    n_or = 1024
+    g = grids.haar(n_or, np.random.default_rng(6))
+    nsteps = 128
     plan = _native.plan_powder(w['I'], w['Q'], g, w['rho0'], w['coil'], DT, nsteps)
     plan.execute()  # warm-up
     t0 = time.perf_counter()
@@ -64,8 +69,8 @@ def test_stepping_norm_bound_and_rate_at_1806_states():
     bound = np.linalg.norm(w['coil']) * np.linalg.norm(w['rho0'])
     assert np.all(np.abs(per) <= bound * (1 + 1e-10))
     assert np.all(np.isfinite(per))
-    print(f'1806 states, 256 orientations x {nsteps} steps: {sec * 1e3:.1f} ms, '
-          f'{256 * nsteps / sec:.3e} orientation-steps/s, {st["step_products"] / sec:.3e} products/s, '
+    print(f'1806 states, {n_or} orientations x {nsteps} steps: {sec * 1e3:.1f} ms, '
+          f'{n_or * nsteps / sec:.3e} orientation-steps/s, {st["step_products"] / sec:.3e} products/s, '
           f'stage ms {st["stage_ms"]}', flush=True)
 
 
